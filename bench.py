@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the point-source model-map path on one MI355X.
+
+A step is one pass of the whole path over the synthetic 100-source workload
+(200x200 CAR image, 0.05 deg pixels, 37 energies, 8x8 Gauss-Legendre pixel
+cubature): livetime gather + exposure contraction, mean-PSF build, and the
+model map (cubature, solid angle, map integral, correction factor,
+exposure).  All inputs are resident in HBM before the timed region; the
+output is a float32 [S, E, H, W] map.
+
+Prints one JSON line.  The CPU baseline is this project's FP64 C/OpenMP
+restatement of the reference algorithm (the reference itself does not build
+here), timed on the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "srcmap pix·E·src evals/s: surrogate vs cubature at ≤1e-4 max rel err, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup(cfg_name: str):
+    import torch
+    import xtsrcmaps_amd as xa
+    from xtsrcmaps_amd import synthetic
+
+    cfg = synthetic.CONFIGS[cfg_name]
+    inp = synthetic.make_inputs(cfg)
+    ctx = xa.Context(0)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    # IRF preparation (per IRF, not per source): normalisation and Aeff
+    king = {}
+    for side, (params, cos, loge) in inp.king_raw.items():
+        king[side] = (t(ctx.normalize_rpsf(params, loge, inp.psf_scale)), t(cos), t(loge))
+    aeff = {s: t(ctx.aeff_value(inp.costh, inp.logE, *g)) for s, g in inp.aeff_grid.items()}
+    ltf = t(xa.livetime_efficiency_factors(inp.logE, *inp.effic))
+    S, E, C = cfg.n_src, cfg.n_energy, len(inp.costh)
+    H = W = cfg.npix
+    d = dict(
+        src=t(inp.src_sph), cube=t(inp.cube), wcube=t(inp.wcube), costh=t(inp.costh),
+        logE=t(inp.logE), af=aeff["front"], ab=aeff["back"], ltf=ltf, king=king,
+        ecb=torch.empty((S, C), dtype=torch.float64, device=dev),
+        wecb=torch.empty((S, C), dtype=torch.float64, device=dev),
+        expo=torch.empty((S, E), dtype=torch.float64, device=dev),
+        upsf=torch.empty((S, 401, E), dtype=torch.float64, device=dev),
+        parint=torch.empty((S, 401, E), dtype=torch.float64, device=dev),
+        out=torch.empty((S, E, H, W), dtype=torch.float32, device=dev),
+    )
+    torch.cuda.synchronize()
+    return cfg, inp, ctx, d
+
+
+def step(ctx, inp, d, precision=0):
+    import xtsrcmaps_amd as xa
+    from xtsrcmaps_amd import capi
+    S = d["src"].shape[0]
+    L = capi.lib()
+    capi.check(L.xts_src_cosbins_dev(ctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["cube"]),
+                                     inp.nside, d["cube"].shape[1], capi.tptr(d["ecb"])))
+    capi.check(L.xts_src_cosbins_dev(ctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["wcube"]),
+                                     inp.nside, d["wcube"].shape[1], capi.tptr(d["wecb"])))
+    ctx.exposure_dev(d["ecb"], d["wecb"], d["af"], d["ab"], d["ltf"], d["expo"], precision)
+    ctx.psf_lookup_dev(d["king"]["front"], d["king"]["back"], d["costh"], d["logE"], d["af"],
+                       d["ab"], d["ecb"], d["wecb"], d["ltf"], d["expo"], d["upsf"], d["parint"],
+                       precision)
+    ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                    d["out"], inp.rule,
+                                    flags=xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32)
+
+
+def cpu_baseline(cfg, inp, gpu_map):
+    """FP64 oracle of the same workload on the host cores (the `port` kind)."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path[:0] = [here, os.path.join(here, "tests")]
+    from oracle import oracle as orc
+    from pipelines import oracle_stages, oracle_rule, oracle_wcs
+    orc.lib()
+    t0 = time.perf_counter()
+    st = oracle_stages(orc, inp)
+    w = oracle_wcs(orc, inp.wcs)
+    ref = orc.point_src_model_map_wcs(cfg.npix, cfg.npix, w, st["src"], st["expo"], st["upsf"],
+                                      st["parint"], oracle_rule(inp))
+    dt = time.perf_counter() - t0
+    got = gpu_map.transpose(0, 2, 3, 1)
+    rel = np.abs(got.astype(np.float64) - ref) / np.abs(ref)
+    return dict(value=cfg.evals / dt, unit="evals/s", cores=orc.num_threads(), kind="port",
+                sample=f"the full {cfg.name} workload ({cfg.n_src} sources), exposure + mean "
+                       f"PSF + model map, FP64, OpenMP schedule(static,16) over sources, "
+                       f"{dt:.1f} s"), float(rel.max()), float(np.median(rel))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 BF16x3 contractions")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by source; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    _log(f"bench: setup {args.config}")
+    cfg, inp, ctx, d = setup(args.config)
+    _log("bench: warmup")
+    for _ in range(args.warmup):
+        step(ctx, inp, d, args.precision)
+    torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    ctx.kernel_times("cubature")
+    ctx.kernel_times("scale")
+    _log(f"bench: timing {args.steps} steps")
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        step(ctx, inp, d, args.precision)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    cub_ms, cub_n = ctx.kernel_times("cubature")
+    sc_ms, sc_n = ctx.kernel_times("scale")
+    ctx.enable_timing(False)
+    evals = cfg.evals
+    cub_avg_s = cub_ms / max(cub_n, 1) / 1e3
+    alg_bytes = 4.0 * evals                       # one FP32 map value per eval
+    achieved = alg_bytes / cub_avg_s / 1e9
+    result = {
+        "metric": METRIC,
+        "value": evals / (ms / 1e3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name}: {cfg.n_src} point sources, {cfg.npix}x{cfg.npix} "
+                               f"CAR {cfg.cdelt} deg, {cfg.n_energy} energies, 8x8 Gauss-"
+                               f"Legendre pixel cubature, FP32 [S,E,H,W] map",
+                   "sources": cfg.n_src, "npix": cfg.npix, "energies": cfg.n_energy,
+                   "rule": cfg.rule, "contraction": "bf16x3" if args.precision else "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "cubature_kernel<float,40>",
+                     "kernel_ms": cub_avg_s * 1e3, "scale_kernel_ms": sc_ms / max(sc_n, 1)},
+    }
+    if not args.no_cpu_baseline:
+        _log("bench: cpu baseline (oracle)")
+        gpu_map = d["out"].cpu().numpy()
+        cb, mx, md = cpu_baseline(cfg, inp, gpu_map)
+        result["cpu_baseline"] = cb
+        result["accuracy"] = {"max_rel_err_vs_fp64_oracle": mx, "median_rel_err": md}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

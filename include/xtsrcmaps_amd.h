@@ -1,0 +1,243 @@
+/*
+ * xtsrcmaps_amd.h -- C ABI of the MI355X (gfx950) point-source model-map path.
+ *
+ * Each entry point replaces one function of the reference xtsrcmaps C++ API
+ * (cited as file:line in Areustle/xtsrcmaps).  Arrays are row-major; sizes are
+ * element counts.  Host-pointer entry points copy in, compute on the device
+ * and copy out before returning.  `_dev` entry points take device pointers,
+ * enqueue on the context's stream and return without synchronising.
+ *
+ * Every function returns XTS_OK (0) or an XTS_ERR_* code; xts_last_error()
+ * gives the message of the last failure on the calling thread.  No C++
+ * exception crosses this boundary.  A context is used by one host thread at
+ * a time.
+ */
+#ifndef XTSRCMAPS_AMD_H
+#define XTSRCMAPS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XTS_ABI_VERSION 1
+
+/* status codes */
+#define XTS_OK 0
+#define XTS_ERR_INVALID 1 /* null pointer, bad size or flag              */
+#define XTS_ERR_DEVICE 2  /* HIP runtime failure / no device             */
+#define XTS_ERR_ALLOC 3   /* device allocation failed                    */
+#define XTS_ERR_RANGE 4   /* a sample lies outside an IRF grid           */
+
+/* projections */
+#define XTS_PROJ_CAR 0
+#define XTS_PROJ_AIT 1
+
+/* model-map flags */
+#define XTS_LAYOUT_SHWE 0x0   /* [S, H, W, E]: the reference's layout          */
+#define XTS_LAYOUT_SEHW 0x1   /* [S, E, H, W]: energy planes, coalesced stores */
+#define XTS_OUT_F64 0x0       /* double output                                 */
+#define XTS_OUT_F32 0x2       /* float output                                  */
+#define XTS_COMPUTE_F32 0x0   /* FP32 cubature (FP64 geometry and scaling)     */
+#define XTS_COMPUTE_F64 0x4   /* FP64 cubature throughout                      */
+
+/* contraction precision for the exposure and mean-PSF builds */
+#define XTS_PREC_FP64 0       /* FP64 FMA                                      */
+#define XTS_PREC_BF16X3 1     /* BF16 hi/mid/lo split on MFMA, FP32 accumulate */
+
+/* scale kinds for xts_scale_map */
+#define XTS_SCALE_PIXEL 0     /* factor [H, W]  (solid angle)                  */
+#define XTS_SCALE_SRC_ENERGY 1/* factor [S, E]  (exposure, correction)         */
+
+typedef struct xts_ctx xts_ctx;
+
+/* WCS of a model-map image.  Same fields as the reference's Fermi::WcsConfig
+ * (sky_geom/sky_geom.hxx:17-25); CROTA2 in degrees, CDELT in degrees. */
+typedef struct {
+    int64_t naxis1, naxis2;
+    double crpix1, crpix2;
+    double crval1, crval2;
+    double cdelt1, cdelt2;
+    double crota2;
+    int32_t proj;
+    int32_t is_galactic;
+} xts_wcs;
+
+/* One section (front or back) of the King PSF IRF grid after
+ * normalisation: Irf::IrfData3 (irf/irf_types.hxx:9-14). */
+typedef struct {
+    const double* params; /* [Mc, Me, 6] NCORE NTAIL SCORE STAIL GCORE GTAIL */
+    const double* cosths; /* [Mc] */
+    const double* logEs;  /* [Me] */
+    int64_t Mc, Me;
+} xts_psf_grid;
+
+/* Effective-area grid: Irf::IrfData3 with one parameter per node. */
+typedef struct {
+    const double* params; /* [Mc, Me] m^2 */
+    const double* cosths; /* [Mc] */
+    const double* logEs;  /* [Me] */
+    int64_t Mc, Me;
+    double min_costheta;
+} xts_aeff_grid;
+
+/* ------------------------------------------------------------------ context */
+int xts_ctx_create(int device, xts_ctx** out);
+int xts_ctx_destroy(xts_ctx* ctx);
+/* Use a caller-owned hipStream_t (NULL restores the context's own stream). */
+int xts_ctx_set_stream(xts_ctx* ctx, void* hip_stream);
+int xts_ctx_synchronize(xts_ctx* ctx);
+const char* xts_last_error(void);
+int xts_abi_version(void);
+/* Instrumentation (no reference counterpart): when enabled, kernel launches
+ * of the named stages ("cubature", "scale") are bracketed by events on the
+ * context stream; xts_ctx_kernel_times synchronises and returns (and
+ * clears) the summed milliseconds and the launch count of one stage. */
+int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
+int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms,
+                         int64_t* count);
+
+/* ------------------------------------------------------------- (a) exposure */
+/* Exposure::costhetas (exposure/exposure.cxx:29-40). */
+int xts_costhetas(int64_t nbins, double cosmin, int32_t thetabin, double* out);
+
+/* Irf::livetime_efficiency_factors (irf/irf.cxx:70-109): p0 -> row 1,
+ * p1 -> row 0, breaks formed in float as the IRF columns are float. */
+int xts_livetime_efficiency_factors(const double* logEs, int64_t E,
+                                    const float* p0, const float* p1,
+                                    double* ltf /* [2, E] */);
+
+/* Irf::aeff_value (irf/irf.aeff_value.cxx:11-55). */
+int xts_aeff_value(xts_ctx* ctx, const double* costh, int64_t C,
+                   const double* logE, int64_t E, const xts_aeff_grid* grid,
+                   double* out /* [C, E] */);
+
+/* Exposure::src_cosbins (exposure/exposure.cxx:42-63) with
+ * Healpix::ang2pix NESTED (healpix/healpix.cxx:182-229). */
+int xts_src_cosbins(xts_ctx* ctx, const double* src_sph /* [S, 2] deg */,
+                    int64_t S, const double* cube /* [12 nside^2, C] */,
+                    int64_t nside, int64_t C, double* out /* [S, C] */);
+int xts_src_cosbins_dev(xts_ctx* ctx, const double* src_sph, int64_t S,
+                        const double* cube, int64_t nside, int64_t C,
+                        double* out);
+
+/* Exposure::contract (exposure/exposure.contract.cpp:13-42). */
+int xts_contract(xts_ctx* ctx, const double* A /* [M, K] */,
+                 const double* B /* [K, N] */, int64_t M, int64_t K,
+                 int64_t N, double* out /* [M, N] */, int32_t precision);
+
+/* Exposure::exposure (exposure/exposure.cxx:69-173): four [S,C].[C,E]
+ * contractions scaled by the front LTF rows and summed. */
+int xts_exposure(xts_ctx* ctx, const double* ecb, const double* wecb,
+                 const double* aeff_front, const double* aeff_back,
+                 const double* front_ltf, int64_t S, int64_t C, int64_t E,
+                 double* exposure /* [S, E] */, int32_t precision);
+int xts_exposure_dev(xts_ctx* ctx, const double* ecb, const double* wecb,
+                     const double* aeff_front, const double* aeff_back,
+                     const double* front_ltf, int64_t S, int64_t C, int64_t E,
+                     double* exposure, int32_t precision);
+
+/* ------------------------------------------------------------- (b) mean PSF */
+/* irf_private::normalize_rpsf (irf/irf.normalize_rpsf.cpp:11-61), in place. */
+int xts_normalize_rpsf(xts_ctx* ctx, double* params /* [Mc, Me, 6] */,
+                       const double* logEs, int64_t Mc, int64_t Me,
+                       float scale0, float scale1, float scale_index);
+
+/* Psf::king (psf/psf.king.cxx:9-43): [401, Mc, Me]. */
+int xts_king(xts_ctx* ctx, const xts_psf_grid* grid, double* out);
+
+/* Psf::psf_lookup_table_and_partial_integrals
+ * (psf/psf.psf_lookup_table_and_partial_integrals.cxx:8-57): King on the
+ * 401 separations, bilinear to (cos theta, log E), livetime x Aeff weighted
+ * contraction, mean over exposure, radial integral and normalisation.
+ * upsf and parint are [S, 401, E]. */
+int xts_psf_lookup_table_and_partial_integrals(
+    xts_ctx* ctx, const xts_psf_grid* front, const xts_psf_grid* back,
+    const double* costh, int64_t C, const double* logE, int64_t E,
+    const double* aeff_front, const double* aeff_back, const double* ecb,
+    const double* wecb, const double* front_ltf, const double* exposure,
+    int64_t S, double* upsf, double* parint, int32_t precision);
+/* Device variant; grid arrays are device pointers too. */
+int xts_psf_lookup_table_and_partial_integrals_dev(
+    xts_ctx* ctx, const xts_psf_grid* front, const xts_psf_grid* back,
+    const double* costh, int64_t C, const double* logE, int64_t E,
+    const double* aeff_front, const double* aeff_back, const double* ecb,
+    const double* wecb, const double* front_ltf, const double* exposure,
+    int64_t S, double* upsf, double* parint, int32_t precision);
+
+/* Psf::partial_total_integral + Psf::normalize
+ * (psf/psf.partial_total_integral.cxx:8-52, psf/psf.normalize.cxx:7-24):
+ * upsf is normalised in place, parint [S, 401, E] and total [S, E] out. */
+int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S,
+                               int64_t E, double* parint, double* total);
+
+/* ------------------------------------------------------------ (c) model map */
+/* ModelMap::point_src_model_map_wcs
+ * (model_map/model_map.point_src_model_map_wcs.cxx:6-45): cubature of the
+ * PSF table over every pixel, x solid angle, x exposure, x correction factor.
+ * rule is [N, 3] (u, v, w) on [-1, 1]^2 with sum w = 4; NULL selects the
+ * reference's production rule (Festa-Sommariva degree 7, 12 points).
+ * out is laid out and typed by flags (XTS_LAYOUT_*, XTS_OUT_*,
+ * XTS_COMPUTE_*). */
+int xts_point_src_model_map_wcs(xts_ctx* ctx, const xts_wcs* wcs,
+                                const double* src_sph, int64_t S,
+                                const double* exposure, const double* upsf,
+                                const double* parint, int64_t E,
+                                const double* rule, int64_t N, int32_t flags,
+                                void* out);
+int xts_point_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* wcs,
+                                    const double* src_sph, int64_t S,
+                                    const double* exposure, const double* upsf,
+                                    const double* parint, int64_t E,
+                                    const double* rule, int64_t N,
+                                    int32_t flags, void* out);
+
+/* cubature_convolve_map_psf (mm_cubature/cubature.implementation.cxx:21-55):
+ * the raw cubature only. */
+int xts_cubature_convolve_map_psf(xts_ctx* ctx, const xts_wcs* wcs,
+                                  const double* src_sph, int64_t S,
+                                  const double* upsf, int64_t E,
+                                  const double* rule, int64_t N,
+                                  int32_t flags, void* out);
+
+/* Math::solid_angle (math/solid_angle.hpp:42-79): [H, W]. */
+int xts_solid_angle(xts_ctx* ctx, const xts_wcs* wcs, double* out);
+
+/* ModelMap::psf_boundary_radius
+ * (model_map/model_map.psf_boundary_radius.cxx:9-62). */
+int xts_psf_boundary_radius(xts_ctx* ctx, const xts_wcs* wcs,
+                            const double* src_sph, int64_t S, double* radius,
+                            uint8_t* in_fov);
+
+/* ModelMap::map_integral (model_map/model_map.map_integral.cxx:4-48):
+ * map [S, H, W, E] double -> inverse integral [S, E]. */
+int xts_map_integral(xts_ctx* ctx, const xts_wcs* wcs, const double* map,
+                     const double* src_sph, int64_t S, int64_t E,
+                     const double* radius, const uint8_t* in_fov,
+                     double* out);
+
+/* ModelMap::map_correction_factor
+ * (model_map/model_map.map_correction_factor.cxx:3-53). */
+int xts_map_correction_factor(xts_ctx* ctx, const double* inv_mapinteg,
+                              const double* radius, const uint8_t* in_fov,
+                              const double* upsf, const double* parint,
+                              int64_t S, int64_t E, double* out);
+
+/* ModelMap::scale_map_by_solid_angle / _by_exposure /
+ * _by_correction_factors (model_map.scale_map_by_*.cxx): in place on a
+ * double [S, H, W, E] map. */
+int xts_scale_map(xts_ctx* ctx, double* map, int64_t S, int64_t H, int64_t W,
+                  int64_t E, const double* factor, int32_t kind);
+
+/* SkyGeom::pix2sph / sph2pix (sky_geom/sky_geom.hxx:179-250) for n points,
+ * evaluated on the device with the same WCS code the kernels use. */
+int xts_pix2sph(xts_ctx* ctx, const xts_wcs* wcs, const double* pix /*[n,2]*/,
+                int64_t n, double* sph /* [n, 2] */);
+int xts_sph2pix(xts_ctx* ctx, const xts_wcs* wcs, const double* sph, int64_t n,
+                double* pix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""GPU parity: every stage of the HIP path against the FP64 oracle on the
+same inputs, and against the reference's own fixtures.  Calls go through the
+C ABI (xtsrcmaps_amd.capi).
+
+Tolerances: FP64 stages 1e-10 relative (different but exact-in-FP64
+summation orders); FP32 cubature 1e-5 relative per map value (the bar for the
+plain FP32 path); BF16x3 contractions 5e-6 relative (FP32-level)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import max_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import xtsrcmaps_amd as xa
+    c = xa.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c0():
+    from xtsrcmaps_amd import synthetic
+    return synthetic.make_inputs(synthetic.CONFIGS["c0"])
+
+
+@pytest.fixture(scope="module")
+def c1():
+    from xtsrcmaps_amd import synthetic
+    return synthetic.make_inputs(synthetic.CONFIGS["c1"])
+
+
+@pytest.fixture(scope="module")
+def c0_oracle(orc, c0):
+    from pipelines import oracle_stages
+    return oracle_stages(orc, c0)
+
+
+def _gpu_stages(ctx, inp, src, precision=0):
+    import xtsrcmaps_amd as xa
+    out = {}
+    king = {}
+    for side, (params, cos, loge) in inp.king_raw.items():
+        king[side] = (ctx.normalize_rpsf(params, loge, inp.psf_scale), cos, loge)
+    out["king"] = king
+    out["aeff"] = {s: ctx.aeff_value(inp.costh, inp.logE, *g) for s, g in inp.aeff_grid.items()}
+    out["ltf"] = xa.livetime_efficiency_factors(inp.logE, *inp.effic)
+    out["ecb"] = ctx.src_cosbins(src, inp.cube, inp.nside)
+    out["wecb"] = ctx.src_cosbins(src, inp.wcube, inp.nside)
+    out["expo"] = ctx.exposure(out["ecb"], out["wecb"], out["aeff"]["front"],
+                               out["aeff"]["back"], out["ltf"], precision)
+    out["upsf"], out["parint"] = ctx.psf_lookup_table_and_partial_integrals(
+        king["front"], king["back"], inp.costh, inp.logE, out["aeff"]["front"],
+        out["aeff"]["back"], out["ecb"], out["wecb"], out["ltf"], out["expo"], precision)
+    return out
+
+
+# ------------------------------------------------------------------ (a), (b)
+def test_gpu_irf_and_exposure_stages(ctx, c0, c0_oracle):
+    _log("test_gpu_irf_and_exposure_stages: c0")
+    g = _gpu_stages(ctx, c0, c0.src_sph)
+    o = c0_oracle
+    for side in ("front", "back"):
+        assert max_rel(g["king"][side][0], o["king"][side][0]) < 1e-12
+        assert max_rel(g["aeff"][side], o["aeff"][side], floor=1e-300) < 1e-13
+    np.testing.assert_array_equal(g["ecb"], o["ecb"])
+    np.testing.assert_array_equal(g["wecb"], o["wecb"])
+    assert max_rel(g["expo"], o["expo"]) < 1e-12
+    assert max_rel(g["upsf"], o["upsf"]) < 1e-10
+    assert max_rel(g["parint"][:, 1:], o["parint"][:, 1:]) < 1e-10
+
+
+def test_gpu_src_cosbins_many_sources(ctx, orc, c1):
+    _log("test_gpu_src_cosbins_many_sources")
+    rng = np.random.default_rng(5)
+    src = np.stack([rng.uniform(0, 360, 5000), np.degrees(np.arcsin(rng.uniform(-1, 1, 5000)))], 1)
+    src[:4] = [[0.0, 90.0], [10.0, -90.0], [359.9999, 0.0], [45.0, 41.8103149]]
+    np.testing.assert_array_equal(ctx.src_cosbins(src, c1.cube, 32),
+                                  orc.src_cosbins(src, c1.cube, 32))
+
+
+def test_gpu_bf16x3_contractions(ctx, c1, orc):
+    from pipelines import oracle_stages
+    _log("test_gpu_bf16x3_contractions: c1, 100 sources")
+    o = oracle_stages(orc, c1)
+    g = _gpu_stages(ctx, c1, c1.src_sph, precision=1)
+    assert max_rel(g["expo"], o["expo"]) < 5e-6
+    # the PSF table carries the contraction error (FP32 level) into every node
+    assert max_rel(g["upsf"], o["upsf"]) < 1e-5
+    rng = np.random.default_rng(0)
+    A, B = rng.uniform(-1, 1, (77, 40)), rng.uniform(-1, 1, (40, 123)) * 1e8
+    ref = A @ B
+    got = ctx.contract(A, B, precision=1)
+    assert np.max(np.abs(got - ref)) < 5e-6 * np.max(np.abs(ref))
+
+
+# ------------------------------------------------------------------ golden chain
+def test_gpu_reference_fixtures(ctx, golden):
+    """The 3C279 fixtures of the reference, computed on the GPU."""
+    _log("test_gpu_reference_fixtures")
+    P, A, X, M = golden("irf_psf"), golden("irf_aeff"), golden("exposure_psf"), golden("mm_single_source")
+    from oracle import oracle as orc
+    import xtsrcmaps_amd as xa
+    grids = {}
+    for k in ("front", "back"):
+        cos, loge, par, _ = orc.prepare_grid(P[k + "_e_lo"], P[k + "_e_hi"], P[k + "_c_lo"],
+                                             P[k + "_c_hi"], P[k + "_raw"])
+        nrm = ctx.normalize_rpsf(par, loge, P[k + "_scale"])
+        grids[k] = (nrm, cos, loge)
+    assert max_rel(ctx.king(*grids["front"]), P["king_front"]) < 1e-6
+    logE = np.log10(X["energies"])
+    costh = xa.costhetas(40, 0.0, True)
+    ltf = xa.livetime_efficiency_factors(logE, A["front_effic_p0"], A["front_effic_p1"])
+    aeff = {}
+    for k in ("front", "back"):
+        cos, loge, par, minc = orc.prepare_grid(A[k + "_e_lo"], A[k + "_e_hi"], A[k + "_c_lo"],
+                                                A[k + "_c_hi"], A[k + "_raw"])
+        aeff[k] = ctx.aeff_value(costh, logE, cos, loge, par[..., 0], minc)
+    ecb, wecb = X["ecb"].astype(float), X["wecb"].astype(float)
+    expo = ctx.exposure(ecb, wecb, aeff["front"], aeff["back"], ltf)
+    assert max_rel(expo, X["exposure"]) < 5e-6
+    upsf, parint = ctx.psf_lookup_table_and_partial_integrals(
+        grids["front"], grids["back"], costh, logE, aeff["front"], aeff["back"], ecb, wecb, ltf,
+        expo)
+    assert max_rel(upsf[:, 0, :], X["upsf_peak"]) < 1e-6
+    # single-source AIT cubature maps (fixture layout [W, H, E])
+    i = int(np.argmin(np.hypot(X["src_dirs"][:, 0] - M["src_sph"][0],
+                               X["src_dirs"][:, 1] - M["src_sph"][1])))
+    w = xa.Wcs.make(M["naxis"], M["crpix"], M["crval"], M["cdelt"], float(M["crota2"]),
+                    str(M["proj"]))
+    lat = M["lattice"]
+    g3 = np.array([[-1, -1, 1], [-1, 1, 1], [1, 1, 1], [1, -1, 1]], float)
+    g3[:, :2] /= np.sqrt(3.0)
+    for name, rule in (("mm_sqd3", g3), ("mm_sqd11", M["rule_square_deg11"]),
+                       ("mm_asqd12_single_src", M["rule_asym_square_d12"])):
+        for flags in (xa.COMPUTE_F64, xa.COMPUTE_F32):
+            m = ctx.cubature_convolve_map_psf(w, np.array([M["src_sph"]]), upsf[i:i + 1], rule,
+                                              flags=flags)[0]
+            got = m.transpose(1, 0, 2)[np.ix_(lat, lat)]
+            assert max_rel(got, M[name]) < 1e-5, (name, flags)
+
+
+# ------------------------------------------------------------------ (c) model map
+def _oracle_model_map(orc, inp, stages, n_src=None):
+    from pipelines import oracle_rule, oracle_wcs
+    S = stages["upsf"].shape[0] if n_src is None else n_src
+    w = oracle_wcs(orc, inp.wcs)
+    return orc.point_src_model_map_wcs(inp.wcs.naxis1, inp.wcs.naxis2, w, stages["src"][:S],
+                                       stages["expo"][:S], stages["upsf"][:S],
+                                       stages["parint"][:S], oracle_rule(inp), details=True)
+
+
+def test_gpu_model_map_c0_fp64_and_fp32(ctx, orc, c0, c0_oracle):
+    import xtsrcmaps_amd as xa
+    _log("test_gpu_model_map_c0: FP64 and FP32 cubature vs oracle")
+    ref, det = _oracle_model_map(orc, c0, c0_oracle)
+    o = c0_oracle
+    m64 = ctx.point_src_model_map_wcs(c0.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
+                                      c0.rule, flags=xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F64)
+    assert max_rel(m64, ref) < 1e-10
+    m32 = ctx.point_src_model_map_wcs(c0.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
+                                      c0.rule, flags=xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F32)
+    rel = np.abs(m32 - ref) / np.abs(ref)
+    _log(f"  c0 FP32 cubature: max rel {rel.max():.3e} median {np.median(rel):.3e}")
+    assert rel.max() < 1e-5
+    # the parts: solid angle, boundary radius, correction factor
+    from pipelines import oracle_wcs
+    w = oracle_wcs(orc, c0.wcs)
+    assert max_rel(ctx.solid_angle(c0.wcs), orc.solid_angle(100, 100, w)) < 1e-12
+    rad, fov = ctx.psf_boundary_radius(c0.wcs, o["src"])
+    np.testing.assert_allclose(rad, det["radius"], rtol=1e-13)
+    np.testing.assert_array_equal(fov, det["in_fov"].astype(bool))
+    cf = ctx.map_correction_factor(det["inv_mapinteg"], det["radius"], det["in_fov"], o["upsf"],
+                                   o["parint"])
+    assert max_rel(cf, det["cf"]) < 1e-12
+
+
+def test_gpu_model_map_c1_subset_fp32(ctx, orc, c1):
+    import xtsrcmaps_amd as xa
+    from pipelines import oracle_stages
+    _log("test_gpu_model_map_c1_subset: 4 sources, 200x200, 8x8 GL, FP32")
+    o = oracle_stages(orc, c1, n_src=4)
+    ref, _ = _oracle_model_map(orc, c1, o)
+    got = ctx.point_src_model_map_wcs(c1.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
+                                      c1.rule, flags=xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32)
+    got = got.transpose(0, 2, 3, 1).astype(np.float64)
+    rel = np.abs(got - ref) / np.abs(ref)
+    _log(f"  c1 subset: max rel {rel.max():.3e} median {np.median(rel):.3e}")
+    assert rel.max() < 1e-5
+
+
+def test_gpu_layouts_and_raw_cubature_agree(ctx, c1):
+    import xtsrcmaps_amd as xa
+    _log("test_gpu_layouts_and_raw_cubature_agree")
+    rng = np.random.default_rng(2)
+    S, E = 3, 41                                # two energy chunks
+    upsf = rng.uniform(0.5, 2.0, (S, 401, E))
+    src = c1.src_sph[:S]
+    a = ctx.cubature_convolve_map_psf(c1.wcs, src, upsf, c1.rule,
+                                      flags=xa.LAYOUT_SHWE | xa.OUT_F32)
+    b = ctx.cubature_convolve_map_psf(c1.wcs, src, upsf, c1.rule,
+                                      flags=xa.LAYOUT_SEHW | xa.OUT_F32)
+    np.testing.assert_array_equal(a, b.transpose(0, 2, 3, 1))
+    # constant table -> constant map (weights sum to 1)
+    const = np.full((S, 401, E), 3.25)
+    m = ctx.cubature_convolve_map_psf(c1.wcs, src, const, c1.rule, flags=xa.OUT_F64)
+    np.testing.assert_allclose(m, 3.25, rtol=2e-6)
+
+
+def test_gpu_edge_cases(ctx, orc, c0):
+    import xtsrcmaps_amd as xa
+    from pipelines import oracle_wcs
+    _log("test_gpu_edge_cases: out-of-FOV source, source on a pixel centre, E=1, 1-point rule")
+    w = oracle_wcs(orc, c0.wcs)
+    src = np.array([[180.0 - 0.1 * 60, 0.0],          # outside the 100x100 0.1 deg image
+                    [180.05, 0.05],                   # on a pixel centre
+                    [180.0, 4.98]])                   # inside the 3.5 px border
+    rng = np.random.default_rng(9)
+    upsf = rng.uniform(0.1, 1.0, (3, 401, 1))
+    upsf, parint, _ = ctx.partial_total_integral(upsf)
+    expo = np.array([[2.0], [3.0], [4.0]])
+    rule = np.array([[0.0, 0.0, 4.0]])
+    ref, det = orc.point_src_model_map_wcs(100, 100, w, src, expo, upsf, parint, rule,
+                                           details=True)
+    assert list(det["in_fov"]) == [0, 1, 0]
+    got = ctx.point_src_model_map_wcs(c0.wcs, src, expo, upsf, parint, rule,
+                                      flags=xa.OUT_F64 | xa.COMPUTE_F64)
+    assert max_rel(got, ref) < 1e-10
+    with pytest.raises(xa.XtsError):
+        ctx.point_src_model_map_wcs(c0.wcs, src, expo, upsf, parint, rule, flags=0x40)
+
+
+def test_gpu_model_map_c1_full_size_properties(ctx, c1):
+    """All 100 sources at full size: finite, positive, layout-independent and
+    within 1e-5 of the FP64 cubature on the same device."""
+    import xtsrcmaps_amd as xa
+    _log("test_gpu_model_map_c1_full: 100 x 37 x 200 x 200")
+    g = _gpu_stages(ctx, c1, c1.src_sph)
+    m32 = ctx.point_src_model_map_wcs(c1.wcs, c1.src_sph, g["expo"], g["upsf"], g["parint"],
+                                      c1.rule, flags=xa.LAYOUT_SEHW | xa.OUT_F32)
+    assert np.isfinite(m32).all() and (m32 > 0).all()
+    m64 = ctx.point_src_model_map_wcs(c1.wcs, c1.src_sph[:10], g["expo"][:10], g["upsf"][:10],
+                                      g["parint"][:10], c1.rule,
+                                      flags=xa.LAYOUT_SEHW | xa.OUT_F64 | xa.COMPUTE_F64)
+    rel = np.abs(m32[:10] - m64) / m64
+    _log(f"  FP32 vs FP64 cubature: max rel {rel.max():.3e}")
+    assert rel.max() < 1e-5

@@ -1,0 +1,314 @@
+"""xtsrcmaps_amd -- MI355X (gfx950) point-source model-map path.
+
+Python host mirror of the reference's C++ interface for this path
+(``Exposure::``, ``Psf::``, ``ModelMap::`` in Areustle/xtsrcmaps), calling the
+C ABI of ``libxtsrcmaps_amd.so`` (declared in ``include/xtsrcmaps_amd.h``).
+Arrays are numpy, row-major, in the reference's layouts.  Every call runs on
+the GPU; there is no CPU fallback and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import capi
+from .capi import (XtsError, Wcs, PROJ_CAR, PROJ_AIT, LAYOUT_SHWE, LAYOUT_SEHW,
+                   OUT_F64, OUT_F32, COMPUTE_F32, COMPUTE_F64, PREC_FP64, PREC_BF16X3)
+
+ND = 401
+
+__all__ = [
+    "Context", "Wcs", "XtsError", "separations", "costhetas",
+    "livetime_efficiency_factors", "PROJ_CAR", "PROJ_AIT", "LAYOUT_SHWE",
+    "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64",
+    "PREC_FP64", "PREC_BF16X3", "ND",
+]
+
+
+def separations() -> np.ndarray:
+    """Psf::separations (psf/psf.hxx:24-31): 0, then 1e-4 .. 70 deg log-spaced."""
+    x = np.log(70.0 / 1e-4) / 399.0
+    out = np.zeros(ND)
+    out[1:] = 1e-4 * np.exp(np.arange(400) * x)
+    return out
+
+
+def costhetas(nbins: int, cosmin: float, thetabin: bool = True) -> np.ndarray:
+    """Exposure::costhetas (exposure/exposure.cxx:29-40)."""
+    out = np.zeros(nbins)
+    capi.check(capi.lib().xts_costhetas(nbins, float(cosmin), int(bool(thetabin)),
+                                        capi.dptr(out)))
+    return out
+
+
+def livetime_efficiency_factors(logEs, p0, p1) -> np.ndarray:
+    """Irf::livetime_efficiency_factors (irf/irf.cxx:70-109) -> [2, E]."""
+    logEs = capi.f64(logEs)
+    p0 = np.ascontiguousarray(p0, np.float32)
+    p1 = np.ascontiguousarray(p1, np.float32)
+    out = np.zeros((2, len(logEs)))
+    capi.check(capi.lib().xts_livetime_efficiency_factors(
+        capi.dptr(logEs), len(logEs), capi.fptr(p0), capi.fptr(p1), capi.dptr(out)))
+    return out
+
+
+class Context:
+    """One device context (stream + device arena) of the C ABI."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        capi.check(capi.lib().xts_ctx_create(int(device), C.byref(self._h)))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            capi.lib().xts_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr: int | None):
+        capi.check(capi.lib().xts_ctx_set_stream(self._h, C.c_void_p(stream_ptr or 0)))
+
+    def synchronize(self):
+        capi.check(capi.lib().xts_ctx_synchronize(self._h))
+
+    def enable_timing(self, on: bool = True):
+        capi.check(capi.lib().xts_ctx_enable_timing(self._h, int(bool(on))))
+
+    def kernel_times(self, name: str):
+        """(total ms, launches) of a timed stage since the last query."""
+        ms, n = C.c_double(), C.c_int64()
+        capi.check(capi.lib().xts_ctx_kernel_times(self._h, name.encode(), C.byref(ms),
+                                                   C.byref(n)))
+        return ms.value, n.value
+
+    # ------------------------------------------------------------ (a) exposure
+    def aeff_value(self, costh, logE, grid_costh, grid_logE, params, min_costheta):
+        """Irf::aeff_value (irf/irf.aeff_value.cxx:11-55) -> [C, E] cm^2."""
+        costh, logE = capi.f64(costh), capi.f64(logE)
+        g = capi.aeff_grid(grid_costh, grid_logE, params, min_costheta)
+        out = np.zeros((len(costh), len(logE)))
+        capi.check(capi.lib().xts_aeff_value(self._h, capi.dptr(costh), len(costh),
+                                             capi.dptr(logE), len(logE), C.byref(g[0]),
+                                             capi.dptr(out)))
+        return out
+
+    def src_cosbins(self, src_sph, cube, nside):
+        """Exposure::src_cosbins (exposure/exposure.cxx:42-63) -> [S, C]."""
+        src, cube = capi.f64(src_sph), capi.f64(cube)
+        S, Cn = src.shape[0], cube.shape[1]
+        out = np.zeros((S, Cn))
+        capi.check(capi.lib().xts_src_cosbins(self._h, capi.dptr(src), S, capi.dptr(cube),
+                                              int(nside), Cn, capi.dptr(out)))
+        return out
+
+    def contract(self, A, B, precision=PREC_FP64):
+        """Exposure::contract (exposure/exposure.contract.cpp:13-42)."""
+        A, B = capi.f64(A), capi.f64(B)
+        out = np.zeros((A.shape[0], B.shape[1]))
+        capi.check(capi.lib().xts_contract(self._h, capi.dptr(A), capi.dptr(B), A.shape[0],
+                                           A.shape[1], B.shape[1], capi.dptr(out),
+                                           int(precision)))
+        return out
+
+    def exposure(self, ecb, wecb, aeff_front, aeff_back, front_ltf, precision=PREC_FP64):
+        """Exposure::exposure (exposure/exposure.cxx:69-173) -> [S, E]."""
+        ecb, wecb, af, ab, ltf = map(capi.f64, (ecb, wecb, aeff_front, aeff_back, front_ltf))
+        S, Cn = ecb.shape
+        E = af.shape[1]
+        out = np.zeros((S, E))
+        capi.check(capi.lib().xts_exposure(self._h, capi.dptr(ecb), capi.dptr(wecb),
+                                           capi.dptr(af), capi.dptr(ab), capi.dptr(ltf), S, Cn,
+                                           E, capi.dptr(out), int(precision)))
+        return out
+
+    # ------------------------------------------------------------ (b) mean PSF
+    def normalize_rpsf(self, params, logEs, scale3):
+        """irf_private::normalize_rpsf (irf/irf.normalize_rpsf.cpp:11-61)."""
+        p = capi.f64(params).copy()
+        logEs = capi.f64(logEs)
+        Mc, Me, _ = p.shape
+        capi.check(capi.lib().xts_normalize_rpsf(self._h, capi.dptr(p), capi.dptr(logEs), Mc, Me,
+                                                 C.c_float(scale3[0]), C.c_float(scale3[1]),
+                                                 C.c_float(scale3[2])))
+        return p
+
+    def king(self, params, cosths, logEs):
+        """Psf::king (psf/psf.king.cxx:9-43) -> [401, Mc, Me]."""
+        g = capi.psf_grid(params, cosths, logEs)
+        Mc, Me = g[0].Mc, g[0].Me
+        out = np.zeros((ND, Mc, Me))
+        capi.check(capi.lib().xts_king(self._h, C.byref(g[0]), capi.dptr(out)))
+        return out
+
+    def psf_lookup_table_and_partial_integrals(self, front, back, costh, logE, aeff_front,
+                                               aeff_back, ecb, wecb, front_ltf, exposure,
+                                               precision=PREC_FP64):
+        """Psf::psf_lookup_table_and_partial_integrals -> (uPsf, partial) [S, 401, E].
+
+        front/back are (params [Mc, Me, 6] normalised, cosths [Mc], logEs [Me])."""
+        gf, gb = capi.psf_grid(*front), capi.psf_grid(*back)
+        costh, logE, af, ab, ecb, wecb, ltf, expo = map(
+            capi.f64, (costh, logE, aeff_front, aeff_back, ecb, wecb, front_ltf, exposure))
+        S, E = expo.shape
+        upsf = np.zeros((S, ND, E))
+        parint = np.zeros((S, ND, E))
+        capi.check(capi.lib().xts_psf_lookup_table_and_partial_integrals(
+            self._h, C.byref(gf[0]), C.byref(gb[0]), capi.dptr(costh), len(costh),
+            capi.dptr(logE), E, capi.dptr(af), capi.dptr(ab), capi.dptr(ecb), capi.dptr(wecb),
+            capi.dptr(ltf), capi.dptr(expo), S, capi.dptr(upsf), capi.dptr(parint),
+            int(precision)))
+        return upsf, parint
+
+    def partial_total_integral(self, upsf):
+        """Psf::partial_total_integral + Psf::normalize -> (uPsf_normed, partial, total)."""
+        u = capi.f64(upsf).copy()
+        S, _, E = u.shape
+        par, tot = np.zeros_like(u), np.zeros((S, E))
+        capi.check(capi.lib().xts_partial_total_integral(self._h, capi.dptr(u), S, E,
+                                                         capi.dptr(par), capi.dptr(tot)))
+        return u, par, tot
+
+    # ------------------------------------------------------------ (c) model map
+    def point_src_model_map_wcs(self, wcs: Wcs, src_sph, exposure, upsf, parint, rule=None,
+                                flags=LAYOUT_SHWE | OUT_F64 | COMPUTE_F32):
+        """ModelMap::point_src_model_map_wcs -> [S, H, W, E] (or [S, E, H, W])."""
+        src, expo, upsf, parint = map(capi.f64, (src_sph, exposure, upsf, parint))
+        S, E = expo.shape
+        out = capi.alloc_map(S, wcs, E, flags)
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_point_src_model_map_wcs(
+            self._h, C.byref(wcs), capi.dptr(src), S, capi.dptr(expo), capi.dptr(upsf),
+            capi.dptr(parint), E, rp, N, int(flags), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def cubature_convolve_map_psf(self, wcs: Wcs, src_sph, upsf, rule=None,
+                                  flags=LAYOUT_SHWE | OUT_F64 | COMPUTE_F32):
+        """cubature_convolve_map_psf (cubature.implementation.cxx:21-55)."""
+        src, upsf = capi.f64(src_sph), capi.f64(upsf)
+        S, _, E = upsf.shape
+        out = capi.alloc_map(S, wcs, E, flags)
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_cubature_convolve_map_psf(
+            self._h, C.byref(wcs), capi.dptr(src), S, capi.dptr(upsf), E, rp, N, int(flags),
+            out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def solid_angle(self, wcs: Wcs):
+        """Math::solid_angle (math/solid_angle.hpp:42-79) -> [H, W] sr."""
+        out = np.zeros((wcs.naxis1, wcs.naxis2))
+        capi.check(capi.lib().xts_solid_angle(self._h, C.byref(wcs), capi.dptr(out)))
+        return out
+
+    def psf_boundary_radius(self, wcs: Wcs, src_sph):
+        """ModelMap::psf_boundary_radius -> (radius [S] deg, in_fov [S] bool)."""
+        src = capi.f64(src_sph)
+        S = src.shape[0]
+        rad, fov = np.zeros(S), np.zeros(S, np.uint8)
+        capi.check(capi.lib().xts_psf_boundary_radius(self._h, C.byref(wcs), capi.dptr(src), S,
+                                                      capi.dptr(rad), capi.u8ptr(fov)))
+        return rad, fov.astype(bool)
+
+    def map_integral(self, wcs: Wcs, model_map, src_sph, radius, in_fov):
+        """ModelMap::map_integral -> inverse map integral [S, E]."""
+        m, src, rad = capi.f64(model_map), capi.f64(src_sph), capi.f64(radius)
+        fov = np.ascontiguousarray(in_fov, np.uint8)
+        S, E = m.shape[0], m.shape[3]
+        out = np.zeros((S, E))
+        capi.check(capi.lib().xts_map_integral(self._h, C.byref(wcs), capi.dptr(m),
+                                               capi.dptr(src), S, E, capi.dptr(rad),
+                                               capi.u8ptr(fov), capi.dptr(out)))
+        return out
+
+    def map_correction_factor(self, inv_mapinteg, radius, in_fov, upsf, parint):
+        """ModelMap::map_correction_factor -> [S, E]."""
+        inv, rad, upsf, parint = map(capi.f64, (inv_mapinteg, radius, upsf, parint))
+        fov = np.ascontiguousarray(in_fov, np.uint8)
+        S, _, E = upsf.shape
+        out = np.zeros((S, E))
+        capi.check(capi.lib().xts_map_correction_factor(self._h, capi.dptr(inv), capi.dptr(rad),
+                                                        capi.u8ptr(fov), capi.dptr(upsf),
+                                                        capi.dptr(parint), S, E, capi.dptr(out)))
+        return out
+
+    def scale_map_by_solid_angle(self, model_map, omega):
+        """ModelMap::scale_map_by_solid_angle, in place on a double [S,H,W,E] map."""
+        return self._scale(model_map, omega, capi.SCALE_PIXEL)
+
+    def scale_map_by_exposure(self, model_map, exposure):
+        """ModelMap::scale_map_by_exposure, in place."""
+        return self._scale(model_map, exposure, capi.SCALE_SRC_ENERGY)
+
+    def scale_map_by_correction_factors(self, model_map, factor):
+        """ModelMap::scale_map_by_correction_factors, in place."""
+        return self._scale(model_map, factor, capi.SCALE_SRC_ENERGY)
+
+    def _scale(self, m, f, kind):
+        if not (isinstance(m, np.ndarray) and m.dtype == np.float64 and m.flags.c_contiguous):
+            raise XtsError(1, "model map must be a C-contiguous float64 array")
+        f = capi.f64(f)
+        S, H, W, E = m.shape
+        capi.check(capi.lib().xts_scale_map(self._h, capi.dptr(m), S, H, W, E, capi.dptr(f),
+                                            int(kind)))
+        return m
+
+    def pix2sph(self, wcs: Wcs, pix):
+        pix = capi.f64(pix).reshape(-1, 2)
+        out = np.zeros_like(pix)
+        capi.check(capi.lib().xts_pix2sph(self._h, C.byref(wcs), capi.dptr(pix), pix.shape[0],
+                                          capi.dptr(out)))
+        return out
+
+    def sph2pix(self, wcs: Wcs, sph):
+        sph = capi.f64(sph).reshape(-1, 2)
+        out = np.zeros_like(sph)
+        capi.check(capi.lib().xts_sph2pix(self._h, C.byref(wcs), capi.dptr(sph), sph.shape[0],
+                                          capi.dptr(out)))
+        return out
+
+    # ------------------------------------------------------------ device variants
+    def exposure_dev(self, ecb, wecb, af, ab, ltf, out, precision=PREC_FP64):
+        """Device-pointer Exposure::exposure on torch CUDA tensors (async)."""
+        S, Cn = ecb.shape
+        E = af.shape[1]
+        capi.check(capi.lib().xts_exposure_dev(self._h, *(capi.tptr(t) for t in
+                                                            (ecb, wecb, af, ab, ltf)),
+                                               S, Cn, E, capi.tptr(out), int(precision)))
+
+    def psf_lookup_dev(self, front, back, costh, logE, af, ab, ecb, wecb, ltf, expo, upsf,
+                       parint, precision=PREC_FP64):
+        """Device-pointer mean-PSF build; front/back are tuples of CUDA tensors."""
+        gf = capi.psf_grid_dev(*front)
+        gb = capi.psf_grid_dev(*back)
+        S, E = expo.shape
+        capi.check(capi.lib().xts_psf_lookup_table_and_partial_integrals_dev(
+            self._h, C.byref(gf), C.byref(gb), capi.tptr(costh), costh.shape[0],
+            capi.tptr(logE), E, capi.tptr(af), capi.tptr(ab), capi.tptr(ecb), capi.tptr(wecb),
+            capi.tptr(ltf), capi.tptr(expo), S, capi.tptr(upsf), capi.tptr(parint),
+            int(precision)))
+
+    def point_src_model_map_wcs_dev(self, wcs: Wcs, src_sph, exposure, upsf, parint, out,
+                                    rule=None, flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
+        """Device-pointer model map on torch CUDA tensors (async on the context stream)."""
+        S, E = exposure.shape
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_point_src_model_map_wcs_dev(
+            self._h, C.byref(wcs), capi.tptr(src_sph), S, capi.tptr(exposure), capi.tptr(upsf),
+            capi.tptr(parint), E, rp, N, int(flags), C.c_void_p(out.data_ptr())))

@@ -1,0 +1,190 @@
+"""ctypes declarations of the C ABI in include/xtsrcmaps_amd.h."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxtsrcmaps_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "xtsrcmaps_amd.h")
+
+PROJ_CAR, PROJ_AIT = 0, 1
+LAYOUT_SHWE, LAYOUT_SEHW = 0x0, 0x1
+OUT_F64, OUT_F32 = 0x0, 0x2
+COMPUTE_F32, COMPUTE_F64 = 0x0, 0x4
+PREC_FP64, PREC_BF16X3 = 0, 1
+SCALE_PIXEL, SCALE_SRC_ENERGY = 0, 1
+
+_d = C.POINTER(C.c_double)
+_f = C.POINTER(C.c_float)
+_u8 = C.POINTER(C.c_uint8)
+_i64 = C.c_int64
+_vp = C.c_void_p
+
+
+class XtsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"xtsrcmaps_amd error {code}: {msg}")
+        self.code = code
+
+
+class Wcs(C.Structure):
+    """xts_wcs: the fields of the reference's Fermi::WcsConfig."""
+    _fields_ = [("naxis1", C.c_int64), ("naxis2", C.c_int64),
+                ("crpix1", C.c_double), ("crpix2", C.c_double),
+                ("crval1", C.c_double), ("crval2", C.c_double),
+                ("cdelt1", C.c_double), ("cdelt2", C.c_double),
+                ("crota2", C.c_double), ("proj", C.c_int32), ("is_galactic", C.c_int32)]
+
+    @classmethod
+    def make(cls, naxis, crpix, crval, cdelt, crota2=0.0, proj="CAR", galactic=False):
+        p = {"CAR": PROJ_CAR, "AIT": PROJ_AIT}[proj] if isinstance(proj, str) else int(proj)
+        return cls(int(naxis[0]), int(naxis[1]), float(crpix[0]), float(crpix[1]),
+                   float(crval[0]), float(crval[1]), float(cdelt[0]), float(cdelt[1]),
+                   float(crota2), p, int(bool(galactic)))
+
+
+class PsfGrid(C.Structure):
+    _fields_ = [("params", _d), ("cosths", _d), ("logEs", _d), ("Mc", C.c_int64),
+                ("Me", C.c_int64)]
+
+
+class AeffGrid(C.Structure):
+    _fields_ = [("params", _d), ("cosths", _d), ("logEs", _d), ("Mc", C.c_int64),
+                ("Me", C.c_int64), ("min_costheta", C.c_double)]
+
+
+# name -> (restype, argtypes); every entry point returns int
+_SIGS = {
+    "xts_ctx_create": [C.c_int, C.POINTER(_vp)],
+    "xts_ctx_destroy": [_vp],
+    "xts_ctx_set_stream": [_vp, _vp],
+    "xts_ctx_synchronize": [_vp],
+    "xts_ctx_enable_timing": [_vp, C.c_int],
+    "xts_ctx_kernel_times": [_vp, C.c_char_p, _d, C.POINTER(_i64)],
+    "xts_abi_version": [],
+    "xts_costhetas": [_i64, C.c_double, C.c_int32, _d],
+    "xts_livetime_efficiency_factors": [_d, _i64, _f, _f, _d],
+    "xts_aeff_value": [_vp, _d, _i64, _d, _i64, C.POINTER(AeffGrid), _d],
+    "xts_src_cosbins": [_vp, _d, _i64, _d, _i64, _i64, _d],
+    "xts_src_cosbins_dev": [_vp, _vp, _i64, _vp, _i64, _i64, _vp],
+    "xts_contract": [_vp, _d, _d, _i64, _i64, _i64, _d, C.c_int32],
+    "xts_exposure": [_vp, _d, _d, _d, _d, _d, _i64, _i64, _i64, _d, C.c_int32],
+    "xts_exposure_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, C.c_int32],
+    "xts_normalize_rpsf": [_vp, _d, _d, _i64, _i64, C.c_float, C.c_float, C.c_float],
+    "xts_king": [_vp, C.POINTER(PsfGrid), _d],
+    "xts_psf_lookup_table_and_partial_integrals": [
+        _vp, C.POINTER(PsfGrid), C.POINTER(PsfGrid), _d, _i64, _d, _i64, _d, _d, _d, _d, _d,
+        _d, _i64, _d, _d, C.c_int32],
+    "xts_psf_lookup_table_and_partial_integrals_dev": [
+        _vp, C.POINTER(PsfGrid), C.POINTER(PsfGrid), _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp,
+        _vp, _vp, _i64, _vp, _vp, C.c_int32],
+    "xts_partial_total_integral": [_vp, _d, _i64, _i64, _d, _d],
+    "xts_point_src_model_map_wcs": [_vp, C.POINTER(Wcs), _d, _i64, _d, _d, _d, _i64, _d, _i64,
+                                    C.c_int32, _vp],
+    "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
+                                        _d, _i64, C.c_int32, _vp],
+    "xts_cubature_convolve_map_psf": [_vp, C.POINTER(Wcs), _d, _i64, _d, _i64, _d, _i64,
+                                      C.c_int32, _vp],
+    "xts_solid_angle": [_vp, C.POINTER(Wcs), _d],
+    "xts_psf_boundary_radius": [_vp, C.POINTER(Wcs), _d, _i64, _d, _u8],
+    "xts_map_integral": [_vp, C.POINTER(Wcs), _d, _d, _i64, _i64, _d, _u8, _d],
+    "xts_map_correction_factor": [_vp, _d, _d, _u8, _d, _d, _i64, _i64, _d],
+    "xts_scale_map": [_vp, _d, _i64, _i64, _i64, _i64, _d, C.c_int32],
+    "xts_pix2sph": [_vp, C.POINTER(Wcs), _d, _i64, _d],
+    "xts_sph2pix": [_vp, C.POINTER(Wcs), _d, _i64, _d],
+}
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m xtsrcmaps_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        L.xts_last_error.restype = C.c_char_p
+        L.xts_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    """Entry points declared in include/xtsrcmaps_amd.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(xts_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise XtsError(rc, lib().xts_last_error().decode())
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_d)
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_f)
+
+
+def u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u8)
+
+
+def tptr(t) -> _vp:
+    """Device pointer of a contiguous torch CUDA tensor."""
+    if not t.is_cuda or not t.is_contiguous():
+        raise XtsError(1, "device entry points need contiguous CUDA tensors")
+    return _vp(t.data_ptr())
+
+
+_keep = []
+
+
+def psf_grid(params, cosths, logEs):
+    p, c, e = f64(params), f64(cosths), f64(logEs)
+    g = PsfGrid(dptr(p), dptr(c), dptr(e), len(c), len(e))
+    return g, (p, c, e)
+
+
+def psf_grid_dev(params, cosths, logEs):
+    g = PsfGrid(C.cast(_vp(params.data_ptr()), _d), C.cast(_vp(cosths.data_ptr()), _d),
+                C.cast(_vp(logEs.data_ptr()), _d), cosths.shape[0], logEs.shape[0])
+    return g
+
+
+def aeff_grid(cosths, logEs, params, min_costheta):
+    p, c, e = f64(params), f64(cosths), f64(logEs)
+    g = AeffGrid(dptr(p), dptr(c), dptr(e), len(c), len(e), float(min_costheta))
+    return g, (p, c, e)
+
+
+def rule_ptr(rule):
+    if rule is None:
+        return C.cast(None, _d), 0
+    r = f64(rule).reshape(-1, 3)
+    _keep.append(r)
+    if len(_keep) > 16:
+        del _keep[:8]
+    return dptr(r), r.shape[0]
+
+
+def alloc_map(S, wcs, E, flags):
+    dt = np.float32 if flags & OUT_F32 else np.float64
+    H, W = wcs.naxis1, wcs.naxis2
+    shape = (S, E, H, W) if flags & LAYOUT_SEHW else (S, H, W, E)
+    return np.zeros(shape, dtype=dt)

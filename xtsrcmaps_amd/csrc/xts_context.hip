@@ -1,0 +1,257 @@
+// xts_context.hip -- context, device arena, error reporting and WCS setup.
+#include <cstdarg>
+#include <cstring>
+
+#include "xts_internal.hpp"
+
+namespace xts {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int activate(xts_ctx* ctx) {
+    XTS_CHECK(ctx != nullptr, "null context");
+    XTS_HIP(hipSetDevice(ctx->device));
+    return XTS_OK;
+}
+
+int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out) {
+    Slot& s = ctx->slots[slot];
+    if (bytes == 0) bytes = 256;
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            XTS_HIP(hipStreamSynchronize(ctx->stream));
+            XTS_HIP(hipFree(s.ptr));
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        // round up so that slowly growing calls do not reallocate each time
+        size_t want = bytes + bytes / 4;
+        want = (want + 4095) & ~size_t(4095);
+        XTS_HIP(hipMalloc(&s.ptr, want));
+        s.bytes = want;
+    }
+    *out = s.ptr;
+    return XTS_OK;
+}
+
+hipEvent_t pooled_event(xts_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int to_device(xts_ctx* ctx, int slot, const void* host, size_t bytes, void** dev) {
+    XTS_TRY(slot_reserve(ctx, slot, bytes, dev));
+    if (bytes) XTS_HIP(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return XTS_OK;
+}
+
+Wcs wcs_setup(const xts_wcs& in) {
+    Wcs w{};
+    w.crpix1 = in.crpix1;
+    w.crpix2 = in.crpix2;
+    w.proj = in.proj;
+    w.nh = (long)in.naxis1;
+    w.nw = (long)in.naxis2;
+    // CROTA2 -> linear transform: x = cdelt1 cos r dp1 - cdelt2 sin r dp2,
+    // y = cdelt1 sin r dp1 + cdelt2 cos r dp2 (Greisen & Calabretta 2002).
+    const double cr = cosd(in.crota2), sr = sind(in.crota2);
+    w.m11 = in.cdelt1 * cr;
+    w.m12 = -in.cdelt2 * sr;
+    w.m21 = in.cdelt1 * sr;
+    w.m22 = in.cdelt2 * cr;
+    const double det = w.m11 * w.m22 - w.m12 * w.m21;
+    w.i11 = w.m22 / det;
+    w.i12 = -w.m12 / det;
+    w.i21 = -w.m21 / det;
+    w.i22 = w.m11 / det;
+    // Celestial pole of the native frame for a reference point at native
+    // (phi0, theta0) = (0, 0): default LONPOLE is 0 when delta0 >= theta0,
+    // else 180; of the two pole latitudes choose the one nearest LATPOLE=90.
+    const double a0 = in.crval1 > 180.0 ? in.crval1 - 360.0 : in.crval1;
+    const double d0 = in.crval2;
+    const double phip = d0 >= 0.0 ? 0.0 : 180.0;
+    const double x = cosd(phip), y = 0.0;
+    const double z = sqrt(x * x + y * y);
+    const double u = atan2(y, x) * kR2D;
+    const double v = acos(sind(d0) / z) * kR2D;
+    double c1 = u + v, c2 = u - v;
+    if (c1 > 180.0) c1 -= 360.0; else if (c1 < -180.0) c1 += 360.0;
+    if (c2 > 180.0) c2 -= 360.0; else if (c2 < -180.0) c2 += 360.0;
+    const bool ok1 = fabs(c1) <= 90.0 + 1e-10, ok2 = fabs(c2) <= 90.0 + 1e-10;
+    double dp = (ok1 && ok2) ? (fabs(c1 - 90.0) < fabs(c2 - 90.0) ? c1 : c2) : (ok1 ? c1 : c2);
+    dp = dp > 90.0 ? 90.0 : (dp < -90.0 ? -90.0 : dp);
+    double ap;
+    const double zz = cosd(dp) * cosd(d0);
+    if (fabs(zz) < 1e-10) {
+        if (fabs(cosd(d0)) < 1e-10) ap = a0;
+        else if (dp > 0) ap = a0 + phip - 180.0;
+        else ap = a0 - phip;
+    } else {
+        const double xx = (0.0 - sind(dp) * sind(d0)) / zz;
+        const double yy = sind(phip) / cosd(d0);
+        ap = a0 - atan2(yy, xx) * kR2D;
+    }
+    w.alphap = ap;
+    w.deltap = dp;
+    w.phip = phip;
+    w.sdp = sind(dp);
+    w.cdp = cosd(dp);
+    return w;
+}
+
+// Published Festa-Sommariva low-cardinality square rule of degree 7
+// (cardinality 12, all points inside, positive weights), as used by the
+// reference's production cubature (mm_cubature/cubature.hpp:209-225).
+const double kSquareDeg7[36] = {
+    3.9383136101878902e-01, -3.6699793434042738e-01, 5.2092862230230441e-01,
+    8.0196847620659251e-01, 8.0992592975809186e-01, 2.3740240561379911e-01,
+    -1.8327327664247231e-02, 9.2596092593108370e-01, 2.4166897208389651e-01,
+    3.6699793434042738e-01, 3.9383136101878902e-01, 5.2092862230230441e-01,
+    -8.0992592975809186e-01, 8.0196847620659251e-01, 2.3740240561379911e-01,
+    -9.2596092593108370e-01, -1.8327327664247231e-02, 2.4166897208389651e-01,
+    -3.9383136101878902e-01, 3.6699793434042738e-01, 5.2092862230230441e-01,
+    -8.0196847620659251e-01, -8.0992592975809186e-01, 2.3740240561379911e-01,
+    1.8327327664247231e-02, -9.2596092593108370e-01, 2.4166897208389651e-01,
+    -3.6699793434042738e-01, -3.9383136101878902e-01, 5.2092862230230441e-01,
+    8.0992592975809186e-01, -8.0196847620659251e-01, 2.3740240561379911e-01,
+    9.2596092593108370e-01, 1.8327327664247231e-02, 2.4166897208389651e-01,
+};
+
+}  // namespace xts
+
+using namespace xts;
+
+extern "C" {
+
+const char* xts_last_error(void) { return g_err; }
+
+int xts_abi_version(void) { return XTS_ABI_VERSION; }
+
+int xts_ctx_create(int device, xts_ctx** out) {
+    XTS_CHECK(out != nullptr, "xts_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    XTS_HIP(hipGetDeviceCount(&n));
+    XTS_CHECK(device >= 0 && device < n, "xts_ctx_create: device %d of %d", device, n);
+    xts_ctx* c = new xts_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(XTS_ERR_DEVICE, "xts_ctx_create: %s", hipGetErrorString(e));
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return XTS_OK;
+}
+
+int xts_ctx_destroy(xts_ctx* ctx) {
+    if (!ctx) return XTS_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& r : ctx->timing_recs) {
+        hipEventDestroy(r.start);
+        hipEventDestroy(r.stop);
+    }
+    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    for (auto& s : ctx->slots)
+        if (s.ptr) hipFree(s.ptr);
+    if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return XTS_OK;
+}
+
+int xts_ctx_set_stream(xts_ctx* ctx, void* stream) {
+    XTS_CHECK(ctx != nullptr, "null context");
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return XTS_OK;
+}
+
+int xts_ctx_enable_timing(xts_ctx* ctx, int enable) {
+    XTS_CHECK(ctx != nullptr, "null context");
+    ctx->timing = enable ? 1 : 0;
+    return XTS_OK;
+}
+
+int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms, int64_t* count) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(name && total_ms && count, "null argument");
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    int64_t n = 0;
+    std::vector<xts_timing_rec> keep;
+    for (auto& r : ctx->timing_recs) {
+        if (strcmp(r.name, name) == 0) {
+            float ms = 0.f;
+            XTS_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+            tot += ms;
+            ++n;
+            ctx->event_pool.push_back(r.start);
+            ctx->event_pool.push_back(r.stop);
+        } else {
+            keep.push_back(r);
+        }
+    }
+    ctx->timing_recs.swap(keep);
+    *total_ms = tot;
+    *count = n;
+    return XTS_OK;
+}
+
+int xts_ctx_synchronize(xts_ctx* ctx) {
+    XTS_TRY(activate(ctx));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+int xts_costhetas(int64_t nbins, double cosmin, int32_t thetabin, double* out) {
+    XTS_CHECK(out != nullptr && nbins > 0, "xts_costhetas: bad arguments");
+    for (int64_t i = 0; i < nbins; ++i) {
+        double f = (i + 0.5) / (double)nbins;
+        f = thetabin ? f * f : f;
+        out[i] = 1. - f * (1. - cosmin);
+    }
+    return XTS_OK;
+}
+
+int xts_livetime_efficiency_factors(const double* logEs, int64_t E, const float* p0,
+                                    const float* p1, double* ltf) {
+    XTS_CHECK(logEs && p0 && p1 && ltf && E > 0, "xts_livetime_efficiency_factors: bad arguments");
+    const float* ps[2] = {p1, p0};  // row 0 from p1, row 1 from p0
+    for (int r = 0; r < 2; ++r) {
+        const float* p = ps[r];
+        const float a0 = p[0], b0 = p[1], a1 = p[2], lb1 = p[3], a2 = p[4], lb2 = p[5];
+        const float b1 = (a0 - a1) * lb1 + b0;
+        const float b2 = (a1 - a2) * lb2 + b1;
+        for (int64_t e = 0; e < E; ++e) {
+            const double x = logEs[e];
+            ltf[r * E + e] = x < lb1 ? a0 * x + b0 : (x < lb2 ? a1 * x + (double)b1 : a2 * x + (double)b2);
+        }
+    }
+    return XTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,465 @@
+// xts_psf.hip -- (b) mean-PSF build on gfx950: King PSF normalisation, the
+// King table on the 401 separations, its bilinear sampling to the
+// observation's (cos theta, log E) points weighted by the effective area, the
+// livetime contraction, the mean over exposure and the FP64 radial integral.
+#include <algorithm>
+#include <vector>
+
+#include "xts_internal.hpp"
+#include "xts_mfma.hpp"
+
+namespace xts {
+
+// irf/irf.hxx:22-45 (gamma == 1 -> 1.001)
+__host__ __device__ inline double king_f(double u, double g) {
+    if (g == 1.0) g = 1.001;
+    return (1.0 - 1.0 / g) * pow(1.0 + u / g, -g);
+}
+
+__host__ __device__ inline double evaluate_king(double sep, double sf, const double* p) {
+    const double score = p[2] * sf, stail = p[3] * sf;
+    const double rc = sep / score, rt = sep / stail;
+    return p[0] * king_f(rc * rc / 2.0, p[4]) + p[1] * p[0] * king_f(rt * rt / 2.0, p[5]);
+}
+
+// irf/irf.hxx:48-77
+__host__ __device__ inline double base_integral(double radius, double sf, const double* p) {
+    const double score = p[2] * sf, stail = p[3] * sf;
+    const double sep = radius * kD2R;
+    const double rc = sep / score, rt = sep / stail;
+    const double uc = rc * rc / 2.0, ut = rt * rt / 2.0;
+    return p[0] * (1.0 - pow(1.0 + uc / p[4], 1.0 - p[4])) * kTwoPi * score * score +
+           p[1] * p[0] * (1.0 - pow(1.0 + ut / p[5], 1.0 - p[5])) * kTwoPi * stail * stail;
+}
+
+// irf/irf.normalize_rpsf.cpp:11-61, one thread per (cos theta, energy) node
+__global__ void normalize_rpsf_kernel(int Mc, int Me, double* __restrict__ params,
+                                      const double* __restrict__ logEs, double sp0, double sp1,
+                                      double si, const double* __restrict__ gl /* [64 x, 64 w] */) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Mc * Me) return;
+    const int e = k % Me;
+    double p[6];
+    for (int i = 0; i < 6; ++i) p[i] = params[(size_t)k * 6 + i];
+    const double energy = pow(10.0, logEs[e]);
+    const double tt = pow(energy * 1.e-2, si);
+    const double sf = sqrt(sp0 * tt * tt + sp1);
+    double norm;
+    if (energy < 120.0) {
+        const double hw = 45.0, mid = 45.0;
+        double acc = 0.0;
+        for (int i = 0; i < 64; ++i) {
+            const double v = hw * gl[i] + mid;
+            const double kv = evaluate_king(v * kD2R, sf, p);
+            acc += gl[64 + i] * (kv * (sin(v * kD2R) * kTwoPi * kD2R));
+        }
+        norm = acc * hw;
+    } else {
+        norm = base_integral(90.0, sf, p);
+    }
+    params[(size_t)k * 6 + 0] = p[0] / norm;
+    params[(size_t)k * 6 + 2] = p[2] * sf;
+    params[(size_t)k * 6 + 3] = p[3] * sf;
+    params[(size_t)k * 6 + 4] = p[4] == 1.0 ? 1.001 : p[4];
+    params[(size_t)k * 6 + 5] = p[5] == 1.0 ? 1.001 : p[5];
+}
+
+// psf/psf.king.cxx:9-43: K[d, c, e] at the separations (radians)
+__global__ void king_kernel(int Mc, int Me, const double* __restrict__ params,
+                            const double* __restrict__ sep, double* __restrict__ out) {
+    const int n = kND * Mc * Me;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int d = k / (Mc * Me), ce = k % (Mc * Me);
+        out[k] = evaluate_king(sep[d] * kD2R, 1.0, params + (size_t)ce * 6);
+    }
+}
+
+__host__ __device__ inline bool lerp_pars_psf(const double* g, int n, double v, double* wt,
+                                              double* cw, int* idx) {
+    int up = 0;
+    while (up < n && !(v < g[up])) ++up;
+    if (up == 0 || up == n) return false;
+    const double tt = (v - g[up - 1]) / (g[up] - g[up - 1]);
+    *wt = tt; *cw = 1.0 - tt; *idx = up;
+    return true;
+}
+
+// psf/psf.bilerp.cxx:6-52 times the effective area
+// (psf.corrected_exposure_psf.cxx:37-44): PA[c, d, e] = bilerp(K)[c, d, e] x
+// aeff[c, e].  One thread per (c, d, e).
+__global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const double* __restrict__ costh,
+                                   const double* __restrict__ logE, const double* __restrict__ pc,
+                                   const double* __restrict__ pe, const double* __restrict__ K,
+                                   const double* __restrict__ aeff, double* __restrict__ out,
+                                   int* __restrict__ err) {
+    const size_t n = (size_t)C * kND * E;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int e = (int)(k % E);
+        const int d = (int)((k / E) % kND);
+        const int c = (int)(k / ((size_t)E * kND));
+        double cw, ccw, ew, ecw;
+        int ci, ei;
+        if (!lerp_pars_psf(pc, Mc, costh[c], &cw, &ccw, &ci) ||
+            !lerp_pars_psf(pe, Me, logE[e], &ew, &ecw, &ei)) {
+            atomicOr(err, 1);
+            out[k] = 0.0;
+            continue;
+        }
+        const double* Kd = K + (size_t)d * Mc * Me;
+        const double v = ccw * ecw * Kd[(ci - 1) * Me + ei - 1] + ccw * ew * Kd[(ci - 1) * Me + ei] +
+                         cw * ecw * Kd[ci * Me + ei - 1] + cw * ew * Kd[ci * Me + ei];
+        out[k] = v * aeff[(size_t)c * E + e];
+    }
+}
+
+// psf.corrected_exposure_psf.cxx:46-117 for both sections and
+// psf.mean_psf.cxx:20-30: for each source and column n = (d, e):
+//   front = (ECB . PAf) LTF0 + (WECB . PAf) LTF1, back likewise with PAb,
+//   mean  = (front + back) / exposure.
+// FP64 FMA; a block covers kSrcTile sources x 256 columns, the livetime rows
+// of its sources staged in LDS.
+constexpr int kSrcTile = 8;
+
+__global__ void __launch_bounds__(256)
+mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
+                const double* __restrict__ wecb, const double* __restrict__ paf,
+                const double* __restrict__ pab, const double* __restrict__ ltf,
+                const double* __restrict__ expo, double* __restrict__ out) {
+    extern __shared__ double sh[];  // [2][kSrcTile][C]
+    const int N = kND * E;
+    const int s0 = blockIdx.y * kSrcTile;
+    const int ns = min(kSrcTile, S - s0);
+    for (int k = threadIdx.x; k < 2 * kSrcTile * C; k += blockDim.x) {
+        const int which = k / (kSrcTile * C), r = (k / C) % kSrcTile, c = k % C;
+        const double* src = which ? wecb : ecb;
+        sh[k] = r < ns ? src[(size_t)(s0 + r) * C + c] : 0.0;
+    }
+    __syncthreads();
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int e = n % E;
+    double ef[kSrcTile], wf[kSrcTile], eb[kSrcTile], wb[kSrcTile];
+#pragma unroll
+    for (int r = 0; r < kSrcTile; ++r) ef[r] = wf[r] = eb[r] = wb[r] = 0.0;
+    for (int c = 0; c < C; ++c) {
+        const double f = paf[(size_t)c * N + n], b = pab[(size_t)c * N + n];
+#pragma unroll
+        for (int r = 0; r < kSrcTile; ++r) {
+            const double x = sh[r * C + c], y = sh[(kSrcTile + r) * C + c];
+            ef[r] = fma(x, f, ef[r]);
+            wf[r] = fma(y, f, wf[r]);
+            eb[r] = fma(x, b, eb[r]);
+            wb[r] = fma(y, b, wb[r]);
+        }
+    }
+    const double l0 = ltf[e], l1 = ltf[E + e];
+#pragma unroll
+    for (int r = 0; r < kSrcTile; ++r) {
+        if (r < ns) {
+            const int s = s0 + r;
+            const double front = ef[r] * l0 + wf[r] * l1;
+            const double back = eb[r] * l0 + wb[r] * l1;
+            out[(size_t)s * N + n] = (front + back) / expo[(size_t)s * E + e];
+        }
+    }
+}
+
+// BF16x3-MFMA variant: P = [ECB; WECB] (2S x C) . [PAf | PAb] (C x 2N)
+// computed by gemm_bf16x3, then this combine.
+__global__ void mean_psf_combine_kernel(int S, int E, const double* __restrict__ P,
+                                        const double* __restrict__ ltf,
+                                        const double* __restrict__ expo, double* __restrict__ out) {
+    const int N = kND * E;
+    const size_t n_all = (size_t)S * N;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_all;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(k / N), n = (int)(k % N), e = n % E;
+        const size_t ld = 2 * (size_t)N;
+        const double ef = P[(size_t)s * ld + n], eb = P[(size_t)s * ld + N + n];
+        const double wf = P[(size_t)(S + s) * ld + n], wb = P[(size_t)(S + s) * ld + N + n];
+        const double l0 = ltf[e], l1 = ltf[E + e];
+        out[k] = ((ef * l0 + wf * l1) + (eb * l0 + wb * l1)) / expo[(size_t)s * E + e];
+    }
+}
+
+__global__ void pack_pa_kernel(int C, int N, const double* __restrict__ a,
+                               const double* __restrict__ b, double* __restrict__ out) {
+    const size_t n = (size_t)C * 2 * N;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = k / (2 * (size_t)N), j = k % (2 * (size_t)N);
+        out[k] = j < (size_t)N ? a[c * N + j] : b[c * N + j - N];
+    }
+}
+
+// psf.partial_total_integral.cxx:8-52 and psf.normalize.cxx:7-24 in FP64:
+// one thread per (source, energy) scans the 400 segments, then normalises
+// the cumulative integral and the PSF by the total.
+__global__ void integral_kernel(int S, int E, double* __restrict__ upsf,
+                                double* __restrict__ parint, double* __restrict__ total,
+                                const double* __restrict__ sep) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S * E) return;
+    const int s = k / E, e = k % E;
+    double* u = upsf + (size_t)s * kND * E + e;
+    double* pi = parint + (size_t)s * kND * E + e;
+    double cum = 0.0;
+    pi[0] = 0.0;
+    double t1 = 0.0, y1 = kTwoPi * u[0] * sin(t1);
+    for (int d = 1; d < kND; ++d) {
+        const double t2 = kD2R * sep[d];
+        const double y2 = kTwoPi * u[(size_t)d * E] * sin(t2);
+        const double m = (y2 - y1) / (t2 - t1);
+        const double b = y1 - t1 * m;
+        cum += 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
+        pi[(size_t)d * E] = cum;
+        t1 = t2;
+        y1 = y2;
+    }
+    if (total) total[k] = cum;
+    if (cum != 0.0)
+        for (int d = 0; d < kND; ++d) pi[(size_t)d * E] /= cum;
+    for (int d = 0; d < kND; ++d) u[(size_t)d * E] /= cum;
+}
+
+static void separations_host(double* sep) {
+    const double xstep = std::log(70.0 / 1e-4) / 399.0;
+    sep[0] = 0.0;
+    for (int i = 0; i < 400; ++i) sep[i + 1] = 1e-4 * std::exp(i * xstep);
+}
+
+// Gauss-Legendre nodes and weights by Newton iteration on P_n.
+static void legendre(int n, double* x, double* w) {
+    for (int i = 1; i <= n; ++i) {
+        double z = std::cos(kPi * (i - 0.25) / (n + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int j = 2; j <= n; ++j) {
+                const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) <= 1e-15) break;
+        }
+        x[i - 1] = z;
+        w[i - 1] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+struct PsfDevGrid {
+    const double* params;
+    const double* cosths;
+    const double* logEs;
+    int Mc, Me;
+};
+
+static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* costh, int C,
+                             const double* logE, int E, const double* af, const double* ab,
+                             const double* ecb, const double* wecb, const double* ltf,
+                             const double* expo, int S, double* upsf, double* parint,
+                             int32_t precision, int* err_dev) {
+    XTS_CHECK(precision == XTS_PREC_FP64 || precision == XTS_PREC_BF16X3, "bad precision %d",
+              precision);
+    const int N = kND * E;
+    // separations table
+    double sep[kND];
+    separations_host(sep);
+    void* dsep;
+    XTS_TRY(to_device(ctx, kSlotWork3, sep, sizeof(sep), &dsep));
+    // King tables and the aeff-weighted bilinear samples, front then back
+    const size_t kf = (size_t)kND * g[0].Mc * g[0].Me, kb = (size_t)kND * g[1].Mc * g[1].Me;
+    const size_t npa = (size_t)C * N;
+    void* w;
+    XTS_TRY(slot_reserve(ctx, kSlotWork0, (kf + kb + 2 * npa) * 8, &w));
+    double* Kf = (double*)w;
+    double* Kb = Kf + kf;
+    double* PAf = Kb + kb;
+    double* PAb = PAf + npa;
+    king_kernel<<<std::min<unsigned>(div_up(kf, 256), 4096), 256, 0, ctx->stream>>>(
+        g[0].Mc, g[0].Me, g[0].params, (double*)dsep, Kf);
+    king_kernel<<<std::min<unsigned>(div_up(kb, 256), 4096), 256, 0, ctx->stream>>>(
+        g[1].Mc, g[1].Me, g[1].params, (double*)dsep, Kb);
+    const unsigned bb = std::min<unsigned>(div_up(npa, 256), 16384);
+    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[0].Mc, g[0].Me, costh, logE,
+                                                    g[0].cosths, g[0].logEs, Kf, af, PAf, err_dev);
+    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[1].Mc, g[1].Me, costh, logE,
+                                                    g[1].cosths, g[1].logEs, Kb, ab, PAb, err_dev);
+    if (precision == XTS_PREC_FP64) {
+        const size_t sm = 2 * kSrcTile * C * 8;
+        mean_psf_kernel<<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
+            S, C, E, ecb, wecb, PAf, PAb, ltf, expo, upsf);
+    } else {
+        void* w2;
+        const size_t nA = 2 * (size_t)S * C, nB = (size_t)C * 2 * N, nP = 2 * (size_t)S * 2 * N;
+        XTS_TRY(slot_reserve(ctx, kSlotWork2, (nA + nB + nP) * 8, &w2));
+        double* A = (double*)w2;
+        double* B = A + nA;
+        double* P = B + nB;
+        pack_rows_kernel<<<div_up(nA, 256), 256, 0, ctx->stream>>>(S, C, ecb, wecb, A);
+        pack_pa_kernel<<<std::min<unsigned>(div_up(nB, 256), 16384), 256, 0, ctx->stream>>>(
+            C, N, PAf, PAb, B);
+        XTS_TRY(gemm_bf16x3(ctx, A, B, P, 2 * S, C, 2 * N));
+        mean_psf_combine_kernel<<<std::min<unsigned>(div_up((int64_t)S * N, 256), 16384), 256, 0,
+                                  ctx->stream>>>(S, E, P, ltf, expo, upsf);
+    }
+    integral_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>(S, E, upsf, parint, nullptr,
+                                                                  (double*)dsep);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
+}  // namespace xts
+
+using namespace xts;
+
+extern "C" {
+
+int xts_normalize_rpsf(xts_ctx* ctx, double* params, const double* logEs, int64_t Mc, int64_t Me,
+                       float scale0, float scale1, float scale_index) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(params && logEs && Mc > 0 && Me > 0, "bad arguments");
+    double gl[128];
+    legendre(64, gl, gl + 64);
+    // IrfScale members are float: the squares are formed in float
+    const float sp0 = scale0 * scale0, sp1 = scale1 * scale1;
+    void *dp, *de, *dg;
+    XTS_TRY(to_device(ctx, kSlotIn0, params, Mc * Me * 6 * 8, &dp));
+    XTS_TRY(to_device(ctx, kSlotIn1, logEs, Me * 8, &de));
+    XTS_TRY(to_device(ctx, kSlotIn2, gl, sizeof(gl), &dg));
+    normalize_rpsf_kernel<<<div_up(Mc * Me, 64), 64, 0, ctx->stream>>>(
+        (int)Mc, (int)Me, (double*)dp, (double*)de, (double)sp0, (double)sp1, (double)scale_index,
+        (double*)dg);
+    XTS_HIP(hipGetLastError());
+    XTS_HIP(hipMemcpyAsync(params, dp, Mc * Me * 6 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+int xts_king(xts_ctx* ctx, const xts_psf_grid* g, double* out) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(g && g->params && out && g->Mc > 0 && g->Me > 0, "bad arguments");
+    double sep[kND];
+    separations_host(sep);
+    void *dsep, *dp, *dout;
+    XTS_TRY(to_device(ctx, kSlotWork3, sep, sizeof(sep), &dsep));
+    XTS_TRY(to_device(ctx, kSlotIn0, g->params, g->Mc * g->Me * 6 * 8, &dp));
+    const size_t n = (size_t)kND * g->Mc * g->Me;
+    XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8, &dout));
+    king_kernel<<<std::min<unsigned>(div_up(n, 256), 4096), 256, 0, ctx->stream>>>(
+        (int)g->Mc, (int)g->Me, (double*)dp, (double*)dsep, (double*)dout);
+    XTS_HIP(hipGetLastError());
+    XTS_HIP(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+static int check_psf_args(const xts_psf_grid* f, const xts_psf_grid* b, int64_t C, int64_t E,
+                          int64_t S) {
+    XTS_CHECK(f && b && f->params && b->params && f->cosths && b->cosths && f->logEs && b->logEs,
+              "null PSF grid");
+    XTS_CHECK(f->Mc > 1 && f->Me > 1 && b->Mc > 1 && b->Me > 1, "bad PSF grid size");
+    XTS_CHECK(C > 0 && E > 0 && S > 0 && (int64_t)kND * E < (1LL << 30), "bad sizes");
+    return XTS_OK;
+}
+
+int xts_psf_lookup_table_and_partial_integrals_dev(
+    xts_ctx* ctx, const xts_psf_grid* front, const xts_psf_grid* back, const double* costh,
+    int64_t C, const double* logE, int64_t E, const double* af, const double* ab,
+    const double* ecb, const double* wecb, const double* ltf, const double* expo, int64_t S,
+    double* upsf, double* parint, int32_t precision) {
+    XTS_TRY(activate(ctx));
+    XTS_TRY(check_psf_args(front, back, C, E, S));
+    XTS_CHECK(costh && logE && af && ab && ecb && wecb && ltf && expo && upsf && parint,
+              "null device pointer");
+    PsfDevGrid g[2] = {{front->params, front->cosths, front->logEs, (int)front->Mc, (int)front->Me},
+                       {back->params, back->cosths, back->logEs, (int)back->Mc, (int)back->Me}};
+    // range errors are reported by the host entry point; here they zero the
+    // offending samples
+    void* derr;
+    XTS_TRY(slot_reserve(ctx, kSlotOut3, 256, &derr));
+    return psf_lookup_device(ctx, g, costh, (int)C, logE, (int)E, af, ab, ecb, wecb, ltf, expo,
+                             (int)S, upsf, parint, precision, (int*)derr);
+}
+
+int xts_psf_lookup_table_and_partial_integrals(
+    xts_ctx* ctx, const xts_psf_grid* front, const xts_psf_grid* back, const double* costh,
+    int64_t C, const double* logE, int64_t E, const double* af, const double* ab,
+    const double* ecb, const double* wecb, const double* ltf, const double* expo, int64_t S,
+    double* upsf, double* parint, int32_t precision) {
+    XTS_TRY(activate(ctx));
+    XTS_TRY(check_psf_args(front, back, C, E, S));
+    XTS_CHECK(costh && logE && af && ab && ecb && wecb && ltf && expo && upsf && parint,
+              "null host pointer");
+    const xts_psf_grid* gs[2] = {front, back};
+    PsfDevGrid g[2];
+    for (int k = 0; k < 2; ++k) {
+        void *dp, *dc, *de;
+        XTS_TRY(to_device(ctx, kSlotIn0 + 3 * k, gs[k]->params, gs[k]->Mc * gs[k]->Me * 6 * 8, &dp));
+        XTS_TRY(to_device(ctx, kSlotIn1 + 3 * k, gs[k]->cosths, gs[k]->Mc * 8, &dc));
+        XTS_TRY(to_device(ctx, kSlotIn2 + 3 * k, gs[k]->logEs, gs[k]->Me * 8, &de));
+        g[k] = {(double*)dp, (double*)dc, (double*)de, (int)gs[k]->Mc, (int)gs[k]->Me};
+    }
+    // small per-call inputs packed into one staging buffer
+    const size_t nC = C, nE = E, nCE = C * E, nSC = S * C, nSE = S * E;
+    std::vector<double> pack(nC + nE + 2 * nCE + 2 * nSC + 2 * nE + nSE);
+    double* q = pack.data();
+    std::copy(costh, costh + nC, q); q += nC;
+    std::copy(logE, logE + nE, q); q += nE;
+    std::copy(af, af + nCE, q); q += nCE;
+    std::copy(ab, ab + nCE, q); q += nCE;
+    std::copy(ecb, ecb + nSC, q); q += nSC;
+    std::copy(wecb, wecb + nSC, q); q += nSC;
+    std::copy(ltf, ltf + 2 * nE, q); q += 2 * nE;
+    std::copy(expo, expo + nSE, q);
+    void* dpk;
+    XTS_TRY(to_device(ctx, kSlotIn6, pack.data(), pack.size() * 8, &dpk));
+    const double* d = (const double*)dpk;
+    const double *dcos = d, *dloge = dcos + nC, *daf = dloge + nE, *dab = daf + nCE,
+                 *decb = dab + nCE, *dwecb = decb + nSC, *dltf = dwecb + nSC, *dexpo = dltf + 2 * nE;
+    const size_t nout = (size_t)S * kND * E;
+    void* dout;
+    XTS_TRY(slot_reserve(ctx, kSlotOut0, nout * 8 * 2, &dout));
+    double* dupsf = (double*)dout;
+    double* dpar = dupsf + nout;
+    const int zero = 0;
+    void* derr;
+    XTS_TRY(to_device(ctx, kSlotOut3, &zero, sizeof(int), &derr));
+    XTS_TRY(psf_lookup_device(ctx, g, dcos, (int)C, dloge, (int)E, daf, dab, decb, dwecb, dltf,
+                              dexpo, (int)S, dupsf, dpar, precision, (int*)derr));
+    int err = 0;
+    XTS_HIP(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipMemcpyAsync(upsf, dupsf, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipMemcpyAsync(parint, dpar, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) return fail(XTS_ERR_RANGE, "psf lookup: a sample lies outside the PSF IRF grid");
+    return XTS_OK;
+}
+
+int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S, int64_t E, double* parint,
+                               double* total) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(upsf && parint && S > 0 && E > 0, "bad arguments");
+    const size_t n = (size_t)S * kND * E;
+    double sep[kND];
+    separations_host(sep);
+    void *dsep, *du, *dout;
+    XTS_TRY(to_device(ctx, kSlotWork3, sep, sizeof(sep), &dsep));
+    XTS_TRY(to_device(ctx, kSlotIn0, upsf, n * 8, &du));
+    XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8 + S * E * 8, &dout));
+    double* dpar = (double*)dout;
+    double* dtot = dpar + n;
+    integral_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>((int)S, (int)E, (double*)du, dpar,
+                                                                  dtot, (double*)dsep);
+    XTS_HIP(hipGetLastError());
+    XTS_HIP(hipMemcpyAsync(upsf, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipMemcpyAsync(parint, dpar, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (total) XTS_HIP(hipMemcpyAsync(total, dtot, S * E * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""Synthetic inputs of the benchmark configurations (fixed seeds).
+
+All images are CAR with CRVAL2 = 0 (plate carree), so pixel <-> sky is the
+closed form lon = crval1 + cdelt1 (p1 - crpix1), lat = cdelt2 (p2 - crpix2).
+The King-PSF and effective-area IRF grids are built in the reference's
+padded layout (irf/irf.prepare_grid.cpp:26-102: bin centres plus -1/+1 in
+cos theta and 0/10 in log10 E, edge-duplicated parameters).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .capi import Wcs
+
+
+@dataclass
+class Config:
+    name: str
+    n_src: int
+    npix: int
+    cdelt: float
+    n_energy: int = 37
+    emin: float = 100.0
+    emax: float = 1e6
+    rule: str = "deg7"          # "deg7" (reference production rule) or "gl8"
+    seed: int = 0
+    out_f32: bool = True
+    src_mode: str = "centre"    # "centre" or "uniform"
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def evals(self) -> int:
+        return self.n_src * self.n_energy * self.npix * self.npix
+
+
+CONFIGS = {
+    # one source at the centre of a 100x100 0.1 deg image, FP64 output
+    "c0": Config("c0", 1, 100, 0.1, rule="deg7", seed=0, out_f32=False, src_mode="centre"),
+    # 100 sources uniform over a 200x200 0.05 deg image, 8x8 Gauss-Legendre, FP32
+    "c1": Config("c1", 100, 200, 0.05, rule="gl8", seed=1, out_f32=True, src_mode="uniform"),
+}
+
+
+def wcs_for(cfg: Config) -> Wcs:
+    n = cfg.npix
+    return Wcs.make((n, n), (n / 2 + 0.5, n / 2 + 0.5), (180.0, 0.0), (-cfg.cdelt, cfg.cdelt),
+                    0.0, "CAR")
+
+
+def pix2sph_car(wcs: Wcs, p1, p2):
+    assert wcs.crval2 == 0.0 and wcs.crota2 == 0.0 and wcs.proj == 0
+    lon = wcs.crval1 + wcs.cdelt1 * (np.asarray(p1) - wcs.crpix1)
+    lat = wcs.cdelt2 * (np.asarray(p2) - wcs.crpix2)
+    return np.mod(lon, 360.0), lat
+
+
+def gauss_legendre_rule(n: int) -> np.ndarray:
+    """n x n tensor Gauss-Legendre rule on [-1, 1]^2 as (u, v, w) rows, sum w = 4."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    u, v = np.meshgrid(x, x, indexing="ij")
+    ww = np.outer(w, w)
+    return np.stack([u.ravel(), v.ravel(), ww.ravel()], axis=1)
+
+
+def rule_for(cfg: Config):
+    """None selects the library's built-in degree-7 rule."""
+    return None if cfg.rule == "deg7" else gauss_legendre_rule(int(cfg.rule[2:]))
+
+
+def energies(cfg: Config) -> np.ndarray:
+    k = np.arange(cfg.n_energy)
+    return cfg.emin * (cfg.emax / cfg.emin) ** (k / (cfg.n_energy - 1))
+
+
+def sources(cfg: Config, wcs: Wcs) -> np.ndarray:
+    rng = np.random.default_rng(cfg.seed + 1000)
+    if cfg.src_mode == "centre":
+        p = np.full((cfg.n_src, 2), cfg.npix / 2 + 0.5)
+    else:
+        p = rng.uniform(0.5, cfg.npix + 0.5, (cfg.n_src, 2))
+    lon, lat = pix2sph_car(wcs, p[:, 0], p[:, 1])
+    return np.stack([lon, lat], axis=1)
+
+
+def _padded_grid(e_edges, c_edges, params_raw):
+    """prepare_grid layout: [Mc0 + 2, Me0 + 2, Ng] with padded axes."""
+    e_lo, e_hi = e_edges[:-1], e_edges[1:]
+    c_lo, c_hi = c_edges[:-1], c_edges[1:]
+    cos = np.concatenate([[-1.0], 0.5 * (c_lo + c_hi), [1.0]])
+    loge = np.concatenate([[0.0], 0.5 * np.log10(e_lo * e_hi), [10.0]])
+    mc0, me0 = len(c_lo), len(e_lo)
+    t_ = np.clip(np.arange(mc0 + 2) - 1, 0, mc0 - 1)
+    e_ = np.clip(np.arange(me0 + 2) - 1, 0, me0 - 1)
+    params = params_raw[np.ix_(t_, e_)]
+    return cos, loge, params, float(c_lo[0])
+
+
+@dataclass
+class Inputs:
+    cfg: Config
+    wcs: Wcs
+    src_sph: np.ndarray
+    logE: np.ndarray
+    costh: np.ndarray
+    king_raw: dict           # side -> (params [Mc,Me,6] before normalisation, cos, loge)
+    psf_scale: tuple
+    aeff_grid: dict          # side -> (cos, loge, params [Mc, Me], minc)
+    effic: tuple             # (p0, p1) float32[6]
+    cube: np.ndarray         # [12 nside^2, C] livetime
+    wcube: np.ndarray
+    nside: int
+    rule: np.ndarray | None
+
+
+def make_inputs(cfg: Config) -> Inputs:
+    rng = np.random.default_rng(cfg.seed)
+    wcs = wcs_for(cfg)
+    src = sources(cfg, wcs)
+    logE = np.log10(energies(cfg))
+    # King PSF IRF: 23 log energy bins x 8 cos theta bins
+    e_edges = 10.0 ** (0.75 + 0.25 * np.arange(24))
+    c_edges = np.linspace(0.2, 1.0, 9)
+    king = {}
+    for side in ("front", "back"):
+        raw = np.zeros((8, 23, 6))
+        raw[..., 0] = 1.0
+        raw[..., 1] = rng.uniform(0.05, 0.5, (8, 23))
+        raw[..., 2] = np.exp(rng.uniform(np.log(1e-3), np.log(1e-1), (8, 23)))
+        raw[..., 3] = np.exp(rng.uniform(np.log(1e-3), np.log(1e-1), (8, 23)))
+        raw[..., 4] = rng.uniform(1.5, 4.0, (8, 23))
+        raw[..., 5] = rng.uniform(1.5, 4.0, (8, 23))
+        cos, loge, params, _ = _padded_grid(e_edges, c_edges, raw)
+        king[side] = (params, cos, loge)
+    # Aeff: smooth log-parabola in E times linear in cos theta
+    ae_edges = 10.0 ** (0.75 + (6.5 - 0.75) * np.arange(41) / 40)
+    ac_edges = np.linspace(0.2, 1.0, 17)
+    ec = np.sqrt(ae_edges[:-1] * ae_edges[1:])
+    cc = 0.5 * (ac_edges[:-1] + ac_edges[1:])
+    shape_e = 0.9 * np.exp(-0.08 * np.log(ec / 3e3) ** 2)
+    aeff = {}
+    for side, scale in (("front", 1.0), ("back", 0.7)):
+        raw = scale * np.outer(0.2 + 0.8 * cc, shape_e)
+        cos, loge, params, minc = _padded_grid(ae_edges, ac_edges, raw)
+        aeff[side] = (cos, loge, params, minc)
+    p0 = np.array([-1.2, 3.9, -0.5, 2.3, -0.07, 3.5], np.float32)
+    p1 = np.array([1.9, -4.2, 0.47, 2.3, 0.066, 3.5], np.float32)
+    nside = 32
+    cube = rng.uniform(0.0, 1e8, (12 * nside * nside, 40))
+    wcube = 0.3 * cube
+    costh = 1.0 - ((np.arange(40) + 0.5) / 40) ** 2      # cosmin 0, sqrt binning
+    return Inputs(cfg, wcs, src, logE, costh, king, (0.0, 1.0, 0.0), aeff, (p0, p1), cube,
+                  wcube, nside, rule_for(cfg))
