@@ -38,7 +38,10 @@ def setup(cfg_name: str):
     cfg = synthetic.CONFIGS[cfg_name]
     inp = synthetic.make_inputs(cfg)
     ctx = xa.Context(0)
-    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    # one non-default stream for the library and for the timing events
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx.set_stream(stream.cuda_stream)
     dev = torch.device("cuda:0")
     t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
     # IRF preparation (per IRF, not per source): normalisation and Aeff

@@ -176,6 +176,10 @@ __device__ inline int sep_index_fast(T s) {
 template <typename T> struct Vec;
 template <> struct Vec<float> { using type = float4; static constexpr int n = 4; };
 template <> struct Vec<double> { using type = double2; static constexpr int n = 2; };
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+constexpr int kNDPad = 404;   // 401 table entries, padded to keep 16-byte alignment
 
 template <typename V, typename T>
 __device__ inline void vfma(V& acc, T w, const V& x);
@@ -189,19 +193,50 @@ __device__ inline void vfma<double2, double>(double2& a, double w, const double2
     a.x = fma(w, x.x, a.x); a.y = fma(w, x.y, a.y);
 }
 
+// asin on [0, 1]: for x < 1/4 the odd Taylor series
+// x + x^3 sum_k c_k x^(2k-2), c_k = (2k)! / (4^k (k!)^2 (2k+1)), through
+// x^13 (the first omitted term is below 1e-10 relative); the library
+// routine above 1/4 (separations beyond 29 degrees).
+__device__ inline float asin_pos(float x) {
+    if (x < 0.25f) {
+        const float y = x * x;
+        float p = 0.017352764423076924f;
+        p = fmaf(p, y, 0.022372159090909092f);
+        p = fmaf(p, y, 0.030381944444444444f);
+        p = fmaf(p, y, 0.044642857142857144f);
+        p = fmaf(p, y, 0.075f);
+        p = fmaf(p, y, 0.16666666666666666f);
+        return fmaf(x * y, p, x);
+    }
+    return asinf(x);
+}
+__device__ inline double asin_pos(double x) { return asin(x); }
+
 // T: cubature arithmetic type; EC: energies per block (multiple of the
-// vector width).  Dynamic LDS: LUT rows [401][EC] of T, then the per-thread
-// node-weight windows [kThreads][kWinStride] of T, then inv_sep [401] of T.
+// vector width).  Dynamic LDS: LUT rows [401][EC] of T, the per-thread
+// node-weight windows [kThreads][kWinStride] (reused for the block
+// reduction), 1/sep [kNDPad], then the rule (b0..b3 vectors, weights).
+//
+// Per pixel the separations of all rule points lie in [c0 - rho, c0 + rho]
+// (c0: chord to the pixel's bilinear centre, rho: largest corner offset
+// from it), so the LUT rows they touch are known before the point loop.
+// Pixels whose points touch at most four consecutive rows accumulate the
+// per-row interpolation weights in four registers (hat functions of the
+// fractional node position); the energy loop then reads four LUT rows.
+// Other pixels (near the source) add each point's two rows directly.
 template <typename T, int EC>
 __global__ void __launch_bounds__(kThreads)
 cubature_kernel(CubArgs a) {
     using V = typename Vec<T>::type;
     constexpr int VN = Vec<T>::n;
     constexpr int NV = EC / VN;
+    using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     T* win_s = lut_s + kND * EC;
     T* isep_s = win_s + kThreads * kWinStride;
+    V4* rb_s = reinterpret_cast<V4*>(isep_s + kNDPad);
+    T* rw_s = reinterpret_cast<T*>(rb_s + a.N);
 
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
@@ -212,7 +247,7 @@ cubature_kernel(CubArgs a) {
     const int t_end = min(ntiles, t_begin + a.tiles_per_block);
     if (t_begin >= ntiles) return;
 
-    // ---- stage this source's PSF rows (energies e0 .. e0+EC) in LDS
+    // ---- stage this source's PSF rows (energies e0 .. e0+EC), 1/sep and the rule
     const T* lut_g = reinterpret_cast<const T*>(a.lut) + (size_t)s * kND * a.Ep + e0;
     for (int k = tid; k < kND * NV; k += kThreads) {
         const int d = k / NV, v = k % NV;
@@ -224,11 +259,23 @@ cubature_kernel(CubArgs a) {
     }
     for (int k = tid; k < kND; k += kThreads)
         isep_s[k] = sizeof(T) == 4 ? (T)a.inv_sep[k] : (T)a.inv_sep64[k];
+    for (int p = tid; p < a.N; p += kThreads) {
+        V4 b;
+        if (sizeof(T) == 4) {
+            const RulePoint32 rp = a.rule32[p];
+            b = V4{(T)rp.b[0], (T)rp.b[1], (T)rp.b[2], (T)rp.b[3]};
+            rw_s[p] = (T)rp.wq;
+        } else {
+            const RulePoint rp = a.rule[p];
+            b = V4{(T)rp.b[0], (T)rp.b[1], (T)rp.b[2], (T)rp.b[3]};
+            rw_s[p] = (T)rp.wq;
+        }
+        rb_s[p] = b;
+    }
     __syncthreads();
 
     const SrcGeom g = a.geo[s];
     const double sx = g.dir[0], sy = g.dir[1], sz = g.dir[2];
-    T* mywin = win_s + tid * kWinStride;
     const int lx = tid % kTileW, ly = tid / kTileW;
     const int Wp1 = a.W + 1;
 
@@ -241,100 +288,95 @@ cubature_kernel(CubArgs a) {
         const int th = t / a.tiles_w, tw = t % a.tiles_w;
         const int h = th * kTileH + ly, w = tw * kTileW + lx;
         const bool valid = h < a.H && w < a.W;
+        // out-of-image lanes compute an in-image pixel and store nothing, so
+        // the point loop runs in uniform control flow
+        const int hc = min(h, a.H - 1), wc = min(w, a.W - 1);
+
+        // corner deltas corner_k - source in FP64, then T (LL LR UR UL)
+        const int c00 = hc * Wp1 + wc;
+        const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};
+        T D[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double* c = a.corners + 3 * cidx[k];
+            D[k][0] = (T)(c[0] - sx);
+            D[k][1] = (T)(c[1] - sy);
+            D[k][2] = (T)(c[2] - sz);
+        }
+        const T cx = T(0.25) * (D[0][0] + D[1][0] + D[2][0] + D[3][0]);
+        const T cy = T(0.25) * (D[0][1] + D[1][1] + D[2][1] + D[3][1]);
+        const T cz = T(0.25) * (D[0][2] + D[1][2] + D[2][2] + D[3][2]);
+        const T c0 = fast_sqrt(cx * cx + cy * cy + cz * cz);
+        T rho = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const T ex = D[k][0] - cx, ey = D[k][1] - cy, ez = D[k][2] - cz;
+            rho = fmax(rho, fast_sqrt(ex * ex + ey * ey + ez * ez));
+        }
+        rho = rho * (T)a.rho_scale * T(1.0001) + T(1e-6) * c0;
+        const T slo = T(2.0 * kR2D) * asin_pos(T(0.5) * fmax(c0 - rho, T(0)));
+        const T shi = T(2.0 * kR2D) * asin_pos(fmin(T(0.5) * (c0 + rho), T(1)));
+        const int ilo = sep_index_fast(slo), ihi = sep_index_fast(shi);
+        // rows touched: ilo-1 .. ihi; inside the table with t in [0, 1]
+        const bool fast = (ihi - ilo) <= 2 && slo >= T(1e-4) && shi < T(69.9);
+        const int r0 = min(max(ilo - 1, 0), kND - 4);
+
         V acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = V{};
-        bool inrad = false;
-        T om = T(1);
-        if (valid) {
-            // corner deltas corner_k - source in FP64, then T
-            const int c00 = h * Wp1 + w;
-            const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};  // LL LR UR UL
-            T D[4][3];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const double* c = a.corners + 3 * cidx[k];
-                D[k][0] = (T)(c[0] - sx);
-                D[k][1] = (T)(c[1] - sy);
-                D[k][2] = (T)(c[2] - sz);
-            }
-            // bound of the point separations: |p - src| in [c0 - rho, c0 + rho]
-            T cx = T(0.25) * (D[0][0] + D[1][0] + D[2][0] + D[3][0]);
-            T cy = T(0.25) * (D[0][1] + D[1][1] + D[2][1] + D[3][1]);
-            T cz = T(0.25) * (D[0][2] + D[1][2] + D[2][2] + D[3][2]);
-            const T c0 = fast_sqrt(cx * cx + cy * cy + cz * cz);
-            T rho = T(0);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T ex = D[k][0] - cx, ey = D[k][1] - cy, ez = D[k][2] - cz;
-                rho = fmax(rho, fast_sqrt(ex * ex + ey * ey + ez * ez));
-            }
-            rho = rho * (T)a.rho_scale * T(1.0001) + T(1e-7) * c0;
-            const T slo = T(2.0 * kR2D) * asin(T(0.5) * fmax(c0 - rho, T(0)));
-            const T shi = T(2.0 * kR2D) * asin(T(0.5) * fmin(c0 + rho, T(2)));
-            const int ilo = sep_index_fast(slo), ihi = sep_index_fast(shi);
-            const int r0 = max(ilo - 2, 0);
-            const bool fast = (ihi + 1 - r0) < kWin;
+        T W0 = T(0), W1 = T(0), W2 = T(0), W3 = T(0);
+        const T fr0 = (T)(r0 + 1);
 
+#pragma unroll 4
+        for (int p = 0; p < a.N; ++p) {
+            const V4 bv = rb_s[p];
+            const T wq = rw_s[p];
+            const T dx = bv.x * D[0][0] + bv.y * D[1][0] + bv.z * D[2][0] + bv.w * D[3][0];
+            const T dy = bv.x * D[0][1] + bv.y * D[1][1] + bv.z * D[2][1] + bv.w * D[3][1];
+            const T dz = bv.x * D[0][2] + bv.y * D[1][2] + bv.z * D[2][2] + bv.w * D[3][2];
+            const T ch = fast_sqrt(dx * dx + dy * dy + dz * dz);
+            const T sd = T(2.0 * kR2D) * asin_pos(fmin(T(0.5) * ch, T(1)));
+            const int i = sep_index_fast(sd);
+            // t = (s - sep[i-1]) / (sep[i] - sep[i-1]) = redm1 (s / sep[i-1] - 1)
+            const T tt = T(kRedm1) * (sd * isep_s[i - 1] - T(1));
             if (fast) {
-#pragma unroll
-                for (int k = 0; k < kWin; ++k) mywin[k] = T(0);
-            }
-            for (int p = 0; p < a.N; ++p) {
-                T b0, b1, b2, b3, wq;
-                if (sizeof(T) == 4) {
-                    const RulePoint32 rp = a.rule32[p];
-                    b0 = (T)rp.b[0]; b1 = (T)rp.b[1]; b2 = (T)rp.b[2]; b3 = (T)rp.b[3];
-                    wq = (T)rp.wq;
-                } else {
-                    const RulePoint rp = a.rule[p];
-                    b0 = (T)rp.b[0]; b1 = (T)rp.b[1]; b2 = (T)rp.b[2]; b3 = (T)rp.b[3];
-                    wq = (T)rp.wq;
-                }
-                const T dx = b0 * D[0][0] + b1 * D[1][0] + b2 * D[2][0] + b3 * D[3][0];
-                const T dy = b0 * D[0][1] + b1 * D[1][1] + b2 * D[2][1] + b3 * D[3][1];
-                const T dz = b0 * D[0][2] + b1 * D[1][2] + b2 * D[2][2] + b3 * D[3][2];
-                const T ch = fast_sqrt(dx * dx + dy * dy + dz * dz);
-                const T sd = T(2.0 * kR2D) * asin(fmin(T(0.5) * ch, T(1)));
-                const int i = sep_index_fast(sd);
-                // t = (s - sep[i-1]) / (sep[i] - sep[i-1]) = redm1 (s / sep[i-1] - 1)
-                const T tt = T(kRedm1) * (sd * isep_s[i - 1] - T(1));
+                // fractional node position relative to row r0
+                const T xx = ((T)i - fr0) + tt;
+                W0 += wq * fmax(T(0), T(1) - fabs(xx));
+                W1 += wq * fmax(T(0), T(1) - fabs(xx - T(1)));
+                W2 += wq * fmax(T(0), T(1) - fabs(xx - T(2)));
+                W3 += wq * fmax(T(0), T(1) - fabs(xx - T(3)));
+            } else {
                 const T wa = wq * (T(1) - tt), wb = wq * tt;
-                if (fast) {
-                    int rel = i - 1 - r0;
-                    rel = rel < 0 ? 0 : (rel > kWin - 2 ? kWin - 2 : rel);
-                    atomicAdd(&mywin[rel], wa);
-                    atomicAdd(&mywin[rel + 1], wb);
-                } else {
-                    const V* L0 = reinterpret_cast<const V*>(lut_s) + (i - 1) * NV;
-                    const V* L1 = L0 + NV;
+                const V* L0 = reinterpret_cast<const V*>(lut_s) + (i - 1) * NV;
+                const V* L1 = L0 + NV;
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        vfma(acc[v], wa, L0[v]);
-                        vfma(acc[v], wb, L1[v]);
-                    }
+                for (int v = 0; v < NV; ++v) {
+                    vfma(acc[v], wa, L0[v]);
+                    vfma(acc[v], wb, L1[v]);
                 }
             }
-            if (fast) {
-                const int nrow = min(kWin, kND - r0);
-                for (int k = 0; k < nrow; ++k) {
-                    const T wk = mywin[k];
-                    if (wk != T(0)) {
-                        const V* L = reinterpret_cast<const V*>(lut_s) + (r0 + k) * NV;
+        }
+        if (fast) {
+            const V* L = reinterpret_cast<const V*>(lut_s) + r0 * NV;
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) vfma(acc[v], wk, L[v]);
-                    }
-                }
+            for (int v = 0; v < NV; ++v) {
+                vfma(acc[v], W0, L[v]);
+                vfma(acc[v], W1, L[NV + v]);
+                vfma(acc[v], W2, L[2 * NV + v]);
+                vfma(acc[v], W3, L[3 * NV + v]);
             }
-            if (a.omega) om = (T)a.omega[h * a.W + w];
-            if (a.mapint && g.in_fov) {
-                const double* c = a.centres + 3 * (h * Wp1 + w);
-                const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
-                const double chd = sqrt(ex * ex + ey * ey + ez * ez);
-                inrad = chd < g.chord_lo ? true
-                      : (chd > g.chord_hi ? false
-                                          : (2.0 * asin(0.5 * chd) <= kD2R * g.radius_deg));
-            }
+        }
+        T om = T(1);
+        if (a.omega) om = (T)a.omega[hc * a.W + wc];
+        bool inrad = false;
+        if (a.mapint && g.in_fov && valid) {
+            const double* c = a.centres + 3 * (hc * Wp1 + wc);
+            const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
+            const double chd = sqrt(ex * ex + ey * ey + ez * ez);
+            inrad = chd < g.chord_lo ? true
+                  : (chd > g.chord_hi ? false
+                                      : (2.0 * asin(0.5 * chd) <= kD2R * g.radius_deg));
         }
         // ---- epilogue: x solid angle, store, in-radius partial sums
         if (valid) {
@@ -440,6 +482,44 @@ __global__ void scale_kernel(OT* __restrict__ out, int S, int E, size_t plane, i
             out[k] = (OT)v;
         } else {
             out[k] = (OT)((double)out[k] * (expo[se] * cf[se]));
+        }
+    }
+}
+
+// [S, E, H, W]: blockIdx.y is the (s, e) plane, factor read once per block
+template <typename OT>
+__global__ void scale_planes_kernel(OT* __restrict__ out, size_t plane,
+                                    const double* __restrict__ expo, const double* __restrict__ cf) {
+    const size_t se = blockIdx.y;
+    OT* p = out + se * plane;
+    if (sizeof(OT) == 8) {
+        const double f0 = expo[se], f1 = cf[se];
+        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < plane;
+             k += (size_t)gridDim.x * blockDim.x) {
+            double v = (double)p[k];
+            v *= f0;
+            v *= f1;
+            p[k] = (OT)v;
+        }
+    } else {
+        const float f = (float)(expo[se] * cf[se]);
+        const size_t n4 = plane / 4;
+        float4* p4 = reinterpret_cast<float4*>(p);
+        const bool aligned = (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+        if (aligned) {
+            for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4;
+                 k += (size_t)gridDim.x * blockDim.x) {
+                float4 v = p4[k];
+                v.x *= f; v.y *= f; v.z *= f; v.w *= f;
+                p4[k] = v;
+            }
+            for (size_t k = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < plane;
+                 k += (size_t)gridDim.x * blockDim.x)
+                p[k] = (OT)((float)p[k] * f);
+        } else {
+            for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < plane;
+                 k += (size_t)gridDim.x * blockDim.x)
+                p[k] = (OT)((float)p[k] * f);
         }
     }
 }
@@ -589,7 +669,7 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RulePoint** 
                        RulePoint32** dev32, double* rho_scale) {
     const double* r = rule ? rule : kSquareDeg7;
     if (!rule) N = 12;
-    XTS_CHECK(N > 0 && N <= 4096, "rule size %lld out of range", (long long)N);
+    XTS_CHECK(N > 0 && N <= 1024, "rule size %lld out of range (1..1024)", (long long)N);
     std::vector<RulePoint> rp((size_t)N);
     double rs = 1.0;
     for (int64_t p = 0; p < N; ++p) {
@@ -695,12 +775,12 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     dim3 grid((ntiles + ca.tiles_per_block - 1) / ca.tiles_per_block, (unsigned)S, nchunks);
     TimedRegion tr_cub(ctx, "cubature");
     if (f64) {
-        const size_t sm = (size_t)(kND * 16 + kThreads * kWinStride + kND) * 8;
+        const size_t sm = (size_t)(kND * 16 + kThreads * kWinStride + kNDPad) * 8 + (size_t)N * 40;
         XTS_HIP(hipFuncSetAttribute((const void*)cubature_kernel<double, 16>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
         cubature_kernel<double, 16><<<grid, kThreads, sm, ctx->stream>>>(ca);
     } else {
-        const size_t sm = (size_t)(kND * 40 + kThreads * kWinStride + kND) * 4;
+        const size_t sm = (size_t)(kND * 40 + kThreads * kWinStride + kNDPad) * 4 + (size_t)N * 20;
         XTS_HIP(hipFuncSetAttribute((const void*)cubature_kernel<float, 40>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
         cubature_kernel<float, 40><<<grid, kThreads, sm, ctx->stream>>>(ca);
@@ -715,12 +795,21 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         const size_t n = (size_t)S * E * plane;
         const unsigned blocks = std::min<unsigned>(div_up((int64_t)n, 256), 65536);
         TimedRegion tr_scale(ctx, "scale");
-        if (of32)
+        if (sehw && S * E <= 65535) {
+            const unsigned bx = std::max(1u, std::min<unsigned>(div_up((int64_t)plane, 1024), 64));
+            if (of32)
+                scale_planes_kernel<float><<<dim3(bx, (unsigned)(S * E)), 256, 0, ctx->stream>>>(
+                    (float*)out, plane, expo, fac);
+            else
+                scale_planes_kernel<double><<<dim3(bx, (unsigned)(S * E)), 256, 0, ctx->stream>>>(
+                    (double*)out, plane, expo, fac);
+        } else if (of32) {
             scale_kernel<float><<<blocks, 256, 0, ctx->stream>>>((float*)out, (int)S, (int)E, plane,
                                                                  sehw, expo, fac);
-        else
+        } else {
             scale_kernel<double><<<blocks, 256, 0, ctx->stream>>>((double*)out, (int)S, (int)E,
                                                                   plane, sehw, expo, fac);
+        }
         XTS_HIP(hipGetLastError());
         tr_scale.stop();
     }

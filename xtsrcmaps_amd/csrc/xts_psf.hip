@@ -193,40 +193,63 @@ __global__ void pack_pa_kernel(int C, int N, const double* __restrict__ a,
     }
 }
 
-// psf.partial_total_integral.cxx:8-52 and psf.normalize.cxx:7-24 in FP64:
-// one thread per (source, energy) scans the 400 segments, then normalises
-// the cumulative integral and the PSF by the total.
-__global__ void integral_kernel(int S, int E, double* __restrict__ upsf,
-                                double* __restrict__ parint, double* __restrict__ total,
-                                const double* __restrict__ sep) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= S * E) return;
-    const int s = k / E, e = k % E;
+// psf.partial_total_integral.cxx:8-52 and psf.normalize.cxx:7-24 in FP64.
+// Each segment d (nodes d-1, d) contributes the exact integral of the
+// linear interpolant of y = 2 pi sin(theta) psf; the cumulative sum is an
+// inclusive prefix scan.  One wave per (source, energy): 64 segments per
+// step, a shuffle scan inside the step and a running carry between steps.
+// tab[d] = theta_d (rad), tab[401 + d] = sin theta_d.
+__global__ void __launch_bounds__(256)
+integral_kernel(int S, int E, double* __restrict__ upsf, double* __restrict__ parint,
+                double* __restrict__ total, const double* __restrict__ tab) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int e = blockIdx.x * 4 + wave, s = blockIdx.y;
+    if (e >= E || s >= S) return;
     double* u = upsf + (size_t)s * kND * E + e;
     double* pi = parint + (size_t)s * kND * E + e;
-    double cum = 0.0;
-    pi[0] = 0.0;
-    double t1 = 0.0, y1 = kTwoPi * u[0] * sin(t1);
-    for (int d = 1; d < kND; ++d) {
-        const double t2 = kD2R * sep[d];
-        const double y2 = kTwoPi * u[(size_t)d * E] * sin(t2);
-        const double m = (y2 - y1) / (t2 - t1);
-        const double b = y1 - t1 * m;
-        cum += 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
-        pi[(size_t)d * E] = cum;
-        t1 = t2;
-        y1 = y2;
+    double carry = 0.0;
+    if (lane == 0) pi[0] = 0.0;
+    for (int d0 = 1; d0 < kND; d0 += 64) {
+        const int d = d0 + lane;
+        double v = 0.0;
+        if (d < kND) {
+            const double t1 = tab[d - 1], t2 = tab[d];
+            const double y1 = kTwoPi * u[(size_t)(d - 1) * E] * tab[kND + d - 1];
+            const double y2 = kTwoPi * u[(size_t)d * E] * tab[kND + d];
+            const double m = (y2 - y1) / (t2 - t1);
+            const double b = y1 - t1 * m;
+            v = 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
+        }
+        // inclusive scan across the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const double w = __shfl_up(v, o, 64);
+            if (lane >= o) v += w;
+        }
+        if (d < kND) pi[(size_t)d * E] = carry + v;
+        carry += __shfl(v, 63, 64);
     }
-    if (total) total[k] = cum;
-    if (cum != 0.0)
-        for (int d = 0; d < kND; ++d) pi[(size_t)d * E] /= cum;
-    for (int d = 0; d < kND; ++d) u[(size_t)d * E] /= cum;
+    const double cum = carry;
+    if (total && lane == 0) total[(size_t)s * E + e] = cum;
+    for (int d = lane; d < kND; d += 64) {
+        if (cum != 0.0 && d > 0) pi[(size_t)d * E] /= cum;
+        u[(size_t)d * E] /= cum;
+    }
 }
 
 static void separations_host(double* sep) {
     const double xstep = std::log(70.0 / 1e-4) / 399.0;
     sep[0] = 0.0;
     for (int i = 0; i < 400; ++i) sep[i + 1] = 1e-4 * std::exp(i * xstep);
+}
+
+// [theta_d (rad), sin theta_d] for the radial integral
+static void theta_table(double* tab) {
+    double sep[kND];
+    separations_host(sep);
+    for (int d = 0; d < kND; ++d) {
+        tab[d] = kD2R * sep[d];
+        tab[kND + d] = std::sin(tab[d]);
+    }
 }
 
 // Gauss-Legendre nodes and weights by Newton iteration on P_n.
@@ -306,8 +329,12 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
         mean_psf_combine_kernel<<<std::min<unsigned>(div_up((int64_t)S * N, 256), 16384), 256, 0,
                                   ctx->stream>>>(S, E, P, ltf, expo, upsf);
     }
-    integral_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>(S, E, upsf, parint, nullptr,
-                                                                  (double*)dsep);
+    double tab[2 * kND];
+    theta_table(tab);
+    void* dtab;
+    XTS_TRY(to_device(ctx, kSlotTab, tab, sizeof(tab), &dtab));
+    integral_kernel<<<dim3(div_up(E, 4), S), 256, 0, ctx->stream>>>(S, E, upsf, parint, nullptr,
+                                                                     (double*)dtab);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -452,8 +479,13 @@ int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S, int64_t E,
     XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8 + S * E * 8, &dout));
     double* dpar = (double*)dout;
     double* dtot = dpar + n;
-    integral_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>((int)S, (int)E, (double*)du, dpar,
-                                                                  dtot, (double*)dsep);
+    double tab[2 * kND];
+    theta_table(tab);
+    void* dtab;
+    XTS_TRY(to_device(ctx, kSlotTab, tab, sizeof(tab), &dtab));
+    (void)dsep;
+    integral_kernel<<<dim3(div_up(E, 4), (unsigned)S), 256, 0, ctx->stream>>>(
+        (int)S, (int)E, (double*)du, dpar, dtot, (double*)dtab);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(upsf, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipMemcpyAsync(parint, dpar, n * 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -1,0 +1,43 @@
+// (c) Point-source model map (reference: model_map/model_map.hxx:11-74).
+// Maps are [S, H, W, E] FP64 host tensors as in the reference.
+#pragma once
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "xtsrcmaps/exposure/exposure.hxx"
+#include "xtsrcmaps/observation/obs_types.hxx"
+#include "xtsrcmaps/psf/psf.hxx"
+#include "xtsrcmaps/sky_geom/sky_geom.hxx"
+#include "xtsrcmaps/source/source.hxx"
+#include "xtsrcmaps/tensor/tensor.hpp"
+
+namespace Fermi::ModelMap {
+
+template <Source::SourceConcept T>
+auto compute_srcmaps(Obs::XtObs const& obs, Source::SourceData<T> src,
+                     Exposure::XtExp const& exp, Psf::XtPsf const& psf) -> Tensor<double, 4>;
+
+auto point_src_model_map_wcs(size_t Nh, size_t Nw, SkyGeom<double> const& skygeom,
+                             Tensor<double, 2> const& src_sph,
+                             std::vector<std::string> const& src_names,
+                             Tensor<double, 2> const& exposures, Tensor<double, 3> const& uPsf,
+                             Tensor<double, 3> const& partial_integrals) -> Tensor<double, 4>;
+
+void scale_map_by_solid_angle(Tensor<double, 4>& model_map, SkyGeom<double> const& skygeom);
+void scale_map_by_exposure(Tensor<double, 4>& model_map, Tensor<double, 2> const& exposures);
+auto map_correction_factor(Tensor<double, 2> const& inv_map_integ,
+                           Tensor<double, 1> const& psf_radius,
+                           std::vector<bool> const& is_in_fov, Tensor<double, 3> const& mean_psf,
+                           Tensor<double, 3> const& partial_integrals) -> Tensor<double, 2>;
+void scale_map_by_correction_factors(Tensor<double, 4>& model_map,
+                                     Tensor<double, 2> const& factor);
+auto psf_boundary_radius(size_t Nh, size_t Nw, Tensor<double, 2> const& src_sph,
+                         SkyGeom<double> const& skygeom)
+    -> std::pair<Tensor<double, 1>, std::vector<bool>>;
+auto map_integral(Tensor<double, 4> const& model_map, Tensor<double, 2> const& src_sph,
+                  SkyGeom<double> const& skygeom, Tensor<double, 1> const& psf_radius,
+                  std::vector<bool> const& is_in_fov) -> Tensor<double, 2>;
+
+}  // namespace Fermi::ModelMap
