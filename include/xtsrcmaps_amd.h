@@ -91,10 +91,15 @@ int xts_ctx_synchronize(xts_ctx* ctx);
 const char* xts_last_error(void);
 int xts_abi_version(void);
 /* Instrumentation (no reference counterpart): when enabled, kernel launches
- * of the named stages ("cubature", "scale") are bracketed by events on the
+ * of the named model-map stages ("cubature": the row-weight pass, "near",
+ * "factor", "emit") are bracketed by events on the
  * context stream; xts_ctx_kernel_times synchronises and returns (and
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
+/* Launch-shape override for the model map (no reference counterpart):
+ * "tiles_per_block" (32x16-pixel tiles per block of the row-weight pass);
+ * 0 restores the default choice.  Results do not depend on it. */
+int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value);
 int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms,
                          int64_t* count);
 

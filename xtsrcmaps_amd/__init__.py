@@ -89,6 +89,10 @@ class Context:
     def synchronize(self):
         capi.check(capi.lib().xts_ctx_synchronize(self._h))
 
+    def set_tuning(self, key: str, value: int):
+        """Launch-shape override ("tiles_per_block"; 0 = default)."""
+        capi.check(capi.lib().xts_ctx_set_tuning(self._h, key.encode(), int(value)))
+
     def enable_timing(self, on: bool = True):
         capi.check(capi.lib().xts_ctx_enable_timing(self._h, int(bool(on))))
 

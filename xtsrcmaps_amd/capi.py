@@ -63,6 +63,7 @@ _SIGS = {
     "xts_ctx_set_stream": [_vp, _vp],
     "xts_ctx_synchronize": [_vp],
     "xts_ctx_enable_timing": [_vp, C.c_int],
+    "xts_ctx_set_tuning": [_vp, C.c_char_p, _i64],
     "xts_ctx_kernel_times": [_vp, C.c_char_p, _d, C.POINTER(_i64)],
     "xts_abi_version": [],
     "xts_costhetas": [_i64, C.c_double, C.c_int32, _d],

@@ -190,6 +190,14 @@ int xts_ctx_set_stream(xts_ctx* ctx, void* stream) {
     return XTS_OK;
 }
 
+int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
+    XTS_CHECK(ctx != nullptr && key != nullptr, "null argument");
+    XTS_CHECK(value >= 0 && value <= 1 << 20, "tuning value %lld out of range", (long long)value);
+    if (strcmp(key, "tiles_per_block") == 0) ctx->tune_tiles_per_block = (int)value;
+    else return fail(XTS_ERR_INVALID, "unknown tuning key '%s'", key);
+    return XTS_OK;
+}
+
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable) {
     XTS_CHECK(ctx != nullptr, "null context");
     ctx->timing = enable ? 1 : 0;

@@ -67,7 +67,7 @@ enum SlotId {
     kSlotIn0 = 0, kSlotIn1, kSlotIn2, kSlotIn3, kSlotIn4, kSlotIn5,
     kSlotIn6, kSlotIn7, kSlotOut0, kSlotOut1, kSlotOut2, kSlotOut3,
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
-    kSlotWork3, kSlotTab, kSlotCount
+    kSlotWork3, kSlotTab, kSlotRows, kSlotCount
 };
 
 }  // namespace xts
@@ -84,6 +84,7 @@ struct xts_ctx {
     xts::Slot slots[xts::kSlotCount];
     int cu_count = 0;
     int timing = 0;                        // record events around kernels
+    int tune_tiles_per_block = 0;          // 0: chosen per call (row-weight pass)
     std::vector<xts_timing_rec> timing_recs;
     std::vector<hipEvent_t> event_pool;
 };

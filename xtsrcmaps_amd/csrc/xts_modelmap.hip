@@ -4,12 +4,15 @@
 //   1. geometry_kernel   FP64 pixel-corner and pixel-centre unit vectors of the
 //                        WCS grid and the per-pixel solid angle (once per image)
 //   2. radius_kernel     in-FOV flag and boundary radius per source
-//   3. cubature_kernel   per (source, pixel tile, energy chunk): PSF table rows
-//                        staged in LDS, cubature of the table over each pixel,
-//                        x solid angle, stores, and the in-radius partial sums
-//                        of the map integral
-//   4. factor_kernel     correction factor x exposure per (source, energy)
-//   5. scale_kernel      in-place multiply of every map value by its factor
+//   3. rowweight_kernel  per (source, pixel): the cubature reduced to four
+//                        PSF-table row weights (x solid angle) and the
+//                        in-radius row histogram of the map integral
+//   4. near_pixel_kernel cubature with direct table reads for the pixels
+//                        whose rule points span more rows (near the source)
+//   5. hist_integral_kernel, factor_kernel
+//                        map integral, correction factor per (source, energy)
+//   6. emit_kernel       every map value once: row weights . table rows
+//                        x exposure x correction
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -115,22 +118,29 @@ __global__ void radius_kernel(Wcs w, int H, int W, int S, const double* __restri
 }
 
 // ------------------------------------------------------------------ cubature
-// Rule point constants: bilinear corner weights b0..b3 of
-// points_weights.hpp:37-51 (corner order LL, LR, UR, UL) and the weight w/4.
-struct RulePoint {
-    double b[4];
-    double wq;
-};
-struct RulePoint32 {
-    float b[4];
-    float wq;
-    float pad[3];
-};
-
-constexpr int kTileW = 16, kTileH = 16;   // pixels per block pass (256 threads)
+// The cubature of the reference (mm_cubature/convolve_pixel_psf_logsep.hpp:
+// 20-80) interpolates the PSF table linearly in separation at every rule
+// point of a pixel: map[e] = sum_p w_p ((1 - t_p) L[i_p - 1][e] + t_p L[i_p][e]).
+// Regrouped by table row this is map[e] = sum_r W_r L[r][e]; every point of a
+// pixel away from the source lands on one of four consecutive rows, so the
+// pixel is described by four row weights and a first row.  Three passes:
+//   rowweight_kernel  point loop per (source, pixel): four row weights x
+//                     solid angle, the in-radius row histogram of the map
+//                     integral, and a list of the pixels whose points span
+//                     more rows (near the source, or outside the table)
+//   near_pixel_kernel point loop with direct table reads for listed pixels,
+//                     their map values and in-radius map-integral sums
+//   emit_kernel       per (source, pixel, energy): the 4-row dot product x
+//                     exposure x correction factor, stored once
+constexpr int kTileW = 32, kTileH = 16;   // pixels per block pass (512 threads)
 constexpr int kThreads = kTileW * kTileH;
-constexpr int kWin = 8;                   // node-weight window per pixel
-constexpr int kWinStride = kWin + 1;      // odd stride: conflict-free lanes
+constexpr int kNearThreads = 256;
+constexpr int kEmitThreads = 512;
+
+// rule point: position (u, v) on [-1, 1]^2 and weight w/4
+struct RuleUVW {
+    double u, v, wq;
+};
 
 struct CubArgs {
     const double* corners;
@@ -138,33 +148,36 @@ struct CubArgs {
     const double* omega;       // may be null (raw cubature)
     const SrcGeom* geo;
     const void* lut;           // [S][401][Ep] of T
-    const RulePoint* rule;
-    const RulePoint32* rule32;
-    const float* inv_sep;      // [401] 1 / sep[i] (deg), FP32 table
+    const RuleUVW* uvw;
+    const float* inv_sep;      // [401] 2 R2D / sep[i] (sep in deg), FP32 table
     const double* inv_sep64;   // [401] FP64 table
-    double* mapint;            // [S][E] or null
+    void* rw;                  // [S][H W] row weights (4 of T) x solid angle
+    int* rw_row;               // [S][H W] first row, -1 for a listed (near) pixel
+    double* hist;              // [S][401] in-radius row-weight sums, or null
+    double* mapint;            // [S][E] in-radius map sums, or null
+    int* near_count;           // [S]
+    int* near_list;            // [S][H W] listed pixel indices
+    const double* expo;        // [S][E] or null
+    const double* cf;          // [S][E] or null
     void* out;
     int S, H, W, E, Ep, N;
     int tiles_w, tiles_h, tiles_per_block;
     int layout_sehw, out_f32;
+    int tensor_nu, tensor_nv;  // rule is an nu x nv grid (p = i nv + j), else 0
     double rho_scale;          // max over points of sum_k |b_k|
+    double wsum;               // sum over points of w/4
+    double j0, jk;             // j = j0 + jk log2(asin(chord / 2))
 };
-
-// upper node index of the reference's logsep lookup
-// (convolve_pixel_psf_logsep.hpp:46-53): 2 below 1e-4 deg, clamped to 400.
-template <typename T>
-__device__ inline int sep_index(T s) {
-    if (s < T(1e-4)) return 2;
-    const T j = T(1) + T(kRecipStep) * (T(kLn1e4) + log(s));
-    int i = 1 + (int)j;
-    return i > 400 ? 400 : i;
-}
 
 __device__ inline float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994530942f; }
 __device__ inline double fast_log(double x) { return log(x); }
 __device__ inline float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double fast_sqrt(double x) { return sqrt(x); }
+__device__ inline float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ inline double fast_log2(double x) { return log2(x); }
 
+// upper node index of the reference's logsep lookup
+// (convolve_pixel_psf_logsep.hpp:46-53): 2 below 1e-4 deg, clamped to 400.
 template <typename T>
 __device__ inline int sep_index_fast(T s) {
     if (s < T(1e-4)) return 2;
@@ -180,6 +193,20 @@ template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<double> { using type = double4; };
 constexpr int kNDPad = 404;   // 401 table entries, padded to keep 16-byte alignment
+
+// Streaming (non-temporal) stores for the map: the map is written once and
+// not re-read by the pass, and stores that bypass L2 allocation overlap far
+// better with the LDS reads of the emit loop.
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef double d4v_t __attribute__((ext_vector_type(4)));
+__device__ inline void stream_store(float* p, float x) { __builtin_nontemporal_store(x, p); }
+__device__ inline void stream_store(double* p, double x) { __builtin_nontemporal_store(x, p); }
+__device__ inline void stream_store4(float* p, float a, float b, float c, float d) {
+    __builtin_nontemporal_store(f4v_t{a, b, c, d}, reinterpret_cast<f4v_t*>(p));
+}
+__device__ inline void stream_store4(double* p, double a, double b, double c, double d) {
+    __builtin_nontemporal_store(d4v_t{a, b, c, d}, reinterpret_cast<d4v_t*>(p));
+}
 
 template <typename V, typename T>
 __device__ inline void vfma(V& acc, T w, const V& x);
@@ -212,217 +239,783 @@ __device__ inline float asin_pos(float x) {
 }
 __device__ inline double asin_pos(double x) { return asin(x); }
 
-// T: cubature arithmetic type; EC: energies per block (multiple of the
-// vector width).  Dynamic LDS: LUT rows [401][EC] of T, the per-thread
-// node-weight windows [kThreads][kWinStride] (reused for the block
-// reduction), 1/sep [kNDPad], then the rule (b0..b3 vectors, weights).
-//
-// Per pixel the separations of all rule points lie in [c0 - rho, c0 + rho]
-// (c0: chord to the pixel's bilinear centre, rho: largest corner offset
-// from it), so the LUT rows they touch are known before the point loop.
-// Pixels whose points touch at most four consecutive rows accumulate the
-// per-row interpolation weights in four registers (hat functions of the
-// fractional node position); the energy loop then reads four LUT rows.
-// Other pixels (near the source) add each point's two rows directly.
-template <typename T, int EC>
-__global__ void __launch_bounds__(kThreads)
-cubature_kernel(CubArgs a) {
+// One far pixel's cubature point: half-chord x = |p - src| / 2 (the blend
+// coefficients are pre-halved), a = asin(x), the lower table row
+// jint = floor(j0 + jk log2 a) (the reference's upper index minus one) and
+// t = redm1 (s / sep - 1) folded into wt = w redm1 (a isep[jint] - 1).
+// Rows r0 .. r0+3 receive sums over the points of each relative row:
+// S_r = sum w, T_r = sum w t.  NP: terms of the asin series (4 below 14
+// degrees, 6 below 28 degrees).
+template <typename T, int NP>
+__device__ inline void far_point(T dx, T dy, T dz, T wq, T wqR, const T* __restrict__ isep_s,
+                                 T j0, T jk, int r0, T& S0, T& T0, T& S1, T& T1, T& Tt) {
+    const T y = dx * dx + dy * dy + dz * dz;
+    const T x = fast_sqrt(y);
+    T q;
+    if (NP == 6) {
+        q = T(0.017352764423076924);
+        q = q * y + T(0.022372159090909092);
+        q = q * y + T(0.030381944444444444);
+    } else {
+        q = T(0.030381944444444444);
+    }
+    q = q * y + T(0.044642857142857144);
+    q = q * y + T(0.075);
+    q = q * y + T(0.16666666666666666);
+    const T a = sizeof(T) == 8 ? (T)asin((double)x) : x + (x * y) * q;
+    const int jint = (int)(j0 + jk * fast_log2(a));
+    const int rel = min(max(jint - r0, 0), 2);
+    const T wt = wqR * (a * isep_s[jint] - T(1));
+    const bool m0 = rel == 0, m1 = rel == 1;
+    S0 += m0 ? wq : T(0);
+    T0 += m0 ? wt : T(0);
+    S1 += m1 ? wq : T(0);
+    T1 += m1 ? wt : T(0);
+    Tt += wt;
+}
+
+// Per-pixel set-up: corner offsets from the source (FP64, then T), half
+// the bilinear blend p(u, v) / 2 = A + u B + v (C + u E) of the reference's
+// point positions (points_weights.hpp:37-51), and bounds slo, shi (deg) on
+// the separation of every point of the pixel.
+template <typename T>
+struct PixelGeom {
+    T A[3], B[3], C[3], E[3];
+    T slo, shi;
+};
+
+template <typename T>
+__device__ inline PixelGeom<T> pixel_geom(const double* __restrict__ corners, int hc, int wc,
+                                          int Wp1, const double* src, double rho_scale) {
+    PixelGeom<T> g;
+    const int c00 = hc * Wp1 + wc;
+    const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};  // LL LR UR UL
+    T D[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double* c = corners + 3 * cidx[k];
+        D[k][0] = (T)(c[0] - src[0]);
+        D[k][1] = (T)(c[1] - src[1]);
+        D[k][2] = (T)(c[2] - src[2]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        g.A[c] = T(0.125) * ((D[0][c] + D[1][c]) + (D[2][c] + D[3][c]));
+        g.B[c] = T(0.125) * ((D[1][c] - D[0][c]) + (D[2][c] - D[3][c]));
+        g.C[c] = T(0.125) * ((D[3][c] - D[0][c]) + (D[2][c] - D[1][c]));
+        g.E[c] = T(0.125) * ((D[0][c] - D[1][c]) + (D[2][c] - D[3][c]));
+    }
+    // half-chord bounds: |p - src| / 2 in [x0 - hr, x0 + hr]
+    const T x0 = fast_sqrt(g.A[0] * g.A[0] + g.A[1] * g.A[1] + g.A[2] * g.A[2]);
+    T hr = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const T ex = T(0.5) * D[k][0] - g.A[0], ey = T(0.5) * D[k][1] - g.A[1],
+                ez = T(0.5) * D[k][2] - g.A[2];
+        hr = fmax(hr, fast_sqrt(ex * ex + ey * ey + ez * ez));
+    }
+    hr = hr * (T)rho_scale * T(1.0001) + T(1e-5) * x0;
+    g.slo = T(2.0 * kR2D) * asin_pos(fmax(x0 - hr, T(0)));
+    g.shi = T(2.0 * kR2D) * asin_pos(fmin(x0 + hr, T(1)));
+    return g;
+}
+
+// Whether the pixel centre lies within the boundary radius
+// (map_integral.cxx:29-36), decided on the chord with an exact check near
+// the threshold.
+__device__ inline bool in_radius(const SrcGeom& g, const double* __restrict__ centres, int idx) {
+    const double* c = centres + 3 * idx;
+    const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
+    const double chd = sqrt(ex * ex + ey * ey + ez * ez);
+    return chd < g.chord_lo ? true
+         : (chd > g.chord_hi ? false : (2.0 * asin(0.5 * chd) <= kD2R * g.radius_deg));
+}
+
+// Flat index of map value (s, pixel, e) in either layout.
+__device__ inline size_t out_index(int layout_sehw, int s, int pix, int e, int E, size_t plane) {
+    return layout_sehw ? ((size_t)s * E + e) * plane + pix : ((size_t)s * plane + pix) * E + e;
+}
+
+__device__ inline void store_value(const CubArgs& a, size_t k, double x) {
+    if (a.out_f32) reinterpret_cast<float*>(a.out)[k] = (float)x;
+    else reinterpret_cast<double*>(a.out)[k] = x;
+}
+
+// Scale factors of one energy chunk: x expo[s, e] then x correction[s, e],
+// the order in which the reference scales the map
+// (model_map.point_src_model_map_wcs.cxx:30-41); FP32 output takes one FP32
+// multiply by the rounded product.
+template <int EC>
+struct ChunkScale {
+    double ex[EC], cf[EC];
+    float f32[EC];
+    // threads 0 .. EC-1 load one energy each (both loads in flight together)
+    __device__ void load(const CubArgs& a, int se0, int ne, int tid) {
+        if (tid < EC) {
+            const int k = se0 + min(tid, ne - 1);
+            double x = a.expo[k], c = a.cf[k];
+            if (tid >= ne) x = c = 1.0;
+            ex[tid] = x;
+            cf[tid] = c;
+            f32[tid] = (float)(x * c);
+        }
+    }
+    template <typename OT, typename T>
+    __device__ OT apply(T x, int e) const {
+        if constexpr (sizeof(OT) == 4) {
+            return (float)x * f32[e];
+        } else {
+            double y = (double)x * ex[e];
+            return y * cf[e];
+        }
+    }
+};
+
+template <typename T>
+__device__ inline void stage_rule(const CubArgs& a, T* isep_s,
+                                  typename Vec4<T>::type* rule_s, int N4, int tid, int nthr) {
+    using V4 = typename Vec4<T>::type;
+    for (int k = tid; k < kND; k += nthr)
+        isep_s[k] = sizeof(T) == 4 ? (T)a.inv_sep[k] : (T)a.inv_sep64[k];
+    for (int p = tid; p < N4; p += nthr) {
+        V4 r = V4{T(0), T(0), T(0), T(0)};
+        if (p < a.N) {
+            const RuleUVW q = a.uvw[p];
+            r = V4{(T)q.u, (T)q.v, (T)q.wq, (T)(q.wq * kRedm1)};
+        }
+        rule_s[p] = r;
+    }
+}
+
+// PSF rows of source s for energies e0 .. e0+EC into [401][LS] (LS >= EC,
+// zero-padded past the last energy).  All of a thread's loads are issued
+// before the first LDS write so their latencies overlap.
+template <typename T, int EC, int NTHR, int LS = EC>
+__device__ inline void stage_lut(const CubArgs& a, int s, int e0, T* lut_s, int tid) {
     using V = typename Vec<T>::type;
     constexpr int VN = Vec<T>::n;
     constexpr int NV = EC / VN;
+    constexpr int ITEMS = (kND * NV + NTHR - 1) / NTHR;
+    const T* lut_g = reinterpret_cast<const T*>(a.lut) + (size_t)s * kND * a.Ep + e0;
+    V buf[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int k = tid + i * NTHR;
+        const int d = k / NV, v = k % NV;
+        const bool ok = k < kND * NV && e0 + v * VN + VN <= a.Ep;
+        // clamped, always in-bounds address; masked afterwards
+        const int dc = ok ? d : 0, vc = ok ? v : 0;
+        buf[i] = *reinterpret_cast<const V*>(lut_g + (size_t)dc * a.Ep + vc * VN);
+        if (!ok) buf[i] = V{};
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int k = tid + i * NTHR;
+        if (k >= kND * NV) break;
+        const int d = k / NV, v = k % NV;
+        if constexpr (LS == EC) {
+            reinterpret_cast<V*>(lut_s)[d * NV + v] = buf[i];
+        } else {
+#pragma unroll
+            for (int j = 0; j < VN; ++j)
+                lut_s[d * LS + v * VN + j] = reinterpret_cast<const T*>(&buf[i])[j];
+        }
+    }
+}
+
+// ---- FP32 point loop for pixels away from the source.
+// With J(s) = ln(s / 1e-4 deg) / step the reference's lower row is
+// 1 + floor(J) and t = (e^(step frac(J)) - 1) / (e^step - 1)
+// (convolve_pixel_psf_logsep.hpp:46-54).  Per pixel r0 is its first row, so
+// Q = 1 + J - r0 lies in [0, 3): rel = floor(Q), f = frac(Q).  With
+// y = |p / 2|^2 (p: the un-normalised point, so s = 2 asin(sqrt y)) and
+// z = y / y_c for the pixel centre's y_c,
+//   Q = Q_c + K1 log2 z + g(y) / step,   K1 = ln 2 / (2 step),
+//   g(y) = ln(asin(sqrt y) / sqrt y) = y (g1 + y (g2 + y (g3 + y g4))),
+// Q_c = 1 + (ln(2 R2D) + ln 1e4 + ln(y_c) / 2) / step - r0 (per pixel, FP64).
+// The relative log keeps Q accurate to ~1e-6 in FP32.  g is a least-squares
+// fit on separations below 14.5 deg (cubic, max error 5e-11) or 28.5 deg
+// (quartic, 1.2e-10); t(f) = f (t1 + f (t2 + f t3)) (max error 5e-8).
+constexpr double kLnTwoR2D = 4.741374145600756;   // ln(2 R2D)
+constexpr double kGCubic[3] = {0.16666666949401615, 0.06110791481077321, 0.03421911418748551};
+constexpr double kGQuartic[4] = {0.1666666648047744, 0.06111208423547483, 0.03360670491188797,
+                                 0.024068853992171683};
+constexpr double kTPoly[3] = {0.9832291565474904, 0.016581961801669792, 0.00018883180009917163};
+
+// z(u, v) = a(u) + v (b(u) + v c(u)), each of a, b, c quadratic in u, plus
+// the Q offset and the g coefficients scaled to z: g_k y^k = g_k' z^k.
+struct LeanPixel {
+    float a0, a1, a2, b0, b1, b2, c0, c1, c2;
+    float q0;
+    float g[4];
+};
+
+template <int NG>
+__device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
+    const float* A = pg.A; const float* B = pg.B; const float* C = pg.C; const float* E = pg.E;
+    auto dot = [](const float* x, const float* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const float yc = dot(A, A);
+    const float iy = 1.0f / yc;
+    LeanPixel lp;
+    lp.a0 = yc * iy;
+    lp.a1 = 2.0f * dot(A, B) * iy;
+    lp.a2 = dot(B, B) * iy;
+    lp.b0 = 2.0f * dot(A, C) * iy;
+    lp.b1 = 2.0f * (dot(A, E) + dot(B, C)) * iy;
+    lp.b2 = 2.0f * dot(B, E) * iy;
+    lp.c0 = dot(C, C) * iy;
+    lp.c1 = 2.0f * dot(C, E) * iy;
+    lp.c2 = dot(E, E) * iy;
+    // log2 z = log2 y + log2 iy exactly as the kernel forms z = y iy
+    lp.q0 = (float)(1.0 + (kLnTwoR2D + kLn1e4) * kRecipStep -
+                    0.5 * kRecipStep * log((double)iy) - (double)r0);
+    const double* gc = NG == 3 ? kGCubic : kGQuartic;
+    double ycp = 1.0 / (double)iy;
+    const double yk = ycp;
+    for (int k = 0; k < 4; ++k) {
+        lp.g[k] = k < NG ? (float)(gc[k] * kRecipStep * ycp) : 0.0f;
+        ycp *= yk;
+    }
+    return lp;
+}
+
+// One point: wq = w/4; tA, tB, tC = wq (t1, t2, t3).  Accumulates the
+// cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1 likewise
+// of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
+// 2^24 (1 - Q) >= 1 whenever Q < 1.
+template <int NG>
+__device__ inline void lean_point(float z, float wq, float tA, float tB, float tC,
+                                  const LeanPixel& lp, float& C0, float& D0, float& C1,
+                                  float& D1, float& Tt) {
+    const float L = __builtin_amdgcn_logf(z);
+    float g = lp.g[NG - 1];
+#pragma unroll
+    for (int k = NG - 2; k >= 0; --k) g = fmaf(g, z, lp.g[k]);
+    const float Q = fmaf(g, z, fmaf((float)(0.5 * kRecipStep * 0.69314718055994530942), L, lp.q0));
+    const float f = __builtin_amdgcn_fractf(Q);
+    const float wt = f * fmaf(f, fmaf(f, tC, tB), tA);
+    const float m0 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 16777216.0f), 0.0f, 1.0f);
+    const float m1 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 33554432.0f), 0.0f, 1.0f);
+    C0 = fmaf(wq, m0, C0);
+    D0 = fmaf(wt, m0, D0);
+    C1 = fmaf(wq, m1, C1);
+    D1 = fmaf(wt, m1, D1);
+    Tt += wt;
+}
+
+template <int NG>
+__device__ inline void lean_tensor_loop(const CubArgs& a, const float4* __restrict__ rule_s,
+                                        const float4* __restrict__ tco_s,
+                                        const PixelGeom<float>& pg, int r0, float& C0, float& D0,
+                                        float& C1, float& D1, float& Tt) {
+    const LeanPixel lp = lean_pixel<NG>(pg, r0);
+    const int nv = a.tensor_nv;
+    for (int i = 0; i < a.tensor_nu; ++i) {
+        const float u = rule_s[i * nv].x;
+        const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
+        const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
+        const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
+        for (int jj = 0; jj < nv; jj += 4) {
+            float4 r[4], tc[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                r[k] = rule_s[i * nv + jj + k];
+                tc[k] = tco_s[i * nv + jj + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float v = r[k].y;
+                lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), r[k].z, tc[k].x, tc[k].y, tc[k].z,
+                               lp, C0, D0, C1, D1, Tt);
+            }
+        }
+    }
+}
+
+template <int NG>
+__device__ inline void lean_list_loop(int N4, const float4* __restrict__ rule_s,
+                                      const float4* __restrict__ tco_s,
+                                      const PixelGeom<float>& pg, int r0, float& C0, float& D0,
+                                      float& C1, float& D1, float& Tt) {
+    const LeanPixel lp = lean_pixel<NG>(pg, r0);
+    for (int p0 = 0; p0 < N4; p0 += 4) {
+        float4 r[4], tc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            r[k] = rule_s[p0 + k];
+            tc[k] = tco_s[p0 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float u = r[k].x, v = r[k].y;
+            const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
+            const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
+            const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
+            lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), r[k].z, tc[k].x, tc[k].y, tc[k].z, lp,
+                           C0, D0, C1, D1, Tt);
+        }
+    }
+}
+
+// Pass 1.  Grid: (tile blocks, S); LDS: the in-radius row histogram
+// [kNDPad] of double, 2 R2D / sep [kNDPad] of T, the rule (u, v, w/4,
+// redm1 w/4) and, for FP32, the t coefficients (w/4) (t1, t2, t3), both
+// padded to a multiple of 4 points.
+//
+// Every point of a pixel has its separation in [slo, shi], so the rows the
+// pixel touches are known before the point loop.  A pixel whose points touch
+// at most four consecutive rows r0 .. r0+3 accumulates, per relative row,
+// the sums of w and w t in registers; the row weights follow from them.
+// Other pixels are listed for near_pixel_kernel.  When every lane of a wave
+// is such a pixel below 28 degrees, the wave runs a branch-free loop over
+// the points in groups of four (grid-structured rules reuse the blend of
+// each rule row).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+rowweight_kernel(CubArgs a) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lut_s = reinterpret_cast<T*>(smem_raw);
-    T* win_s = lut_s + kND * EC;
-    T* isep_s = win_s + kThreads * kWinStride;
-    V4* rb_s = reinterpret_cast<V4*>(isep_s + kNDPad);
-    T* rw_s = reinterpret_cast<T*>(rb_s + a.N);
+    double* hist_s = reinterpret_cast<double*>(smem_raw);
+    T* isep_s = reinterpret_cast<T*>(hist_s + kNDPad);
+    V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
+    const int N4 = (a.N + 3) & ~3;
+    float4* tco_s = reinterpret_cast<float4*>(rule_s + N4);
 
     const int s = blockIdx.y;
-    const int e0 = blockIdx.z * EC;
-    const int ne = min(EC, a.E - e0);
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_w * a.tiles_h;
     const int t_begin = blockIdx.x * a.tiles_per_block;
     const int t_end = min(ntiles, t_begin + a.tiles_per_block);
     if (t_begin >= ntiles) return;
 
-    // ---- stage this source's PSF rows (energies e0 .. e0+EC), 1/sep and the rule
-    const T* lut_g = reinterpret_cast<const T*>(a.lut) + (size_t)s * kND * a.Ep + e0;
-    for (int k = tid; k < kND * NV; k += kThreads) {
-        const int d = k / NV, v = k % NV;
-        V val;
-        const T* src = lut_g + (size_t)d * a.Ep + v * VN;
-        if (e0 + v * VN + VN <= a.Ep) val = *reinterpret_cast<const V*>(src);
-        else { T tmp[VN] = {}; val = *reinterpret_cast<V*>(tmp); }
-        reinterpret_cast<V*>(lut_s)[d * NV + v] = val;
-    }
-    for (int k = tid; k < kND; k += kThreads)
-        isep_s[k] = sizeof(T) == 4 ? (T)a.inv_sep[k] : (T)a.inv_sep64[k];
-    for (int p = tid; p < a.N; p += kThreads) {
-        V4 b;
-        if (sizeof(T) == 4) {
-            const RulePoint32 rp = a.rule32[p];
-            b = V4{(T)rp.b[0], (T)rp.b[1], (T)rp.b[2], (T)rp.b[3]};
-            rw_s[p] = (T)rp.wq;
-        } else {
-            const RulePoint rp = a.rule[p];
-            b = V4{(T)rp.b[0], (T)rp.b[1], (T)rp.b[2], (T)rp.b[3]};
-            rw_s[p] = (T)rp.wq;
+    stage_rule<T>(a, isep_s, rule_s, N4, tid, kThreads);
+    if constexpr (sizeof(T) == 4) {
+        for (int p = tid; p < N4; p += kThreads) {
+            const float wq = p < a.N ? (float)a.uvw[p].wq : 0.0f;
+            tco_s[p] = float4{wq * (float)kTPoly[0], wq * (float)kTPoly[1], wq * (float)kTPoly[2],
+                              0.0f};
         }
-        rb_s[p] = b;
     }
+    for (int k = tid; k < kND; k += kThreads) hist_s[k] = 0.0;
     __syncthreads();
 
     const SrcGeom g = a.geo[s];
-    const double sx = g.dir[0], sy = g.dir[1], sz = g.dir[2];
+    const bool do_hist = a.hist && g.in_fov;
     const int lx = tid % kTileW, ly = tid / kTileW;
     const int Wp1 = a.W + 1;
-
-    // per-thread running sums of the map integral (pixels within the radius)
-    T isum[EC];
-#pragma unroll
-    for (int e = 0; e < EC; ++e) isum[e] = T(0);
+    const T j0 = (T)a.j0, jk = (T)a.jk, wsum = (T)a.wsum;
+    const size_t plane = (size_t)a.H * a.W;
+    V4* rw = reinterpret_cast<V4*>(a.rw) + (size_t)s * plane;
+    int* rw_row = a.rw_row + (size_t)s * plane;
+    int* near_list = a.near_list + (size_t)s * plane;
 
     for (int t = t_begin; t < t_end; ++t) {
         const int th = t / a.tiles_w, tw = t % a.tiles_w;
         const int h = th * kTileH + ly, w = tw * kTileW + lx;
         const bool valid = h < a.H && w < a.W;
         // out-of-image lanes compute an in-image pixel and store nothing, so
-        // the point loop runs in uniform control flow
+        // the point loops run in uniform control flow
         const int hc = min(h, a.H - 1), wc = min(w, a.W - 1);
-
-        // corner deltas corner_k - source in FP64, then T (LL LR UR UL)
-        const int c00 = hc * Wp1 + wc;
-        const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};
-        T D[4][3];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double* c = a.corners + 3 * cidx[k];
-            D[k][0] = (T)(c[0] - sx);
-            D[k][1] = (T)(c[1] - sy);
-            D[k][2] = (T)(c[2] - sz);
-        }
-        const T cx = T(0.25) * (D[0][0] + D[1][0] + D[2][0] + D[3][0]);
-        const T cy = T(0.25) * (D[0][1] + D[1][1] + D[2][1] + D[3][1]);
-        const T cz = T(0.25) * (D[0][2] + D[1][2] + D[2][2] + D[3][2]);
-        const T c0 = fast_sqrt(cx * cx + cy * cy + cz * cz);
-        T rho = T(0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const T ex = D[k][0] - cx, ey = D[k][1] - cy, ez = D[k][2] - cz;
-            rho = fmax(rho, fast_sqrt(ex * ex + ey * ey + ez * ez));
-        }
-        rho = rho * (T)a.rho_scale * T(1.0001) + T(1e-6) * c0;
-        const T slo = T(2.0 * kR2D) * asin_pos(T(0.5) * fmax(c0 - rho, T(0)));
-        const T shi = T(2.0 * kR2D) * asin_pos(fmin(T(0.5) * (c0 + rho), T(1)));
-        const int ilo = sep_index_fast(slo), ihi = sep_index_fast(shi);
+        const PixelGeom<T> pg = pixel_geom<T>(a.corners, hc, wc, Wp1, g.dir, a.rho_scale);
+        const int ilo = sep_index_fast(pg.slo), ihi = sep_index_fast(pg.shi);
         // rows touched: ilo-1 .. ihi; inside the table with t in [0, 1]
-        const bool fast = (ihi - ilo) <= 2 && slo >= T(1e-4) && shi < T(69.9);
+        const bool fast = (ihi - ilo) <= 2 && pg.slo >= T(1e-4) && pg.shi < T(69.9);
         const int r0 = min(max(ilo - 1, 0), kND - 4);
 
+        // S0 = sum_{rel=0} w, S01 = sum_{rel<=1} w, likewise T0, T01 of w t
+        T S0 = T(0), T0 = T(0), S01 = T(0), T01 = T(0), Tt = T(0);
+        const bool lean6 = __all(fast && pg.shi < T(28.0));
+        const bool lean4 = lean6 && __all(pg.shi < T(14.0));
+        if constexpr (sizeof(T) == 4) {
+            if (lean6) {
+                if (a.tensor_nu > 0) {
+                    if (lean4) lean_tensor_loop<3>(a, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
+                    else lean_tensor_loop<4>(a, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
+                } else {
+                    if (lean4) lean_list_loop<3>(N4, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
+                    else lean_list_loop<4>(N4, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
+                }
+            }
+        } else if (lean6) {
+            // FP64: exact asin, row from the log of the separation
+            for (int p0 = 0; p0 < N4; p0 += 4) {
+                V4 r[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[k] = rule_s[p0 + k];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T u = r[k].x, v = r[k].y;
+                    const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+                    const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+                    const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+                    far_point<T, 6>(dx, dy, dz, r[k].z, r[k].w, isep_s, j0, jk, r0, S0, T0, S01,
+                                    T01, Tt);
+                }
+            }
+            S01 += S0;
+            T01 += T0;
+        }
+        if (!lean6 && __any(fast)) {
+            // ---- far pixels beyond 28 degrees or sharing a wave with near ones
+            for (int p = 0; p < a.N; ++p) {
+                const V4 r = rule_s[p];
+                const T u = r.x, v = r.y, wq = r.z;
+                const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+                const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+                const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+                const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
+                const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
+                const int i = sep_index_fast(sd);
+                // t = (s - sep[i-1]) / (sep[i] - sep[i-1]) = redm1 (s / sep[i-1] - 1)
+                const T tt = T(kRedm1) * (sd * (isep_s[i - 1] * T(0.5 / kR2D)) - T(1));
+                const int rel = min(max(i - 1 - r0, 0), 2);
+                const T wt = wq * tt;
+                S0 += rel == 0 ? wq : T(0);
+                T0 += rel == 0 ? wt : T(0);
+                S01 += rel <= 1 ? wq : T(0);
+                T01 += rel <= 1 ? wt : T(0);
+                Tt += wt;
+            }
+        }
+        if (valid) {
+            const int pix = h * a.W + w;
+            if (fast) {
+                // row weights from the cumulative sums, x solid angle
+                const T S1 = S01 - S0, T1 = T01 - T0, S2 = wsum - S01, T2 = Tt - T01;
+                const T om = a.omega ? (T)a.omega[pix] : T(1);
+                const V4 wv = V4{om * (S0 - T0), om * (T0 + (S1 - T1)), om * (T1 + (S2 - T2)),
+                                 om * T2};
+                rw[pix] = wv;
+                rw_row[pix] = r0;
+                if (do_hist && in_radius(g, a.centres, h * Wp1 + w)) {
+                    atomicAdd(&hist_s[r0], (double)wv.x);
+                    atomicAdd(&hist_s[r0 + 1], (double)wv.y);
+                    atomicAdd(&hist_s[r0 + 2], (double)wv.z);
+                    atomicAdd(&hist_s[r0 + 3], (double)wv.w);
+                }
+            } else {
+                rw_row[pix] = -1;
+                near_list[atomicAdd(&a.near_count[s], 1)] = pix;
+            }
+        }
+    }
+    if (do_hist) {
+        __syncthreads();
+        for (int k = tid; k < kND; k += kThreads)
+            if (hist_s[k] != 0.0) atomicAdd(&a.hist[(size_t)s * kND + k], hist_s[k]);
+    }
+}
+
+// Pass 2: the listed pixels of each source, every point adding its two
+// table rows directly; values x solid angle, stored unscaled (emit_kernel
+// scales them).  Grid: (blocks striding over the list, S, energy chunks of
+// EC).  In-radius values are summed per energy into mapint.
+template <typename T, int EC>
+__global__ void __launch_bounds__(kNearThreads)
+near_pixel_kernel(CubArgs a) {
+    using V = typename Vec<T>::type;
+    constexpr int VN = Vec<T>::n;
+    constexpr int NV = EC / VN;
+    using V4 = typename Vec4<T>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* isum_s = reinterpret_cast<double*>(smem_raw);            // [EC]
+    T* lut_s = reinterpret_cast<T*>(isum_s + EC);
+    T* isep_s = lut_s + kND * EC;
+    V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
+    const int N4 = (a.N + 3) & ~3;
+
+    const int s = blockIdx.y;
+    const int e0 = blockIdx.z * EC;
+    const int ne = min(EC, a.E - e0);
+    const int count = a.near_count[s];
+    if ((int)blockIdx.x * kNearThreads >= count) return;
+    const int tid = threadIdx.x;
+    stage_lut<T, EC, kNearThreads>(a, s, e0, lut_s, tid);
+    stage_rule<T>(a, isep_s, rule_s, N4, tid, kNearThreads);
+    for (int k = tid; k < EC; k += kNearThreads) isum_s[k] = 0.0;
+    __syncthreads();
+
+    const SrcGeom g = a.geo[s];
+    const bool do_sum = a.mapint && g.in_fov;
+    const size_t plane = (size_t)a.H * a.W;
+    const int Wp1 = a.W + 1;
+    const int* list = a.near_list + (size_t)s * plane;
+    for (int first = blockIdx.x * kNearThreads; first < count;
+         first += gridDim.x * kNearThreads) {
+        const int idx = first + tid;
+        const bool live = idx < count;
+        const int pix = live ? list[idx] : list[first];
+        const int h = pix / a.W, w = pix % a.W;
+        const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, Wp1, g.dir, a.rho_scale);
+        const T om = a.omega ? (T)a.omega[pix] : T(1);
         V acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = V{};
-        T W0 = T(0), W1 = T(0), W2 = T(0), W3 = T(0);
-        const T fr0 = (T)(r0 + 1);
-
-#pragma unroll 4
         for (int p = 0; p < a.N; ++p) {
-            const V4 bv = rb_s[p];
-            const T wq = rw_s[p];
-            const T dx = bv.x * D[0][0] + bv.y * D[1][0] + bv.z * D[2][0] + bv.w * D[3][0];
-            const T dy = bv.x * D[0][1] + bv.y * D[1][1] + bv.z * D[2][1] + bv.w * D[3][1];
-            const T dz = bv.x * D[0][2] + bv.y * D[1][2] + bv.z * D[2][2] + bv.w * D[3][2];
-            const T ch = fast_sqrt(dx * dx + dy * dy + dz * dz);
-            const T sd = T(2.0 * kR2D) * asin_pos(fmin(T(0.5) * ch, T(1)));
+            const V4 r = rule_s[p];
+            const T u = r.x, v = r.y, wq = r.z;
+            const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+            const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+            const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+            const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
+            const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
             const int i = sep_index_fast(sd);
-            // t = (s - sep[i-1]) / (sep[i] - sep[i-1]) = redm1 (s / sep[i-1] - 1)
-            const T tt = T(kRedm1) * (sd * isep_s[i - 1] - T(1));
-            if (fast) {
-                // fractional node position relative to row r0
-                const T xx = ((T)i - fr0) + tt;
-                W0 += wq * fmax(T(0), T(1) - fabs(xx));
-                W1 += wq * fmax(T(0), T(1) - fabs(xx - T(1)));
-                W2 += wq * fmax(T(0), T(1) - fabs(xx - T(2)));
-                W3 += wq * fmax(T(0), T(1) - fabs(xx - T(3)));
-            } else {
-                const T wa = wq * (T(1) - tt), wb = wq * tt;
-                const V* L0 = reinterpret_cast<const V*>(lut_s) + (i - 1) * NV;
-                const V* L1 = L0 + NV;
+            const T tt = T(kRedm1) * (sd * (isep_s[i - 1] * T(0.5 / kR2D)) - T(1));
+            const T wa = wq * (T(1) - tt), wb = wq * tt;
+            const V* L0 = reinterpret_cast<const V*>(lut_s) + (i - 1) * NV;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    vfma(acc[v], wa, L0[v]);
-                    vfma(acc[v], wb, L1[v]);
-                }
+            for (int vv = 0; vv < NV; ++vv) {
+                vfma(acc[vv], wa, L0[vv]);
+                vfma(acc[vv], wb, L0[NV + vv]);
             }
         }
-        if (fast) {
-            const V* L = reinterpret_cast<const V*>(lut_s) + r0 * NV;
+        const bool inrad = live && do_sum && in_radius(g, a.centres, h * Wp1 + w);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                vfma(acc[v], W0, L[v]);
-                vfma(acc[v], W1, L[NV + v]);
-                vfma(acc[v], W2, L[2 * NV + v]);
-                vfma(acc[v], W3, L[3 * NV + v]);
-            }
-        }
-        T om = T(1);
-        if (a.omega) om = (T)a.omega[hc * a.W + wc];
-        bool inrad = false;
-        if (a.mapint && g.in_fov && valid) {
-            const double* c = a.centres + 3 * (hc * Wp1 + wc);
-            const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
-            const double chd = sqrt(ex * ex + ey * ey + ez * ez);
-            inrad = chd < g.chord_lo ? true
-                  : (chd > g.chord_hi ? false
-                                      : (2.0 * asin(0.5 * chd) <= kD2R * g.radius_deg));
-        }
-        // ---- epilogue: x solid angle, store, in-radius partial sums
-        if (valid) {
-            const size_t plane = (size_t)a.H * a.W;
-            const size_t base = a.layout_sehw
-                                    ? ((size_t)s * a.E + e0) * plane + (size_t)h * a.W + w
-                                    : (((size_t)s * a.H + h) * a.W + w) * a.E + e0;
-            const size_t step = a.layout_sehw ? plane : 1;
+        for (int v = 0; v < NV; ++v) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
+            for (int j = 0; j < VN; ++j) {
+                const int e = v * VN + j;
+                const T x = reinterpret_cast<const T*>(&acc[v])[j] * om;
+                if (live && e < ne) store_value(a, out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane), x);
+                if (do_sum) {
+                    // wave sum of the in-radius values, one LDS add per wave
+                    double y = inrad ? (double)x : 0.0;
 #pragma unroll
-                for (int j = 0; j < VN; ++j) {
-                    const int e = v * VN + j;
-                    if (e < ne) {
-                        const T val = reinterpret_cast<const T*>(&acc[v])[j] * om;
-                        if (a.out_f32) reinterpret_cast<float*>(a.out)[base + e * step] = (float)val;
-                        else reinterpret_cast<double*>(a.out)[base + e * step] = (double)val;
-                        if (inrad) isum[e] += val;
-                    }
+                    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o);
+                    if ((tid & 63) == 0 && y != 0.0) atomicAdd(&isum_s[e], y);
                 }
             }
         }
     }
+    if (do_sum) {
+        __syncthreads();
+        for (int e = tid; e < ne; e += kNearThreads)
+            if (isum_s[e] != 0.0) atomicAdd(&a.mapint[(size_t)s * a.E + e0 + e], isum_s[e]);
+    }
+}
 
-    // ---- map integral: reduce the per-thread sums over the block, then one
-    // FP64 atomic per (source, energy) per block.
-    if (a.mapint && g.in_fov) {
+// map_integral.cxx:21-44 for the pixels of pass 1: their in-radius sum is
+// sum_d hist[s, d] uPsf[s, d, e] (FP64 table); pass 2 has already added
+// the listed pixels.  One block per source: each thread takes one energy
+// and every P-th row (consecutive threads read consecutive energies), the
+// P partial sums are added in LDS.
+constexpr int kHistThreads = 256;
+__global__ void __launch_bounds__(kHistThreads)
+hist_integral_kernel(int E, const double* __restrict__ hist, const double* __restrict__ upsf,
+                     double* __restrict__ mapint) {
+    __shared__ double h_s[kNDPad];
+    __shared__ double part[kHistThreads];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    for (int d = tid; d < kND; d += kHistThreads) h_s[d] = hist[(size_t)s * kND + d];
+    const int Ec = min(E, kHistThreads);
+    const int P = kHistThreads / Ec;
+    const int el = tid % Ec, ph = tid / Ec;
+    const double* u = upsf + (size_t)s * kND * E;
+    for (int eb = 0; eb < E; eb += Ec) {
         __syncthreads();
-        double* red = reinterpret_cast<double*>(win_s);  // reuse the window space
-        const int wave = tid / 64, lane = tid % 64;
-        const int nwaves = kThreads / 64;
-#pragma unroll
-        for (int e = 0; e < EC; ++e) {
-            if (e < ne) {
-                T v = isum[e];
-                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-                if (lane == 0) red[wave * EC + e] = (double)v;
-            }
-        }
+        const int e = eb + el;
+        double acc = 0.0;
+        if (ph < P && e < E)
+            for (int d = ph; d < kND; d += P) acc = fma(h_s[d], u[(size_t)d * E + e], acc);
+        part[tid] = acc;
         __syncthreads();
-        if (tid < ne) {
+        if (ph == 0 && e < E) {
             double tot = 0.0;
-            for (int k = 0; k < nwaves; ++k) tot += red[k * EC + tid];
-            atomicAdd(&a.mapint[(size_t)s * a.E + e0 + tid], tot);
+            for (int q = 0; q < P; ++q) tot += part[q * Ec + el];
+            mapint[(size_t)s * E + e] += tot;
         }
     }
+}
+
+// Pass 3: per (source, pixel range, energy chunk): map = sum_k W_k L[r0+k]
+// for pass-1 pixels, x exposure x correction when SCALED.  Listed pixels are
+// left to near_scale_kernel.  OT: output type.  Each thread first loads the
+// row weights of its kEmitPix pixels and then only stores: a load issued
+// after stores would wait for them to drain (the vector memory counter
+// counts both).
+//
+// [S, E, H, W]: the energy loop is outermost, so a block writes runs of
+// kEmitThreads consecutive values of one plane; the table rows are staged
+// with an odd stride (EC + 1) so lanes on different rows read different LDS
+// banks.  [S, H, W, E]: each thread writes its pixels' energies as vectors.
+constexpr int kEmitPix = 4;
+constexpr int kEmitBlockPix = kEmitThreads * kEmitPix;
+constexpr int kEmitEC32 = 40, kEmitEC64 = 16;   // energies per block (LDS: 401 rows of EC + 1)
+template <typename T, int EC, typename OT, bool SCALED>
+__global__ void __launch_bounds__(kEmitThreads)
+emit_sehw_kernel(CubArgs a) {
+    using V4 = typename Vec4<T>::type;
+    constexpr int LS = EC + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lut_s = reinterpret_cast<T*>(smem_raw);
+    __shared__ ChunkScale<EC> sc;
+
+    const int s = blockIdx.y;
+    const int e0 = blockIdx.z * EC;
+    const int ne = min(EC, a.E - e0);
+    const int tid = threadIdx.x;
+    const int plane = a.H * a.W;
+    const int se0 = s * a.E + e0;
+    const V4* rw = reinterpret_cast<const V4*>(a.rw) + (size_t)s * plane;
+    const int* rw_row = a.rw_row + (size_t)s * plane;
+    // planes of a multiple of 4 pixels: each thread takes 4 consecutive
+    // pixels and stores them as one vector per energy
+    const bool vec = (plane & 3) == 0;
+    int pix[kEmitPix], lrow[kEmitPix];
+    bool out_ok[kEmitPix];
+    V4 wv[kEmitPix];
+    int r0[kEmitPix];
+#pragma unroll
+    for (int k = 0; k < kEmitPix; ++k) {
+        pix[k] = blockIdx.x * kEmitBlockPix + (vec ? 4 * tid + k : k * kEmitThreads + tid);
+        // loads from a clamped index, unconditionally, so they overlap
+        const int pc = min(pix[k], plane - 1);
+        r0[k] = rw_row[pc];
+        wv[k] = rw[pc];
+    }
+#pragma unroll
+    for (int k = 0; k < kEmitPix; ++k) {
+        out_ok[k] = pix[k] < plane && r0[k] >= 0;
+        lrow[k] = out_ok[k] ? r0[k] * LS : 0;
+        if (!out_ok[k]) wv[k] = V4{T(0), T(0), T(0), T(0)};
+    }
+    const bool all_ok = out_ok[0] && out_ok[1] && out_ok[2] && out_ok[3];
+    stage_lut<T, EC, kEmitThreads, LS>(a, s, e0, lut_s, tid);
+    if (SCALED) sc.load(a, se0, ne, tid);
+    __syncthreads();
+
+    OT* po = reinterpret_cast<OT*>(a.out) + (size_t)se0 * plane;
+#pragma unroll 2
+    for (int e = 0; e < ne; ++e, po += plane) {
+        OT y[kEmitPix];
+#pragma unroll
+        for (int k = 0; k < kEmitPix; ++k) {
+            const T* L = lut_s + lrow[k] + e;
+            T x = wv[k].x * L[0];
+            x = fma(wv[k].y, L[LS], x);
+            x = fma(wv[k].z, L[2 * LS], x);
+            x = fma(wv[k].w, L[3 * LS], x);
+            y[k] = SCALED ? sc.template apply<OT>(x, e) : (OT)x;
+        }
+        if (vec && all_ok) {
+            stream_store4(po + pix[0], y[0], y[1], y[2], y[3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kEmitPix; ++k)
+                if (out_ok[k]) stream_store(po + pix[k], y[k]);
+        }
+    }
+}
+
+template <typename T, int EC, typename OT, bool SCALED>
+__global__ void __launch_bounds__(kEmitThreads)
+emit_shwe_kernel(CubArgs a) {
+    using V = typename Vec<T>::type;
+    constexpr int VN = Vec<T>::n;
+    constexpr int NV = EC / VN;
+    using V4 = typename Vec4<T>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lut_s = reinterpret_cast<T*>(smem_raw);
+    __shared__ ChunkScale<EC> sc;
+
+    const int s = blockIdx.y;
+    const int e0 = blockIdx.z * EC;
+    const int ne = min(EC, a.E - e0);
+    const int tid = threadIdx.x;
+    const int plane = a.H * a.W;
+    const int base = blockIdx.x * kEmitBlockPix;
+    const int se0 = s * a.E + e0;
+    const V4* rw = reinterpret_cast<const V4*>(a.rw) + (size_t)s * plane;
+    const int* rw_row = a.rw_row + (size_t)s * plane;
+    int pix[kEmitPix], r0[kEmitPix];
+    V4 wv[kEmitPix];
+#pragma unroll
+    for (int k = 0; k < kEmitPix; ++k) {
+        pix[k] = base + k * kEmitThreads + tid;
+        const int pc = min(pix[k], plane - 1);
+        r0[k] = rw_row[pc];
+        wv[k] = rw[pc];
+    }
+#pragma unroll
+    for (int k = 0; k < kEmitPix; ++k)
+        if (pix[k] >= plane) r0[k] = -1;
+    stage_lut<T, EC, kEmitThreads>(a, s, e0, lut_s, tid);
+    if (SCALED) sc.load(a, se0, ne, tid);
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < kEmitPix; ++k) {
+        if (r0[k] < 0) continue;
+        const V* L = reinterpret_cast<const V*>(lut_s) + r0[k] * NV;
+        OT* po = reinterpret_cast<OT*>(a.out) + ((size_t)s * plane + pix[k]) * a.E + e0;
+#pragma unroll 2
+        for (int v = 0; v < NV; ++v) {
+            V acc = V{};
+            vfma(acc, wv[k].x, L[v]);
+            vfma(acc, wv[k].y, L[NV + v]);
+            vfma(acc, wv[k].z, L[2 * NV + v]);
+            vfma(acc, wv[k].w, L[3 * NV + v]);
+#pragma unroll
+            for (int j = 0; j < VN; ++j) {
+                const int e = v * VN + j;
+                if (e < ne) {
+                    const T x = reinterpret_cast<const T*>(&acc)[j];
+                    stream_store(po + e, SCALED ? sc.template apply<OT>(x, e) : (OT)x);
+                }
+            }
+        }
+    }
+}
+
+// Listed pixels: x exposure x correction in place (their unscaled values
+// come from near_pixel_kernel).  Grid: (blocks striding over the source's
+// (pixel, energy) pairs, S).
+template <typename OT>
+__global__ void near_scale_kernel(CubArgs a) {
+    const int s = blockIdx.y;
+    const int count = a.near_count[s];
+    const size_t plane = (size_t)a.H * a.W;
+    const int* list = a.near_list + (size_t)s * plane;
+    OT* out = reinterpret_cast<OT*>(a.out);
+    const int64_t n = (int64_t)count * a.E;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        // consecutive threads: consecutive list entries (mostly neighbouring
+        // pixels) of one energy
+        const int e = (int)(k / count), idx = (int)(k % count);
+        const int pix = list[idx];
+        const size_t o = a.layout_sehw ? ((size_t)s * a.E + e) * plane + pix
+                                       : ((size_t)s * plane + pix) * a.E + e;
+        const int se = s * a.E + e;
+        if constexpr (sizeof(OT) == 4) {
+            out[o] = out[o] * (float)(a.expo[se] * a.cf[se]);
+        } else {
+            double y = out[o] * a.expo[se];
+            out[o] = y * a.cf[se];
+        }
+    }
+}
+
+template <typename T, int EC, typename OT, bool SCALED>
+static int launch_emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid, bool sehw) {
+    const size_t sm = (size_t)kND * (sehw ? EC + 1 : EC) * sizeof(T);
+    const void* fn = sehw ? (const void*)emit_sehw_kernel<T, EC, OT, SCALED>
+                          : (const void*)emit_shwe_kernel<T, EC, OT, SCALED>;
+    XTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    if (sehw) emit_sehw_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
+    else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
+template <typename T, int EC>
+static int emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid, bool scaled) {
+    const bool sehw = ca.layout_sehw != 0;
+    if (ca.out_f32)
+        return scaled ? launch_emit<T, EC, float, true>(ctx, ca, grid, sehw)
+                      : launch_emit<T, EC, float, false>(ctx, ca, grid, sehw);
+    return scaled ? launch_emit<T, EC, double, true>(ctx, ca, grid, sehw)
+                  : launch_emit<T, EC, double, false>(ctx, ca, grid, sehw);
 }
 
 // ------------------------------------------------------------------ factors
@@ -433,8 +1026,8 @@ __global__ void factor_kernel(int S, int E, const SrcGeom* __restrict__ geo,
                               const double* __restrict__ invmi_in, const double* __restrict__ upsf,
                               const double* __restrict__ parint, const double* __restrict__ sep,
                               const double* __restrict__ expo, double* __restrict__ cf_out,
-                              double* __restrict__ invmi_out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                              double* __restrict__ invmi_out, int s_begin = 0) {
+    const int k = s_begin * E + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S * E) return;
     const int s = k / E, e = k % E;
     const SrcGeom g = geo[s];
@@ -462,66 +1055,6 @@ __global__ void factor_kernel(int S, int E, const SrcGeom* __restrict__ geo,
     }
     if (cf_out) cf_out[k] = cf;
     if (invmi_out) invmi_out[k] = invmi;
-}
-
-// map[s, ...] *= expo[s, e] then *= cf[s, e] (two roundings, as the
-// reference applies them in two passes).
-template <typename OT>
-__global__ void scale_kernel(OT* __restrict__ out, int S, int E, size_t plane, int layout_sehw,
-                             const double* __restrict__ expo, const double* __restrict__ cf) {
-    const size_t n = (size_t)S * E * plane;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (size_t)gridDim.x * blockDim.x) {
-        size_t se;
-        if (layout_sehw) se = k / plane;
-        else { const size_t s = k / (plane * E); se = s * E + (k % E); }
-        if (sizeof(OT) == 8) {
-            double v = (double)out[k];
-            v *= expo[se];
-            v *= cf[se];
-            out[k] = (OT)v;
-        } else {
-            out[k] = (OT)((double)out[k] * (expo[se] * cf[se]));
-        }
-    }
-}
-
-// [S, E, H, W]: blockIdx.y is the (s, e) plane, factor read once per block
-template <typename OT>
-__global__ void scale_planes_kernel(OT* __restrict__ out, size_t plane,
-                                    const double* __restrict__ expo, const double* __restrict__ cf) {
-    const size_t se = blockIdx.y;
-    OT* p = out + se * plane;
-    if (sizeof(OT) == 8) {
-        const double f0 = expo[se], f1 = cf[se];
-        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < plane;
-             k += (size_t)gridDim.x * blockDim.x) {
-            double v = (double)p[k];
-            v *= f0;
-            v *= f1;
-            p[k] = (OT)v;
-        }
-    } else {
-        const float f = (float)(expo[se] * cf[se]);
-        const size_t n4 = plane / 4;
-        float4* p4 = reinterpret_cast<float4*>(p);
-        const bool aligned = (reinterpret_cast<uintptr_t>(p) & 15) == 0;
-        if (aligned) {
-            for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4;
-                 k += (size_t)gridDim.x * blockDim.x) {
-                float4 v = p4[k];
-                v.x *= f; v.y *= f; v.z *= f; v.w *= f;
-                p4[k] = v;
-            }
-            for (size_t k = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < plane;
-                 k += (size_t)gridDim.x * blockDim.x)
-                p[k] = (OT)((float)p[k] * f);
-        } else {
-            for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < plane;
-                 k += (size_t)gridDim.x * blockDim.x)
-                p[k] = (OT)((float)p[k] * f);
-        }
-    }
 }
 
 // pixel-factor scaling on a double [S,H,W,E] map (solid angle)
@@ -638,8 +1171,9 @@ static int sep_tables(xts_ctx* ctx, SepTables* t) {
         const double xstep = std::log(70.0 / 1e-4) / 399.0;
         sep[0] = 0.0;
         for (int i = 0; i < 400; ++i) sep[i + 1] = 1e-4 * std::exp(i * xstep);
+        // 2 R2D / sep: multiplies asin(chord / 2) into s / sep
         for (int i = 0; i < kND; ++i) {
-            inv[i] = sep[i] > 0 ? 1.0 / sep[i] : 0.0;
+            inv[i] = sep[i] > 0 ? 2.0 * kR2D / sep[i] : 0.0;
             inv32[i] = (float)inv[i];
         }
     }
@@ -665,36 +1199,43 @@ __global__ void lut_convert_kernel(const double* __restrict__ upsf, int S, int E
     }
 }
 
-static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RulePoint** dev,
-                       RulePoint32** dev32, double* rho_scale) {
+static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** dev,
+                       double* rho_scale, double* wsum, int* tnu, int* tnv) {
     const double* r = rule ? rule : kSquareDeg7;
     if (!rule) N = 12;
     XTS_CHECK(N > 0 && N <= 1024, "rule size %lld out of range (1..1024)", (long long)N);
-    std::vector<RulePoint> rp((size_t)N);
+    std::vector<RuleUVW> rp((size_t)N);
     double rs = 1.0;
     for (int64_t p = 0; p < N; ++p) {
         const double u = r[3 * p], v = r[3 * p + 1];
-        rp[p].b[0] = 0.25 * (1.0 - u) * (1.0 - v);
-        rp[p].b[1] = 0.25 * (1.0 + u) * (1.0 - v);
-        rp[p].b[2] = 0.25 * (1.0 + u) * (1.0 + v);
-        rp[p].b[3] = 0.25 * (1.0 - u) * (1.0 + v);
-        rp[p].wq = 0.25 * r[3 * p + 2];
-        rs = std::max(rs, std::fabs(rp[p].b[0]) + std::fabs(rp[p].b[1]) +
-                              std::fabs(rp[p].b[2]) + std::fabs(rp[p].b[3]));
+        rp[p] = {u, v, 0.25 * r[3 * p + 2]};
+        // sum of |bilinear corner weights|: 1 inside the square
+        const double b = 0.25 * (std::fabs((1 - u) * (1 - v)) + std::fabs((1 + u) * (1 - v)) +
+                                 std::fabs((1 + u) * (1 + v)) + std::fabs((1 - u) * (1 + v)));
+        rs = std::max(rs, b);
     }
-    std::vector<char> buf(rp.size() * (sizeof(RulePoint) + sizeof(RulePoint32)));
-    memcpy(buf.data(), rp.data(), rp.size() * sizeof(RulePoint));
-    RulePoint32* r32 = (RulePoint32*)(buf.data() + rp.size() * sizeof(RulePoint));
-    for (size_t p = 0; p < rp.size(); ++p) {
-        for (int k = 0; k < 4; ++k) r32[p].b[k] = (float)rp[p].b[k];
-        r32[p].wq = (float)rp[p].wq;
-        r32[p].pad[0] = r32[p].pad[1] = r32[p].pad[2] = 0.f;
+    double ws = 0.0;
+    for (auto& q : rp) ws += q.wq;
+    // grid rule in row-major order: u depends on p / nv only, v on p % nv only
+    int nu = 0, nv = 0;
+    for (int64_t p = 1; p < N && nv == 0; ++p)
+        if (rp[p].u != rp[0].u) nv = (int)p;
+    if (nv > 0 && N % nv == 0 && nv % 4 == 0) {
+        nu = (int)(N / nv);
+        bool grid = true;
+        for (int64_t p = 0; p < N && grid; ++p)
+            grid = rp[p].u == rp[(p / nv) * nv].u && rp[p].v == rp[p % nv].v;
+        if (!grid) nu = nv = 0;
+    } else {
+        nv = 0;
     }
     void* d;
-    XTS_TRY(to_device(ctx, kSlotWork2, buf.data(), buf.size(), &d));
-    *dev = (RulePoint*)d;
-    *dev32 = (RulePoint32*)((char*)d + rp.size() * sizeof(RulePoint));
+    XTS_TRY(to_device(ctx, kSlotWork2, rp.data(), rp.size() * sizeof(RuleUVW), &d));
+    *dev = (RuleUVW*)d;
     *rho_scale = rs;
+    *wsum = ws;
+    *tnu = nu;
+    *tnv = nv;
     return XTS_OK;
 }
 
@@ -712,6 +1253,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_CHECK(!full || (expo && parint), "null exposure/parint");
     XTS_CHECK((flags & ~0x7) == 0, "unknown flags 0x%x", flags);
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
+    XTS_CHECK((int64_t)H * W <= (int64_t)1 << 30, "image of %lld pixels exceeds 2^30",
+              (long long)H * W);
     const bool f64 = (flags & XTS_COMPUTE_F64) != 0;
     const int sehw = (flags & XTS_LAYOUT_SEHW) ? 1 : 0;
     const int of32 = (flags & XTS_OUT_F32) ? 1 : 0;
@@ -720,10 +1263,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig));
     SepTables st;
     XTS_TRY(sep_tables(ctx, &st));
-    RulePoint* rp;
-    RulePoint32* rp32;
-    double rho_scale;
-    XTS_TRY(upload_rule(ctx, rule, N, &rp, &rp32, &rho_scale));
+    RuleUVW* rp;
+    double rho_scale, wsum;
+    int tnu, tnv;
+    XTS_TRY(upload_rule(ctx, rule, N, &rp, &rho_scale, &wsum, &tnu, &tnv));
     if (!rule) N = 12;
 
     // PSF table in the cubature precision
@@ -735,16 +1278,20 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     lut_convert_kernel<<<std::min<unsigned>(div_up(lut_elems, 256), 8192), 256, 0, ctx->stream>>>(
         upsf, (int)S, (int)E, Ep, f64 ? nullptr : (float*)lut, f64 ? (double*)lut : nullptr);
 
-    double* mapint = nullptr;
-    double* fac = nullptr;
-    if (full) {
-        void* acc;
-        XTS_TRY(slot_reserve(ctx, kSlotAcc, (size_t)S * E * 8 * 2, &acc));
-        mapint = (double*)acc;
-        fac = mapint + S * E;
-        zero_kernel<<<std::min<unsigned>(div_up(S * E, 256), 1024), 256, 0, ctx->stream>>>(
-            mapint, (size_t)S * E);
-    }
+    // per-pixel row weights, first rows and the near-pixel lists
+    const size_t plane = (size_t)H * W;
+    const size_t rw_bytes = (size_t)S * plane * (f64 ? 32 : 16);
+    void* rows;
+    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 8 + 512, &rows));
+    // counters and FP64 accumulators: near_count [S] (int, padded), hist
+    // [S][401], mapint [S][E], factor [S][E]
+    const size_t cnt_d = (size_t)(S + 1) / 2;
+    const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
+    void* accp;
+    XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
+    double* acc = (double*)accp;
+    const size_t nzero = full ? cnt_d + (size_t)S * kND + (size_t)S * E : cnt_d;
+    zero_kernel<<<std::min<unsigned>(div_up(nzero, 256), 4096), 256, 0, ctx->stream>>>(acc, nzero);
 
     CubArgs ca{};
     ca.corners = ig.corners;
@@ -752,66 +1299,100 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.omega = full ? ig.omega : nullptr;
     ca.geo = ig.geo;
     ca.lut = lut;
-    ca.rule = rp;
-    ca.rule32 = rp32;
+    ca.uvw = rp;
     ca.inv_sep = st.inv32;
     ca.inv_sep64 = st.inv64;
-    ca.mapint = mapint;
+    ca.rw = rows;
+    ca.rw_row = (int*)((char*)rows + rw_bytes);
+    ca.near_list = ca.rw_row + (size_t)S * plane;
+    ca.near_count = (int*)acc;
+    ca.hist = full ? acc + cnt_d : nullptr;
+    ca.mapint = full ? ca.hist + (size_t)S * kND : nullptr;
+    double* fac = full ? ca.mapint + (size_t)S * E : nullptr;
+    ca.expo = full ? expo : nullptr;
+    ca.cf = fac;
     ca.out = out;
     ca.S = (int)S; ca.H = H; ca.W = W; ca.E = (int)E; ca.Ep = Ep; ca.N = (int)N;
     ca.tiles_w = (W + kTileW - 1) / kTileW;
     ca.tiles_h = (H + kTileH - 1) / kTileH;
-    const int ntiles = ca.tiles_w * ca.tiles_h;
-    // About eight blocks per CU (two resident at a time), each amortising its
-    // LUT staging over several tiles of the same source.
-    const int nchunks = (int)((E + EC - 1) / EC);
-    const int64_t work = (int64_t)ntiles * S * nchunks;
-    int64_t tpb = (work + (int64_t)ctx->cu_count * 8 - 1) / ((int64_t)ctx->cu_count * 8);
-    tpb = std::max<int64_t>(tpb, std::min<int64_t>(4, ntiles));
-    ca.tiles_per_block = (int)std::min<int64_t>(std::max<int64_t>(tpb, 1), ntiles);
     ca.layout_sehw = sehw;
     ca.out_f32 = of32;
     ca.rho_scale = rho_scale;
-    dim3 grid((ntiles + ca.tiles_per_block - 1) / ca.tiles_per_block, (unsigned)S, nchunks);
-    TimedRegion tr_cub(ctx, "cubature");
-    if (f64) {
-        const size_t sm = (size_t)(kND * 16 + kThreads * kWinStride + kNDPad) * 8 + (size_t)N * 40;
-        XTS_HIP(hipFuncSetAttribute((const void*)cubature_kernel<double, 16>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        cubature_kernel<double, 16><<<grid, kThreads, sm, ctx->stream>>>(ca);
-    } else {
-        const size_t sm = (size_t)(kND * 40 + kThreads * kWinStride + kNDPad) * 4 + (size_t)N * 20;
-        XTS_HIP(hipFuncSetAttribute((const void*)cubature_kernel<float, 40>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        cubature_kernel<float, 40><<<grid, kThreads, sm, ctx->stream>>>(ca);
-    }
-    XTS_HIP(hipGetLastError());
-    tr_cub.stop();
+    ca.wsum = wsum;
+    ca.tensor_nu = tnu;
+    ca.tensor_nv = tnv;
+    // j = 1 + recipstep (ln 1e4 + ln(2 R2D) + ln 2 log2(asin(chord / 2)))
+    ca.j0 = 1.0 + kRecipStep * (kLn1e4 + std::log(2.0 * kR2D));
+    ca.jk = kRecipStep * 0.69314718055994530942;
 
-    if (full) {
-        factor_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>(
-            (int)S, (int)E, ig.geo, mapint, nullptr, upsf, parint, st.sep, expo, fac, nullptr);
-        const size_t plane = (size_t)H * W;
-        const size_t n = (size_t)S * E * plane;
-        const unsigned blocks = std::min<unsigned>(div_up((int64_t)n, 256), 65536);
-        TimedRegion tr_scale(ctx, "scale");
-        if (sehw && S * E <= 65535) {
-            const unsigned bx = std::max(1u, std::min<unsigned>(div_up((int64_t)plane, 1024), 64));
-            if (of32)
-                scale_planes_kernel<float><<<dim3(bx, (unsigned)(S * E)), 256, 0, ctx->stream>>>(
-                    (float*)out, plane, expo, fac);
-            else
-                scale_planes_kernel<double><<<dim3(bx, (unsigned)(S * E)), 256, 0, ctx->stream>>>(
-                    (double*)out, plane, expo, fac);
-        } else if (of32) {
-            scale_kernel<float><<<blocks, 256, 0, ctx->stream>>>((float*)out, (int)S, (int)E, plane,
-                                                                 sehw, expo, fac);
+    // pass 1: each block takes a run of tiles of one source, enough blocks
+    // to fill every CU a few times over
+    const int ntiles = ca.tiles_w * ca.tiles_h;
+    const int64_t want_blocks = (int64_t)ctx->cu_count * 4;
+    const int64_t tpb = ctx->tune_tiles_per_block > 0
+                            ? ctx->tune_tiles_per_block
+                            : ((int64_t)ntiles * S + want_blocks - 1) / want_blocks;
+    ca.tiles_per_block = (int)std::min<int64_t>(std::max<int64_t>(tpb, 1), ntiles);
+    const int N4 = (int)((N + 3) / 4 * 4);
+    const size_t tsz = f64 ? 8 : 4;
+    {
+        const size_t sm = (size_t)kNDPad * 8 + (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz +
+                          (size_t)N4 * 16;
+        const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
+        TimedRegion tr(ctx, "cubature");
+        if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        else rowweight_kernel<float><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        XTS_HIP(hipGetLastError());
+    }
+    // pass 2: blocks stride over each source's list
+    const int nchunks = (int)((E + EC - 1) / EC);
+    {
+        const size_t sm = (size_t)EC * 8 + ((size_t)kND * EC + kNDPad) * tsz + (size_t)N4 * 4 * tsz;
+        const int64_t gx = std::min<int64_t>(
+            std::max<int64_t>(1, want_blocks / ((int64_t)S * nchunks)), div_up(plane, kNearThreads));
+        const dim3 grid((unsigned)gx, (unsigned)S, (unsigned)nchunks);
+        TimedRegion tr(ctx, "near");
+        if (f64) {
+            XTS_HIP(hipFuncSetAttribute((const void*)near_pixel_kernel<double, 16>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            near_pixel_kernel<double, 16><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
         } else {
-            scale_kernel<double><<<blocks, 256, 0, ctx->stream>>>((double*)out, (int)S, (int)E,
-                                                                  plane, sehw, expo, fac);
+            XTS_HIP(hipFuncSetAttribute((const void*)near_pixel_kernel<float, 40>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            near_pixel_kernel<float, 40><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
         }
         XTS_HIP(hipGetLastError());
-        tr_scale.stop();
+    }
+    if (full) {
+        TimedRegion tr(ctx, "factor");
+        const int nse = (int)(S * E);
+        hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>((int)E, ca.hist, upsf,
+                                                                            ca.mapint);
+        factor_kernel<<<div_up(nse, 256), 256, 0, ctx->stream>>>((int)S, (int)E, ig.geo, ca.mapint,
+                                                                  nullptr, upsf, parint, st.sep,
+                                                                  expo, fac, nullptr);
+        XTS_HIP(hipGetLastError());
+    }
+    // pass 3: blocks of kEmitBlockPix pixels of one source
+    {
+        const dim3 grid32((unsigned)div_up(plane, kEmitBlockPix), (unsigned)S,
+                          (unsigned)div_up(E, kEmitEC32));
+        const dim3 grid64((unsigned)div_up(plane, kEmitBlockPix), (unsigned)S,
+                          (unsigned)div_up(E, kEmitEC64));
+        TimedRegion tr(ctx, "emit");
+        if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx, ca, grid64, full)));
+        else XTS_TRY((emit<float, kEmitEC32>(ctx, ca, grid32, full)));
+    }
+    if (full) {
+        // listed pixels: a few elements per thread
+        TimedRegion tr(ctx, "near_scale");
+        const unsigned gx = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>(div_up((int64_t)plane * E, 256 * 4),
+                                 std::max<int64_t>(1, want_blocks * 4 / S)));
+        const dim3 grid(gx, (unsigned)S);
+        if (of32) near_scale_kernel<float><<<grid, 256, 0, ctx->stream>>>(ca);
+        else near_scale_kernel<double><<<grid, 256, 0, ctx->stream>>>(ca);
+        XTS_HIP(hipGetLastError());
     }
     return XTS_OK;
 }
