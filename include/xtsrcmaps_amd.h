@@ -92,7 +92,7 @@ const char* xts_last_error(void);
 int xts_abi_version(void);
 /* Instrumentation (no reference counterpart): when enabled, kernel launches
  * of the named model-map stages ("cubature": the row-weight pass, "near",
- * "factor", "emit") are bracketed by events on the
+ * "factor", "emit", "near_scale") are bracketed by events on the
  * context stream; xts_ctx_kernel_times synchronises and returns (and
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
@@ -170,6 +170,31 @@ int xts_psf_lookup_table_and_partial_integrals_dev(
     const double* aeff_front, const double* aeff_back, const double* ecb,
     const double* wecb, const double* front_ltf, const double* exposure,
     int64_t S, double* upsf, double* parint, int32_t precision);
+
+/* Psf::bilerp (psf/psf.bilerp.cxx:6-52): King table kings [401, Mc, Me]
+ * on the grid's (cos theta, log E) nodes -> [C, 401, E] at the observation
+ * points; grid->params is not read.  XTS_ERR_RANGE if a point lies outside
+ * the grid. */
+int xts_bilerp(xts_ctx* ctx, const double* costh, int64_t C, const double* logE,
+               int64_t E, const xts_psf_grid* grid, const double* kings,
+               double* out);
+
+/* Psf::corrected_exposure_psf (psf/psf.corrected_exposure_psf.cxx:12-120):
+ * (ECB . (psf x aeff)) LTF0 + (WECB . (psf x aeff)) LTF1 -> [S, 401, E]. */
+int xts_corrected_exposure_psf(xts_ctx* ctx, const double* obs_psf /* [C, 401, E] */,
+                               const double* obs_aeff /* [C, E] */,
+                               const double* ecb, const double* wecb,
+                               const double* front_ltf /* [2, E] */, int64_t S,
+                               int64_t C, int64_t E, double* out);
+
+/* Psf::mean_psf (psf/psf.mean_psf.cxx:6-33): (front + back) / exposure. */
+int xts_mean_psf(xts_ctx* ctx, const double* front, const double* back,
+                 const double* exposure /* [S, E] */, int64_t S, int64_t E,
+                 double* out /* [S, 401, E] */);
+
+/* Psf::normalize (psf/psf.normalize.cxx:7-24): upsf /= total, in place. */
+int xts_normalize(xts_ctx* ctx, double* upsf /* [S, 401, E] */,
+                  const double* total /* [S, E] */, int64_t S, int64_t E);
 
 /* Psf::partial_total_integral + Psf::normalize
  * (psf/psf.partial_total_integral.cxx:8-52, psf/psf.normalize.cxx:7-24):

@@ -211,6 +211,46 @@ def psf_lookup_table_and_partial_integrals(front, back, costh, logE, af, ab,
     return upsf, parint
 
 
+def bilerp(costh, logE, pc, pe, kings):
+    """psf/psf.bilerp.cxx:6-52 -> [C, 401, E]."""
+    costh, logE, pc, pe, k = map(_c, (costh, logE, pc, pe, kings))
+    out = np.zeros((len(costh), ND, len(logE)))
+    rc = lib().or_bilerp(_p(costh), _l(len(costh)), _p(logE), _l(len(logE)), _p(pc),
+                         _l(len(pc)), _p(pe), _l(len(pe)), _p(k), _p(out))
+    if rc:
+        raise ValueError("bilerp: sample outside the IRF grid")
+    return out
+
+
+def corrected_exposure_psf(obs_psf, aeff, ecb, wecb, ltf_):
+    """psf/psf.corrected_exposure_psf.cxx:12-120 -> [S, 401, E]."""
+    obs_psf, aeff, ecb, wecb, ltf_ = map(_c, (obs_psf, aeff, ecb, wecb, ltf_))
+    C_, D, E = obs_psf.shape
+    S = ecb.shape[0]
+    out = np.zeros((S, D, E))
+    lib().or_corrected_exposure_psf(_p(obs_psf), _p(aeff), _p(ecb), _p(wecb), _p(ltf_), _l(S),
+                                    _l(C_), _l(D), _l(E), _p(out))
+    return out
+
+
+def mean_psf(front, back, expo):
+    """psf/psf.mean_psf.cxx:6-33."""
+    front, back, expo = map(_c, (front, back, expo))
+    S, D, E = front.shape
+    out = np.zeros_like(front)
+    lib().or_mean_psf(_p(front), _p(back), _p(expo), _l(S), _l(D), _l(E), _p(out))
+    return out
+
+
+def normalize(upsf, total):
+    """psf/psf.normalize.cxx:7-24 (returns a normalised copy)."""
+    u = _c(upsf).copy()
+    t = _c(total)
+    S, D, E = u.shape
+    lib().or_normalize(_p(u), _p(t), _l(S), _l(D), _l(E))
+    return u
+
+
 def partial_total_integral(upsf):
     u = _c(upsf)
     S, D, E = u.shape

@@ -55,3 +55,53 @@ def oracle_wcs(orc, wcs):
     return orc.make_wcs((wcs.naxis1, wcs.naxis2), (wcs.crpix1, wcs.crpix2),
                         (wcs.crval1, wcs.crval2), (wcs.cdelt1, wcs.cdelt2), wcs.crota2,
                         "CAR" if wcs.proj == 0 else "AIT")
+
+
+# ---------------------------------------------------------------- XTSA files
+# The array container of tests/cpp/fermi_pipeline.cpp: "XTSA", u32 count,
+# then per array u32 name length, name, u32 rank, u64 extents, f64 data.
+def write_xtsa(path, arrays: dict):
+    import struct
+    with open(path, "wb") as f:
+        f.write(b"XTSA")
+        f.write(struct.pack("<I", len(arrays)))
+        for name, a in arrays.items():
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            nb = name.encode()
+            f.write(struct.pack("<I", len(nb)) + nb)
+            f.write(struct.pack("<I", a.ndim))
+            f.write(struct.pack(f"<{a.ndim}Q", *a.shape))
+            f.write(a.tobytes())
+
+
+def read_xtsa(path) -> dict:
+    import struct
+    out = {}
+    with open(path, "rb") as f:
+        assert f.read(4) == b"XTSA"
+        (n,) = struct.unpack("<I", f.read(4))
+        for _ in range(n):
+            (ln,) = struct.unpack("<I", f.read(4))
+            name = f.read(ln).decode()
+            (rank,) = struct.unpack("<I", f.read(4))
+            shape = struct.unpack(f"<{rank}Q", f.read(8 * rank))
+            count = int(np.prod(shape)) if rank else 1
+            out[name] = np.frombuffer(f.read(8 * count), dtype=np.float64).reshape(shape)
+    return out
+
+
+def xtsa_inputs(inp: synthetic.Inputs) -> dict:
+    """The synthetic inputs in the driver's naming (raw King rows, Aeff grids)."""
+    w = inp.wcs
+    a = {"wcs": [w.naxis1, w.naxis2, w.crpix1, w.crpix2, w.crval1, w.crval2, w.cdelt1, w.cdelt2,
+                 w.crota2, w.proj],
+         "logE": inp.logE, "cube": inp.cube, "wcube": inp.wcube, "nside": [inp.nside],
+         "src_sph": inp.src_sph, "psf_scale": list(inp.psf_scale),
+         "effic_p0": inp.effic[0], "effic_p1": inp.effic[1]}
+    for side, (params, cos, loge) in inp.king_raw.items():
+        a[f"psf_{side}_cosths"], a[f"psf_{side}_logEs"], a[f"psf_{side}_params"] = cos, loge, params
+    for side, (cos, loge, params, minc) in inp.aeff_grid.items():
+        a[f"aeff_{side}_cosths"], a[f"aeff_{side}_logEs"] = cos, loge
+        a[f"aeff_{side}_params"] = np.asarray(params)[:, :, None]
+        a[f"aeff_{side}_minc"] = [minc]
+    return a

@@ -81,6 +81,33 @@ def test_gpu_irf_and_exposure_stages(ctx, c0, c0_oracle):
     assert max_rel(g["parint"][:, 1:], o["parint"][:, 1:]) < 1e-10
 
 
+def test_gpu_psf_stage_entry_points(ctx, orc, c1):
+    """bilerp, corrected_exposure_psf, mean_psf and normalize one by one
+    (psf.hxx:50-84) against the oracle, and their chain against the fused
+    psf_lookup_table_and_partial_integrals."""
+    _log("test_gpu_psf_stage_entry_points: c1 (100 sources)")
+    g = _gpu_stages(ctx, c1, c1.src_sph)
+    corr = {}
+    for side in ("front", "back"):
+        params, cos, loge = g["king"][side]
+        kings = ctx.king(params, cos, loge)
+        b = ctx.bilerp(c1.costh, c1.logE, cos, loge, kings)
+        assert max_rel(b, orc.bilerp(c1.costh, c1.logE, cos, loge, kings), floor=1e-300) < 1e-13
+        corr[side] = ctx.corrected_exposure_psf(b, g["aeff"][side], g["ecb"], g["wecb"], g["ltf"])
+        ref = orc.corrected_exposure_psf(b, g["aeff"][side], g["ecb"], g["wecb"], g["ltf"])
+        assert max_rel(corr[side], ref, floor=1e-300) < 1e-12
+    mean = ctx.mean_psf(corr["front"], corr["back"], g["expo"])
+    assert max_rel(mean, orc.mean_psf(corr["front"], corr["back"], g["expo"])) < 1e-15
+    par, tot = orc.partial_total_integral(mean)
+    normed = ctx.normalize(mean, tot)
+    assert max_rel(normed, orc.normalize(mean, tot)) < 1e-15
+    # the chain equals the fused entry point
+    assert max_rel(normed, g["upsf"]) < 1e-11
+    with pytest.raises(Exception):
+        params, cos, loge = g["king"]["front"]
+        ctx.bilerp(np.array([2.0]), c1.logE, cos, loge, ctx.king(params, cos, loge))
+
+
 def test_gpu_src_cosbins_many_sources(ctx, orc, c1):
     _log("test_gpu_src_cosbins_many_sources")
     rng = np.random.default_rng(5)

@@ -181,6 +181,45 @@ class Context:
             int(precision)))
         return upsf, parint
 
+    def bilerp(self, costh, logE, cosths, logEs, kings):
+        """Psf::bilerp (psf/psf.bilerp.cxx:6-52): kings [401, Mc, Me] -> [C, 401, E]."""
+        costh, logE, k = capi.f64(costh), capi.f64(logE), capi.f64(kings)
+        g = capi.psf_grid(np.zeros((len(cosths), len(logEs), 6)), cosths, logEs)
+        out = np.zeros((len(costh), ND, len(logE)))
+        capi.check(capi.lib().xts_bilerp(self._h, capi.dptr(costh), len(costh), capi.dptr(logE),
+                                         len(logE), C.byref(g[0]), capi.dptr(k),
+                                         capi.dptr(out)))
+        return out
+
+    def corrected_exposure_psf(self, obs_psf, obs_aeff, ecb, wecb, front_ltf):
+        """Psf::corrected_exposure_psf (psf/psf.corrected_exposure_psf.cxx:12-120)."""
+        obs_psf, obs_aeff, ecb, wecb, ltf = map(capi.f64, (obs_psf, obs_aeff, ecb, wecb,
+                                                           front_ltf))
+        Cn, _, E = obs_psf.shape
+        S = ecb.shape[0]
+        out = np.zeros((S, ND, E))
+        capi.check(capi.lib().xts_corrected_exposure_psf(
+            self._h, capi.dptr(obs_psf), capi.dptr(obs_aeff), capi.dptr(ecb), capi.dptr(wecb),
+            capi.dptr(ltf), S, Cn, E, capi.dptr(out)))
+        return out
+
+    def mean_psf(self, front, back, exposure):
+        """Psf::mean_psf (psf/psf.mean_psf.cxx:6-33)."""
+        front, back, expo = map(capi.f64, (front, back, exposure))
+        S, _, E = front.shape
+        out = np.zeros_like(front)
+        capi.check(capi.lib().xts_mean_psf(self._h, capi.dptr(front), capi.dptr(back),
+                                           capi.dptr(expo), S, E, capi.dptr(out)))
+        return out
+
+    def normalize(self, upsf, total):
+        """Psf::normalize (psf/psf.normalize.cxx:7-24); returns the normalised table."""
+        u = capi.f64(upsf).copy()
+        t = capi.f64(total)
+        S, _, E = u.shape
+        capi.check(capi.lib().xts_normalize(self._h, capi.dptr(u), capi.dptr(t), S, E))
+        return u
+
     def partial_total_integral(self, upsf):
         """Psf::partial_total_integral + Psf::normalize -> (uPsf_normed, partial, total)."""
         u = capi.f64(upsf).copy()

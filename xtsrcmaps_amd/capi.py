@@ -83,6 +83,10 @@ _SIGS = {
         _vp, C.POINTER(PsfGrid), C.POINTER(PsfGrid), _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp,
         _vp, _vp, _i64, _vp, _vp, C.c_int32],
     "xts_partial_total_integral": [_vp, _d, _i64, _i64, _d, _d],
+    "xts_bilerp": [_vp, _d, _i64, _d, _i64, C.POINTER(PsfGrid), _d, _d],
+    "xts_corrected_exposure_psf": [_vp, _d, _d, _d, _d, _d, _i64, _i64, _i64, _d],
+    "xts_mean_psf": [_vp, _d, _d, _d, _i64, _i64, _d],
+    "xts_normalize": [_vp, _d, _d, _i64, _i64],
     "xts_point_src_model_map_wcs": [_vp, C.POINTER(Wcs), _d, _i64, _d, _d, _d, _i64, _d, _i64,
                                     C.c_int32, _vp],
     "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,

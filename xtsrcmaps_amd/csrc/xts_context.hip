@@ -66,60 +66,6 @@ int to_device(xts_ctx* ctx, int slot, const void* host, size_t bytes, void** dev
     return XTS_OK;
 }
 
-Wcs wcs_setup(const xts_wcs& in) {
-    Wcs w{};
-    w.crpix1 = in.crpix1;
-    w.crpix2 = in.crpix2;
-    w.proj = in.proj;
-    w.nh = (long)in.naxis1;
-    w.nw = (long)in.naxis2;
-    // CROTA2 -> linear transform: x = cdelt1 cos r dp1 - cdelt2 sin r dp2,
-    // y = cdelt1 sin r dp1 + cdelt2 cos r dp2 (Greisen & Calabretta 2002).
-    const double cr = cosd(in.crota2), sr = sind(in.crota2);
-    w.m11 = in.cdelt1 * cr;
-    w.m12 = -in.cdelt2 * sr;
-    w.m21 = in.cdelt1 * sr;
-    w.m22 = in.cdelt2 * cr;
-    const double det = w.m11 * w.m22 - w.m12 * w.m21;
-    w.i11 = w.m22 / det;
-    w.i12 = -w.m12 / det;
-    w.i21 = -w.m21 / det;
-    w.i22 = w.m11 / det;
-    // Celestial pole of the native frame for a reference point at native
-    // (phi0, theta0) = (0, 0): default LONPOLE is 0 when delta0 >= theta0,
-    // else 180; of the two pole latitudes choose the one nearest LATPOLE=90.
-    const double a0 = in.crval1 > 180.0 ? in.crval1 - 360.0 : in.crval1;
-    const double d0 = in.crval2;
-    const double phip = d0 >= 0.0 ? 0.0 : 180.0;
-    const double x = cosd(phip), y = 0.0;
-    const double z = sqrt(x * x + y * y);
-    const double u = atan2(y, x) * kR2D;
-    const double v = acos(sind(d0) / z) * kR2D;
-    double c1 = u + v, c2 = u - v;
-    if (c1 > 180.0) c1 -= 360.0; else if (c1 < -180.0) c1 += 360.0;
-    if (c2 > 180.0) c2 -= 360.0; else if (c2 < -180.0) c2 += 360.0;
-    const bool ok1 = fabs(c1) <= 90.0 + 1e-10, ok2 = fabs(c2) <= 90.0 + 1e-10;
-    double dp = (ok1 && ok2) ? (fabs(c1 - 90.0) < fabs(c2 - 90.0) ? c1 : c2) : (ok1 ? c1 : c2);
-    dp = dp > 90.0 ? 90.0 : (dp < -90.0 ? -90.0 : dp);
-    double ap;
-    const double zz = cosd(dp) * cosd(d0);
-    if (fabs(zz) < 1e-10) {
-        if (fabs(cosd(d0)) < 1e-10) ap = a0;
-        else if (dp > 0) ap = a0 + phip - 180.0;
-        else ap = a0 - phip;
-    } else {
-        const double xx = (0.0 - sind(dp) * sind(d0)) / zz;
-        const double yy = sind(phip) / cosd(d0);
-        ap = a0 - atan2(yy, xx) * kR2D;
-    }
-    w.alphap = ap;
-    w.deltap = dp;
-    w.phip = phip;
-    w.sdp = sind(dp);
-    w.cdp = cosd(dp);
-    return w;
-}
-
 // Published Festa-Sommariva low-cardinality square rule of degree 7
 // (cardinality 12, all points inside, positive weights), as used by the
 // reference's production cubature (mm_cubature/cubature.hpp:209-225).

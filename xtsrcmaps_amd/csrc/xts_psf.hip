@@ -86,7 +86,8 @@ __host__ __device__ inline bool lerp_pars_psf(const double* g, int n, double v, 
 
 // psf/psf.bilerp.cxx:6-52 times the effective area
 // (psf.corrected_exposure_psf.cxx:37-44): PA[c, d, e] = bilerp(K)[c, d, e] x
-// aeff[c, e].  One thread per (c, d, e).
+// aeff[c, e] (aeff null: the bilinear samples alone).  One thread per
+// (c, d, e).
 __global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const double* __restrict__ costh,
                                    const double* __restrict__ logE, const double* __restrict__ pc,
                                    const double* __restrict__ pe, const double* __restrict__ K,
@@ -109,7 +110,7 @@ __global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const double* _
         const double* Kd = K + (size_t)d * Mc * Me;
         const double v = ccw * ecw * Kd[(ci - 1) * Me + ei - 1] + ccw * ew * Kd[(ci - 1) * Me + ei] +
                          cw * ecw * Kd[ci * Me + ei - 1] + cw * ew * Kd[ci * Me + ei];
-        out[k] = v * aeff[(size_t)c * E + e];
+        out[k] = aeff ? v * aeff[(size_t)c * E + e] : v;
     }
 }
 
@@ -121,6 +122,9 @@ __global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const double* _
 // of its sources staged in LDS.
 constexpr int kSrcTile = 8;
 
+// kMean false: one section only, out = (ECB . PA) LTF0 + (WECB . PA) LTF1
+// (psf.corrected_exposure_psf.cxx:46-117 alone).
+template <bool kMean>
 __global__ void __launch_bounds__(256)
 mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
                 const double* __restrict__ wecb, const double* __restrict__ paf,
@@ -143,7 +147,7 @@ mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
 #pragma unroll
     for (int r = 0; r < kSrcTile; ++r) ef[r] = wf[r] = eb[r] = wb[r] = 0.0;
     for (int c = 0; c < C; ++c) {
-        const double f = paf[(size_t)c * N + n], b = pab[(size_t)c * N + n];
+        const double f = paf[(size_t)c * N + n], b = kMean ? pab[(size_t)c * N + n] : 0.0;
 #pragma unroll
         for (int r = 0; r < kSrcTile; ++r) {
             const double x = sh[r * C + c], y = sh[(kSrcTile + r) * C + c];
@@ -159,8 +163,12 @@ mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
         if (r < ns) {
             const int s = s0 + r;
             const double front = ef[r] * l0 + wf[r] * l1;
-            const double back = eb[r] * l0 + wb[r] * l1;
-            out[(size_t)s * N + n] = (front + back) / expo[(size_t)s * E + e];
+            if (kMean) {
+                const double back = eb[r] * l0 + wb[r] * l1;
+                out[(size_t)s * N + n] = (front + back) / expo[(size_t)s * E + e];
+            } else {
+                out[(size_t)s * N + n] = front;
+            }
         }
     }
 }
@@ -280,6 +288,43 @@ struct PsfDevGrid {
     int Mc, Me;
 };
 
+// psf.corrected_exposure_psf.cxx:37-44: PA[c, d, e] = psf[c, d, e] aeff[c, e]
+__global__ void psf_aeff_kernel(int C, int E, const double* __restrict__ psf,
+                                const double* __restrict__ aeff, double* __restrict__ out) {
+    const size_t n = (size_t)C * kND * E;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int e = (int)(k % E);
+        const int c = (int)(k / ((size_t)E * kND));
+        out[k] = psf[k] * aeff[(size_t)c * E + e];
+    }
+}
+
+// psf.mean_psf.cxx:6-33: (front + back) / exposure[s, e]
+__global__ void mean_kernel(int S, int E, const double* __restrict__ front,
+                            const double* __restrict__ back, const double* __restrict__ expo,
+                            double* __restrict__ out) {
+    const size_t n = (size_t)S * kND * E;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int e = (int)(k % E);
+        const size_t s = k / ((size_t)E * kND);
+        out[k] = (front[k] + back[k]) / expo[s * E + e];
+    }
+}
+
+// psf.normalize.cxx:7-24: uPsf[s, d, e] /= total[s, e]
+__global__ void normalize_kernel(int S, int E, double* __restrict__ upsf,
+                                 const double* __restrict__ total) {
+    const size_t n = (size_t)S * kND * E;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int e = (int)(k % E);
+        const size_t s = k / ((size_t)E * kND);
+        upsf[k] /= total[s * E + e];
+    }
+}
+
 static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* costh, int C,
                              const double* logE, int E, const double* af, const double* ab,
                              const double* ecb, const double* wecb, const double* ltf,
@@ -313,7 +358,7 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
                                                     g[1].cosths, g[1].logEs, Kb, ab, PAb, err_dev);
     if (precision == XTS_PREC_FP64) {
         const size_t sm = 2 * kSrcTile * C * 8;
-        mean_psf_kernel<<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
+        mean_psf_kernel<true><<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
             S, C, E, ecb, wecb, PAf, PAb, ltf, expo, upsf);
     } else {
         void* w2;
@@ -471,10 +516,7 @@ int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S, int64_t E,
     XTS_TRY(activate(ctx));
     XTS_CHECK(upsf && parint && S > 0 && E > 0, "bad arguments");
     const size_t n = (size_t)S * kND * E;
-    double sep[kND];
-    separations_host(sep);
-    void *dsep, *du, *dout;
-    XTS_TRY(to_device(ctx, kSlotWork3, sep, sizeof(sep), &dsep));
+    void *du, *dout;
     XTS_TRY(to_device(ctx, kSlotIn0, upsf, n * 8, &du));
     XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8 + S * E * 8, &dout));
     double* dpar = (double*)dout;
@@ -483,13 +525,100 @@ int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S, int64_t E,
     theta_table(tab);
     void* dtab;
     XTS_TRY(to_device(ctx, kSlotTab, tab, sizeof(tab), &dtab));
-    (void)dsep;
     integral_kernel<<<dim3(div_up(E, 4), (unsigned)S), 256, 0, ctx->stream>>>(
         (int)S, (int)E, (double*)du, dpar, dtot, (double*)dtab);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(upsf, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipMemcpyAsync(parint, dpar, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     if (total) XTS_HIP(hipMemcpyAsync(total, dtot, S * E * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+int xts_bilerp(xts_ctx* ctx, const double* costh, int64_t C, const double* logE, int64_t E,
+               const xts_psf_grid* grid, const double* kings, double* out) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(costh && logE && grid && grid->cosths && grid->logEs && kings && out, "null pointer");
+    XTS_CHECK(C > 0 && E > 0 && grid->Mc > 1 && grid->Me > 1, "bad sizes");
+    const size_t nk = (size_t)kND * grid->Mc * grid->Me, n = (size_t)C * kND * E;
+    void *dc, *de, *dpc, *dpe, *dk, *dout, *derr;
+    XTS_TRY(to_device(ctx, kSlotIn0, costh, C * 8, &dc));
+    XTS_TRY(to_device(ctx, kSlotIn1, logE, E * 8, &de));
+    XTS_TRY(to_device(ctx, kSlotIn2, grid->cosths, grid->Mc * 8, &dpc));
+    XTS_TRY(to_device(ctx, kSlotIn3, grid->logEs, grid->Me * 8, &dpe));
+    XTS_TRY(to_device(ctx, kSlotIn4, kings, nk * 8, &dk));
+    XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8, &dout));
+    const int zero = 0;
+    XTS_TRY(to_device(ctx, kSlotOut3, &zero, sizeof(int), &derr));
+    bilerp_aeff_kernel<<<std::min<unsigned>(div_up(n, 256), 16384), 256, 0, ctx->stream>>>(
+        (int)C, (int)E, (int)grid->Mc, (int)grid->Me, (double*)dc, (double*)de, (double*)dpc,
+        (double*)dpe, (double*)dk, nullptr, (double*)dout, (int*)derr);
+    XTS_HIP(hipGetLastError());
+    int err = 0;
+    XTS_HIP(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) return fail(XTS_ERR_RANGE, "bilerp: a sample lies outside the PSF IRF grid");
+    return XTS_OK;
+}
+
+int xts_corrected_exposure_psf(xts_ctx* ctx, const double* obs_psf, const double* obs_aeff,
+                               const double* ecb, const double* wecb, const double* front_ltf,
+                               int64_t S, int64_t C, int64_t E, double* out) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(obs_psf && obs_aeff && ecb && wecb && front_ltf && out, "null pointer");
+    XTS_CHECK(S > 0 && C > 0 && E > 0 && (int64_t)kND * E < (1LL << 30), "bad sizes");
+    const size_t npa = (size_t)C * kND * E, nout = (size_t)S * kND * E;
+    void *dpsf, *daeff, *decb, *dwecb, *dltf, *dpa, *dout;
+    XTS_TRY(to_device(ctx, kSlotIn0, obs_psf, npa * 8, &dpsf));
+    XTS_TRY(to_device(ctx, kSlotIn1, obs_aeff, C * E * 8, &daeff));
+    XTS_TRY(to_device(ctx, kSlotIn2, ecb, S * C * 8, &decb));
+    XTS_TRY(to_device(ctx, kSlotIn3, wecb, S * C * 8, &dwecb));
+    XTS_TRY(to_device(ctx, kSlotIn4, front_ltf, 2 * E * 8, &dltf));
+    XTS_TRY(slot_reserve(ctx, kSlotWork0, npa * 8, &dpa));
+    XTS_TRY(slot_reserve(ctx, kSlotOut0, nout * 8, &dout));
+    psf_aeff_kernel<<<std::min<unsigned>(div_up(npa, 256), 16384), 256, 0, ctx->stream>>>(
+        (int)C, (int)E, (double*)dpsf, (double*)daeff, (double*)dpa);
+    const int N = kND * (int)E;
+    const size_t sm = 2 * kSrcTile * C * 8;
+    mean_psf_kernel<false><<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
+        (int)S, (int)C, (int)E, (double*)decb, (double*)dwecb, (double*)dpa, nullptr,
+        (double*)dltf, nullptr, (double*)dout);
+    XTS_HIP(hipGetLastError());
+    XTS_HIP(hipMemcpyAsync(out, dout, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+int xts_mean_psf(xts_ctx* ctx, const double* front, const double* back, const double* exposure,
+                 int64_t S, int64_t E, double* out) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(front && back && exposure && out && S > 0 && E > 0, "bad arguments");
+    const size_t n = (size_t)S * kND * E;
+    void *df, *db, *dx, *dout;
+    XTS_TRY(to_device(ctx, kSlotIn0, front, n * 8, &df));
+    XTS_TRY(to_device(ctx, kSlotIn1, back, n * 8, &db));
+    XTS_TRY(to_device(ctx, kSlotIn2, exposure, S * E * 8, &dx));
+    XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8, &dout));
+    mean_kernel<<<std::min<unsigned>(div_up(n, 256), 16384), 256, 0, ctx->stream>>>(
+        (int)S, (int)E, (double*)df, (double*)db, (double*)dx, (double*)dout);
+    XTS_HIP(hipGetLastError());
+    XTS_HIP(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XTS_HIP(hipStreamSynchronize(ctx->stream));
+    return XTS_OK;
+}
+
+int xts_normalize(xts_ctx* ctx, double* upsf, const double* total, int64_t S, int64_t E) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(upsf && total && S > 0 && E > 0, "bad arguments");
+    const size_t n = (size_t)S * kND * E;
+    void *du, *dt;
+    XTS_TRY(to_device(ctx, kSlotIn0, upsf, n * 8, &du));
+    XTS_TRY(to_device(ctx, kSlotIn1, total, S * E * 8, &dt));
+    normalize_kernel<<<std::min<unsigned>(div_up(n, 256), 16384), 256, 0, ctx->stream>>>(
+        (int)S, (int)E, (double*)du, (double*)dt);
+    XTS_HIP(hipGetLastError());
+    XTS_HIP(hipMemcpyAsync(upsf, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipStreamSynchronize(ctx->stream));
     return XTS_OK;
 }

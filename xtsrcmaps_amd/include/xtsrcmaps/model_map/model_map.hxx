@@ -15,10 +15,6 @@
 
 namespace Fermi::ModelMap {
 
-template <Source::SourceConcept T>
-auto compute_srcmaps(Obs::XtObs const& obs, Source::SourceData<T> src,
-                     Exposure::XtExp const& exp, Psf::XtPsf const& psf) -> Tensor<double, 4>;
-
 auto point_src_model_map_wcs(size_t Nh, size_t Nw, SkyGeom<double> const& skygeom,
                              Tensor<double, 2> const& src_sph,
                              std::vector<std::string> const& src_names,
@@ -39,5 +35,13 @@ auto psf_boundary_radius(size_t Nh, size_t Nw, Tensor<double, 2> const& src_sph,
 auto map_integral(Tensor<double, 4> const& model_map, Tensor<double, 2> const& src_sph,
                   SkyGeom<double> const& skygeom, Tensor<double, 1> const& psf_radius,
                   std::vector<bool> const& is_in_fov) -> Tensor<double, 2>;
+
+// model_map.compute_src_maps.cxx:6-22
+template <Source::SourceConcept T>
+auto compute_srcmaps(Obs::XtObs const& obs, Source::SourceData<T> src,
+                     Exposure::XtExp const& exp, Psf::XtPsf const& psf) -> Tensor<double, 4> {
+    return point_src_model_map_wcs((size_t)obs.Nh, (size_t)obs.Nw, obs.skygeom, src.sph_locs,
+                                   src.names, exp.exposure, psf.uPsf, psf.partial_psf_integral);
+}
 
 }  // namespace Fermi::ModelMap

@@ -29,7 +29,27 @@ inline auto separations(double xmin = 1e-4, double xmax = 70.) -> SepArr {
     return sep;
 }
 
+// separation of fractional node index x and its inverse (psf/psf.hxx:33-43)
+inline auto separation(double x) -> double {
+    const double delta = std::exp(std::log(7e5) / 399.);
+    return 1e-4 * (x >= 1. ? std::pow(delta, x - 1.) : x);
+}
+inline auto inverse_separation(double s) -> double {
+    return s < 1e-4 ? 1e4 * s : 1. + std::log(s * 1e4) / (std::log(7e5) / 399.);
+}
+
 auto fast_separation_lower_index(Tensor<double, 1> seps) -> Tensor<int, 1>;
+auto bilerp(std::vector<double> const& costhetas, std::vector<double> const& logEs,
+            Tensor<double, 1> const& par_cosths, Tensor<double, 1> const& par_logEs,
+            Tensor<double, 3> const& kings) -> Tensor<double, 3>;  // [C, 401, E]
+auto corrected_exposure_psf(Tensor<double, 3> const& obs_psf,  // [C, 401, E]
+                            Tensor<double, 2> const& obs_aeff,  // [C, E]
+                            Tensor<double, 2> const& src_exposure_cosbins,
+                            Tensor<double, 2> const& src_weighted_exposure_cosbins,
+                            Tensor<double, 2> const& front_LTF) -> Tensor<double, 3>;
+auto mean_psf(Tensor<double, 3> const& front_corrected_psf,
+              Tensor<double, 3> const& back_corrected_psf,
+              Tensor<double, 2> const& exposures) -> Tensor<double, 3>;  // [S, 401, E]
 auto king(Irf::psf::Data const& data) -> Tensor<double, 3>;  // [401, Mc, Me]
 auto partial_total_integral(Tensor<double, 3> const& mean_psf)
     -> std::pair<Tensor<double, 3>, Tensor<double, 2>>;

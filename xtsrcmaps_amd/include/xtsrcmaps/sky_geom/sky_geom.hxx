@@ -62,6 +62,10 @@ class SkyGeom {
         const T x = L[0] - R[0], y = L[1] - R[1], z = L[2] - R[2];
         return T(2) * std::asin(T(0.5) * std::sqrt(x * x + y * y + z * z));
     }
+    static auto pix_diff(std::array<T, 2> const& L, std::array<T, 2> const& R,
+                         SkyGeom const& skygeom) -> T {
+        return dir_diff(skygeom.pix2dir(L), skygeom.pix2dir(R));
+    }
     auto wcs_config() const -> WcsConfig;
 
     // the C-ABI description of this geometry
