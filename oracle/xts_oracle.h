@@ -9,6 +9,26 @@
  *
  * Every function cites the reference file:line range it restates.  Arrays are
  * row-major, sizes in elements.  All arithmetic is IEEE double.
+ *
+ * Pins (tests/test_oracle_golden.py, tests/test_oracle_invariants.py): the
+ * reference's fixtures pin the separations, prepare_grid, normalize_rpsf,
+ * king, LTF, costhetas, aeff_value, the exposure contractions, the mean-PSF
+ * peak, the pixel cubature maps (logsep, searchsep, three rules) and the
+ * FermiTools in-FOV flags and radii; closed forms and invariants pin
+ * ang2pix, CAR, the WCS round trip and CROTA2, the Gauss-Legendre rule, the
+ * planar King integral, the cumulative radial integral, constant-LUT
+ * cubature, logsep == searchsep inside the grid, a brute-force pixel
+ * integral, and map_integral / map_correction_factor through the
+ * constant-LUT pipeline identity at a 1-degree radius.
+ *
+ * Parity unpinned (restated from the cited lines, no independent check):
+ *   - or_solid_angle: the neighbouring-centre quad of math/solid_angle.hpp:
+ *     50-75 is only checked against the exact CAR pixel area to O(pixel^2),
+ *     which a corner quad would pass too;
+ *   - or_map_correction_factor for radii below 1 degree (the index-1 branch
+ *     of psf.fast_separation_lower_index.cxx:15-18);
+ *   - or_pixel_logsep for separations outside [1e-4, 70] degrees (the
+ *     extrapolation of convolve_pixel_psf_logsep.hpp:46-54).
  */
 #ifndef XTS_ORACLE_H
 #define XTS_ORACLE_H
