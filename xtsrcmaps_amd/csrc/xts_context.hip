@@ -2,6 +2,8 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <cmath>
+
 #include "xts_internal.hpp"
 
 namespace xts {
@@ -46,6 +48,38 @@ int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out) {
         s.bytes = want;
     }
     *out = s.ptr;
+    return XTS_OK;
+}
+
+int const_tables(xts_ctx* ctx, const xts_const_tables** out) {
+    if (!ctx->consts_ready) {
+        // separations (psf/psf.hxx:24-31), 2 R2D / sep, theta and sin theta
+        std::vector<double> h(kND * 4 + kND / 2 + 1);
+        double* sep = h.data();
+        double* inv = sep + kND;
+        double* th = inv + kND;
+        float* inv32 = reinterpret_cast<float*>(th + 2 * kND);
+        const double xstep = std::log(70.0 / 1e-4) / 399.0;
+        sep[0] = 0.0;
+        for (int i = 0; i < 400; ++i) sep[i + 1] = 1e-4 * std::exp(i * xstep);
+        for (int i = 0; i < kND; ++i) {
+            inv[i] = sep[i] > 0 ? 2.0 * kR2D / sep[i] : 0.0;
+            inv32[i] = (float)inv[i];
+            th[i] = kD2R * sep[i];
+            th[kND + i] = std::sin(th[i]);
+        }
+        void* d;
+        XTS_TRY(slot_reserve(ctx, kSlotConst, h.size() * 8, &d));
+        // a blocking copy: done once, before any kernel reads the tables
+        XTS_HIP(hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        double* dd = (double*)d;
+        ctx->consts.sep = dd;
+        ctx->consts.inv64 = dd + kND;
+        ctx->consts.theta = dd + 2 * kND;
+        ctx->consts.inv32 = reinterpret_cast<float*>(dd + 4 * kND);
+        ctx->consts_ready = true;
+    }
+    *out = &ctx->consts;
     return XTS_OK;
 }
 
@@ -132,7 +166,14 @@ int xts_ctx_destroy(xts_ctx* ctx) {
 
 int xts_ctx_set_stream(xts_ctx* ctx, void* stream) {
     XTS_CHECK(ctx != nullptr, "null context");
-    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    hipStream_t next = stream ? (hipStream_t)stream : ctx->own_stream;
+    if (next != ctx->stream) {
+        // cached geometry and rule were written in the old stream's order:
+        // rebuild them in the new one's
+        ctx->geom_valid = false;
+        ctx->rule.dev = nullptr;
+    }
+    ctx->stream = next;
     return XTS_OK;
 }
 

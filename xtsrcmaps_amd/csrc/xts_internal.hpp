@@ -63,7 +63,7 @@ enum SlotId {
     kSlotIn0 = 0, kSlotIn1, kSlotIn2, kSlotIn3, kSlotIn4, kSlotIn5,
     kSlotIn6, kSlotIn7, kSlotOut0, kSlotOut1, kSlotOut2, kSlotOut3,
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
-    kSlotWork3, kSlotTab, kSlotRows, kSlotCount
+    kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotCount
 };
 
 }  // namespace xts
@@ -71,6 +71,22 @@ enum SlotId {
 struct xts_timing_rec {
     const char* name;
     hipEvent_t start, stop;
+};
+
+// Device copies of constant tables, uploaded once per context.
+struct xts_const_tables {
+    double* sep = nullptr;       // [401] separations (deg)
+    double* inv64 = nullptr;     // [401] 2 R2D / sep (0 at node 0)
+    float* inv32 = nullptr;      // [401] the same in FP32
+    double* theta = nullptr;     // [2 * 401] theta (rad), then sin theta
+};
+
+// The last cubature rule uploaded, with the quantities derived from it.
+struct xts_rule_cache {
+    std::vector<double> host;    // [N, 3] as given
+    void* dev = nullptr;         // per-point records on the device
+    double rho_scale = 0.0, wsum = 0.0;
+    int nu = 0, nv = 0;
 };
 
 struct xts_ctx {
@@ -83,11 +99,19 @@ struct xts_ctx {
     int tune_tiles_per_block = 0;          // 0: chosen per call (row-weight pass)
     std::vector<xts_timing_rec> timing_recs;
     std::vector<hipEvent_t> event_pool;
+    bool consts_ready = false;
+    xts_const_tables consts;
+    xts_rule_cache rule;
+    bool geom_valid = false;               // pixel geometry in kSlotGeom is for geom_wcs
+    xts_wcs geom_wcs{};
 };
 
 namespace xts {
 
 int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out);
+
+// Constant tables on the device (uploaded on the first call).
+int const_tables(xts_ctx* ctx, const xts_const_tables** out);
 
 // Brackets kernel launches with events on the context stream when timing is
 // enabled (xts_ctx_enable_timing); no effect otherwise.

@@ -15,6 +15,8 @@
 //                        x exposure x correction
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "xts_internal.hpp"
@@ -996,12 +998,25 @@ __global__ void near_scale_kernel(CubArgs a) {
     }
 }
 
+// Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
+// and size: the attribute belongs to the function).
+static int allow_lds(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> done;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& d : done)
+        if (d.first == fn && d.second >= bytes) return XTS_OK;
+    XTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done.emplace_back(fn, bytes);
+    return XTS_OK;
+}
+
 template <typename T, int EC, typename OT, bool SCALED>
 static int launch_emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid, bool sehw) {
     const size_t sm = (size_t)kND * (sehw ? EC + 1 : EC) * sizeof(T);
     const void* fn = sehw ? (const void*)emit_sehw_kernel<T, EC, OT, SCALED>
                           : (const void*)emit_shwe_kernel<T, EC, OT, SCALED>;
-    XTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    XTS_TRY(allow_lds(fn, sm));
     if (sehw) emit_sehw_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
     else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
     XTS_HIP(hipGetLastError());
@@ -1134,54 +1149,30 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
                           ImageGeom* ig) {
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     const size_t ncorn = (size_t)(H + 1) * (W + 1);
-    const size_t bytes = ncorn * 3 * 8 * 2 + (size_t)H * W * 8 + (size_t)std::max(S, 1) * sizeof(SrcGeom) + 1024;
+    const size_t bytes = ncorn * 3 * 8 * 2 + (size_t)H * W * 8;
     void* base;
     XTS_TRY(slot_reserve(ctx, kSlotGeom, bytes, &base));
     char* p = (char*)base;
     ig->corners = (double*)p; p += ncorn * 3 * 8;
     ig->centres = (double*)p; p += ncorn * 3 * 8;
-    ig->omega = (double*)p; p += (size_t)H * W * 8;
-    p = (char*)(((uintptr_t)p + 255) & ~uintptr_t(255));
-    ig->geo = (SrcGeom*)p;
+    ig->omega = (double*)p;
+    void* g;
+    XTS_TRY(slot_reserve(ctx, kSlotSrcGeo, (size_t)std::max(S, 1) * sizeof(SrcGeom), &g));
+    ig->geo = (SrcGeom*)g;
     const Wcs w = wcs_setup(*wcs);
-    geometry_kernel<<<std::min<unsigned>(div_up(ncorn, 256), 4096), 256, 0, ctx->stream>>>(
-        w, H, W, ig->corners, ig->centres);
-    solid_angle_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0, ctx->stream>>>(
-        H, W, ig->centres, ig->omega);
+    // the pixel geometry depends on the WCS alone: recomputed only when the
+    // image changes (the slot keeps the last image's)
+    if (!ctx->geom_valid || std::memcmp(&ctx->geom_wcs, wcs, sizeof(xts_wcs)) != 0) {
+        geometry_kernel<<<std::min<unsigned>(div_up(ncorn, 256), 4096), 256, 0, ctx->stream>>>(
+            w, H, W, ig->corners, ig->centres);
+        solid_angle_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
+                             ctx->stream>>>(H, W, ig->centres, ig->omega);
+        ctx->geom_wcs = *wcs;
+        ctx->geom_valid = true;
+    }
     if (S > 0 && src_sph_dev)
         radius_kernel<<<S, 256, 0, ctx->stream>>>(w, H, W, S, src_sph_dev, ig->corners, ig->geo);
     XTS_HIP(hipGetLastError());
-    return XTS_OK;
-}
-
-// Separation tables (FP64 and FP32 inverse) in the kSlotWork3 slot.
-struct SepTables {
-    double* sep;
-    double* inv64;
-    float* inv32;
-};
-
-static int sep_tables(xts_ctx* ctx, SepTables* t) {
-    static std::vector<char> host;
-    if (host.empty()) {
-        host.resize(kND * (8 + 8 + 4));
-        double* sep = (double*)host.data();
-        double* inv = sep + kND;
-        float* inv32 = (float*)(inv + kND);
-        const double xstep = std::log(70.0 / 1e-4) / 399.0;
-        sep[0] = 0.0;
-        for (int i = 0; i < 400; ++i) sep[i + 1] = 1e-4 * std::exp(i * xstep);
-        // 2 R2D / sep: multiplies asin(chord / 2) into s / sep
-        for (int i = 0; i < kND; ++i) {
-            inv[i] = sep[i] > 0 ? 2.0 * kR2D / sep[i] : 0.0;
-            inv32[i] = (float)inv[i];
-        }
-    }
-    void* d;
-    XTS_TRY(to_device(ctx, kSlotWork3, host.data(), host.size(), &d));
-    t->sep = (double*)d;
-    t->inv64 = t->sep + kND;
-    t->inv32 = (float*)(t->inv64 + kND);
     return XTS_OK;
 }
 
@@ -1199,43 +1190,56 @@ __global__ void lut_convert_kernel(const double* __restrict__ upsf, int S, int E
     }
 }
 
+// The rule as per-point records on the device, with its bilinear-weight
+// bound, weight sum and grid structure; re-uploaded only when its content
+// changes.
 static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** dev,
                        double* rho_scale, double* wsum, int* tnu, int* tnv) {
     const double* r = rule ? rule : kSquareDeg7;
     if (!rule) N = 12;
     XTS_CHECK(N > 0 && N <= 1024, "rule size %lld out of range (1..1024)", (long long)N);
-    std::vector<RuleUVW> rp((size_t)N);
-    double rs = 1.0;
-    for (int64_t p = 0; p < N; ++p) {
-        const double u = r[3 * p], v = r[3 * p + 1];
-        rp[p] = {u, v, 0.25 * r[3 * p + 2]};
-        // sum of |bilinear corner weights|: 1 inside the square
-        const double b = 0.25 * (std::fabs((1 - u) * (1 - v)) + std::fabs((1 + u) * (1 - v)) +
-                                 std::fabs((1 + u) * (1 + v)) + std::fabs((1 - u) * (1 + v)));
-        rs = std::max(rs, b);
+    xts_rule_cache& rc = ctx->rule;
+    if (!(rc.dev && rc.host.size() == (size_t)(3 * N) &&
+          std::memcmp(rc.host.data(), r, (size_t)(3 * N) * sizeof(double)) == 0)) {
+        std::vector<RuleUVW> rp((size_t)N);
+        double rs = 1.0;
+        for (int64_t p = 0; p < N; ++p) {
+            const double u = r[3 * p], v = r[3 * p + 1];
+            rp[p] = {u, v, 0.25 * r[3 * p + 2]};
+            // sum of |bilinear corner weights|: 1 inside the square
+            const double b = 0.25 * (std::fabs((1 - u) * (1 - v)) + std::fabs((1 + u) * (1 - v)) +
+                                     std::fabs((1 + u) * (1 + v)) + std::fabs((1 - u) * (1 + v)));
+            rs = std::max(rs, b);
+        }
+        double ws = 0.0;
+        for (auto& q : rp) ws += q.wq;
+        // grid rule in row-major order: u depends on p / nv only, v on p % nv only
+        int nu = 0, nv = 0;
+        for (int64_t p = 1; p < N && nv == 0; ++p)
+            if (rp[p].u != rp[0].u) nv = (int)p;
+        if (nv > 0 && N % nv == 0 && nv % 4 == 0) {
+            nu = (int)(N / nv);
+            bool grid = true;
+            for (int64_t p = 0; p < N && grid; ++p)
+                grid = rp[p].u == rp[(p / nv) * nv].u && rp[p].v == rp[p % nv].v;
+            if (!grid) nu = nv = 0;
+        } else {
+            nv = 0;
+        }
+        void* d;
+        XTS_TRY(to_device(ctx, kSlotRule, rp.data(), rp.size() * sizeof(RuleUVW), &d));
+        rc.host.assign(r, r + 3 * N);
+        rc.dev = d;
+        rc.rho_scale = rs;
+        rc.wsum = ws;
+        rc.nu = nu;
+        rc.nv = nv;
     }
-    double ws = 0.0;
-    for (auto& q : rp) ws += q.wq;
-    // grid rule in row-major order: u depends on p / nv only, v on p % nv only
-    int nu = 0, nv = 0;
-    for (int64_t p = 1; p < N && nv == 0; ++p)
-        if (rp[p].u != rp[0].u) nv = (int)p;
-    if (nv > 0 && N % nv == 0 && nv % 4 == 0) {
-        nu = (int)(N / nv);
-        bool grid = true;
-        for (int64_t p = 0; p < N && grid; ++p)
-            grid = rp[p].u == rp[(p / nv) * nv].u && rp[p].v == rp[p % nv].v;
-        if (!grid) nu = nv = 0;
-    } else {
-        nv = 0;
-    }
-    void* d;
-    XTS_TRY(to_device(ctx, kSlotWork2, rp.data(), rp.size() * sizeof(RuleUVW), &d));
-    *dev = (RuleUVW*)d;
-    *rho_scale = rs;
-    *wsum = ws;
-    *tnu = nu;
-    *tnv = nv;
+    *dev = (RuleUVW*)rc.dev;
+    *rho_scale = rc.rho_scale;
+    *wsum = rc.wsum;
+    *tnu = rc.nu;
+    *tnv = rc.nv;
     return XTS_OK;
 }
 
@@ -1261,8 +1265,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
 
     ImageGeom ig;
     XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig));
-    SepTables st;
-    XTS_TRY(sep_tables(ctx, &st));
+    const xts_const_tables* ct;
+    XTS_TRY(const_tables(ctx, &ct));
     RuleUVW* rp;
     double rho_scale, wsum;
     int tnu, tnv;
@@ -1300,8 +1304,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.geo = ig.geo;
     ca.lut = lut;
     ca.uvw = rp;
-    ca.inv_sep = st.inv32;
-    ca.inv_sep64 = st.inv64;
+    ca.inv_sep = ct->inv32;
+    ca.inv_sep64 = ct->inv64;
     ca.rw = rows;
     ca.rw_row = (int*)((char*)rows + rw_bytes);
     ca.near_list = ca.rw_row + (size_t)S * plane;
@@ -1353,12 +1357,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         const dim3 grid((unsigned)gx, (unsigned)S, (unsigned)nchunks);
         TimedRegion tr(ctx, "near");
         if (f64) {
-            XTS_HIP(hipFuncSetAttribute((const void*)near_pixel_kernel<double, 16>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            XTS_TRY(allow_lds((const void*)near_pixel_kernel<double, 16>, sm));
             near_pixel_kernel<double, 16><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
         } else {
-            XTS_HIP(hipFuncSetAttribute((const void*)near_pixel_kernel<float, 40>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            XTS_TRY(allow_lds((const void*)near_pixel_kernel<float, 40>, sm));
             near_pixel_kernel<float, 40><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
         }
         XTS_HIP(hipGetLastError());
@@ -1369,7 +1371,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>((int)E, ca.hist, upsf,
                                                                             ca.mapint);
         factor_kernel<<<div_up(nse, 256), 256, 0, ctx->stream>>>((int)S, (int)E, ig.geo, ca.mapint,
-                                                                  nullptr, upsf, parint, st.sep,
+                                                                  nullptr, upsf, parint, ct->sep,
                                                                   expo, fac, nullptr);
         XTS_HIP(hipGetLastError());
     }
@@ -1531,11 +1533,11 @@ int xts_map_correction_factor(xts_ctx* ctx, const double* invmi, const double* r
     XTS_TRY(to_device(ctx, kSlotIn1, upsf, (size_t)S * kND * E * 8, &dup));
     XTS_TRY(to_device(ctx, kSlotIn2, parint, (size_t)S * kND * E * 8, &dpar));
     XTS_TRY(to_device(ctx, kSlotIn3, invmi, S * E * 8, &dinv));
-    SepTables st;
-    XTS_TRY(sep_tables(ctx, &st));
+    const xts_const_tables* ct;
+    XTS_TRY(const_tables(ctx, &ct));
     XTS_TRY(slot_reserve(ctx, kSlotOut0, S * E * 8, &dout));
     factor_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>(
-        (int)S, (int)E, (SrcGeom*)dg, nullptr, (double*)dinv, (double*)dup, (double*)dpar, st.sep,
+        (int)S, (int)E, (SrcGeom*)dg, nullptr, (double*)dinv, (double*)dup, (double*)dpar, ct->sep,
         nullptr, (double*)dout, nullptr);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(out, dout, S * E * 8, hipMemcpyDeviceToHost, ctx->stream));

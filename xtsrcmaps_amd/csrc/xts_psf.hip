@@ -244,22 +244,6 @@ integral_kernel(int S, int E, double* __restrict__ upsf, double* __restrict__ pa
     }
 }
 
-static void separations_host(double* sep) {
-    const double xstep = std::log(70.0 / 1e-4) / 399.0;
-    sep[0] = 0.0;
-    for (int i = 0; i < 400; ++i) sep[i + 1] = 1e-4 * std::exp(i * xstep);
-}
-
-// [theta_d (rad), sin theta_d] for the radial integral
-static void theta_table(double* tab) {
-    double sep[kND];
-    separations_host(sep);
-    for (int d = 0; d < kND; ++d) {
-        tab[d] = kD2R * sep[d];
-        tab[kND + d] = std::sin(tab[d]);
-    }
-}
-
 // Gauss-Legendre nodes and weights by Newton iteration on P_n.
 static void legendre(int n, double* x, double* w) {
     for (int i = 1; i <= n; ++i) {
@@ -333,11 +317,9 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
     XTS_CHECK(precision == XTS_PREC_FP64 || precision == XTS_PREC_BF16X3, "bad precision %d",
               precision);
     const int N = kND * E;
-    // separations table
-    double sep[kND];
-    separations_host(sep);
-    void* dsep;
-    XTS_TRY(to_device(ctx, kSlotWork3, sep, sizeof(sep), &dsep));
+    const xts_const_tables* ct;
+    XTS_TRY(const_tables(ctx, &ct));
+    const double* dsep = ct->sep;
     // King tables and the aeff-weighted bilinear samples, front then back
     const size_t kf = (size_t)kND * g[0].Mc * g[0].Me, kb = (size_t)kND * g[1].Mc * g[1].Me;
     const size_t npa = (size_t)C * N;
@@ -348,9 +330,9 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
     double* PAf = Kb + kb;
     double* PAb = PAf + npa;
     king_kernel<<<std::min<unsigned>(div_up(kf, 256), 4096), 256, 0, ctx->stream>>>(
-        g[0].Mc, g[0].Me, g[0].params, (double*)dsep, Kf);
+        g[0].Mc, g[0].Me, g[0].params, dsep, Kf);
     king_kernel<<<std::min<unsigned>(div_up(kb, 256), 4096), 256, 0, ctx->stream>>>(
-        g[1].Mc, g[1].Me, g[1].params, (double*)dsep, Kb);
+        g[1].Mc, g[1].Me, g[1].params, dsep, Kb);
     const unsigned bb = std::min<unsigned>(div_up(npa, 256), 16384);
     bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[0].Mc, g[0].Me, costh, logE,
                                                     g[0].cosths, g[0].logEs, Kf, af, PAf, err_dev);
@@ -374,12 +356,8 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
         mean_psf_combine_kernel<<<std::min<unsigned>(div_up((int64_t)S * N, 256), 16384), 256, 0,
                                   ctx->stream>>>(S, E, P, ltf, expo, upsf);
     }
-    double tab[2 * kND];
-    theta_table(tab);
-    void* dtab;
-    XTS_TRY(to_device(ctx, kSlotTab, tab, sizeof(tab), &dtab));
     integral_kernel<<<dim3(div_up(E, 4), S), 256, 0, ctx->stream>>>(S, E, upsf, parint, nullptr,
-                                                                     (double*)dtab);
+                                                                     ct->theta);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -414,15 +392,14 @@ int xts_normalize_rpsf(xts_ctx* ctx, double* params, const double* logEs, int64_
 int xts_king(xts_ctx* ctx, const xts_psf_grid* g, double* out) {
     XTS_TRY(activate(ctx));
     XTS_CHECK(g && g->params && out && g->Mc > 0 && g->Me > 0, "bad arguments");
-    double sep[kND];
-    separations_host(sep);
-    void *dsep, *dp, *dout;
-    XTS_TRY(to_device(ctx, kSlotWork3, sep, sizeof(sep), &dsep));
+    const xts_const_tables* ct;
+    XTS_TRY(const_tables(ctx, &ct));
+    void *dp, *dout;
     XTS_TRY(to_device(ctx, kSlotIn0, g->params, g->Mc * g->Me * 6 * 8, &dp));
     const size_t n = (size_t)kND * g->Mc * g->Me;
     XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8, &dout));
     king_kernel<<<std::min<unsigned>(div_up(n, 256), 4096), 256, 0, ctx->stream>>>(
-        (int)g->Mc, (int)g->Me, (double*)dp, (double*)dsep, (double*)dout);
+        (int)g->Mc, (int)g->Me, (double*)dp, ct->sep, (double*)dout);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipStreamSynchronize(ctx->stream));
@@ -521,12 +498,10 @@ int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S, int64_t E,
     XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8 + S * E * 8, &dout));
     double* dpar = (double*)dout;
     double* dtot = dpar + n;
-    double tab[2 * kND];
-    theta_table(tab);
-    void* dtab;
-    XTS_TRY(to_device(ctx, kSlotTab, tab, sizeof(tab), &dtab));
+    const xts_const_tables* ct;
+    XTS_TRY(const_tables(ctx, &ct));
     integral_kernel<<<dim3(div_up(E, 4), (unsigned)S), 256, 0, ctx->stream>>>(
-        (int)S, (int)E, (double*)du, dpar, dtot, (double*)dtab);
+        (int)S, (int)E, (double*)du, dpar, dtot, ct->theta);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(upsf, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipMemcpyAsync(parint, dpar, n * 8, hipMemcpyDeviceToHost, ctx->stream));

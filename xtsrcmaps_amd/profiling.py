@@ -1,0 +1,145 @@
+"""rocprofv3 helpers behind the committed profile summaries.
+
+Workloads to profile (run under ``rocprofv3 ... -- python3 -m
+xtsrcmaps_amd.profiling <workload>``):
+
+    modelmap --config c1 --calls 3   the model-map passes on resident inputs
+    copy --mib 592 --calls 3         an FP32 copy kernel of known size: the
+                                     calibration of FETCH_SIZE / WRITE_SIZE
+
+Summaries of rocprofv3 CSV output (plain Python, no GPU):
+
+    stats <kernel_stats.csv> --cmd CMD           Markdown table of a --stats run
+    traffic --mm-fetch D --mm-write D --copy-fetch D --copy-write D
+            --copy-mib M --calls N --cmd CMD     JSON: HBM bytes per model map
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+import sys
+
+MM_KERNELS = r"rowweight_kernel|near_pixel_kernel|hist_integral_kernel|factor_kernel|emit_s|near_scale_kernel"
+COPY_KERNELS = r"elementwise|vectorized|copy"
+
+
+# ---------------------------------------------------------------- workloads
+def _modelmap(config: str, calls: int) -> None:
+    import torch
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    import xtsrcmaps_amd as xa
+    cfg, inp, ctx, d = bench.setup(config)
+    bench.step(ctx, inp, d)
+    torch.cuda.synchronize()
+    for _ in range(calls):
+        ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                        d["out"], inp.rule,
+                                        flags=xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32)
+    torch.cuda.synchronize()
+    print(f"modelmap: {calls} calls of {config}", flush=True)
+
+
+def _copy(mib: int, calls: int) -> None:
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(calls):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"copy: {calls} x {mib} MiB", flush=True)
+
+
+# ---------------------------------------------------------------- summaries
+def _one(pattern: str) -> str:
+    found = sorted(glob.glob(pattern, recursive=True))
+    if not found:
+        raise FileNotFoundError(pattern)
+    return found[0]
+
+
+def stats_markdown(path: str, cmd: str, top: int = 16) -> str:
+    rows = list(csv.DictReader(open(path)))
+    lines = [f"Command: `{cmd}`", "",
+             "| kernel | calls | avg µs | total µs | % |", "|---|---|---|---|---|"]
+    for r in rows[:top]:
+        name = r["Name"].replace("|", "\\|")
+        if len(name) > 90:
+            name = name[:87] + "..."
+        lines.append(f"| {name} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['TotalDurationNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    return "\n".join(lines) + "\n"
+
+
+def _counter_sum(run_dir: str, counter: str, kernel_rx: str) -> tuple[float, int]:
+    """Sum of `counter` (KiB per dispatch) over dispatches whose kernel name
+    matches kernel_rx; returns (total KiB, dispatch count)."""
+    path = _one(os.path.join(run_dir, "**", "*counter_collection.csv"))
+    per_dispatch = {}
+    rx = re.compile(kernel_rx)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter or not rx.search(r["Kernel_Name"]):
+            continue
+        key = r["Dispatch_Id"]
+        per_dispatch[key] = per_dispatch.get(key, 0.0) + float(r["Counter_Value"])
+    return sum(per_dispatch.values()), len(per_dispatch)
+
+
+def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
+                 copy_mib: int, calls: int, cmd: str) -> dict:
+    copy_bytes = copy_mib * (1 << 20)
+    cf, nf = _counter_sum(copy_fetch, "FETCH_SIZE", COPY_KERNELS)
+    cw, nw = _counter_sum(copy_write, "WRITE_SIZE", COPY_KERNELS)
+    # the copy kernel reads and writes copy_bytes per call
+    f_fetch = cf * 1024 / (nf * copy_bytes)
+    f_write = cw * 1024 / (nw * copy_bytes)
+    mf, mnf = _counter_sum(mm_fetch, "FETCH_SIZE", MM_KERNELS)
+    mw, mnw = _counter_sum(mm_write, "WRITE_SIZE", MM_KERNELS)
+    fetch_b = mf * 1024 / calls / f_fetch
+    write_b = mw * 1024 / calls / f_write
+    return {"command": cmd, "copy_mib": copy_mib, "fetch_factor": f_fetch,
+            "write_factor": f_write, "model_map_calls": calls,
+            "dispatches_fetch": mnf, "dispatches_write": mnw,
+            "fetch_bytes_per_call": fetch_b, "write_bytes_per_call": write_b,
+            "traffic_bytes_per_call": fetch_b + write_b}
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser(prog="python -m xtsrcmaps_amd.profiling")
+    sub = ap.add_subparsers(dest="what", required=True)
+    p = sub.add_parser("modelmap")
+    p.add_argument("--config", default="c1")
+    p.add_argument("--calls", type=int, default=3)
+    p = sub.add_parser("copy")
+    p.add_argument("--mib", type=int, default=592)
+    p.add_argument("--calls", type=int, default=3)
+    p = sub.add_parser("stats")
+    p.add_argument("csv")
+    p.add_argument("--cmd", required=True)
+    p = sub.add_parser("traffic")
+    for k in ("mm-fetch", "mm-write", "copy-fetch", "copy-write"):
+        p.add_argument("--" + k, required=True)
+    p.add_argument("--copy-mib", type=int, required=True)
+    p.add_argument("--calls", type=int, required=True)
+    p.add_argument("--cmd", required=True)
+    a = ap.parse_args(argv)
+    if a.what == "modelmap":
+        _modelmap(a.config, a.calls)
+    elif a.what == "copy":
+        _copy(a.mib, a.calls)
+    elif a.what == "stats":
+        sys.stdout.write(stats_markdown(a.csv, a.cmd))
+    else:
+        print(json.dumps(traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write,
+                                      a.copy_mib, a.calls, a.cmd), indent=1))
+
+
+if __name__ == "__main__":
+    main()
