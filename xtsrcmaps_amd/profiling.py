@@ -11,7 +11,7 @@ Summaries of rocprofv3 CSV output (plain Python, no GPU):
 
     stats <kernel_stats.csv> --cmd CMD           Markdown table of a --stats run
     traffic --mm-fetch D --mm-write D --copy-fetch D --copy-write D
-            --copy-mib M --calls N --cmd CMD     JSON: HBM bytes per model map
+            --copy-mib M --cmd CMD               JSON: HBM bytes per model map
 """
 from __future__ import annotations
 
@@ -24,7 +24,7 @@ import re
 import sys
 
 MM_KERNELS = r"rowweight_kernel|near_pixel_kernel|hist_integral_kernel|factor_kernel|emit_s|near_scale_kernel"
-COPY_KERNELS = r"elementwise|vectorized|copy"
+COPY_KERNELS = r"MulFunctor"          # torch.mul(src, 1.0, out=dst): reads and writes N bytes
 
 
 # ---------------------------------------------------------------- workloads
@@ -93,7 +93,7 @@ def _counter_sum(run_dir: str, counter: str, kernel_rx: str) -> tuple[float, int
 
 
 def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
-                 copy_mib: int, calls: int, cmd: str) -> dict:
+                 copy_mib: int, cmd: str) -> dict:
     copy_bytes = copy_mib * (1 << 20)
     cf, nf = _counter_sum(copy_fetch, "FETCH_SIZE", COPY_KERNELS)
     cw, nw = _counter_sum(copy_write, "WRITE_SIZE", COPY_KERNELS)
@@ -102,13 +102,23 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
     f_write = cw * 1024 / (nw * copy_bytes)
     mf, mnf = _counter_sum(mm_fetch, "FETCH_SIZE", MM_KERNELS)
     mw, mnw = _counter_sum(mm_write, "WRITE_SIZE", MM_KERNELS)
+    # one emit dispatch per model map
+    _, calls = _counter_sum(mm_write, "WRITE_SIZE", r"emit_s")
     fetch_b = mf * 1024 / calls / f_fetch
     write_b = mw * 1024 / calls / f_write
-    return {"command": cmd, "copy_mib": copy_mib, "fetch_factor": f_fetch,
-            "write_factor": f_write, "model_map_calls": calls,
+    per_kernel = {}
+    for name, rx in (("rowweight", "rowweight_kernel"), ("near", "near_pixel_kernel"),
+                     ("hist_factor", "hist_integral_kernel|factor_kernel"),
+                     ("emit", "emit_s"), ("near_scale", "near_scale_kernel")):
+        f, _ = _counter_sum(mm_fetch, "FETCH_SIZE", rx)
+        w, _ = _counter_sum(mm_write, "WRITE_SIZE", rx)
+        per_kernel[name] = {"fetch_bytes": f * 1024 / calls / f_fetch,
+                            "write_bytes": w * 1024 / calls / f_write}
+    return {"command": cmd, "copy_mib": copy_mib,
+            "fetch_factor": f_fetch, "write_factor": f_write, "model_map_calls": calls,
             "dispatches_fetch": mnf, "dispatches_write": mnw,
             "fetch_bytes_per_call": fetch_b, "write_bytes_per_call": write_b,
-            "traffic_bytes_per_call": fetch_b + write_b}
+            "traffic_bytes_per_call": fetch_b + write_b, "per_kernel": per_kernel}
 
 
 def main(argv=None) -> None:
@@ -127,7 +137,6 @@ def main(argv=None) -> None:
     for k in ("mm-fetch", "mm-write", "copy-fetch", "copy-write"):
         p.add_argument("--" + k, required=True)
     p.add_argument("--copy-mib", type=int, required=True)
-    p.add_argument("--calls", type=int, required=True)
     p.add_argument("--cmd", required=True)
     a = ap.parse_args(argv)
     if a.what == "modelmap":
@@ -138,7 +147,7 @@ def main(argv=None) -> None:
         sys.stdout.write(stats_markdown(a.csv, a.cmd))
     else:
         print(json.dumps(traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write,
-                                      a.copy_mib, a.calls, a.cmd), indent=1))
+                                      a.copy_mib, a.cmd), indent=1))
 
 
 if __name__ == "__main__":
