@@ -26,7 +26,7 @@ import numpy as np
 METRIC = "srcmap pix·E·src evals/s: surrogate vs cubature at ≤1e-4 max rel err, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 FP32_VALU_PEAK_TFS = 157.3
-STAGES = ("cubature", "near", "factor", "emit", "near_scale")
+STAGES = ("cubature", "near", "factor", "emit", "near_emit")
 
 
 def _log(msg):

@@ -7,12 +7,13 @@
 //   3. rowweight_kernel  per (source, pixel): the cubature reduced to four
 //                        PSF-table row weights (x solid angle) and the
 //                        in-radius row histogram of the map integral
-//   4. near_pixel_kernel cubature with direct table reads for the pixels
-//                        whose rule points span more rows (near the source)
+//   4. near_rows_kernel  the same for the pixels whose rule points span more
+//                        rows (near the source), up to 48 rows per pixel
 //   5. hist_integral_kernel, factor_kernel
 //                        map integral, correction factor per (source, energy)
-//   6. emit_kernel       every map value once: row weights . table rows
+//   6. emit_*_kernel     every map value once: row weights . table rows
 //                        x exposure x correction
+//   7. near_emit_kernel  the same for the near pixels
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -125,15 +126,17 @@ __global__ void radius_kernel(Wcs w, int H, int W, int S, const double* __restri
 // point of a pixel: map[e] = sum_p w_p ((1 - t_p) L[i_p - 1][e] + t_p L[i_p][e]).
 // Regrouped by table row this is map[e] = sum_r W_r L[r][e]; every point of a
 // pixel away from the source lands on one of four consecutive rows, so the
-// pixel is described by four row weights and a first row.  Three passes:
+// pixel is described by four row weights and a first row.  The passes:
 //   rowweight_kernel  point loop per (source, pixel): four row weights x
 //                     solid angle, the in-radius row histogram of the map
 //                     integral, and a list of the pixels whose points span
 //                     more rows (near the source, or outside the table)
-//   near_pixel_kernel point loop with direct table reads for listed pixels,
-//                     their map values and in-radius map-integral sums
-//   emit_kernel       per (source, pixel, energy): the 4-row dot product x
+//   near_rows_kernel  up to 48 row weights per listed pixel (pooled), into
+//                     the same histogram
+//   emit_*_kernel     per (source, pixel, energy): the 4-row dot product x
 //                     exposure x correction factor, stored once
+//   near_emit_kernel  the listed pixels' values from their pooled rows (or
+//                     the direct cubature for the pixel holding the source)
 constexpr int kTileW = 32, kTileH = 16;   // pixels per block pass (512 threads)
 constexpr int kThreads = kTileW * kTileH;
 constexpr int kNearThreads = 256;
@@ -156,9 +159,14 @@ struct CubArgs {
     void* rw;                  // [S][H W] row weights (4 of T) x solid angle
     int* rw_row;               // [S][H W] first row, -1 for a listed (near) pixel
     double* hist;              // [S][401] in-radius row-weight sums, or null
-    double* mapint;            // [S][E] in-radius map sums, or null
+    double* mapint;            // [S][E] in-radius map sums, or null (full model map only)
     int* near_count;           // [S]
     int* near_list;            // [S][H W] listed pixel indices
+    int* near_off;             // [S][H W] pool offset per listed pixel (-1: wide)
+    int* near_rr;              // [S][H W] rbase * 256 + R per listed pixel
+    void* near_pool;           // pooled row weights of listed pixels (T)
+    int* near_pool_count;      // [S] pool entries taken in each source's segment
+    int near_pool_cap;         // entries per source
     const double* expo;        // [S][E] or null
     const double* cf;          // [S][E] or null
     void* out;
@@ -568,7 +576,7 @@ __device__ inline void lean_list_loop(int N4, const float4* __restrict__ rule_s,
 // pixel touches are known before the point loop.  A pixel whose points touch
 // at most four consecutive rows r0 .. r0+3 accumulates, per relative row,
 // the sums of w and w t in registers; the row weights follow from them.
-// Other pixels are listed for near_pixel_kernel.  When every lane of a wave
+// Other pixels are listed for near_rows_kernel.  When every lane of a wave
 // is such a pixel below 28 degrees, the wave runs a branch-free loop over
 // the points in groups of four (grid-structured rules reuse the blend of
 // each rule row).
@@ -708,23 +716,270 @@ rowweight_kernel(CubArgs a) {
     }
 }
 
-// Pass 2: the listed pixels of each source, every point adding its two
-// table rows directly; values x solid angle, stored unscaled (emit_kernel
-// scales them).  Grid: (blocks striding over the list, S, energy chunks of
-// EC).  In-radius values are summed per energy into mapint.
-template <typename T, int EC>
+// Pass 2a (before the correction factor): the listed pixels of each source.
+// A pixel whose points touch at most kNearRows consecutive rows rbase ..
+// rbase+R-1 gathers its row weights and stores them, x solid angle, in a
+// pool (R values at an offset in its source's segment); its in-radius
+// weights go to the source's row histogram.  kNearSplit lanes of one wave
+// share a pixel, each taking every kNearSplit-th rule point into a
+// lane-private LDS histogram (rows x threads, no two lanes on one word); the
+// lanes then sum the kNearSplit partial rows.  Below 14 degrees the FP32 row
+// and fraction come from the relative log of lean_point.  A pixel touching
+// more rows (the one holding the source), or finding the pool full, is
+// "wide" (pool offset -1): only its in-radius weights are added here, point
+// by point, and wide_pixel_kernel evaluates it.  Grid: (blocks striding over
+// the list, S).
+constexpr int kNearRows = 48;
+constexpr int kNearSplit = 4;
+constexpr int kNearPix = kNearThreads / kNearSplit;   // pixels per block pass
+template <typename T>
 __global__ void __launch_bounds__(kNearThreads)
-near_pixel_kernel(CubArgs a) {
+near_rows_kernel(CubArgs a) {
+    using V4 = typename Vec4<T>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* isep_s = reinterpret_cast<T*>(smem_raw);
+    V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
+    const int N4 = (a.N + 3) & ~3;
+    float4* tco_s = reinterpret_cast<float4*>(rule_s + N4);
+    T* ph = reinterpret_cast<T*>(tco_s + N4);   // [kNearRows][kNearThreads]
+
+    const int s = blockIdx.y;
+    const int count = a.near_count[s];
+    if ((int)blockIdx.x * kNearPix >= count) return;
+    const int tid = threadIdx.x;
+    const int slot = tid / kNearSplit, part = tid % kNearSplit;
+    stage_rule<T>(a, isep_s, rule_s, N4, tid, kNearThreads);
+    for (int p = tid; p < N4; p += kNearThreads) {
+        const float wq = p < a.N ? (float)a.uvw[p].wq : 0.0f;
+        tco_s[p] = float4{wq * (float)kTPoly[0], wq * (float)kTPoly[1], wq * (float)kTPoly[2], 0.0f};
+    }
+    __syncthreads();
+
+    const SrcGeom g = a.geo[s];
+    const bool do_hist = a.hist && g.in_fov;
+    double* hist = do_hist ? a.hist + (size_t)s * kND : nullptr;
+    const size_t plane = (size_t)a.H * a.W;
+    const int Wp1 = a.W + 1;
+    const int* list = a.near_list + (size_t)s * plane;
+    int* meta_off = a.near_off + (size_t)s * plane;
+    int* meta_rr = a.near_rr + (size_t)s * plane;
+    T* pool = reinterpret_cast<T*>(a.near_pool);
+    T* myh = ph + tid;                              // row r at myh[r * kNearThreads]
+    const int lane0 = (tid & 63) & ~(kNearSplit - 1);
+    for (int first = blockIdx.x * kNearPix; first < count; first += gridDim.x * kNearPix) {
+        const int idx = first + slot;
+        const bool live = idx < count;
+        const int pix = live ? list[idx] : list[first];
+        const int h = pix / a.W, w = pix % a.W;
+        const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, Wp1, g.dir, a.rho_scale);
+        const T om = a.omega ? (T)a.omega[pix] : T(1);
+        const bool inrad = live && do_hist && in_radius(g, a.centres, h * Wp1 + w);
+        // rows touched: rbase = lower row of slo .. the upper row of shi
+        const int rbase = sep_index_fast(pg.slo) - 1;
+        const int R = sep_index_fast(pg.shi) - rbase + 1;
+        int off = -1;
+        if (part == 0 && live) {
+            if (R <= kNearRows) {
+                // this source's segment of the pool
+                off = atomicAdd(&a.near_pool_count[s], R);
+                off = off + R > a.near_pool_cap ? -1 : off + s * a.near_pool_cap;
+            }
+        }
+        off = __shfl(off, lane0);
+        const bool narrow = off >= 0;
+        if (narrow)
+            for (int r = 0; r < R; ++r) myh[r * kNearThreads] = T(0);
+        bool lean = false;
+        if constexpr (sizeof(T) == 4) lean = narrow && pg.slo >= T(1e-4) && pg.shi < T(14.0);
+        if (lean) {
+            if constexpr (sizeof(T) == 4) {
+                const LeanPixel lp = lean_pixel<3>(pg, rbase);
+#pragma unroll 4
+                for (int p = part; p < a.N; p += kNearSplit) {
+                    const V4 rr = rule_s[p];
+                    const float4 tc = tco_s[p];
+                    const float u = rr.x, v = rr.y;
+                    const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
+                    const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
+                    const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
+                    const float z = fmaf(fmaf(cu, v, bu), v, au);
+                    const float L = __builtin_amdgcn_logf(z);
+                    const float gq = fmaf(fmaf(lp.g[2], z, lp.g[1]), z, lp.g[0]);
+                    const float Q = fmaf(gq, z, fmaf((float)(0.5 * kRecipStep * 0.69314718055994530942),
+                                                    L, lp.q0));
+                    const float f = __builtin_amdgcn_fractf(Q);
+                    const float wt = f * fmaf(f, fmaf(f, tc.z, tc.y), tc.x);
+                    const int l = min(max((int)floorf(Q), 0), R - 2);
+                    myh[l * kNearThreads] += (T)(rr.z - wt);
+                    myh[(l + 1) * kNearThreads] += (T)wt;
+                }
+            }
+        } else {
+            for (int p = part; p < a.N; p += kNearSplit) {
+                const V4 rr = rule_s[p];
+                const T u = rr.x, v = rr.y, wq = rr.z;
+                const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+                const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+                const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+                const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
+                const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
+                const int i = sep_index_fast(sd);
+                const T tt = T(kRedm1) * (sd * (isep_s[i - 1] * T(0.5 / kR2D)) - T(1));
+                const T wa = wq * (T(1) - tt), wb = wq * tt;
+                if (narrow) {
+                    const int l = min(max(i - 1 - rbase, 0), R - 2);
+                    myh[l * kNearThreads] += wa;
+                    myh[(l + 1) * kNearThreads] += wb;
+                } else if (inrad) {
+                    atomicAdd(&hist[i - 1], (double)(om * wa));
+                    atomicAdd(&hist[i], (double)(om * wb));
+                }
+            }
+        }
+        if (narrow) {
+            // the pixel's kNearSplit partial histograms are in this wave's
+            // lanes lane0 .. lane0+kNearSplit-1 (LDS order within a wave)
+            T* base = ph + (tid - part);
+            for (int r = part; r < R; r += kNearSplit) {
+                T W = T(0);
+#pragma unroll
+                for (int q = 0; q < kNearSplit; ++q) W += base[r * kNearThreads + q];
+                W *= om;
+                pool[off + r] = W;
+                if (inrad) atomicAdd(&hist[rbase + r], (double)W);
+            }
+        }
+        if (live && part == 0) {
+            meta_off[idx] = off;
+            meta_rr[idx] = rbase * 256 + min(R, 255);
+        }
+    }
+}
+
+// Wide listed pixels (pool offset -1): one wave per pixel.  Every wave scans
+// the source's list; the k-th wide entry goes to wave k mod (waves per
+// source).  The wave's lanes take the rule points, adding each point's two
+// row weights to a 401-row LDS histogram; then each lane takes every 64th
+// table row, loads its EC energies as vectors and accumulates W_r L[r], and
+// the wave sums the 64 partial vectors.  x solid angle, x exposure x
+// correction when SCALED.  Grid: (kWideBlocks, S, energy chunks).
+constexpr int kWideBlocks = 4;
+template <typename T, int EC, typename OT, bool SCALED>
+__global__ void __launch_bounds__(256)
+wide_pixel_kernel(CubArgs a) {
     using V = typename Vec<T>::type;
     constexpr int VN = Vec<T>::n;
     constexpr int NV = EC / VN;
-    using V4 = typename Vec4<T>::type;
+    __shared__ T hw[4][kNDPad];
+    const int s = blockIdx.y;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int gw = blockIdx.x * 4 + wave, nwaves = gridDim.x * 4;
+    const int count = a.near_count[s];
+    const int e0 = blockIdx.z * EC;
+    const int ne = min(EC, a.E - e0);
+    const SrcGeom g = a.geo[s];
+    const size_t plane = (size_t)a.H * a.W;
+    const int* list = a.near_list + (size_t)s * plane;
+    const int* meta_off = a.near_off + (size_t)s * plane;
+    const T* lut_s0 = reinterpret_cast<const T*>(a.lut) + (size_t)s * kND * a.Ep + e0;
+    T* H = hw[wave];
+    int rank = 0;   // wide entries seen before this chunk
+    constexpr int kPre = 8;   // chunks whose entries are loaded together
+    int pre[kPre];
+    for (int c = 0; c < count; c += 64) {
+        const int q = (c / 64) % kPre;
+        if (q == 0) {
+#pragma unroll
+            for (int j = 0; j < kPre; ++j) {
+                const int k = c + j * 64 + lane;
+                pre[j] = k < count ? meta_off[k] : 0;
+            }
+        }
+        int mine = pre[0];
+#pragma unroll
+        for (int j = 1; j < kPre; ++j) mine = q == j ? pre[j] : mine;
+        const uint64_t wide = __ballot(mine < 0);
+        uint64_t todo = wide;
+        while (todo) {
+            const int b = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int my = rank + __popcll(wide & ((1ull << b) - 1));
+            if (my % nwaves != gw) continue;
+            const int pix = list[c + b];
+            const int h = pix / a.W, w = pix % a.W;
+            const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, a.W + 1, g.dir, a.rho_scale);
+            const T om = a.omega ? (T)a.omega[pix] : T(1);
+            for (int r = lane; r < kND; r += 64) H[r] = T(0);
+            for (int p = lane; p < a.N; p += 64) {
+                const RuleUVW q = a.uvw[p];
+                const T u = (T)q.u, v = (T)q.v, wq = (T)q.wq;
+                const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+                const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+                const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+                const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
+                const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
+                const int i = sep_index_fast(sd);
+                const T isep = sizeof(T) == 4 ? (T)a.inv_sep[i - 1] : (T)a.inv_sep64[i - 1];
+                const T tt = T(kRedm1) * (sd * (isep * T(0.5 / kR2D)) - T(1));
+                atomicAdd(&H[i - 1], wq * (T(1) - tt));
+                atomicAdd(&H[i], wq * tt);
+            }
+            V acc[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = V{};
+            for (int r = lane; r < kND; r += 64) {
+                const T W = H[r];
+                if (W != T(0)) {
+                    const V* L = reinterpret_cast<const V*>(lut_s0 + (size_t)r * a.Ep);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        if (e0 + v * VN < a.Ep) vfma(acc[v], W, L[v]);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+#pragma unroll
+                for (int j = 0; j < VN; ++j) {
+                    T x = reinterpret_cast<const T*>(&acc[v])[j];
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+                    const int e = v * VN + j;
+                    if (lane == 0 && e < ne) {
+                        x *= om;
+                        OT* po = reinterpret_cast<OT*>(a.out) +
+                                 out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
+                        if (SCALED) {
+                            const int se = s * a.E + e0 + e;
+                            if constexpr (sizeof(OT) == 4) {
+                                *po = (float)x * (float)(a.expo[se] * a.cf[se]);
+                            } else {
+                                double y = (double)x * a.expo[se];
+                                *po = y * a.cf[se];
+                            }
+                        } else {
+                            *po = (OT)x;
+                        }
+                    }
+                }
+            }
+        }
+        rank += __popcll(wide);
+    }
+}
+
+// Pass 2b (after the correction factor): the pooled listed pixels' map
+// values, written once: sum_r W_r L[rbase + r] (W already x solid angle),
+// x exposure x correction when SCALED.  Grid: (blocks striding over the
+// list, S, energy chunks).
+template <typename T, int EC, typename OT, bool SCALED>
+__global__ void __launch_bounds__(kNearThreads)
+near_emit_kernel(CubArgs a) {
+    using V = typename Vec<T>::type;
+    constexpr int VN = Vec<T>::n;
+    constexpr int NV = EC / VN;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* isum_s = reinterpret_cast<double*>(smem_raw);            // [EC]
-    T* lut_s = reinterpret_cast<T*>(isum_s + EC);
-    T* isep_s = lut_s + kND * EC;
-    V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
-    const int N4 = (a.N + 3) & ~3;
+    T* lut_s = reinterpret_cast<T*>(smem_raw);
+    __shared__ ChunkScale<EC> sc;
 
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
@@ -733,72 +988,63 @@ near_pixel_kernel(CubArgs a) {
     if ((int)blockIdx.x * kNearThreads >= count) return;
     const int tid = threadIdx.x;
     stage_lut<T, EC, kNearThreads>(a, s, e0, lut_s, tid);
-    stage_rule<T>(a, isep_s, rule_s, N4, tid, kNearThreads);
-    for (int k = tid; k < EC; k += kNearThreads) isum_s[k] = 0.0;
+    if (SCALED) sc.load(a, s * a.E + e0, ne, tid);
     __syncthreads();
 
-    const SrcGeom g = a.geo[s];
-    const bool do_sum = a.mapint && g.in_fov;
     const size_t plane = (size_t)a.H * a.W;
-    const int Wp1 = a.W + 1;
     const int* list = a.near_list + (size_t)s * plane;
+    const int* meta_off = a.near_off + (size_t)s * plane;
+    const int* meta_rr = a.near_rr + (size_t)s * plane;
+    const T* pool = reinterpret_cast<const T*>(a.near_pool);
     for (int first = blockIdx.x * kNearThreads; first < count;
          first += gridDim.x * kNearThreads) {
         const int idx = first + tid;
         const bool live = idx < count;
-        const int pix = live ? list[idx] : list[first];
-        const int h = pix / a.W, w = pix % a.W;
-        const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, Wp1, g.dir, a.rho_scale);
-        const T om = a.omega ? (T)a.omega[pix] : T(1);
+        const int ic = live ? idx : first;
+        const int pix = list[ic];
+        const int off = meta_off[ic];
         V acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = V{};
-        for (int p = 0; p < a.N; ++p) {
-            const V4 r = rule_s[p];
-            const T u = r.x, v = r.y, wq = r.z;
-            const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
-            const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
-            const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
-            const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
-            const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
-            const int i = sep_index_fast(sd);
-            const T tt = T(kRedm1) * (sd * (isep_s[i - 1] * T(0.5 / kR2D)) - T(1));
-            const T wa = wq * (T(1) - tt), wb = wq * tt;
-            const V* L0 = reinterpret_cast<const V*>(lut_s) + (i - 1) * NV;
+        if (off < 0) continue;            // wide: wide_pixel_kernel
+        {
+            const int rr = meta_rr[ic];
+            const int rbase = rr >> 8, R = rr & 255;
+            const V* L = reinterpret_cast<const V*>(lut_s) + rbase * NV;
+            for (int r0 = 0; r0 < R; r0 += 8) {
+                // eight pool loads in flight, then their rows
+                T Wb[8];
 #pragma unroll
-            for (int vv = 0; vv < NV; ++vv) {
-                vfma(acc[vv], wa, L0[vv]);
-                vfma(acc[vv], wb, L0[NV + vv]);
+                for (int k = 0; k < 8; ++k) Wb[k] = r0 + k < R ? pool[off + r0 + k] : T(0);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (r0 + k < R) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) vfma(acc[v], Wb[k], L[(r0 + k) * NV + v]);
+                    }
+                }
             }
         }
-        const bool inrad = live && do_sum && in_radius(g, a.centres, h * Wp1 + w);
+        if (!live) continue;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
 #pragma unroll
             for (int j = 0; j < VN; ++j) {
                 const int e = v * VN + j;
-                const T x = reinterpret_cast<const T*>(&acc[v])[j] * om;
-                if (live && e < ne) store_value(a, out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane), x);
-                if (do_sum) {
-                    // wave sum of the in-radius values, one LDS add per wave
-                    double y = inrad ? (double)x : 0.0;
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o);
-                    if ((tid & 63) == 0 && y != 0.0) atomicAdd(&isum_s[e], y);
+                if (e < ne) {
+                    const T x = reinterpret_cast<const T*>(&acc[v])[j];
+                    OT* po = reinterpret_cast<OT*>(a.out) +
+                             out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
+                    *po = SCALED ? sc.template apply<OT>(x, e) : (OT)x;
                 }
             }
         }
     }
-    if (do_sum) {
-        __syncthreads();
-        for (int e = tid; e < ne; e += kNearThreads)
-            if (isum_s[e] != 0.0) atomicAdd(&a.mapint[(size_t)s * a.E + e0 + e], isum_s[e]);
-    }
 }
 
-// map_integral.cxx:21-44 for the pixels of pass 1: their in-radius sum is
-// sum_d hist[s, d] uPsf[s, d, e] (FP64 table); pass 2 has already added
-// the listed pixels.  One block per source: each thread takes one energy
+// map_integral.cxx:21-44 through the row histogram: the in-radius sum of
+// the (solid-angle scaled) map is sum_d hist[s, d] uPsf[s, d, e] (FP64
+// table), listed pixels included.  One block per source: each thread takes one energy
 // and every P-th row (consecutive threads read consecutive energies), the
 // P partial sums are added in LDS.
 constexpr int kHistThreads = 256;
@@ -824,14 +1070,14 @@ hist_integral_kernel(int E, const double* __restrict__ hist, const double* __res
         if (ph == 0 && e < E) {
             double tot = 0.0;
             for (int q = 0; q < P; ++q) tot += part[q * Ec + el];
-            mapint[(size_t)s * E + e] += tot;
+            mapint[(size_t)s * E + e] = tot;
         }
     }
 }
 
 // Pass 3: per (source, pixel range, energy chunk): map = sum_k W_k L[r0+k]
 // for pass-1 pixels, x exposure x correction when SCALED.  Listed pixels are
-// left to near_scale_kernel.  OT: output type.  Each thread first loads the
+// left to near_emit_kernel.  OT: output type.  Each thread first loads the
 // row weights of its kEmitPix pixels and then only stores: a load issued
 // after stores would wait for them to drain (the vector memory counter
 // counts both).
@@ -969,35 +1215,6 @@ emit_shwe_kernel(CubArgs a) {
     }
 }
 
-// Listed pixels: x exposure x correction in place (their unscaled values
-// come from near_pixel_kernel).  Grid: (blocks striding over the source's
-// (pixel, energy) pairs, S).
-template <typename OT>
-__global__ void near_scale_kernel(CubArgs a) {
-    const int s = blockIdx.y;
-    const int count = a.near_count[s];
-    const size_t plane = (size_t)a.H * a.W;
-    const int* list = a.near_list + (size_t)s * plane;
-    OT* out = reinterpret_cast<OT*>(a.out);
-    const int64_t n = (int64_t)count * a.E;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        // consecutive threads: consecutive list entries (mostly neighbouring
-        // pixels) of one energy
-        const int e = (int)(k / count), idx = (int)(k % count);
-        const int pix = list[idx];
-        const size_t o = a.layout_sehw ? ((size_t)s * a.E + e) * plane + pix
-                                       : ((size_t)s * plane + pix) * a.E + e;
-        const int se = s * a.E + e;
-        if constexpr (sizeof(OT) == 4) {
-            out[o] = out[o] * (float)(a.expo[se] * a.cf[se]);
-        } else {
-            double y = out[o] * a.expo[se];
-            out[o] = y * a.cf[se];
-        }
-    }
-}
-
 // Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
 // and size: the attribute belongs to the function).
 static int allow_lds(const void* fn, size_t bytes) {
@@ -1021,6 +1238,28 @@ static int launch_emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid, bool sehw) {
     else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
+}
+
+template <typename T, int EC, typename OT, bool SCALED>
+static int launch_near_emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid) {
+    const size_t sm = (size_t)kND * EC * sizeof(T);
+    XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED>, sm));
+    near_emit_kernel<T, EC, OT, SCALED><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
+    // wide pixels: a few waves per source scan the list
+    const dim3 wgrid(kWideBlocks, grid.y, grid.z);
+    wide_pixel_kernel<T, EC, OT, SCALED><<<wgrid, 256, 0, ctx->stream>>>(ca);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
+template <typename T, int EC>
+static int near_emit(xts_ctx* ctx, const CubArgs& ca, int64_t gx, bool scaled) {
+    const dim3 grid((unsigned)gx, (unsigned)ca.S, (unsigned)div_up(ca.E, EC));
+    if (ca.out_f32)
+        return scaled ? launch_near_emit<T, EC, float, true>(ctx, ca, grid)
+                      : launch_near_emit<T, EC, float, false>(ctx, ca, grid);
+    return scaled ? launch_near_emit<T, EC, double, true>(ctx, ca, grid)
+                  : launch_near_emit<T, EC, double, false>(ctx, ca, grid);
 }
 
 template <typename T, int EC>
@@ -1274,7 +1513,6 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     if (!rule) N = 12;
 
     // PSF table in the cubature precision
-    const int EC = f64 ? 16 : 40;
     const int Ep = (int)((E + 3) / 4 * 4);
     void* lut;
     const size_t lut_elems = (size_t)S * kND * Ep;
@@ -1282,20 +1520,28 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     lut_convert_kernel<<<std::min<unsigned>(div_up(lut_elems, 256), 8192), 256, 0, ctx->stream>>>(
         upsf, (int)S, (int)E, Ep, f64 ? nullptr : (float*)lut, f64 ? (double*)lut : nullptr);
 
-    // per-pixel row weights, first rows and the near-pixel lists
+    // per-pixel row weights and first rows; listed (near) pixels, their pool
+    // offsets and row ranges
     const size_t plane = (size_t)H * W;
     const size_t rw_bytes = (size_t)S * plane * (f64 ? 32 : 16);
     void* rows;
-    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 8 + 512, &rows));
-    // counters and FP64 accumulators: near_count [S] (int, padded), hist
-    // [S][401], mapint [S][E], factor [S][E]
-    const size_t cnt_d = (size_t)(S + 1) / 2;
+    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 16 + 512, &rows));
+    // counters [2 S] (int, padded to doubles: near counts, pool entries taken
+    // per source) and FP64 accumulators: hist [S][401], mapint [S][E],
+    // factor [S][E]
+    const size_t cnt_d = (size_t)S;
     const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
     void* accp;
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
     double* acc = (double*)accp;
-    const size_t nzero = full ? cnt_d + (size_t)S * kND + (size_t)S * E : cnt_d;
+    const size_t nzero = full ? cnt_d + (size_t)S * kND : cnt_d;
     zero_kernel<<<std::min<unsigned>(div_up(nzero, 256), 4096), 256, 0, ctx->stream>>>(acc, nzero);
+    // pool of listed pixels' row weights, one segment per source with room
+    // for 2048 pixels of 16 rows; a pixel that finds its segment full is
+    // evaluated directly
+    const int64_t pool_cap = std::min<int64_t>(2048 * 16, ((int64_t)1 << 30) / S);
+    void* pool;
+    XTS_TRY(slot_reserve(ctx, kSlotNearPool, (size_t)(pool_cap * S) * (f64 ? 8 : 4), &pool));
 
     CubArgs ca{};
     ca.corners = ig.corners;
@@ -1309,7 +1555,12 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.rw = rows;
     ca.rw_row = (int*)((char*)rows + rw_bytes);
     ca.near_list = ca.rw_row + (size_t)S * plane;
+    ca.near_off = ca.near_list + (size_t)S * plane;
+    ca.near_rr = ca.near_off + (size_t)S * plane;
     ca.near_count = (int*)acc;
+    ca.near_pool_count = ca.near_count + S;
+    ca.near_pool = pool;
+    ca.near_pool_cap = (int)pool_cap;
     ca.hist = full ? acc + cnt_d : nullptr;
     ca.mapint = full ? ca.hist + (size_t)S * kND : nullptr;
     double* fac = full ? ca.mapint + (size_t)S * E : nullptr;
@@ -1348,20 +1599,20 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         else rowweight_kernel<float><<<grid, kThreads, sm, ctx->stream>>>(ca);
         XTS_HIP(hipGetLastError());
     }
-    // pass 2: blocks stride over each source's list
-    const int nchunks = (int)((E + EC - 1) / EC);
+    // pass 2a: blocks stride over each source's list
+    const int64_t near_gx = std::min<int64_t>(std::max<int64_t>(1, 2 * want_blocks / S),
+                                              div_up(plane, kNearPix));
     {
-        const size_t sm = (size_t)EC * 8 + ((size_t)kND * EC + kNDPad) * tsz + (size_t)N4 * 4 * tsz;
-        const int64_t gx = std::min<int64_t>(
-            std::max<int64_t>(1, want_blocks / ((int64_t)S * nchunks)), div_up(plane, kNearThreads));
-        const dim3 grid((unsigned)gx, (unsigned)S, (unsigned)nchunks);
+        const size_t sm = (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz + (size_t)N4 * 16 +
+                          (size_t)kNearRows * kNearThreads * tsz;
+        const dim3 grid((unsigned)near_gx, (unsigned)S);
         TimedRegion tr(ctx, "near");
         if (f64) {
-            XTS_TRY(allow_lds((const void*)near_pixel_kernel<double, 16>, sm));
-            near_pixel_kernel<double, 16><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
+            XTS_TRY(allow_lds((const void*)near_rows_kernel<double>, sm));
+            near_rows_kernel<double><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
         } else {
-            XTS_TRY(allow_lds((const void*)near_pixel_kernel<float, 40>, sm));
-            near_pixel_kernel<float, 40><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
+            XTS_TRY(allow_lds((const void*)near_rows_kernel<float>, sm));
+            near_rows_kernel<float><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
         }
         XTS_HIP(hipGetLastError());
     }
@@ -1385,16 +1636,11 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx, ca, grid64, full)));
         else XTS_TRY((emit<float, kEmitEC32>(ctx, ca, grid32, full)));
     }
-    if (full) {
-        // listed pixels: a few elements per thread
-        TimedRegion tr(ctx, "near_scale");
-        const unsigned gx = (unsigned)std::max<int64_t>(
-            1, std::min<int64_t>(div_up((int64_t)plane * E, 256 * 4),
-                                 std::max<int64_t>(1, want_blocks * 4 / S)));
-        const dim3 grid(gx, (unsigned)S);
-        if (of32) near_scale_kernel<float><<<grid, 256, 0, ctx->stream>>>(ca);
-        else near_scale_kernel<double><<<grid, 256, 0, ctx->stream>>>(ca);
-        XTS_HIP(hipGetLastError());
+    // pass 2b: listed pixels, scaled, written once
+    {
+        TimedRegion tr(ctx, "near_emit");
+        if (f64) XTS_TRY((near_emit<double, 16>(ctx, ca, near_gx, full)));
+        else XTS_TRY((near_emit<float, 40>(ctx, ca, near_gx, full)));
     }
     return XTS_OK;
 }
