@@ -63,7 +63,7 @@ enum SlotId {
     kSlotIn0 = 0, kSlotIn1, kSlotIn2, kSlotIn3, kSlotIn4, kSlotIn5,
     kSlotIn6, kSlotIn7, kSlotOut0, kSlotOut1, kSlotOut2, kSlotOut3,
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
-    kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotCount
+    kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotBilerp, kSlotCount
 };
 
 }  // namespace xts

@@ -1128,11 +1128,11 @@ emit_sehw_kernel(CubArgs a) {
         if (!out_ok[k]) wv[k] = V4{T(0), T(0), T(0), T(0)};
     }
     const bool all_ok = out_ok[0] && out_ok[1] && out_ok[2] && out_ok[3];
+    OT* po = reinterpret_cast<OT*>(a.out) + (size_t)se0 * plane;
     stage_lut<T, EC, kEmitThreads, LS>(a, s, e0, lut_s, tid);
     if (SCALED) sc.load(a, se0, ne, tid);
     __syncthreads();
 
-    OT* po = reinterpret_cast<OT*>(a.out) + (size_t)se0 * plane;
 #pragma unroll 2
     for (int e = 0; e < ne; ++e, po += plane) {
         OT y[kEmitPix];

@@ -84,32 +84,57 @@ __host__ __device__ inline bool lerp_pars_psf(const double* g, int n, double v, 
     return true;
 }
 
-// psf/psf.bilerp.cxx:6-52 times the effective area
-// (psf.corrected_exposure_psf.cxx:37-44): PA[c, d, e] = bilerp(K)[c, d, e] x
-// aeff[c, e] (aeff null: the bilinear samples alone).  One thread per
-// (c, d, e).
-__global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const double* __restrict__ costh,
-                                   const double* __restrict__ logE, const double* __restrict__ pc,
-                                   const double* __restrict__ pe, const double* __restrict__ K,
-                                   const double* __restrict__ aeff, double* __restrict__ out,
-                                   int* __restrict__ err) {
+// psf/psf.bilerp.cxx:6-52: the bilinear weights and base node of each
+// observation point (c, e), shared by all 401 separations.
+struct BilerpPoint {
+    double w00, w01, w10, w11;   // (1-tc)(1-te), (1-tc) te, tc (1-te), tc te
+    int base;                    // (ci - 1) Me + ei - 1, or -1 outside the grid
+    int pad;
+};
+
+__global__ void bilerp_points_kernel(int C, int E, int Mc, int Me, const double* __restrict__ costh,
+                                     const double* __restrict__ logE, const double* __restrict__ pc,
+                                     const double* __restrict__ pe, BilerpPoint* __restrict__ pts,
+                                     int* __restrict__ err) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= C * E) return;
+    const int c = k / E, e = k % E;
+    double cw, ccw, ew, ecw;
+    int ci, ei;
+    BilerpPoint q{};
+    if (!lerp_pars_psf(pc, Mc, costh[c], &cw, &ccw, &ci) ||
+        !lerp_pars_psf(pe, Me, logE[e], &ew, &ecw, &ei)) {
+        atomicOr(err, 1);
+        q.base = -1;
+    } else {
+        q.w00 = ccw * ecw;
+        q.w01 = ccw * ew;
+        q.w10 = cw * ecw;
+        q.w11 = cw * ew;
+        q.base = (ci - 1) * Me + ei - 1;
+    }
+    pts[k] = q;
+}
+
+// times the effective area (psf.corrected_exposure_psf.cxx:37-44):
+// PA[c, d, e] = bilerp(K)[c, d, e] x aeff[c, e] (aeff null: the bilinear
+// samples alone).  One thread per (c, d, e); K is [401][Mc][Me].
+__global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const BilerpPoint* __restrict__ pts,
+                                   const double* __restrict__ K, const double* __restrict__ aeff,
+                                   double* __restrict__ out) {
     const size_t n = (size_t)C * kND * E;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (size_t)gridDim.x * blockDim.x) {
         const int e = (int)(k % E);
         const int d = (int)((k / E) % kND);
         const int c = (int)(k / ((size_t)E * kND));
-        double cw, ccw, ew, ecw;
-        int ci, ei;
-        if (!lerp_pars_psf(pc, Mc, costh[c], &cw, &ccw, &ci) ||
-            !lerp_pars_psf(pe, Me, logE[e], &ew, &ecw, &ei)) {
-            atomicOr(err, 1);
+        const BilerpPoint q = pts[c * E + e];
+        if (q.base < 0) {
             out[k] = 0.0;
             continue;
         }
-        const double* Kd = K + (size_t)d * Mc * Me;
-        const double v = ccw * ecw * Kd[(ci - 1) * Me + ei - 1] + ccw * ew * Kd[(ci - 1) * Me + ei] +
-                         cw * ecw * Kd[ci * Me + ei - 1] + cw * ew * Kd[ci * Me + ei];
+        const double* Kd = K + (size_t)d * Mc * Me + q.base;
+        const double v = q.w00 * Kd[0] + q.w01 * Kd[1] + q.w10 * Kd[Me] + q.w11 * Kd[Me + 1];
         out[k] = aeff ? v * aeff[(size_t)c * E + e] : v;
     }
 }
@@ -146,6 +171,7 @@ mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
     double ef[kSrcTile], wf[kSrcTile], eb[kSrcTile], wb[kSrcTile];
 #pragma unroll
     for (int r = 0; r < kSrcTile; ++r) ef[r] = wf[r] = eb[r] = wb[r] = 0.0;
+#pragma unroll 4
     for (int c = 0; c < C; ++c) {
         const double f = paf[(size_t)c * N + n], b = kMean ? pab[(size_t)c * N + n] : 0.0;
 #pragma unroll
@@ -203,45 +229,68 @@ __global__ void pack_pa_kernel(int C, int N, const double* __restrict__ a,
 
 // psf.partial_total_integral.cxx:8-52 and psf.normalize.cxx:7-24 in FP64.
 // Each segment d (nodes d-1, d) contributes the exact integral of the
-// linear interpolant of y = 2 pi sin(theta) psf; the cumulative sum is an
-// inclusive prefix scan.  One wave per (source, energy): 64 segments per
-// step, a shuffle scan inside the step and a running carry between steps.
+// linear interpolant of y = 2 pi sin(theta) psf; the cumulative sum over d
+// is normalised by its last value, and so is uPsf.  A block takes kIntE
+// energies of one source; its threads split the 400 segments into kIntP
+// contiguous parts (consecutive threads: consecutive energies, so every
+// load is coalesced).  Pass 1 sums each part, the parts' exclusive prefix
+// comes from LDS, pass 2 writes the running sums, pass 3 normalises uPsf.
 // tab[d] = theta_d (rad), tab[401 + d] = sin theta_d.
-__global__ void __launch_bounds__(256)
+constexpr int kIntE = 16, kIntP = 32;
+
+__device__ inline double segment_integral(const double* __restrict__ tab, int d, double u1,
+                                          double u2) {
+    const double t1 = tab[d - 1], t2 = tab[d];
+    const double y1 = kTwoPi * u1 * tab[kND + d - 1];
+    const double y2 = kTwoPi * u2 * tab[kND + d];
+    const double m = (y2 - y1) / (t2 - t1);
+    const double b = y1 - t1 * m;
+    return 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
+}
+
+__global__ void __launch_bounds__(kIntE * kIntP)
 integral_kernel(int S, int E, double* __restrict__ upsf, double* __restrict__ parint,
                 double* __restrict__ total, const double* __restrict__ tab) {
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int e = blockIdx.x * 4 + wave, s = blockIdx.y;
-    if (e >= E || s >= S) return;
-    double* u = upsf + (size_t)s * kND * E + e;
-    double* pi = parint + (size_t)s * kND * E + e;
-    double carry = 0.0;
-    if (lane == 0) pi[0] = 0.0;
-    for (int d0 = 1; d0 < kND; d0 += 64) {
-        const int d = d0 + lane;
-        double v = 0.0;
-        if (d < kND) {
-            const double t1 = tab[d - 1], t2 = tab[d];
-            const double y1 = kTwoPi * u[(size_t)(d - 1) * E] * tab[kND + d - 1];
-            const double y2 = kTwoPi * u[(size_t)d * E] * tab[kND + d];
-            const double m = (y2 - y1) / (t2 - t1);
-            const double b = y1 - t1 * m;
-            v = 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
-        }
-        // inclusive scan across the wave
-        for (int o = 1; o < 64; o <<= 1) {
-            const double w = __shfl_up(v, o, 64);
-            if (lane >= o) v += w;
-        }
-        if (d < kND) pi[(size_t)d * E] = carry + v;
-        carry += __shfl(v, 63, 64);
+    __shared__ double psum[kIntP][kIntE];
+    const int s = blockIdx.y;
+    const int el = threadIdx.x % kIntE, part = threadIdx.x / kIntE;
+    const int e = blockIdx.x * kIntE + el;
+    const bool valid = e < E;
+    const int ec = valid ? e : E - 1;
+    double* u = upsf + (size_t)s * kND * E + ec;
+    double* pi = parint + (size_t)s * kND * E + ec;
+    constexpr int nseg = kND - 1;
+    const int d_lo = 1 + part * nseg / kIntP, d_hi = 1 + (part + 1) * nseg / kIntP;
+    double acc = 0.0;
+    double prev = u[(size_t)(d_lo - 1) * E];
+    for (int d = d_lo; d < d_hi; ++d) {
+        const double cur = u[(size_t)d * E];
+        acc += segment_integral(tab, d, prev, cur);
+        prev = cur;
     }
-    const double cum = carry;
-    if (total && lane == 0) total[(size_t)s * E + e] = cum;
-    for (int d = lane; d < kND; d += 64) {
-        if (cum != 0.0 && d > 0) pi[(size_t)d * E] /= cum;
-        u[(size_t)d * E] /= cum;
+    psum[part][el] = acc;
+    __syncthreads();
+    double carry = 0.0, tot = 0.0;
+    for (int q = 0; q < kIntP; ++q) {
+        const double v = psum[q][el];
+        if (q < part) carry += v;
+        tot += v;
     }
+    // running sums, normalised by the total (node 0 stays 0)
+    double run = carry;
+    prev = u[(size_t)(d_lo - 1) * E];
+    if (valid && part == 0) pi[0] = 0.0;
+    for (int d = d_lo; d < d_hi; ++d) {
+        const double cur = u[(size_t)d * E];
+        run += segment_integral(tab, d, prev, cur);
+        prev = cur;
+        if (valid) pi[(size_t)d * E] = tot != 0.0 ? run / tot : run;
+    }
+    if (valid && part == 0 && total) total[(size_t)s * E + e] = tot;
+    __syncthreads();
+    if (!valid) return;
+    if (part == 0) u[0] /= tot;
+    for (int d = d_lo; d < d_hi; ++d) u[(size_t)d * E] /= tot;
 }
 
 // Gauss-Legendre nodes and weights by Newton iteration on P_n.
@@ -333,11 +382,17 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
         g[0].Mc, g[0].Me, g[0].params, dsep, Kf);
     king_kernel<<<std::min<unsigned>(div_up(kb, 256), 4096), 256, 0, ctx->stream>>>(
         g[1].Mc, g[1].Me, g[1].params, dsep, Kb);
+    void* bp;
+    XTS_TRY(slot_reserve(ctx, kSlotBilerp, (size_t)2 * C * E * sizeof(BilerpPoint), &bp));
+    BilerpPoint* bpf = (BilerpPoint*)bp;
+    BilerpPoint* bpb = bpf + (size_t)C * E;
+    bilerp_points_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
+        C, E, g[0].Mc, g[0].Me, costh, logE, g[0].cosths, g[0].logEs, bpf, err_dev);
+    bilerp_points_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
+        C, E, g[1].Mc, g[1].Me, costh, logE, g[1].cosths, g[1].logEs, bpb, err_dev);
     const unsigned bb = std::min<unsigned>(div_up(npa, 256), 16384);
-    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[0].Mc, g[0].Me, costh, logE,
-                                                    g[0].cosths, g[0].logEs, Kf, af, PAf, err_dev);
-    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[1].Mc, g[1].Me, costh, logE,
-                                                    g[1].cosths, g[1].logEs, Kb, ab, PAb, err_dev);
+    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[0].Mc, g[0].Me, bpf, Kf, af, PAf);
+    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[1].Mc, g[1].Me, bpb, Kb, ab, PAb);
     if (precision == XTS_PREC_FP64) {
         const size_t sm = 2 * kSrcTile * C * 8;
         mean_psf_kernel<true><<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
@@ -356,8 +411,8 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
         mean_psf_combine_kernel<<<std::min<unsigned>(div_up((int64_t)S * N, 256), 16384), 256, 0,
                                   ctx->stream>>>(S, E, P, ltf, expo, upsf);
     }
-    integral_kernel<<<dim3(div_up(E, 4), S), 256, 0, ctx->stream>>>(S, E, upsf, parint, nullptr,
-                                                                     ct->theta);
+    integral_kernel<<<dim3(div_up(E, kIntE), S), kIntE * kIntP, 0, ctx->stream>>>(
+        S, E, upsf, parint, nullptr, ct->theta);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -500,7 +555,7 @@ int xts_partial_total_integral(xts_ctx* ctx, double* upsf, int64_t S, int64_t E,
     double* dtot = dpar + n;
     const xts_const_tables* ct;
     XTS_TRY(const_tables(ctx, &ct));
-    integral_kernel<<<dim3(div_up(E, 4), (unsigned)S), 256, 0, ctx->stream>>>(
+    integral_kernel<<<dim3(div_up(E, kIntE), (unsigned)S), kIntE * kIntP, 0, ctx->stream>>>(
         (int)S, (int)E, (double*)du, dpar, dtot, ct->theta);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(upsf, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -525,9 +580,14 @@ int xts_bilerp(xts_ctx* ctx, const double* costh, int64_t C, const double* logE,
     XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8, &dout));
     const int zero = 0;
     XTS_TRY(to_device(ctx, kSlotOut3, &zero, sizeof(int), &derr));
-    bilerp_aeff_kernel<<<std::min<unsigned>(div_up(n, 256), 16384), 256, 0, ctx->stream>>>(
+    void* bp;
+    XTS_TRY(slot_reserve(ctx, kSlotBilerp, (size_t)C * E * sizeof(BilerpPoint), &bp));
+    bilerp_points_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
         (int)C, (int)E, (int)grid->Mc, (int)grid->Me, (double*)dc, (double*)de, (double*)dpc,
-        (double*)dpe, (double*)dk, nullptr, (double*)dout, (int*)derr);
+        (double*)dpe, (BilerpPoint*)bp, (int*)derr);
+    bilerp_aeff_kernel<<<std::min<unsigned>(div_up(n, 256), 16384), 256, 0, ctx->stream>>>(
+        (int)C, (int)E, (int)grid->Mc, (int)grid->Me, (BilerpPoint*)bp, (double*)dk, nullptr,
+        (double*)dout);
     XTS_HIP(hipGetLastError());
     int err = 0;
     XTS_HIP(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
