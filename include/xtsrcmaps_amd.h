@@ -91,8 +91,9 @@ int xts_ctx_synchronize(xts_ctx* ctx);
 const char* xts_last_error(void);
 int xts_abi_version(void);
 /* Instrumentation (no reference counterpart): when enabled, kernel launches
- * of the named model-map stages ("cubature": the row-weight pass, "near",
- * "factor", "emit", "near_scale") are bracketed by events on the
+ * of the named model-map stages ("cubature": the row-weight pass, "near":
+ * the near-pixel rows, "factor", "emit", "near_emit": the near and wide
+ * pixels' map values) are bracketed by events on the
  * context stream; xts_ctx_kernel_times synchronises and returns (and
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);

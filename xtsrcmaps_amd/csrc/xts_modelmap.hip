@@ -292,6 +292,7 @@ template <typename T>
 struct PixelGeom {
     T A[3], B[3], C[3], E[3];
     T slo, shi;
+    T x0, hr;   // half-chord of the centre, bound on the points' offset from it
 };
 
 template <typename T>
@@ -325,6 +326,8 @@ __device__ inline PixelGeom<T> pixel_geom(const double* __restrict__ corners, in
         hr = fmax(hr, fast_sqrt(ex * ex + ey * ey + ez * ez));
     }
     hr = hr * (T)rho_scale * T(1.0001) + T(1e-5) * x0;
+    g.x0 = x0;
+    g.hr = hr;
     g.slo = T(2.0 * kR2D) * asin_pos(fmax(x0 - hr, T(0)));
     g.shi = T(2.0 * kR2D) * asin_pos(fmin(x0 + hr, T(1)));
     return g;
@@ -454,11 +457,18 @@ constexpr double kTPoly[3] = {0.9832291565474904, 0.016581961801669792, 0.000188
 
 // z(u, v) = a(u) + v (b(u) + v c(u)), each of a, b, c quadratic in u, plus
 // the Q offset and the g coefficients scaled to z: g_k y^k = g_k' z^k.
+// NG = 1 is g linearised about the pixel centre,
+//   g(y) ~ g(y_c) + g'(y_c) y_c (z - 1),
+// whose error g''(y) (y - y_c)^2 / 2 is bounded by lin_q_error.
 struct LeanPixel {
     float a0, a1, a2, b0, b1, b2, c0, c1, c2;
     float q0;
     float g[4];
 };
+
+// Upper bound of g'' on separations below 28.5 deg (g'' rises from 0.1206
+// at y = 0 to 0.1366 at sin^2(14.25 deg)).
+constexpr double kG2Max = 0.14;
 
 template <int NG>
 __device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
@@ -477,33 +487,51 @@ __device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
     lp.c1 = 2.0f * dot(C, E) * iy;
     lp.c2 = dot(E, E) * iy;
     // log2 z = log2 y + log2 iy exactly as the kernel forms z = y iy
-    lp.q0 = (float)(1.0 + (kLnTwoR2D + kLn1e4) * kRecipStep -
-                    0.5 * kRecipStep * log((double)iy) - (double)r0);
-    const double* gc = NG == 3 ? kGCubic : kGQuartic;
-    double ycp = 1.0 / (double)iy;
-    const double yk = ycp;
-    for (int k = 0; k < 4; ++k) {
-        lp.g[k] = k < NG ? (float)(gc[k] * kRecipStep * ycp) : 0.0f;
-        ycp *= yk;
+    double q0 = 1.0 + (kLnTwoR2D + kLn1e4) * kRecipStep -
+                0.5 * kRecipStep * log((double)iy) - (double)r0;
+    const double ycd = 1.0 / (double)iy;
+    if constexpr (NG == 1) {
+        const double* gc = kGQuartic;
+        const double gv = ycd * (gc[0] + ycd * (gc[1] + ycd * (gc[2] + ycd * gc[3])));
+        const double gd = gc[0] + ycd * (2.0 * gc[1] + ycd * (3.0 * gc[2] + ycd * 4.0 * gc[3]));
+        q0 += kRecipStep * (gv - ycd * gd);
+        lp.g[0] = (float)(kRecipStep * ycd * gd);
+        lp.g[1] = lp.g[2] = lp.g[3] = 0.0f;
+    } else {
+        const double* gc = NG == 3 ? kGCubic : kGQuartic;
+        double ycp = ycd;
+        for (int k = 0; k < 4; ++k) {
+            lp.g[k] = k < NG ? (float)(gc[k] * kRecipStep * ycp) : 0.0f;
+            ycp *= ycd;
+        }
     }
+    lp.q0 = (float)q0;
     return lp;
 }
 
-// One point: wq = w/4; tA, tB, tC = wq (t1, t2, t3).  Accumulates the
-// cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1 likewise
-// of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
+// Bound on the error in Q of the linearised g: the points' y lie within
+// (x0 + hr)^2 - x0^2 of y_c = x0^2.
+__device__ inline float lin_q_error(const PixelGeom<float>& pg) {
+    const float dy = pg.hr * (2.0f * pg.x0 + pg.hr);
+    return (float)(0.5 * kG2Max * kRecipStep) * dy * dy;
+}
+constexpr float kLinQTol = 1e-7f;
+
+// One point of weight wq = w/4: t = f (t1 + f (t2 + f t3)).  Accumulates
+// the cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1
+// likewise of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
 // 2^24 (1 - Q) >= 1 whenever Q < 1.
 template <int NG>
-__device__ inline void lean_point(float z, float wq, float tA, float tB, float tC,
-                                  const LeanPixel& lp, float& C0, float& D0, float& C1,
-                                  float& D1, float& Tt) {
+__device__ inline void lean_point(float z, float wq, const LeanPixel& lp, float& C0, float& D0,
+                                  float& C1, float& D1, float& Tt) {
     const float L = __builtin_amdgcn_logf(z);
     float g = lp.g[NG - 1];
 #pragma unroll
     for (int k = NG - 2; k >= 0; --k) g = fmaf(g, z, lp.g[k]);
     const float Q = fmaf(g, z, fmaf((float)(0.5 * kRecipStep * 0.69314718055994530942), L, lp.q0));
     const float f = __builtin_amdgcn_fractf(Q);
-    const float wt = f * fmaf(f, fmaf(f, tC, tB), tA);
+    const float tt = f * fmaf(f, fmaf(f, (float)kTPoly[2], (float)kTPoly[1]), (float)kTPoly[0]);
+    const float wt = wq * tt;
     const float m0 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 16777216.0f), 0.0f, 1.0f);
     const float m1 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 33554432.0f), 0.0f, 1.0f);
     C0 = fmaf(wq, m0, C0);
@@ -513,75 +541,76 @@ __device__ inline void lean_point(float z, float wq, float tA, float tB, float t
     Tt += wt;
 }
 
+// FP32 rule in LDS as three arrays pu, pv, pw (u, v, w/4), so that a group
+// of four points takes one 16-byte read per array.
+struct LeanRule {
+    const float* pu;
+    const float* pv;
+    const float* pw;
+};
+
 template <int NG>
-__device__ inline void lean_tensor_loop(const CubArgs& a, const float4* __restrict__ rule_s,
-                                        const float4* __restrict__ tco_s,
+__device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
                                         const PixelGeom<float>& pg, int r0, float& C0, float& D0,
                                         float& C1, float& D1, float& Tt) {
     const LeanPixel lp = lean_pixel<NG>(pg, r0);
     const int nv = a.tensor_nv;
     for (int i = 0; i < a.tensor_nu; ++i) {
-        const float u = rule_s[i * nv].x;
+        const float u = lr.pu[i * nv];
         const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
         const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
         const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
         for (int jj = 0; jj < nv; jj += 4) {
-            float4 r[4], tc[4];
+            const float4 v4 = *reinterpret_cast<const float4*>(lr.pv + i * nv + jj);
+            const float4 w4 = *reinterpret_cast<const float4*>(lr.pw + i * nv + jj);
+            const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+            const float ww[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                r[k] = rule_s[i * nv + jj + k];
-                tc[k] = tco_s[i * nv + jj + k];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float v = r[k].y;
-                lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), r[k].z, tc[k].x, tc[k].y, tc[k].z,
-                               lp, C0, D0, C1, D1, Tt);
-            }
+            for (int k = 0; k < 4; ++k)
+                lean_point<NG>(fmaf(fmaf(cu, vv[k], bu), vv[k], au), ww[k], lp, C0, D0, C1, D1,
+                               Tt);
         }
     }
 }
 
 template <int NG>
-__device__ inline void lean_list_loop(int N4, const float4* __restrict__ rule_s,
-                                      const float4* __restrict__ tco_s,
-                                      const PixelGeom<float>& pg, int r0, float& C0, float& D0,
-                                      float& C1, float& D1, float& Tt) {
+__device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeom<float>& pg,
+                                      int r0, float& C0, float& D0, float& C1, float& D1,
+                                      float& Tt) {
     const LeanPixel lp = lean_pixel<NG>(pg, r0);
     for (int p0 = 0; p0 < N4; p0 += 4) {
-        float4 r[4], tc[4];
+        const float4 u4 = *reinterpret_cast<const float4*>(lr.pu + p0);
+        const float4 v4 = *reinterpret_cast<const float4*>(lr.pv + p0);
+        const float4 w4 = *reinterpret_cast<const float4*>(lr.pw + p0);
+        const float uu[4] = {u4.x, u4.y, u4.z, u4.w};
+        const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+        const float ww[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            r[k] = rule_s[p0 + k];
-            tc[k] = tco_s[p0 + k];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float u = r[k].x, v = r[k].y;
+            const float u = uu[k], v = vv[k];
             const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
             const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
             const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
-            lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), r[k].z, tc[k].x, tc[k].y, tc[k].z, lp,
-                           C0, D0, C1, D1, Tt);
+            lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), ww[k], lp, C0, D0, C1, D1, Tt);
         }
     }
 }
 
 // Pass 1.  Grid: (tile blocks, S); LDS: the in-radius row histogram
 // [kNDPad] of double, 2 R2D / sep [kNDPad] of T, the rule (u, v, w/4,
-// redm1 w/4) and, for FP32, the t coefficients (w/4) (t1, t2, t3), both
+// redm1 w/4) and, for FP32, the rule's u, v, w/4 as separate arrays, both
 // padded to a multiple of 4 points.
 //
 // Every point of a pixel has its separation in [slo, shi], so the rows the
 // pixel touches are known before the point loop.  A pixel whose points touch
 // at most four consecutive rows r0 .. r0+3 accumulates, per relative row,
 // the sums of w and w t in registers; the row weights follow from them.
-// Other pixels are listed for near_rows_kernel.  When every lane of a wave
-// is such a pixel below 28 degrees, the wave runs a branch-free loop over
-// the points in groups of four (grid-structured rules reuse the blend of
-// each rule row).
+// Other pixels are listed for near_rows_kernel.  When every such pixel of a
+// wave lies below 28 degrees, the wave runs a branch-free FP32 loop over the
+// points in groups of four (grid-structured rules reuse the blend of each
+// rule row).
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 rowweight_kernel(CubArgs a) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -589,7 +618,10 @@ rowweight_kernel(CubArgs a) {
     T* isep_s = reinterpret_cast<T*>(hist_s + kNDPad);
     V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
     const int N4 = (a.N + 3) & ~3;
-    float4* tco_s = reinterpret_cast<float4*>(rule_s + N4);
+    // FP32: the rule again as u, v, w/4 arrays [N4] each
+    const LeanRule lr{reinterpret_cast<const float*>(rule_s + N4),
+                      reinterpret_cast<const float*>(rule_s + N4) + N4,
+                      reinterpret_cast<const float*>(rule_s + N4) + 2 * N4};
 
     const int s = blockIdx.y;
     const int tid = threadIdx.x;
@@ -600,10 +632,12 @@ rowweight_kernel(CubArgs a) {
 
     stage_rule<T>(a, isep_s, rule_s, N4, tid, kThreads);
     if constexpr (sizeof(T) == 4) {
+        float* lean_s = reinterpret_cast<float*>(rule_s + N4);
         for (int p = tid; p < N4; p += kThreads) {
-            const float wq = p < a.N ? (float)a.uvw[p].wq : 0.0f;
-            tco_s[p] = float4{wq * (float)kTPoly[0], wq * (float)kTPoly[1], wq * (float)kTPoly[2],
-                              0.0f};
+            const bool in = p < a.N;
+            lean_s[p] = in ? (float)a.uvw[p].u : 0.0f;
+            lean_s[N4 + p] = in ? (float)a.uvw[p].v : 0.0f;
+            lean_s[2 * N4 + p] = in ? (float)a.uvw[p].wq : 0.0f;
         }
     }
     for (int k = tid; k < kND; k += kThreads) hist_s[k] = 0.0;
@@ -634,16 +668,24 @@ rowweight_kernel(CubArgs a) {
 
         // S0 = sum_{rel=0} w, S01 = sum_{rel<=1} w, likewise T0, T01 of w t
         T S0 = T(0), T0 = T(0), S01 = T(0), T01 = T(0), Tt = T(0);
-        const bool lean6 = __all(fast && pg.shi < T(28.0));
-        const bool lean4 = lean6 && __all(pg.shi < T(14.0));
+        // FP32: lanes that are not fast are listed for the near pass and
+        // their sums are discarded, so they may run the lean loop (which
+        // reads no memory that depends on the lane) on meaningless values.
+        // FP64 indexes the table per point and keeps every lane in range.
+        const bool lean6 = sizeof(T) == 4 ? __any(fast) && __all(!fast || pg.shi < T(28.0))
+                                          : __all(fast && pg.shi < T(28.0));
+        const bool lean4 = lean6 && __all(!fast || pg.shi < T(14.0));
         if constexpr (sizeof(T) == 4) {
             if (lean6) {
+                const bool lin = __all(!fast || lin_q_error(pg) < kLinQTol);
                 if (a.tensor_nu > 0) {
-                    if (lean4) lean_tensor_loop<3>(a, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
-                    else lean_tensor_loop<4>(a, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
+                    if (lin) lean_tensor_loop<1>(a, lr, pg, r0, S0, T0, S01, T01, Tt);
+                    else if (lean4) lean_tensor_loop<3>(a, lr, pg, r0, S0, T0, S01, T01, Tt);
+                    else lean_tensor_loop<4>(a, lr, pg, r0, S0, T0, S01, T01, Tt);
                 } else {
-                    if (lean4) lean_list_loop<3>(N4, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
-                    else lean_list_loop<4>(N4, rule_s, tco_s, pg, r0, S0, T0, S01, T01, Tt);
+                    if (lin) lean_list_loop<1>(N4, lr, pg, r0, S0, T0, S01, T01, Tt);
+                    else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, S0, T0, S01, T01, Tt);
+                    else lean_list_loop<4>(N4, lr, pg, r0, S0, T0, S01, T01, Tt);
                 }
             }
         } else if (lean6) {
@@ -1580,19 +1622,17 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.j0 = 1.0 + kRecipStep * (kLn1e4 + std::log(2.0 * kR2D));
     ca.jk = kRecipStep * 0.69314718055994530942;
 
-    // pass 1: each block takes a run of tiles of one source, enough blocks
-    // to fill every CU a few times over
+    // pass 1: one tile per block by default (many short blocks keep the
+    // last round of the grid short); a run of tiles per block on request
     const int ntiles = ca.tiles_w * ca.tiles_h;
     const int64_t want_blocks = (int64_t)ctx->cu_count * 4;
-    const int64_t tpb = ctx->tune_tiles_per_block > 0
-                            ? ctx->tune_tiles_per_block
-                            : ((int64_t)ntiles * S + want_blocks - 1) / want_blocks;
+    const int64_t tpb = ctx->tune_tiles_per_block > 0 ? ctx->tune_tiles_per_block : 1;
     ca.tiles_per_block = (int)std::min<int64_t>(std::max<int64_t>(tpb, 1), ntiles);
     const int N4 = (int)((N + 3) / 4 * 4);
     const size_t tsz = f64 ? 8 : 4;
     {
         const size_t sm = (size_t)kNDPad * 8 + (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz +
-                          (size_t)N4 * 16;
+                          (size_t)N4 * 12;
         const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
         if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
