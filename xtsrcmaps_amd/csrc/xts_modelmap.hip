@@ -1271,47 +1271,59 @@ static int allow_lds(const void* fn, size_t bytes) {
 }
 
 template <typename T, int EC, typename OT, bool SCALED>
-static int launch_emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid, bool sehw) {
+static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) {
     const size_t sm = (size_t)kND * (sehw ? EC + 1 : EC) * sizeof(T);
     const void* fn = sehw ? (const void*)emit_sehw_kernel<T, EC, OT, SCALED>
                           : (const void*)emit_shwe_kernel<T, EC, OT, SCALED>;
     XTS_TRY(allow_lds(fn, sm));
-    if (sehw) emit_sehw_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
-    else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, ctx->stream>>>(ca);
+    if (sehw) emit_sehw_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, st>>>(ca);
+    else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, st>>>(ca);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
 
+// Pooled near pixels (wide = false) or wide pixels (wide = true: a few
+// waves per source scan the list).
 template <typename T, int EC, typename OT, bool SCALED>
-static int launch_near_emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid) {
-    const size_t sm = (size_t)kND * EC * sizeof(T);
-    XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED>, sm));
-    near_emit_kernel<T, EC, OT, SCALED><<<grid, kNearThreads, sm, ctx->stream>>>(ca);
-    // wide pixels: a few waves per source scan the list
-    const dim3 wgrid(kWideBlocks, grid.y, grid.z);
-    wide_pixel_kernel<T, EC, OT, SCALED><<<wgrid, 256, 0, ctx->stream>>>(ca);
+static int launch_near_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool wide) {
+    if (wide) {
+        const dim3 wgrid(kWideBlocks, grid.y, grid.z);
+        wide_pixel_kernel<T, EC, OT, SCALED><<<wgrid, 256, 0, st>>>(ca);
+    } else {
+        const size_t sm = (size_t)kND * EC * sizeof(T);
+        XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED>, sm));
+        near_emit_kernel<T, EC, OT, SCALED><<<grid, kNearThreads, sm, st>>>(ca);
+    }
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
 
 template <typename T, int EC>
-static int near_emit(xts_ctx* ctx, const CubArgs& ca, int64_t gx, bool scaled) {
+static int near_emit(hipStream_t st, const CubArgs& ca, int64_t gx, bool scaled, bool wide) {
     const dim3 grid((unsigned)gx, (unsigned)ca.S, (unsigned)div_up(ca.E, EC));
     if (ca.out_f32)
-        return scaled ? launch_near_emit<T, EC, float, true>(ctx, ca, grid)
-                      : launch_near_emit<T, EC, float, false>(ctx, ca, grid);
-    return scaled ? launch_near_emit<T, EC, double, true>(ctx, ca, grid)
-                  : launch_near_emit<T, EC, double, false>(ctx, ca, grid);
+        return scaled ? launch_near_emit<T, EC, float, true>(st, ca, grid, wide)
+                      : launch_near_emit<T, EC, float, false>(st, ca, grid, wide);
+    return scaled ? launch_near_emit<T, EC, double, true>(st, ca, grid, wide)
+                  : launch_near_emit<T, EC, double, false>(st, ca, grid, wide);
+}
+
+// The context's second stream and its two events (created on first use).
+static int aux_stream(xts_ctx* ctx) {
+    if (!ctx->aux_stream) XTS_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    for (auto& e : ctx->aux_events)
+        if (!e) XTS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return XTS_OK;
 }
 
 template <typename T, int EC>
-static int emit(xts_ctx* ctx, const CubArgs& ca, dim3 grid, bool scaled) {
+static int emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool scaled) {
     const bool sehw = ca.layout_sehw != 0;
     if (ca.out_f32)
-        return scaled ? launch_emit<T, EC, float, true>(ctx, ca, grid, sehw)
-                      : launch_emit<T, EC, float, false>(ctx, ca, grid, sehw);
-    return scaled ? launch_emit<T, EC, double, true>(ctx, ca, grid, sehw)
-                  : launch_emit<T, EC, double, false>(ctx, ca, grid, sehw);
+        return scaled ? launch_emit<T, EC, float, true>(st, ca, grid, sehw)
+                      : launch_emit<T, EC, float, false>(st, ca, grid, sehw);
+    return scaled ? launch_emit<T, EC, double, true>(st, ca, grid, sehw)
+                  : launch_emit<T, EC, double, false>(st, ca, grid, sehw);
 }
 
 // ------------------------------------------------------------------ factors
@@ -1666,6 +1678,22 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                                                                   expo, fac, nullptr);
         XTS_HIP(hipGetLastError());
     }
+    // Wide pixels depend only on the factors and the near lists, and write
+    // values no other pass writes: they run on the second stream while the
+    // far pixels are emitted (on the context stream when stage timing is
+    // on, whose events are on the context stream).
+    hipStream_t wide_st = ctx->stream;
+    if (!ctx->timing) {
+        XTS_TRY(aux_stream(ctx));
+        wide_st = ctx->aux_stream;
+        XTS_HIP(hipEventRecord(ctx->aux_events[0], ctx->stream));
+        XTS_HIP(hipStreamWaitEvent(wide_st, ctx->aux_events[0], 0));
+    }
+    if (wide_st != ctx->stream) {
+        if (f64) XTS_TRY((near_emit<double, 16>(wide_st, ca, near_gx, full, true)));
+        else XTS_TRY((near_emit<float, 40>(wide_st, ca, near_gx, full, true)));
+        XTS_HIP(hipEventRecord(ctx->aux_events[1], wide_st));
+    }
     // pass 3: blocks of kEmitBlockPix pixels of one source
     {
         const dim3 grid32((unsigned)div_up(plane, kEmitBlockPix), (unsigned)S,
@@ -1673,15 +1701,20 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         const dim3 grid64((unsigned)div_up(plane, kEmitBlockPix), (unsigned)S,
                           (unsigned)div_up(E, kEmitEC64));
         TimedRegion tr(ctx, "emit");
-        if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx, ca, grid64, full)));
-        else XTS_TRY((emit<float, kEmitEC32>(ctx, ca, grid32, full)));
+        if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx->stream, ca, grid64, full)));
+        else XTS_TRY((emit<float, kEmitEC32>(ctx->stream, ca, grid32, full)));
     }
     // pass 2b: listed pixels, scaled, written once
     {
         TimedRegion tr(ctx, "near_emit");
-        if (f64) XTS_TRY((near_emit<double, 16>(ctx, ca, near_gx, full)));
-        else XTS_TRY((near_emit<float, 40>(ctx, ca, near_gx, full)));
+        if (f64) XTS_TRY((near_emit<double, 16>(ctx->stream, ca, near_gx, full, false)));
+        else XTS_TRY((near_emit<float, 40>(ctx->stream, ca, near_gx, full, false)));
+        if (wide_st == ctx->stream) {
+            if (f64) XTS_TRY((near_emit<double, 16>(ctx->stream, ca, near_gx, full, true)));
+            else XTS_TRY((near_emit<float, 40>(ctx->stream, ca, near_gx, full, true)));
+        }
     }
+    if (wide_st != ctx->stream) XTS_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_events[1], 0));
     return XTS_OK;
 }
 
