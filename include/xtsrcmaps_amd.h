@@ -98,7 +98,7 @@ int xts_abi_version(void);
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
 /* Launch-shape override for the model map (no reference counterpart):
- * "tiles_per_block" (32x16-pixel tiles per block of the row-weight pass);
+ * "tiles_per_block" (16x16-pixel tiles per block of the row-weight pass);
  * 0 restores the default choice.  Results do not depend on it. */
 int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value);
 int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms,

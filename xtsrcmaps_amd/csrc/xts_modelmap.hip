@@ -137,7 +137,7 @@ __global__ void radius_kernel(Wcs w, int H, int W, int S, const double* __restri
 //                     exposure x correction factor, stored once
 //   near_emit_kernel  the listed pixels' values from their pooled rows (or
 //                     the direct cubature for the pixel holding the source)
-constexpr int kTileW = 32, kTileH = 16;   // pixels per block pass (512 threads)
+constexpr int kTileW = 16, kTileH = 16;   // pixels per block pass (256 threads; a wave: 16x4)
 constexpr int kThreads = kTileW * kTileH;
 constexpr int kNearThreads = 256;
 constexpr int kEmitThreads = 512;
@@ -336,12 +336,20 @@ __device__ inline PixelGeom<T> pixel_geom(const double* __restrict__ corners, in
 // Whether the pixel centre lies within the boundary radius
 // (map_integral.cxx:29-36), decided on the chord with an exact check near
 // the threshold.
+// The exact test inside the bracket (rare), kept out of line so that it is
+// branched to rather than evaluated for every pixel.
+__device__ __attribute__((noinline)) bool in_radius_exact(double c2, double radius_deg) {
+    return 2.0 * asin(0.5 * sqrt(c2)) <= kD2R * radius_deg;
+}
+
 __device__ inline bool in_radius(const SrcGeom& g, const double* __restrict__ centres, int idx) {
     const double* c = centres + 3 * idx;
     const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
-    const double chd = sqrt(ex * ex + ey * ey + ez * ez);
-    return chd < g.chord_lo ? true
-         : (chd > g.chord_hi ? false : (2.0 * asin(0.5 * chd) <= kD2R * g.radius_deg));
+    const double c2 = ex * ex + ey * ey + ez * ez;
+    // the bracket's 1e-12 margins dwarf the rounding of the squares
+    if (c2 < g.chord_lo * g.chord_lo) return true;
+    if (c2 > g.chord_hi * g.chord_hi) return false;
+    return in_radius_exact(c2, g.radius_deg);
 }
 
 // Flat index of map value (s, pixel, e) in either layout.
@@ -466,6 +474,27 @@ struct LeanPixel {
     float g[4];
 };
 
+// Natural log of a positive normal double: x = m 2^k with m in
+// [sqrt(1/2), sqrt(2)), ln x = k ln 2 + 2 atanh(t), t = (m - 1) / (m + 1),
+// |t| <= 0.1716; the atanh series to t^11 leaves < 3e-11 absolute.
+__device__ inline double ln_pos(double x) {
+    const long long bits = __double_as_longlong(x);
+    int k = (int)((bits >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
+    if (m > 1.4142135623730951) {
+        m *= 0.5;
+        ++k;
+    }
+    const double t = (m - 1.0) / (m + 1.0), t2 = t * t;
+    double p = 1.0 / 11.0;
+    p = fma(p, t2, 1.0 / 9.0);
+    p = fma(p, t2, 1.0 / 7.0);
+    p = fma(p, t2, 1.0 / 5.0);
+    p = fma(p, t2, 1.0 / 3.0);
+    p = fma(p, t2, 1.0);
+    return fma((double)k, 0.69314718055994530942, 2.0 * t * p);
+}
+
 // Upper bound of g'' on separations below 28.5 deg (g'' rises from 0.1206
 // at y = 0 to 0.1366 at sin^2(14.25 deg)).
 constexpr double kG2Max = 0.14;
@@ -488,7 +517,7 @@ __device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
     lp.c2 = dot(E, E) * iy;
     // log2 z = log2 y + log2 iy exactly as the kernel forms z = y iy
     double q0 = 1.0 + (kLnTwoR2D + kLn1e4) * kRecipStep -
-                0.5 * kRecipStep * log((double)iy) - (double)r0;
+                0.5 * kRecipStep * ln_pos((double)iy) - (double)r0;
     const double ycd = 1.0 / (double)iy;
     if constexpr (NG == 1) {
         const double* gc = kGQuartic;
@@ -517,21 +546,21 @@ __device__ inline float lin_q_error(const PixelGeom<float>& pg) {
 }
 constexpr float kLinQTol = 1e-7f;
 
-// One point of weight wq = w/4: t = f (t1 + f (t2 + f t3)).  Accumulates
-// the cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1
-// likewise of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
+// One point: wq = w/4; tA, tB, tC = wq (t1, t2, t3).  Accumulates the
+// cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1 likewise
+// of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
 // 2^24 (1 - Q) >= 1 whenever Q < 1.
 template <int NG>
-__device__ inline void lean_point(float z, float wq, const LeanPixel& lp, float& C0, float& D0,
-                                  float& C1, float& D1, float& Tt) {
+__device__ inline void lean_point(float z, float wq, float tA, float tB, float tC,
+                                  const LeanPixel& lp, float& C0, float& D0, float& C1,
+                                  float& D1, float& Tt) {
     const float L = __builtin_amdgcn_logf(z);
     float g = lp.g[NG - 1];
 #pragma unroll
     for (int k = NG - 2; k >= 0; --k) g = fmaf(g, z, lp.g[k]);
     const float Q = fmaf(g, z, fmaf((float)(0.5 * kRecipStep * 0.69314718055994530942), L, lp.q0));
     const float f = __builtin_amdgcn_fractf(Q);
-    const float tt = f * fmaf(f, fmaf(f, (float)kTPoly[2], (float)kTPoly[1]), (float)kTPoly[0]);
-    const float wt = wq * tt;
+    const float wt = f * fmaf(f, fmaf(f, tC, tB), tA);
     const float m0 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 16777216.0f), 0.0f, 1.0f);
     const float m1 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 33554432.0f), 0.0f, 1.0f);
     C0 = fmaf(wq, m0, C0);
@@ -541,13 +570,21 @@ __device__ inline void lean_point(float z, float wq, const LeanPixel& lp, float&
     Tt += wt;
 }
 
-// FP32 rule in LDS as three arrays pu, pv, pw (u, v, w/4), so that a group
-// of four points takes one 16-byte read per array.
+// FP32 rule in LDS as arrays [N4] of u, v, w/4 and w/4 (t1, t2, t3), so that
+// a group of four points takes one 16-byte read per array.
 struct LeanRule {
-    const float* pu;
-    const float* pv;
-    const float* pw;
+    const float* p[6];   // u, v, wq, tA, tB, tC
 };
+constexpr int kLeanArrays = 6;
+
+__device__ inline void lean_group(const LeanRule& lr, int p0, float4 (&r)[kLeanArrays]) {
+#pragma unroll
+    for (int j = 0; j < kLeanArrays; ++j) r[j] = *reinterpret_cast<const float4*>(lr.p[j] + p0);
+}
+
+__device__ inline float lane(const float4& x, int k) {
+    return k == 0 ? x.x : (k == 1 ? x.y : (k == 2 ? x.z : x.w));
+}
 
 template <int NG>
 __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
@@ -556,19 +593,22 @@ __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
     const LeanPixel lp = lean_pixel<NG>(pg, r0);
     const int nv = a.tensor_nv;
     for (int i = 0; i < a.tensor_nu; ++i) {
-        const float u = lr.pu[i * nv];
+        const float u = lr.p[0][i * nv];
         const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
         const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
         const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
         for (int jj = 0; jj < nv; jj += 4) {
-            const float4 v4 = *reinterpret_cast<const float4*>(lr.pv + i * nv + jj);
-            const float4 w4 = *reinterpret_cast<const float4*>(lr.pw + i * nv + jj);
-            const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
-            const float ww[4] = {w4.x, w4.y, w4.z, w4.w};
+            const int p0 = i * nv + jj;
+            float4 r[kLeanArrays - 1];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                lean_point<NG>(fmaf(fmaf(cu, vv[k], bu), vv[k], au), ww[k], lp, C0, D0, C1, D1,
-                               Tt);
+            for (int j = 1; j < kLeanArrays; ++j)
+                r[j - 1] = *reinterpret_cast<const float4*>(lr.p[j] + p0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float v = lane(r[0], k);
+                lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), lane(r[1], k), lane(r[2], k),
+                               lane(r[3], k), lane(r[4], k), lp, C0, D0, C1, D1, Tt);
+            }
         }
     }
 }
@@ -579,27 +619,24 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
                                       float& Tt) {
     const LeanPixel lp = lean_pixel<NG>(pg, r0);
     for (int p0 = 0; p0 < N4; p0 += 4) {
-        const float4 u4 = *reinterpret_cast<const float4*>(lr.pu + p0);
-        const float4 v4 = *reinterpret_cast<const float4*>(lr.pv + p0);
-        const float4 w4 = *reinterpret_cast<const float4*>(lr.pw + p0);
-        const float uu[4] = {u4.x, u4.y, u4.z, u4.w};
-        const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
-        const float ww[4] = {w4.x, w4.y, w4.z, w4.w};
+        float4 r[kLeanArrays];
+        lean_group(lr, p0, r);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float u = uu[k], v = vv[k];
+            const float u = lane(r[0], k), v = lane(r[1], k);
             const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
             const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
             const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
-            lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), ww[k], lp, C0, D0, C1, D1, Tt);
+            lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), lane(r[2], k), lane(r[3], k),
+                           lane(r[4], k), lane(r[5], k), lp, C0, D0, C1, D1, Tt);
         }
     }
 }
 
 // Pass 1.  Grid: (tile blocks, S); LDS: the in-radius row histogram
 // [kNDPad] of double, 2 R2D / sep [kNDPad] of T, the rule (u, v, w/4,
-// redm1 w/4) and, for FP32, the rule's u, v, w/4 as separate arrays, both
-// padded to a multiple of 4 points.
+// redm1 w/4) and, for FP32, the arrays of LeanRule, both padded to a
+// multiple of 4 points.
 //
 // Every point of a pixel has its separation in [slo, shi], so the rows the
 // pixel touches are known before the point loop.  A pixel whose points touch
@@ -610,7 +647,7 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
 // points in groups of four (grid-structured rules reuse the blend of each
 // rule row).
 template <typename T>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5)))
 rowweight_kernel(CubArgs a) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -618,10 +655,11 @@ rowweight_kernel(CubArgs a) {
     T* isep_s = reinterpret_cast<T*>(hist_s + kNDPad);
     V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
     const int N4 = (a.N + 3) & ~3;
-    // FP32: the rule again as u, v, w/4 arrays [N4] each
-    const LeanRule lr{reinterpret_cast<const float*>(rule_s + N4),
-                      reinterpret_cast<const float*>(rule_s + N4) + N4,
-                      reinterpret_cast<const float*>(rule_s + N4) + 2 * N4};
+    // FP32: the rule again as kLeanArrays arrays [N4]
+    float* lean_s = reinterpret_cast<float*>(rule_s + N4);
+    LeanRule lr;
+#pragma unroll
+    for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = lean_s + j * N4;
 
     const int s = blockIdx.y;
     const int tid = threadIdx.x;
@@ -632,12 +670,14 @@ rowweight_kernel(CubArgs a) {
 
     stage_rule<T>(a, isep_s, rule_s, N4, tid, kThreads);
     if constexpr (sizeof(T) == 4) {
-        float* lean_s = reinterpret_cast<float*>(rule_s + N4);
         for (int p = tid; p < N4; p += kThreads) {
             const bool in = p < a.N;
+            const float wq = in ? (float)a.uvw[p].wq : 0.0f;
             lean_s[p] = in ? (float)a.uvw[p].u : 0.0f;
             lean_s[N4 + p] = in ? (float)a.uvw[p].v : 0.0f;
-            lean_s[2 * N4 + p] = in ? (float)a.uvw[p].wq : 0.0f;
+            lean_s[2 * N4 + p] = wq;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lean_s[(3 + k) * N4 + p] = wq * (float)kTPoly[k];
         }
     }
     for (int k = tid; k < kND; k += kThreads) hist_s[k] = 0.0;
@@ -1644,7 +1684,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const size_t tsz = f64 ? 8 : 4;
     {
         const size_t sm = (size_t)kNDPad * 8 + (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz +
-                          (size_t)N4 * 12;
+                          (size_t)N4 * 4 * kLeanArrays;
         const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
         if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);

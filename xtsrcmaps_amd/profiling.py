@@ -12,6 +12,7 @@ Summaries of rocprofv3 CSV output (plain Python, no GPU):
     stats <kernel_stats.csv> --cmd CMD           Markdown table of a --stats run
     traffic --mm-fetch D --mm-write D --copy-fetch D --copy-write D
             --copy-mib M --cmd CMD               JSON: HBM bytes per model map
+    valu <pmc run dir> --kernel K --cmd CMD      JSON: VALU issue of one kernel
 """
 from __future__ import annotations
 
@@ -23,7 +24,8 @@ import os
 import re
 import sys
 
-MM_KERNELS = r"rowweight_kernel|near_pixel_kernel|hist_integral_kernel|factor_kernel|emit_s|near_scale_kernel"
+MM_KERNELS = (r"rowweight_kernel|near_rows_kernel|hist_integral_kernel|factor_kernel|emit_s|"
+              r"near_emit_kernel|wide_pixel_kernel")
 COPY_KERNELS = r"MulFunctor"          # torch.mul(src, 1.0, out=dst): reads and writes N bytes
 
 
@@ -107,9 +109,10 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
     fetch_b = mf * 1024 / calls / f_fetch
     write_b = mw * 1024 / calls / f_write
     per_kernel = {}
-    for name, rx in (("rowweight", "rowweight_kernel"), ("near", "near_pixel_kernel"),
+    for name, rx in (("rowweight", "rowweight_kernel"), ("near_rows", "near_rows_kernel"),
                      ("hist_factor", "hist_integral_kernel|factor_kernel"),
-                     ("emit", "emit_s"), ("near_scale", "near_scale_kernel")):
+                     ("emit", "emit_s"), ("near_emit", "near_emit_kernel"),
+                     ("wide_pixel", "wide_pixel_kernel")):
         f, _ = _counter_sum(mm_fetch, "FETCH_SIZE", rx)
         w, _ = _counter_sum(mm_write, "WRITE_SIZE", rx)
         per_kernel[name] = {"fetch_bytes": f * 1024 / calls / f_fetch,
@@ -119,6 +122,41 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
             "dispatches_fetch": mnf, "dispatches_write": mnw,
             "fetch_bytes_per_call": fetch_b, "write_bytes_per_call": write_b,
             "traffic_bytes_per_call": fetch_b + write_b, "per_kernel": per_kernel}
+
+
+# VALU counters of one kernel (one rocprofv3 --pmc pass with SQ_WAVES,
+# SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_WAIT_INST_ANY, SQ_WAVE_CYCLES and
+# GRBM_GUI_ACTIVE).  GRBM_GUI_ACTIVE is summed over the 8 XCDs, so the
+# shader clock is GRBM_GUI_ACTIVE / 8 / duration; a wave64 VALU instruction
+# occupies one of the 1024 SIMDs for 4 cycles (transcendentals count once).
+N_XCD = 8
+N_SIMD = 1024
+
+
+def valu_json(run_dir: str, kernel_rx: str, cmd: str) -> dict:
+    path = _one(os.path.join(run_dir, "**", "*counter_collection.csv"))
+    rx = re.compile(kernel_rx)
+    disp = {}
+    for r in csv.DictReader(open(path)):
+        if not rx.search(r["Kernel_Name"]):
+            continue
+        d = disp.setdefault(r["Dispatch_Id"], {"ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+                                                "name": r["Kernel_Name"]})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    if not disp:
+        raise ValueError(f"no dispatch of {kernel_rx} in {path}")
+    rows = list(disp.values())
+    mean = lambda k: sum(d[k] for d in rows) / len(rows)
+    ns = mean("ns")
+    clk_ghz = mean("GRBM_GUI_ACTIVE") / N_XCD / ns
+    valu = mean("SQ_INSTS_VALU")
+    util = valu * 4 / (N_SIMD * mean("GRBM_GUI_ACTIVE") / N_XCD)
+    return {"command": cmd, "kernel": rows[0]["name"], "dispatches": len(rows),
+            "duration_us": ns / 1e3, "clock_ghz": clk_ghz, "waves": mean("SQ_WAVES"),
+            "valu_insts": valu, "lds_insts": mean("SQ_INSTS_LDS"),
+            "valu_insts_per_wave": valu / mean("SQ_WAVES"),
+            "wait_any_per_wave_cycles": mean("SQ_WAIT_INST_ANY") / mean("SQ_WAVE_CYCLES"),
+            "valu_issue_frac": util}
 
 
 def main(argv=None) -> None:
@@ -138,6 +176,10 @@ def main(argv=None) -> None:
         p.add_argument("--" + k, required=True)
     p.add_argument("--copy-mib", type=int, required=True)
     p.add_argument("--cmd", required=True)
+    p = sub.add_parser("valu")
+    p.add_argument("run_dir")
+    p.add_argument("--kernel", default="rowweight_kernel<float>")
+    p.add_argument("--cmd", required=True)
     a = ap.parse_args(argv)
     if a.what == "modelmap":
         _modelmap(a.config, a.calls)
@@ -145,6 +187,8 @@ def main(argv=None) -> None:
         _copy(a.mib, a.calls)
     elif a.what == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.cmd))
+    elif a.what == "valu":
+        print(json.dumps(valu_json(a.run_dir, re.escape(a.kernel), a.cmd), indent=1))
     else:
         print(json.dumps(traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write,
                                       a.copy_mib, a.cmd), indent=1))
