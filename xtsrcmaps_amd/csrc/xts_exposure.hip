@@ -119,24 +119,43 @@ __global__ void contract_kernel(int M, int K, int N, const double* __restrict__ 
 
 // exposure.cxx:96-169 fused: the four contractions, the front LTF rows and
 // the sum ((lef + leb) + lwf) + lwb.
-__global__ void exposure_kernel(int S, int C, int E, const double* __restrict__ ecb,
-                                const double* __restrict__ wecb, const double* __restrict__ af,
-                                const double* __restrict__ ab, const double* __restrict__ ltf,
-                                double* __restrict__ out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= S * E) return;
-    const int s = k / E, e = k % E;
+constexpr int kExpoThreads = 256, kExpoE = 64, kExpoG = kExpoThreads / kExpoE;
+__global__ void __launch_bounds__(kExpoThreads)
+exposure_kernel(int S, int C, int E, const double* __restrict__ ecb,
+                const double* __restrict__ wecb, const double* __restrict__ af,
+                const double* __restrict__ ab, const double* __restrict__ ltf,
+                double* __restrict__ out) {
+    // block: one source, kExpoE energies; kExpoG groups of threads take every
+    // kExpoG-th cos theta bin, their partial sums are added in group order
+    __shared__ double part[4][kExpoG][kExpoE];
+    const int s = blockIdx.x, tx = threadIdx.x % kExpoE, g = threadIdx.x / kExpoE;
+    const int e = blockIdx.y * kExpoE + tx;
     double ef = 0, wf = 0, eb = 0, wb = 0;
-    for (int c = 0; c < C; ++c) {
-        const double x = ecb[(size_t)s * C + c], y = wecb[(size_t)s * C + c];
-        const double f = af[(size_t)c * E + e], b = ab[(size_t)c * E + e];
-        ef = fma(x, f, ef);
-        wf = fma(y, f, wf);
-        eb = fma(x, b, eb);
-        wb = fma(y, b, wb);
+    if (e < E) {
+        for (int c = g; c < C; c += kExpoG) {
+            const double x = ecb[(size_t)s * C + c], y = wecb[(size_t)s * C + c];
+            const double f = af[(size_t)c * E + e], b = ab[(size_t)c * E + e];
+            ef = fma(x, f, ef);
+            wf = fma(y, f, wf);
+            eb = fma(x, b, eb);
+            wb = fma(y, b, wb);
+        }
     }
-    const double l0 = ltf[e], l1 = ltf[E + e];
-    out[k] = ((ef * l0 + eb * l0) + wf * l1) + wb * l1;
+    part[0][g][tx] = ef;
+    part[1][g][tx] = wf;
+    part[2][g][tx] = eb;
+    part[3][g][tx] = wb;
+    __syncthreads();
+    if (g == 0 && e < E) {
+        for (int q = 1; q < kExpoG; ++q) {
+            ef += part[0][q][tx];
+            wf += part[1][q][tx];
+            eb += part[2][q][tx];
+            wb += part[3][q][tx];
+        }
+        const double l0 = ltf[e], l1 = ltf[E + e];
+        out[(size_t)s * E + e] = ((ef * l0 + eb * l0) + wf * l1) + wb * l1;
+    }
 }
 
 // BF16x3 variant: the sums over cos theta run on MFMA (see xts_mfma.hpp),
@@ -170,8 +189,9 @@ int exposure_device(xts_ctx* ctx, const double* ecb, const double* wecb, const d
     XTS_CHECK(precision == XTS_PREC_FP64 || precision == XTS_PREC_BF16X3, "bad precision %d",
               precision);
     if (precision == XTS_PREC_FP64) {
-        exposure_kernel<<<div_up(S * E, 256), 256, 0, ctx->stream>>>((int)S, (int)C, (int)E, ecb,
-                                                                      wecb, af, ab, ltf, out);
+        XTS_CHECK(S <= 0x7fffffff && E <= 65535LL * kExpoE, "bad sizes");
+        exposure_kernel<<<dim3((unsigned)S, (unsigned)div_up(E, kExpoE)), kExpoThreads, 0,
+                          ctx->stream>>>((int)S, (int)C, (int)E, ecb, wecb, af, ab, ltf, out);
         XTS_HIP(hipGetLastError());
         return XTS_OK;
     }

@@ -1129,7 +1129,7 @@ near_emit_kernel(CubArgs a) {
 // table), listed pixels included.  One block per source: each thread takes one energy
 // and every P-th row (consecutive threads read consecutive energies), the
 // P partial sums are added in LDS.
-constexpr int kHistThreads = 256;
+constexpr int kHistThreads = 1024;
 __global__ void __launch_bounds__(kHistThreads)
 hist_integral_kernel(int E, const double* __restrict__ hist, const double* __restrict__ upsf,
                      double* __restrict__ mapint) {
@@ -1145,8 +1145,10 @@ hist_integral_kernel(int E, const double* __restrict__ hist, const double* __res
         __syncthreads();
         const int e = eb + el;
         double acc = 0.0;
-        if (ph < P && e < E)
+        if (ph < P && e < E) {
+#pragma unroll 4
             for (int d = ph; d < kND; d += P) acc = fma(h_s[d], u[(size_t)d * E + e], acc);
+        }
         part[tid] = acc;
         __syncthreads();
         if (ph == 0 && e < E) {

@@ -65,12 +65,21 @@ __global__ void normalize_rpsf_kernel(int Mc, int Me, double* __restrict__ param
 }
 
 // psf/psf.king.cxx:9-43: K[d, c, e] at the separations (radians)
-__global__ void king_kernel(int Mc, int Me, const double* __restrict__ params,
-                            const double* __restrict__ sep, double* __restrict__ out) {
-    const int n = kND * Mc * Me;
+// Kernels over the PSF sections (front, back) take one argument set per
+// section; blockIdx.y selects it, so both sections share one launch.
+struct KingSec {
+    const double* params;
+    double* out;
+    int Mc, Me;
+};
+struct KingSecs { KingSec s[2]; };
+
+__global__ void king_kernel(KingSecs ks, const double* __restrict__ sep) {
+    const KingSec q = ks.s[blockIdx.y];
+    const int n = kND * q.Mc * q.Me;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int d = k / (Mc * Me), ce = k % (Mc * Me);
-        out[k] = evaluate_king(sep[d] * kD2R, 1.0, params + (size_t)ce * 6);
+        const int d = k / (q.Mc * q.Me), ce = k % (q.Mc * q.Me);
+        q.out[k] = evaluate_king(sep[d] * kD2R, 1.0, q.params + (size_t)ce * 6);
     }
 }
 
@@ -92,18 +101,26 @@ struct BilerpPoint {
     int pad;
 };
 
-__global__ void bilerp_points_kernel(int C, int E, int Mc, int Me, const double* __restrict__ costh,
-                                     const double* __restrict__ logE, const double* __restrict__ pc,
-                                     const double* __restrict__ pe, BilerpPoint* __restrict__ pts,
+struct BilerpSec {
+    const double* pc;   // grid cos theta nodes [Mc]
+    const double* pe;   // grid log E nodes [Me]
+    BilerpPoint* pts;   // [C * E]
+    int Mc, Me;
+};
+struct BilerpSecs { BilerpSec s[2]; };
+
+__global__ void bilerp_points_kernel(int C, int E, const double* __restrict__ costh,
+                                     const double* __restrict__ logE, BilerpSecs bs,
                                      int* __restrict__ err) {
+    const BilerpSec g = bs.s[blockIdx.y];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= C * E) return;
     const int c = k / E, e = k % E;
     double cw, ccw, ew, ecw;
     int ci, ei;
     BilerpPoint q{};
-    if (!lerp_pars_psf(pc, Mc, costh[c], &cw, &ccw, &ci) ||
-        !lerp_pars_psf(pe, Me, logE[e], &ew, &ecw, &ei)) {
+    if (!lerp_pars_psf(g.pc, g.Mc, costh[c], &cw, &ccw, &ci) ||
+        !lerp_pars_psf(g.pe, g.Me, logE[e], &ew, &ecw, &ei)) {
         atomicOr(err, 1);
         q.base = -1;
     } else {
@@ -111,31 +128,39 @@ __global__ void bilerp_points_kernel(int C, int E, int Mc, int Me, const double*
         q.w01 = ccw * ew;
         q.w10 = cw * ecw;
         q.w11 = cw * ew;
-        q.base = (ci - 1) * Me + ei - 1;
+        q.base = (ci - 1) * g.Me + ei - 1;
     }
-    pts[k] = q;
+    g.pts[k] = q;
 }
 
 // times the effective area (psf.corrected_exposure_psf.cxx:37-44):
 // PA[c, d, e] = bilerp(K)[c, d, e] x aeff[c, e] (aeff null: the bilinear
 // samples alone).  One thread per (c, d, e); K is [401][Mc][Me].
-__global__ void bilerp_aeff_kernel(int C, int E, int Mc, int Me, const BilerpPoint* __restrict__ pts,
-                                   const double* __restrict__ K, const double* __restrict__ aeff,
-                                   double* __restrict__ out) {
+struct AeffSec {
+    const BilerpPoint* pts;
+    const double* K;      // [401][Mc][Me]
+    const double* aeff;   // [C][E] or null
+    double* out;          // [C][401][E]
+    int Mc, Me;
+};
+struct AeffSecs { AeffSec s[2]; };
+
+__global__ void bilerp_aeff_kernel(int C, int E, AeffSecs as) {
+    const AeffSec g = as.s[blockIdx.y];
     const size_t n = (size_t)C * kND * E;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (size_t)gridDim.x * blockDim.x) {
         const int e = (int)(k % E);
         const int d = (int)((k / E) % kND);
         const int c = (int)(k / ((size_t)E * kND));
-        const BilerpPoint q = pts[c * E + e];
+        const BilerpPoint q = g.pts[c * E + e];
         if (q.base < 0) {
-            out[k] = 0.0;
+            g.out[k] = 0.0;
             continue;
         }
-        const double* Kd = K + (size_t)d * Mc * Me + q.base;
-        const double v = q.w00 * Kd[0] + q.w01 * Kd[1] + q.w10 * Kd[Me] + q.w11 * Kd[Me + 1];
-        out[k] = aeff ? v * aeff[(size_t)c * E + e] : v;
+        const double* Kd = g.K + (size_t)d * g.Mc * g.Me + q.base;
+        const double v = q.w00 * Kd[0] + q.w01 * Kd[1] + q.w10 * Kd[g.Me] + q.w11 * Kd[g.Me + 1];
+        g.out[k] = g.aeff ? v * g.aeff[(size_t)c * E + e] : v;
     }
 }
 
@@ -378,21 +403,20 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
     double* Kb = Kf + kf;
     double* PAf = Kb + kb;
     double* PAb = PAf + npa;
-    king_kernel<<<std::min<unsigned>(div_up(kf, 256), 4096), 256, 0, ctx->stream>>>(
-        g[0].Mc, g[0].Me, g[0].params, dsep, Kf);
-    king_kernel<<<std::min<unsigned>(div_up(kb, 256), 4096), 256, 0, ctx->stream>>>(
-        g[1].Mc, g[1].Me, g[1].params, dsep, Kb);
+    const KingSecs ks{{{g[0].params, Kf, g[0].Mc, g[0].Me}, {g[1].params, Kb, g[1].Mc, g[1].Me}}};
+    king_kernel<<<dim3(std::min<unsigned>(div_up(std::max(kf, kb), 256), 4096), 2), 256, 0,
+                  ctx->stream>>>(ks, dsep);
     void* bp;
     XTS_TRY(slot_reserve(ctx, kSlotBilerp, (size_t)2 * C * E * sizeof(BilerpPoint), &bp));
     BilerpPoint* bpf = (BilerpPoint*)bp;
     BilerpPoint* bpb = bpf + (size_t)C * E;
-    bilerp_points_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
-        C, E, g[0].Mc, g[0].Me, costh, logE, g[0].cosths, g[0].logEs, bpf, err_dev);
-    bilerp_points_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
-        C, E, g[1].Mc, g[1].Me, costh, logE, g[1].cosths, g[1].logEs, bpb, err_dev);
-    const unsigned bb = std::min<unsigned>(div_up(npa, 256), 16384);
-    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[0].Mc, g[0].Me, bpf, Kf, af, PAf);
-    bilerp_aeff_kernel<<<bb, 256, 0, ctx->stream>>>(C, E, g[1].Mc, g[1].Me, bpb, Kb, ab, PAb);
+    const BilerpSecs bs{{{g[0].cosths, g[0].logEs, bpf, g[0].Mc, g[0].Me},
+                         {g[1].cosths, g[1].logEs, bpb, g[1].Mc, g[1].Me}}};
+    bilerp_points_kernel<<<dim3(div_up(C * E, 256), 2), 256, 0, ctx->stream>>>(C, E, costh, logE,
+                                                                              bs, err_dev);
+    const AeffSecs as{{{bpf, Kf, af, PAf, g[0].Mc, g[0].Me}, {bpb, Kb, ab, PAb, g[1].Mc, g[1].Me}}};
+    bilerp_aeff_kernel<<<dim3(std::min<unsigned>(div_up(npa, 256), 16384), 2), 256, 0,
+                         ctx->stream>>>(C, E, as);
     if (precision == XTS_PREC_FP64) {
         const size_t sm = 2 * kSrcTile * C * 8;
         mean_psf_kernel<true><<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
@@ -453,8 +477,9 @@ int xts_king(xts_ctx* ctx, const xts_psf_grid* g, double* out) {
     XTS_TRY(to_device(ctx, kSlotIn0, g->params, g->Mc * g->Me * 6 * 8, &dp));
     const size_t n = (size_t)kND * g->Mc * g->Me;
     XTS_TRY(slot_reserve(ctx, kSlotOut0, n * 8, &dout));
-    king_kernel<<<std::min<unsigned>(div_up(n, 256), 4096), 256, 0, ctx->stream>>>(
-        (int)g->Mc, (int)g->Me, (double*)dp, ct->sep, (double*)dout);
+    const KingSecs ks{{{(double*)dp, (double*)dout, (int)g->Mc, (int)g->Me}, {}}};
+    king_kernel<<<dim3(std::min<unsigned>(div_up(n, 256), 4096), 1), 256, 0, ctx->stream>>>(
+        ks, ct->sep);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipStreamSynchronize(ctx->stream));
@@ -582,12 +607,14 @@ int xts_bilerp(xts_ctx* ctx, const double* costh, int64_t C, const double* logE,
     XTS_TRY(to_device(ctx, kSlotOut3, &zero, sizeof(int), &derr));
     void* bp;
     XTS_TRY(slot_reserve(ctx, kSlotBilerp, (size_t)C * E * sizeof(BilerpPoint), &bp));
-    bilerp_points_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
-        (int)C, (int)E, (int)grid->Mc, (int)grid->Me, (double*)dc, (double*)de, (double*)dpc,
-        (double*)dpe, (BilerpPoint*)bp, (int*)derr);
-    bilerp_aeff_kernel<<<std::min<unsigned>(div_up(n, 256), 16384), 256, 0, ctx->stream>>>(
-        (int)C, (int)E, (int)grid->Mc, (int)grid->Me, (BilerpPoint*)bp, (double*)dk, nullptr,
-        (double*)dout);
+    const BilerpSecs bs{{{(double*)dpc, (double*)dpe, (BilerpPoint*)bp, (int)grid->Mc,
+                          (int)grid->Me}, {}}};
+    bilerp_points_kernel<<<dim3(div_up(C * E, 256), 1), 256, 0, ctx->stream>>>(
+        (int)C, (int)E, (double*)dc, (double*)de, bs, (int*)derr);
+    const AeffSecs as{{{(BilerpPoint*)bp, (double*)dk, nullptr, (double*)dout, (int)grid->Mc,
+                        (int)grid->Me}, {}}};
+    bilerp_aeff_kernel<<<dim3(std::min<unsigned>(div_up(n, 256), 16384), 1), 256, 0,
+                         ctx->stream>>>((int)C, (int)E, as);
     XTS_HIP(hipGetLastError());
     int err = 0;
     XTS_HIP(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
