@@ -268,6 +268,22 @@ int xts_pix2sph(xts_ctx* ctx, const xts_wcs* wcs, const double* pix /*[n,2]*/,
 int xts_sph2pix(xts_ctx* ctx, const xts_wcs* wcs, const double* sph, int64_t n,
                 double* pix);
 
+/* ------------------------------------------------------------- output */
+/* Fermi::fits::write_src_model (fits/fits.write_src_model.cxx:10-66): a FITS
+ * file with a PRIMARY HDU without data (HDUNAME = 'PRIMARY') and one IMAGE
+ * HDU per source, EXTNAME = names[s], holding the source's cube in FITS axis
+ * order NAXIS1 = H (the image's first, longitude axis), NAXIS2 = W,
+ * NAXIS3 = E, with CHECKSUM and DATASUM cards.  map: double [S, H, W, E]
+ * (XTS_LAYOUT_SHWE, the reference's) or [S, E, H, W] (XTS_LAYOUT_SEHW);
+ * XTS_OUT_F32 writes BITPIX -32 (gtsrcmaps' type), otherwise -64 (the
+ * reference's).  wcs, if not NULL, adds the image's celestial WCS cards to
+ * each image HDU.  The reference declares NAXIS1 = W and NAXIS2 = H for the
+ * same data (the same file for square images); this writer keeps NAXIS1 as
+ * the fastest data axis.  Host only: needs no context or device. */
+int xts_write_src_model(const char* path, const double* map, int64_t S, int64_t H,
+                        int64_t W, int64_t E, const char* const* names, int32_t flags,
+                        const xts_wcs* wcs);
+
 #ifdef __cplusplus
 }
 #endif

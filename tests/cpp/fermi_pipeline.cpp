@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "xtsrcmaps/exposure/exposure.hxx"
+#include "xtsrcmaps/fits/fits.hxx"
 #include "xtsrcmaps/irf/irf.hxx"
 #include "xtsrcmaps/model_map/mm_cubature/cubature.hpp"
 #include "xtsrcmaps/model_map/model_map.hxx"
@@ -97,8 +98,9 @@ static Fermi::Irf::IrfData3 grid(const Arrays& in, const std::string& side, cons
 }
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s <inputs.xtsa> <outputs.xtsa>\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: %s <inputs.xtsa> <outputs.xtsa> [<srcmaps.fits>]\n",
+                     argv[0]);
         return 2;
     }
     try {
@@ -152,7 +154,9 @@ int main(int argc, char** argv) {
         // the pipeline of main.cxx
         const auto exp = Fermi::Exposure::compute_exposure(obs, src, irf);
         const auto psf = Fermi::Psf::compute_psf_data(obs, irf, exp);
-        const auto map = Fermi::ModelMap::compute_srcmaps(obs, src, exp, psf);
+        auto map = Fermi::ModelMap::compute_srcmaps(obs, src, exp, psf);
+        // the output step main.cxx:40 leaves commented out
+        if (argc == 4) Fermi::fits::write_src_model(argv[3], map, src.names);
 
         // mean PSF stage by stage
         Fermi::Tensor<double, 3> corr[2];

@@ -7,8 +7,10 @@ Run once, in a container where the reference tree is mounted:
 It reads only DATA: the numeric initialiser lists in the reference's
 ``tests/fits/*.hxx`` files (parsed as numbers, no code is compiled or run),
 the raw little-endian binaries in ``tests/expected/*.bin``, the FITS header
-cards of ``analysis/3C279_binned_ccube.fits`` and the source position in
-``analysis/single_point.xml``.  Outputs are ``.npz`` files next to this
+cards of ``analysis/3C279_binned_ccube.fits``, two HDUs of FermiTools'
+``analysis/canonical_srcmap.fits`` and the source position in
+``analysis/single_point.xml``.  ``--canonical-only`` writes only the
+canonical_srcmap fixture.  Outputs are ``.npz`` files next to this
 script; large maps are subsampled on a fixed pixel lattice so the fixtures
 stay small.
 """
@@ -209,5 +211,53 @@ def main(ref: str) -> None:
     np.savez_compressed(os.path.join(HERE, "fermitools_boundary.npz"), **out)
 
 
+def _fits_hdus(path):
+    """(header bytes, data bytes) of every HDU of a FITS file, read as bytes."""
+    buf = open(path, "rb").read()
+    pos, out = 0, []
+    while pos < len(buf):
+        start = pos
+        hdr = {}
+        while True:
+            block = buf[pos:pos + 2880]
+            pos += 2880
+            cards = [block[i:i + 80].decode("ascii") for i in range(0, 2880, 80)]
+            for c in cards:
+                if c[8:10] == "= ":
+                    hdr[c[:8].strip()] = c[10:].split("/")[0].strip().strip("'").strip()
+            if any(c.startswith("END     ") or c.rstrip() == "END" for c in cards):
+                break
+        header = buf[start:pos]
+        naxis = int(hdr.get("NAXIS", 0))
+        n = abs(int(hdr["BITPIX"])) // 8 if naxis else 0
+        for k in range(naxis):
+            n *= int(hdr[f"NAXIS{k + 1}"])
+        n += int(hdr.get("PCOUNT", 0))
+        n = (n + 2879) // 2880 * 2880
+        out.append((header, buf[pos:pos + n]))
+        pos += n
+    return out
+
+
+def canonical_srcmap(ref: str) -> None:
+    """FermiTools' gtsrcmaps output for the 3C279 ROI: the EBOUNDS table HDU
+    (bytes, for its CHECKSUM/DATASUM cards) and the first source's image HDU
+    (header bytes and the float32 [38, 100, 100] cube)."""
+    hdus = _fits_hdus(os.path.join(ref, "analysis", "canonical_srcmap.fits"))
+    eb_h, eb_d = hdus[2]
+    src_h, src_d = hdus[3]
+    cube = np.frombuffer(src_d[:38 * 100 * 100 * 4], dtype=">f4").reshape(38, 100, 100)
+    out = {
+        "ebounds_hdu": np.frombuffer(eb_h + eb_d, dtype=np.uint8),
+        "src_header": np.frombuffer(src_h, dtype=np.uint8),
+        "src_cube": cube.astype(np.float32),
+    }
+    np.savez_compressed(os.path.join(HERE, "canonical_srcmap.npz"), **out)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    ref = args[0] if args else "/root/reference"
+    if "--canonical-only" not in sys.argv:
+        main(ref)
+    canonical_srcmap(ref)

@@ -105,3 +105,43 @@ def xtsa_inputs(inp: synthetic.Inputs) -> dict:
         a[f"aeff_{side}_params"] = np.asarray(params)[:, :, None]
         a[f"aeff_{side}_minc"] = [minc]
     return a
+
+
+# ------------------------------------------------- the gtsrcmaps comparison
+def canonical_source(orc, golden):
+    """Inputs of FermiTools' canonical source map (analysis/canonical_srcmap.fits,
+    source 4FGL J1118.2-0415 of the 3C279 ROI): its exposure and mean PSF
+    from the reference's IRF and livetime fixtures through the oracle, the
+    ROI's WCS fields and the production degree-7 rule."""
+    P, A, X, M = golden("irf_psf"), golden("irf_aeff"), golden("exposure_psf"), \
+        golden("mm_single_source")
+    king = {}
+    for k in ("front", "back"):
+        cos, loge, par, _ = orc.prepare_grid(P[k + "_e_lo"], P[k + "_e_hi"], P[k + "_c_lo"],
+                                             P[k + "_c_hi"], P[k + "_raw"])
+        king[k] = (orc.normalize_rpsf(par, loge, P[k + "_scale"]), cos, loge)
+    logE = np.log10(X["energies"])
+    costh = orc.costhetas(40, 0.0, True)
+    ltf = orc.ltf(logE, A["front_effic_p0"], A["front_effic_p1"])
+    aeff = {}
+    for k in ("front", "back"):
+        cos, loge, par, minc = orc.prepare_grid(A[k + "_e_lo"], A[k + "_e_hi"], A[k + "_c_lo"],
+                                                A[k + "_c_hi"], A[k + "_raw"])
+        aeff[k] = orc.aeff_value(costh, logE, cos, loge, par[..., 0], minc)
+    dirs = X["src_dirs"]
+    i = int(np.argmin(np.hypot(dirs[:, 0] - M["src_sph"][0], dirs[:, 1] - M["src_sph"][1])))
+    ecb, wecb = X["ecb"][i:i + 1].astype(float), X["wecb"][i:i + 1].astype(float)
+    expo = orc.exposure(ecb, wecb, aeff["front"], aeff["back"], ltf)
+    upsf, parint = orc.psf_lookup_table_and_partial_integrals(
+        king["front"], king["back"], costh, logE, aeff["front"], aeff["back"], ecb, wecb, ltf,
+        expo)
+    wcs = dict(naxis=tuple(int(x) for x in M["naxis"]), crpix=tuple(M["crpix"]),
+               crval=tuple(M["crval"]), cdelt=tuple(M["cdelt"]), crota2=float(M["crota2"]),
+               proj=str(M["proj"]))
+    return dict(src=np.array([M["src_sph"]]), expo=expo, upsf=upsf, parint=parint, wcs=wcs,
+                rule=M["rule_square_deg7"], name="4FGL J1118.2-0415")
+
+
+def canonical_cube(golden):
+    """gtsrcmaps' float32 cube of that source, [E, W, H] (FITS axis order)."""
+    return golden("canonical_srcmap")["src_cube"]

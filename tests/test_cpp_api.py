@@ -39,10 +39,19 @@ def test_cpp_pipeline_matches_oracle(orc, tmp_path):
     inp.wcube = inp.wcube.astype(np.float32).astype(np.float64)
     write_xtsa(tmp_path / "in.xtsa", xtsa_inputs(inp))
     print("test_cpp_pipeline_matches_oracle: running the C++ driver", flush=True)
-    r = subprocess.run([DRIVER, str(tmp_path / "in.xtsa"), str(tmp_path / "out.xtsa")],
+    r = subprocess.run([DRIVER, str(tmp_path / "in.xtsa"), str(tmp_path / "out.xtsa"),
+                        str(tmp_path / "srcmaps.fits")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = read_xtsa(tmp_path / "out.xtsa")
+    # Fermi::fits::write_src_model of the same map: [E, W, H] cubes, FP64
+    import fits_reader as fr
+    hdus = fr.split_hdus((tmp_path / "srcmaps.fits").read_bytes())
+    assert len(hdus) == 1 + got["map"].shape[0]
+    for s, (h, d) in enumerate(hdus[1:]):
+        assert fr.cards(h)["EXTNAME"] == f"src{s}"
+        np.testing.assert_array_equal(fr.image(h, d), got["map"][s].transpose(2, 1, 0))
+        assert fr.verify(h, d)["hdu_sum_is_minus_zero"]
 
     st = oracle_stages(orc, inp)
     assert max_rel(got["exposure"], st["expo"]) < 1e-12

@@ -23,7 +23,7 @@ __all__ = [
     "Context", "Wcs", "XtsError", "separations", "costhetas",
     "livetime_efficiency_factors", "PROJ_CAR", "PROJ_AIT", "LAYOUT_SHWE",
     "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64",
-    "PREC_FP64", "PREC_BF16X3", "ND",
+    "PREC_FP64", "PREC_BF16X3", "ND", "write_src_model",
 ]
 
 
@@ -52,6 +52,29 @@ def livetime_efficiency_factors(logEs, p0, p1) -> np.ndarray:
     capi.check(capi.lib().xts_livetime_efficiency_factors(
         capi.dptr(logEs), len(logEs), capi.fptr(p0), capi.fptr(p1), capi.dptr(out)))
     return out
+
+
+def write_src_model(path, model_map, src_names, layout: int = LAYOUT_SHWE,
+                    float32: bool = False, wcs: Wcs | None = None) -> None:
+    """fits::write_src_model (fits/fits.write_src_model.cxx:10-66): one IMAGE
+    HDU per source, EXTNAME = its name, axes (H, W, E) in FITS order, with
+    CHECKSUM/DATASUM.  model_map: [S, H, W, E] (LAYOUT_SHWE) or [S, E, H, W]
+    (LAYOUT_SEHW); float32 writes BITPIX -32.  Host only."""
+    m = capi.f64(model_map)
+    if m.ndim != 4:
+        raise ValueError("model_map must have 4 axes")
+    S = m.shape[0]
+    if layout == LAYOUT_SEHW:
+        E, H, W = m.shape[1:]
+    else:
+        H, W, E = m.shape[1:]
+    if len(src_names) != S:
+        raise ValueError(f"{len(src_names)} names for {S} sources")
+    names = (C.c_char_p * S)(*[str(n).encode() for n in src_names])
+    flags = int(layout) | (OUT_F32 if float32 else 0)
+    capi.check(capi.lib().xts_write_src_model(os.fsencode(path), capi.dptr(m), S, H, W, E,
+                                              names, flags,
+                                              C.byref(wcs) if wcs is not None else None))
 
 
 class Context:

@@ -99,6 +99,8 @@ _SIGS = {
     "xts_map_correction_factor": [_vp, _d, _d, _u8, _d, _d, _i64, _i64, _d],
     "xts_scale_map": [_vp, _d, _i64, _i64, _i64, _i64, _d, C.c_int32],
     "xts_pix2sph": [_vp, C.POINTER(Wcs), _d, _i64, _d],
+    "xts_write_src_model": [C.c_char_p, _d, _i64, _i64, _i64, _i64, C.POINTER(C.c_char_p),
+                            C.c_int32, C.POINTER(Wcs)],
     "xts_sph2pix": [_vp, C.POINTER(Wcs), _d, _i64, _d],
 }
 

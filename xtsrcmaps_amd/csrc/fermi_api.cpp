@@ -20,6 +20,7 @@
 
 #include "xts_wcs.hpp"
 #include "xtsrcmaps/exposure/exposure.hxx"
+#include "xtsrcmaps/fits/fits.hxx"
 #include "xtsrcmaps/irf/irf.hxx"
 #include "xtsrcmaps/model_map/mm_cubature/cubature.hpp"
 #include "xtsrcmaps/model_map/model_map.hxx"
@@ -531,4 +532,22 @@ auto map_integral(Tensor<double, 4> const& model_map, Tensor<double, 2> const& s
 }
 
 }  // namespace ModelMap
+
+// ---------------------------------------------------------------- output
+namespace fits {
+
+auto write_src_model(std::string const& filename, Tensor<double, 4>& model_map,
+                     std::vector<std::string> const& srcs) -> void {
+    const size_t S = model_map.extent(0);
+    require(srcs.size() == S, "write_src_model: one name per source");
+    std::vector<const char*> names(S);
+    for (size_t s = 0; s < S; ++s) names[s] = srcs[s].c_str();
+    check(xts_write_src_model(filename.c_str(), model_map.data(), (int64_t)S,
+                              (int64_t)model_map.extent(1), (int64_t)model_map.extent(2),
+                              (int64_t)model_map.extent(3), names.data(),
+                              XTS_LAYOUT_SHWE | XTS_OUT_F64, nullptr),
+          "write_src_model");
+}
+
+}  // namespace fits
 }  // namespace Fermi

@@ -1,0 +1,260 @@
+// xts_fits.cpp -- the source-map FITS writer of the reference
+// (fits/fits.write_src_model.cxx:10-66) without cfitsio.
+//
+// A FITS file is a sequence of HDUs; each is a header of 80-character cards
+// padded with spaces to a multiple of 2880 bytes, then big-endian IEEE data
+// padded with zeros to a multiple of 2880 bytes.  Every HDU carries the
+// DATASUM and CHECKSUM cards of the FITS checksum convention: DATASUM is the
+// 32-bit ones'-complement sum of the data as big-endian words, and CHECKSUM
+// is the 16-character ASCII encoding of the complement of the header's sum
+// plus DATASUM, which makes the sum of the whole HDU -0.
+#include <cinttypes>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "xtsrcmaps_amd.h"
+
+namespace xts {
+int fail(int code, const char* fmt, ...);
+
+namespace {
+
+constexpr size_t kBlock = 2880;
+constexpr size_t kCard = 80;
+
+// ------------------------------------------------------------------- cards
+std::string pad_card(std::string c) {
+    c.resize(kCard, ' ');
+    return c;
+}
+
+std::string key8(const char* key) {
+    std::string k(key);
+    k.resize(8, ' ');
+    return k;
+}
+
+// Fixed format: value right-justified in columns 11-30.
+std::string card_value(const char* key, const std::string& value, const char* comment) {
+    std::string c = key8(key) + "= ";
+    char v[32];
+    std::snprintf(v, sizeof v, "%20s", value.c_str());
+    c += v;
+    if (comment && *comment) c += std::string(" / ") + comment;
+    return pad_card(c);
+}
+
+std::string card_logical(const char* key, bool v, const char* comment = nullptr) {
+    return card_value(key, v ? "T" : "F", comment);
+}
+
+std::string card_int(const char* key, long long v, const char* comment = nullptr) {
+    return card_value(key, std::to_string(v), comment);
+}
+
+std::string card_real(const char* key, double v, const char* comment = nullptr) {
+    // the shortest of 15 to 17 significant digits that reads back as v
+    char b[40];
+    for (int digits = 15; digits <= 17; ++digits) {
+        std::snprintf(b, sizeof b, "%.*G", digits, v);
+        if (std::strtod(b, nullptr) == v) break;
+    }
+    std::string s(b);
+    if (s.find_first_of(".EN") == std::string::npos) s += ".";   // a real, not an integer
+    return card_value(key, s, comment);
+}
+
+// String value: a quote in column 11, the text (quotes doubled) padded to
+// at least 8 characters, a closing quote.
+std::string card_str(const char* key, const std::string& v, const char* comment = nullptr) {
+    std::string q;
+    for (char ch : v) {
+        q += ch;
+        if (ch == '\'') q += '\'';
+    }
+    if (q.size() < 8) q.resize(8, ' ');
+    std::string c = key8(key) + "= '" + q + "'";
+    if (c.size() < 30) c.resize(30, ' ');
+    if (comment && *comment) c += std::string(" / ") + comment;
+    return pad_card(c);
+}
+
+// ---------------------------------------------------------------- checksum
+// Ones'-complement sum of big-endian 32-bit words (len a multiple of 4).
+uint32_t ones_sum(const unsigned char* p, size_t len, uint32_t start = 0) {
+    uint64_t s = start;
+    for (size_t i = 0; i < len; i += 4) {
+        s += ((uint64_t)p[i] << 24) | ((uint64_t)p[i + 1] << 16) | ((uint64_t)p[i + 2] << 8) |
+             (uint64_t)p[i + 3];
+        if (s >> 32) s = (s & 0xffffffffu) + (s >> 32);
+    }
+    while (s >> 32) s = (s & 0xffffffffu) + (s >> 32);
+    return (uint32_t)s;
+}
+
+uint32_t ones_add(uint32_t a, uint32_t b) {
+    uint64_t s = (uint64_t)a + b;
+    while (s >> 32) s = (s & 0xffffffffu) + (s >> 32);
+    return (uint32_t)s;
+}
+
+// 16 characters whose words, added to a sum, give -0: each byte of the
+// complement splits into four characters of about byte / 4 + '0' (the
+// remainder on the first), pairs are nudged off the punctuation between the
+// digits and the letters, and the string is rotated right by one character
+// so that the 4-byte words line up with the card's alignment.
+void checksum_ascii(uint32_t sum, char out[16]) {
+    static const int kExclude[] = {0x3a, 0x3b, 0x3c, 0x3d, 0x3e, 0x3f, 0x40,
+                                   0x5b, 0x5c, 0x5d, 0x5e, 0x5f, 0x60};
+    const uint32_t value = ~sum;
+    char asc[16];
+    for (int i = 0; i < 4; ++i) {
+        const int byte = (int)((value >> (24 - 8 * i)) & 0xff);
+        const int q = byte / 4 + 0x30, r = byte % 4;
+        int ch[4] = {q + r, q, q, q};
+        for (bool moved = true; moved;) {
+            moved = false;
+            for (int x : kExclude)
+                for (int j = 0; j < 4; j += 2)
+                    if (ch[j] == x || ch[j + 1] == x) {
+                        ++ch[j];
+                        --ch[j + 1];
+                        moved = true;
+                    }
+        }
+        for (int j = 0; j < 4; ++j) asc[4 * j + i] = (char)ch[j];
+    }
+    for (int i = 0; i < 16; ++i) out[i] = asc[(i + 15) % 16];
+}
+
+// --------------------------------------------------------------------- HDU
+struct Hdu {
+    std::vector<std::string> cards;   // without END, CHECKSUM, DATASUM
+    std::vector<unsigned char> data;  // big-endian, unpadded
+};
+
+// Header and data blocks of one HDU with its DATASUM and CHECKSUM.
+std::vector<unsigned char> encode_hdu(const Hdu& h) {
+    std::vector<unsigned char> data = h.data;
+    data.resize((data.size() + kBlock - 1) / kBlock * kBlock, 0);
+    const uint32_t datasum = ones_sum(data.data(), data.size());
+    std::string hdr;
+    for (auto& c : h.cards) hdr += c;
+    const size_t ck_pos = hdr.size();
+    hdr += card_str("CHECKSUM", "0000000000000000", "HDU checksum");
+    hdr += card_str("DATASUM", std::to_string(datasum), "data unit checksum");
+    hdr += pad_card("END");
+    hdr.resize((hdr.size() + kBlock - 1) / kBlock * kBlock, ' ');
+    const uint32_t sum =
+        ones_add(ones_sum(reinterpret_cast<const unsigned char*>(hdr.data()), hdr.size()),
+                 datasum);
+    char enc[16];
+    checksum_ascii(sum, enc);
+    std::memcpy(&hdr[ck_pos + 11], enc, 16);   // between the quotes of the value
+    std::vector<unsigned char> out(hdr.begin(), hdr.end());
+    out.insert(out.end(), data.begin(), data.end());
+    return out;
+}
+
+void put_be(unsigned char* p, double v, bool f32) {
+    if (f32) {
+        const float f = (float)v;
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        for (int k = 0; k < 4; ++k) p[k] = (unsigned char)(u >> (24 - 8 * k));
+    } else {
+        uint64_t u;
+        std::memcpy(&u, &v, 8);
+        for (int k = 0; k < 8; ++k) p[k] = (unsigned char)(u >> (56 - 8 * k));
+    }
+}
+
+void wcs_cards(std::vector<std::string>& c, const xts_wcs* w) {
+    const char* proj = w->proj == XTS_PROJ_AIT ? "AIT" : "CAR";
+    const std::string lon = w->is_galactic ? "GLON" : "RA--";
+    const std::string lat = w->is_galactic ? "GLAT" : "DEC-";
+    c.push_back(card_str("CTYPE1", lon + "-" + proj));
+    c.push_back(card_real("CRPIX1", w->crpix1));
+    c.push_back(card_real("CRVAL1", w->crval1));
+    c.push_back(card_real("CDELT1", w->cdelt1));
+    c.push_back(card_str("CUNIT1", "deg"));
+    c.push_back(card_str("CTYPE2", lat + "-" + proj));
+    c.push_back(card_real("CRPIX2", w->crpix2));
+    c.push_back(card_real("CRVAL2", w->crval2));
+    c.push_back(card_real("CDELT2", w->cdelt2));
+    c.push_back(card_str("CUNIT2", "deg"));
+    c.push_back(card_real("CROTA2", w->crota2));
+    if (!w->is_galactic) c.push_back(card_real("EQUINOX", 2000.0));
+}
+
+struct FileCloser {
+    void operator()(FILE* f) const {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+}  // namespace xts
+
+using namespace xts;
+
+extern "C" int xts_write_src_model(const char* path, const double* map, int64_t S, int64_t H,
+                                   int64_t W, int64_t E, const char* const* names, int32_t flags,
+                                   const xts_wcs* wcs) {
+    if (!path || !map || !names) return fail(XTS_ERR_INVALID, "xts_write_src_model: null pointer");
+    if (S <= 0 || H <= 0 || W <= 0 || E <= 0)
+        return fail(XTS_ERR_INVALID, "xts_write_src_model: bad sizes S=%lld H=%lld W=%lld E=%lld",
+                    (long long)S, (long long)H, (long long)W, (long long)E);
+    if ((flags & ~(XTS_LAYOUT_SEHW | XTS_OUT_F32)) != 0)
+        return fail(XTS_ERR_INVALID, "xts_write_src_model: unknown flags 0x%x", flags);
+    if (wcs && (wcs->naxis1 != H || wcs->naxis2 != W))
+        return fail(XTS_ERR_INVALID, "xts_write_src_model: wcs is %lldx%lld, map %lldx%lld",
+                    (long long)wcs->naxis1, (long long)wcs->naxis2, (long long)H, (long long)W);
+    for (int64_t s = 0; s < S; ++s)
+        if (!names[s]) return fail(XTS_ERR_INVALID, "xts_write_src_model: null name %lld", (long long)s);
+    const bool sehw = (flags & XTS_LAYOUT_SEHW) != 0, f32 = (flags & XTS_OUT_F32) != 0;
+    const size_t esz = f32 ? 4 : 8;
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "wb"));
+    if (!f) return fail(XTS_ERR_INVALID, "xts_write_src_model: cannot open '%s'", path);
+    auto put = [&](const std::vector<unsigned char>& b) {
+        return std::fwrite(b.data(), 1, b.size(), f.get()) == b.size();
+    };
+    // a primary HDU without data, named as the reference names it
+    Hdu prim;
+    prim.cards = {card_logical("SIMPLE", true, "conforms to FITS standard"),
+                  card_int("BITPIX", f32 ? -32 : -64), card_int("NAXIS", 0),
+                  card_logical("EXTEND", true), card_str("HDUNAME", "PRIMARY")};
+    if (!put(encode_hdu(prim))) return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
+    const size_t plane = (size_t)H * W;
+    for (int64_t s = 0; s < S; ++s) {
+        Hdu img;
+        img.cards = {card_str("XTENSION", "IMAGE", "image extension"),
+                     card_int("BITPIX", f32 ? -32 : -64), card_int("NAXIS", 3),
+                     card_int("NAXIS1", H), card_int("NAXIS2", W), card_int("NAXIS3", E),
+                     card_int("PCOUNT", 0), card_int("GCOUNT", 1),
+                     card_str("EXTNAME", names[s], "source name")};
+        if (wcs) wcs_cards(img.cards, wcs);
+        // the source's cube as [E][W][H], H fastest (FITS axis 1)
+        img.data.resize((size_t)E * plane * esz);
+        const double* m = map + (size_t)s * plane * E;
+        unsigned char* p = img.data.data();
+        for (int64_t e = 0; e < E; ++e)
+            for (int64_t w = 0; w < W; ++w)
+                for (int64_t h = 0; h < H; ++h, p += esz) {
+                    const double v = sehw ? m[((size_t)e * H + h) * W + w]
+                                          : m[((size_t)h * W + w) * E + e];
+                    put_be(p, v, f32);
+                }
+        if (!put(encode_hdu(img)))
+            return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
+    }
+    if (std::fflush(f.get()) != 0) return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
+    return XTS_OK;
+}
