@@ -284,6 +284,37 @@ int xts_write_src_model(const char* path, const double* map, int64_t S, int64_t 
                         int64_t W, int64_t E, const char* const* names, int32_t flags,
                         const xts_wcs* wcs);
 
+/* ------------------------------------------------------------- input */
+/* fits::read_irf_pars (fits/fits.read_irf_pars.cxx:12-126): the binary
+ * table named tblname (EXTNAME, case-insensitive) whose columns are all
+ * float32 ('E'), as rows of the columns' elements concatenated in column
+ * order.  With rowdata NULL it only reports the number of columns, their
+ * repeat counts (the first cap_cols go to extents), the number of rows and
+ * the row width; then call again with rowdata of nrows x row_width floats.
+ * A table with a column of another type is an error, as in the reference.
+ * Host only. */
+int xts_read_irf_pars(const char* path, const char* tblname, int64_t* ncols,
+                      int64_t* extents, int64_t cap_cols, int64_t* nrows,
+                      int64_t* row_width, float* rowdata);
+
+/* Livetime-cube table keywords: Obs::ExposureCubeData
+ * (observation/obs_types.hxx:10-20) without the arrays. */
+typedef struct {
+    int64_t nside, nbrbins;
+    double cosmin;
+    int32_t thetabin; /* THETABIN other than 'COSTHETA' */
+    char ordering[16], coordsys[16];
+} xts_expcube_info;
+
+/* fits::read_expcube (fits/fits.read_expcube.cxx:8-116): the keywords of the
+ * HEALPix livetime table tblname ("EXPOSURE", "WEIGHTED_EXPOSURE") and, for
+ * each non-NULL pointer, its columns 1-3 read as float: cosbins
+ * [12 nside^2 * nbrbins] (pixel-major), ra and dec [12 nside^2].  Only
+ * ORDERING = NESTED and COORDSYS = EQU cubes are accepted, as in the
+ * reference.  Host only. */
+int xts_read_expcube(const char* path, const char* tblname, xts_expcube_info* info,
+                     float* cosbins, float* ra, float* dec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,7 +4,14 @@
 // the stage-by-stage forms of the mean PSF (psf.hxx:50-84) and of the model map
 // (model_map.point_src_model_map_wcs.cxx:6-45).
 //
-//   fermi_pipeline <inputs.xtsa> <outputs.xtsa>
+//   fermi_pipeline <inputs.xtsa> <outputs.xtsa> [<srcmaps.fits>]
+//   fermi_pipeline --fits <aeff.fits> <psf.fits> <ltcube.fits> <inputs.xtsa>
+//                  <outputs.xtsa> [<srcmaps.fits>]
+//
+// With --fits the IRFs come from Irf::load_aeff / load_psf (irf/irf.cxx:24-68)
+// and the livetime cubes from fits::read_expcube (tables EXPOSURE and
+// WEIGHTED_EXPOSURE), as the reference's collect_irf_data and observation
+// setup read them; the inputs then hold only the sources, energies and WCS.
 //
 // Arrays travel in a small binary container: "XTSA", u32 count, then per
 // array u32 name length, name, u32 rank, u64 extents[rank], f64 data.
@@ -98,32 +105,45 @@ static Fermi::Irf::IrfData3 grid(const Arrays& in, const std::string& side, cons
 }
 
 int main(int argc, char** argv) {
-    if (argc != 3 && argc != 4) {
-        std::fprintf(stderr, "usage: %s <inputs.xtsa> <outputs.xtsa> [<srcmaps.fits>]\n",
-                     argv[0]);
+    const bool from_fits = argc > 1 && std::string(argv[1]) == "--fits";
+    if (from_fits ? (argc != 7 && argc != 8) : (argc != 3 && argc != 4)) {
+        std::fprintf(stderr,
+                     "usage: %s <inputs.xtsa> <outputs.xtsa> [<srcmaps.fits>]\n"
+                     "       %s --fits <aeff.fits> <psf.fits> <ltcube.fits> <inputs.xtsa> "
+                     "<outputs.xtsa> [<srcmaps.fits>]\n",
+                     argv[0], argv[0]);
         return 2;
     }
+    // file arguments after the optional FITS inputs
+    const int a0 = from_fits ? 5 : 1;
+    const char* fits_out = argc > a0 + 2 ? argv[a0 + 2] : nullptr;
     try {
-        const Arrays in = read_arrays(argv[1]);
+        const Arrays in = read_arrays(argv[a0]);
         const auto& w = in.at("wcs").data;  // naxis1 naxis2 crpix1 crpix2 crval1 crval2 cdelt1 cdelt2 crota2 proj
         const long Nh = (long)w[0], Nw = (long)w[1];
         const std::vector<double> logE = in.at("logE").data;
-        const auto& cube = in.at("cube");
-        const auto& wcube = in.at("wcube");
-        const unsigned nside = (unsigned)in.at("nside").data[0];
 
         // observation (obs_types.hxx) and sources (source.hxx)
         Fermi::Obs::XtObs obs{Nh, Nw, {}, logE, {}, {},
                               Fermi::SkyGeom<double>(Nh, Nw, w[9] == 0 ? "CAR" : "AIT", false,
                                                      w[2], w[3], w[4], w[5], w[6], w[7], w[8])};
         for (double le : logE) obs.energies.push_back(std::pow(10.0, le));
-        for (auto* c : {&obs.exp_cube, &obs.weighted_exp_cube}) {
-            const auto& src = c == &obs.exp_cube ? cube : wcube;
-            c->cosbins.assign(src.data.begin(), src.data.end());
-            c->nside = nside;
-            c->nbrbins = (unsigned)src.shape[1];
-            c->cosmin = 0.0;
-            c->thetabin = true;
+        if (from_fits) {
+            auto e = Fermi::fits::read_expcube(argv[4], "EXPOSURE");
+            auto we = Fermi::fits::read_expcube(argv[4], "WEIGHTED_EXPOSURE");
+            if (!e || !we) throw std::runtime_error("cannot read the livetime cube");
+            obs.exp_cube = std::move(*e);
+            obs.weighted_exp_cube = std::move(*we);
+        } else {
+            const unsigned nside = (unsigned)in.at("nside").data[0];
+            for (auto* c : {&obs.exp_cube, &obs.weighted_exp_cube}) {
+                const auto& a = in.at(c == &obs.exp_cube ? "cube" : "wcube");
+                c->cosbins.assign(a.data.begin(), a.data.end());
+                c->nside = nside;
+                c->nbrbins = (unsigned)a.shape[1];
+                c->cosmin = 0.0;
+                c->thetabin = true;
+            }
         }
         Fermi::Source::SourceData<Fermi::Source::PointSource> src;
         src.sph_locs = tensor<2>(in.at("src_sph"));
@@ -134,29 +154,38 @@ int main(int argc, char** argv) {
 
         // IRFs: King sections normalised in place as the reference's loader does
         Fermi::Irf::XtIrf irf;
-        const auto& sc = in.at("psf_scale").data;
-        for (const std::string side : {"front", "back"}) {
-            auto& pd = side == "front" ? irf.psf_irf.front : irf.psf_irf.back;
-            pd.rpsf = grid(in, side, "psf");
-            pd.psf_scaling_params = {(float)sc[0], (float)sc[1], (float)sc[2]};
-            Fermi::Irf::normalize_rpsf(pd);
-            auto& ad = side == "front" ? irf.aeff_irf.front : irf.aeff_irf.back;
-            ad.effective_area = grid(in, side, "aeff");
+        if (from_fits) {
+            auto a = Fermi::Irf::load_aeff(argv[2]);
+            auto p = Fermi::Irf::load_psf(argv[3]);
+            if (!a || !p) throw std::runtime_error("cannot read the IRF files");
+            irf.aeff_irf = std::move(*a);
+            irf.psf_irf = std::move(*p);
+        } else {
+            const auto& sc = in.at("psf_scale").data;
+            for (const std::string side : {"front", "back"}) {
+                auto& pd = side == "front" ? irf.psf_irf.front : irf.psf_irf.back;
+                pd.rpsf = grid(in, side, "psf");
+                pd.psf_scaling_params = {(float)sc[0], (float)sc[1], (float)sc[2]};
+                Fermi::Irf::normalize_rpsf(pd);
+                auto& ad = side == "front" ? irf.aeff_irf.front : irf.aeff_irf.back;
+                ad.effective_area = grid(in, side, "aeff");
+            }
+            Fermi::Irf::IrfEffic effic{};
+            for (int k = 0; k < 6; ++k) {
+                effic.p0[k] = (float)in.at("effic_p0").data[k];
+                effic.p1[k] = (float)in.at("effic_p1").data[k];
+            }
+            irf.aeff_irf.front.efficiency_params = effic;
         }
-        Fermi::Irf::IrfEffic effic{};
-        for (int k = 0; k < 6; ++k) {
-            effic.p0[k] = (float)in.at("effic_p0").data[k];
-            effic.p1[k] = (float)in.at("effic_p1").data[k];
-        }
-        irf.aeff_irf.front.efficiency_params = effic;
-        irf.front_LTF = Fermi::Irf::livetime_efficiency_factors(obs.logEs, effic);
+        irf.front_LTF = Fermi::Irf::livetime_efficiency_factors(
+            obs.logEs, irf.aeff_irf.front.efficiency_params);
 
         // the pipeline of main.cxx
         const auto exp = Fermi::Exposure::compute_exposure(obs, src, irf);
         const auto psf = Fermi::Psf::compute_psf_data(obs, irf, exp);
         auto map = Fermi::ModelMap::compute_srcmaps(obs, src, exp, psf);
         // the output step main.cxx:40 leaves commented out
-        if (argc == 4) Fermi::fits::write_src_model(argv[3], map, src.names);
+        if (fits_out) Fermi::fits::write_src_model(fits_out, map, src.names);
 
         // mean PSF stage by stage
         Fermi::Tensor<double, 3> corr[2];
@@ -214,7 +243,12 @@ int main(int argc, char** argv) {
         out["correction"] = from_tensor(cf);
         out["row_sph"] = from_tensor(sph);
         out["row_pix"] = from_tensor(pix);
-        write_arrays(argv[2], out);
+        out["aeff_front_params"] = from_tensor(irf.aeff_irf.front.effective_area.params);
+        out["aeff_front_cosths"] = from_tensor(irf.aeff_irf.front.effective_area.cosths);
+        out["aeff_front_logEs"] = from_tensor(irf.aeff_irf.front.effective_area.logEs);
+        out["psf_back_params"] = from_tensor(irf.psf_irf.back.rpsf.params);
+        out["ltf"] = from_tensor(irf.front_LTF);
+        write_arrays(argv[a0 + 1], out);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "fermi_pipeline: %s\n", e.what());
         return 1;

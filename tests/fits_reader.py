@@ -2,7 +2,8 @@
 HDUs as (header bytes, data bytes), header cards as a dict, image cubes as
 numpy arrays, and the FITS checksum convention (the 32-bit ones'-complement
 sum of an HDU is -0 when its CHECKSUM card is right; DATASUM is the sum of
-the data blocks)."""
+the data blocks).  write_bintables makes test inputs for the library's
+table readers."""
 from __future__ import annotations
 
 import numpy as np
@@ -119,3 +120,46 @@ def verify(header: bytes, data: bytes) -> dict:
             "datasum_ok": str(h.get("DATASUM")) == str(datasum),
             "reencoded": encode_checksum(ones_sum(hdr0, datasum)),
             "checksum": h.get("CHECKSUM")}
+
+
+def _card(key: str, value) -> str:
+    if isinstance(value, bool):
+        v = f"{'T' if value else 'F':>20}"
+    elif isinstance(value, (int, np.integer)):
+        v = f"{int(value):>20}"
+    elif isinstance(value, float):
+        v = f"{repr(value):>20}"
+    else:
+        q = str(value).replace("'", "''")
+        v = "'" + q.ljust(8) + "'"
+    return f"{key:<8}= {v}".ljust(80)
+
+
+def write_bintables(path, tables: list[tuple[str, list[tuple[str, np.ndarray]], dict]]):
+    """A FITS file of binary tables: (EXTNAME, [(TTYPE, array [rows, repeat]
+    of float32 / float64 / int32)], extra keywords)."""
+    fmt = {np.dtype(np.float32): ("E", ">f4"), np.dtype(np.float64): ("D", ">f8"),
+           np.dtype(np.int32): ("J", ">i4")}
+    out = b"".join(_card(k, v).encode() for k, v in
+                   (("SIMPLE", True), ("BITPIX", 8), ("NAXIS", 0), ("EXTEND", True)))
+    out += "END".ljust(80).encode()
+    out += b" " * (-len(out) % BLOCK)
+    for name, cols, keys in tables:
+        nrows = cols[0][1].shape[0]
+        row = []
+        cards = [("XTENSION", "BINTABLE"), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", 0),
+                 ("NAXIS2", nrows), ("PCOUNT", 0), ("GCOUNT", 1), ("TFIELDS", len(cols))]
+        width = 0
+        for i, (ttype, a) in enumerate(cols, 1):
+            a = np.asarray(a).reshape(nrows, -1)
+            letter, be = fmt[a.dtype]
+            cards += [(f"TTYPE{i}", ttype), (f"TFORM{i}", f"{a.shape[1]}{letter}")]
+            row.append(a.astype(be))
+            width += a.shape[1] * a.dtype.itemsize
+        cards[3] = ("NAXIS1", width)
+        cards += [("EXTNAME", name)] + list(keys.items())
+        hdr = "".join(_card(k, v) for k, v in cards) + "END".ljust(80)
+        hdr = hdr.encode() + b" " * (-len(hdr) % BLOCK)
+        data = b"".join(b"".join(r[i].tobytes() for r in row) for i in range(nrows))
+        out += hdr + data + b"\0" * (-len(data) % BLOCK)
+    open(path, "wb").write(out)

@@ -8,9 +8,9 @@ It reads only DATA: the numeric initialiser lists in the reference's
 ``tests/fits/*.hxx`` files (parsed as numbers, no code is compiled or run),
 the raw little-endian binaries in ``tests/expected/*.bin``, the FITS header
 cards of ``analysis/3C279_binned_ccube.fits``, two HDUs of FermiTools'
-``analysis/canonical_srcmap.fits`` and the source position in
-``analysis/single_point.xml``.  ``--canonical-only`` writes only the
-canonical_srcmap fixture.  Outputs are ``.npz`` files next to this
+``analysis/canonical_srcmap.fits``, the P8R3 IRF FITS files (copied) and
+the source position in ``analysis/single_point.xml``.
+``--canonical-only`` / ``--irf-only`` write only those fixtures.  Outputs are ``.npz`` files next to this
 script; large maps are subsampled on a fixed pixel lattice so the fixtures
 stay small.
 """
@@ -239,6 +239,14 @@ def _fits_hdus(path):
     return out
 
 
+def irf_files(ref: str) -> None:
+    """The P8R3 SOURCE IRF files the reference's tests read
+    (config/config.hxx:28-29), copied as data for the FITS readers."""
+    for name in ("aeff_P8R3_SOURCE_V2_FB.fits", "psf_P8R3_SOURCE_V2_FB.fits"):
+        data = open(os.path.join(ref, "analysis", name), "rb").read()
+        open(os.path.join(HERE, name), "wb").write(data)
+
+
 def canonical_srcmap(ref: str) -> None:
     """FermiTools' gtsrcmaps output for the 3C279 ROI: the EBOUNDS table HDU
     (bytes, for its CHECKSUM/DATASUM cards) and the first source's image HDU
@@ -258,6 +266,9 @@ def canonical_srcmap(ref: str) -> None:
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     ref = args[0] if args else "/root/reference"
-    if "--canonical-only" not in sys.argv:
+    if "--canonical-only" not in sys.argv and "--irf-only" not in sys.argv:
         main(ref)
-    canonical_srcmap(ref)
+    if "--irf-only" not in sys.argv:
+        canonical_srcmap(ref)
+    if "--canonical-only" not in sys.argv:
+        irf_files(ref)

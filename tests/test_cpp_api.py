@@ -79,3 +79,77 @@ def test_cpp_pipeline_matches_oracle(orc, tmp_path):
         assert abs(got["row_sph"][j, 0] - lon) < 1e-9 and abs(got["row_sph"][j, 1] - lat) < 1e-9
     np.testing.assert_allclose(got["row_pix"][:, 0], 1.0, atol=1e-8)
     np.testing.assert_allclose(got["row_pix"][:, 1], np.arange(1, 65), atol=1e-8)
+
+
+@pytest.mark.gpu
+def test_cpp_pipeline_from_fits_inputs(orc, tmp_path):
+    """main.cxx's pipeline with the IRFs read by Irf::load_aeff / load_psf from
+    the P8R3 files and the livetime cubes by fits::read_expcube from a FITS
+    file, against the oracle fed the same tables (read_irf_pars output, which
+    test_fits_readers pins to the reference's fixtures)."""
+    import fits_reader as fr
+    import xtsrcmaps_amd as xa
+    from pipelines import DEG7, oracle_wcs, read_xtsa, write_xtsa
+    gold = os.path.join(ROOT, "tests", "golden")
+    aeff_f = os.path.join(gold, "aeff_P8R3_SOURCE_V2_FB.fits")
+    psf_f = os.path.join(gold, "psf_P8R3_SOURCE_V2_FB.fits")
+    rng = np.random.default_rng(12)
+    nside, nbr = 8, 40
+    npix = 12 * nside * nside
+    cubes = {}
+    tabs = []
+    for name in ("EXPOSURE", "WEIGHTED_EXPOSURE"):
+        cos = rng.uniform(0, 1e7, (npix, nbr)).astype(np.float32)
+        keys = {"NSIDE": nside, "NBRBINS": nbr, "COSMIN": 0.0, "ORDERING": "NESTED",
+                "COORDSYS": "EQU", "THETABIN": "SQRT(1-COSTHETA)"}
+        tabs.append((name, [("COSBINS", cos), ("RA", np.zeros((npix, 1), np.float32)),
+                            ("DEC", np.zeros((npix, 1), np.float32))], keys))
+        cubes[name] = cos.astype(np.float64)
+    fr.write_bintables(tmp_path / "ltcube.fits", tabs)
+    wcs = xa.Wcs.make((48, 48), (24.5, 24.5), (150.0, 20.0), (-0.1, 0.1), 0.0, "CAR")
+    logE = np.linspace(2.0, 5.5, 11)
+    src = np.array([[150.3, 20.2], [149.1, 19.0], [151.0, 21.5]])
+    write_xtsa(tmp_path / "in.xtsa", {
+        "wcs": [wcs.naxis1, wcs.naxis2, wcs.crpix1, wcs.crpix2, wcs.crval1, wcs.crval2,
+                wcs.cdelt1, wcs.cdelt2, wcs.crota2, wcs.proj],
+        "logE": logE, "src_sph": src})
+    print("test_cpp_pipeline_from_fits_inputs: running the C++ driver", flush=True)
+    r = subprocess.run([DRIVER, "--fits", aeff_f, psf_f, str(tmp_path / "ltcube.fits"),
+                        str(tmp_path / "in.xtsa"), str(tmp_path / "out.xtsa")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = read_xtsa(tmp_path / "out.xtsa")
+
+    # the oracle on the same tables
+    def grid(path, table):
+        ext, off, rows = xa.read_irf_pars(path, table)
+        cols = [rows[0][o:o + e] for e, o in zip(ext, off)]
+        raw = np.stack([c.reshape(ext[2], ext[0]) for c in cols[4:]])
+        return orc.prepare_grid(cols[0], cols[1], cols[2], cols[3], raw)
+    aeff, king = {}, {}
+    costh = orc.costhetas(nbr, 0.0, True)
+    for side in ("front", "back"):
+        cos, loge, par, minc = grid(aeff_f, f"EFFECTIVE AREA_{side.upper()}")
+        if side == "front":
+            np.testing.assert_array_equal(got["aeff_front_params"], par)
+            np.testing.assert_array_equal(got["aeff_front_cosths"], cos)
+            np.testing.assert_array_equal(got["aeff_front_logEs"], loge)
+        aeff[side] = orc.aeff_value(costh, logE, cos, loge, par[..., 0], minc)
+        cos, loge, par, _ = grid(psf_f, f"RPSF_{side.upper()}")
+        _, _, scale = xa.read_irf_pars(psf_f, f"PSF_SCALING_PARAMS_{side.upper()}")
+        king[side] = (orc.normalize_rpsf(par, loge, scale[0]), cos, loge)
+    assert max_rel(got["psf_back_params"], king["back"][0]) < 1e-12
+    _, _, effic = xa.read_irf_pars(aeff_f, "EFFICIENCY_PARAMS_FRONT")
+    ltf = orc.ltf(logE, effic[0], effic[1])
+    np.testing.assert_allclose(got["ltf"], ltf, rtol=1e-14)
+    ecb = orc.src_cosbins(src, cubes["EXPOSURE"], nside)
+    wecb = orc.src_cosbins(src, cubes["WEIGHTED_EXPOSURE"], nside)
+    expo = orc.exposure(ecb, wecb, aeff["front"], aeff["back"], ltf)
+    assert max_rel(got["exposure"], expo) < 1e-12
+    upsf, parint = orc.psf_lookup_table_and_partial_integrals(
+        king["front"], king["back"], costh, logE, aeff["front"], aeff["back"], ecb, wecb, ltf,
+        expo)
+    assert max_rel(got["upsf"], upsf) < 1e-10
+    ref = orc.point_src_model_map_wcs(48, 48, oracle_wcs(orc, wcs), src, expo, upsf, parint,
+                                      DEG7)
+    assert max_rel(got["map"], ref) < 1e-5

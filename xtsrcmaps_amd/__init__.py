@@ -23,7 +23,7 @@ __all__ = [
     "Context", "Wcs", "XtsError", "separations", "costhetas",
     "livetime_efficiency_factors", "PROJ_CAR", "PROJ_AIT", "LAYOUT_SHWE",
     "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64",
-    "PREC_FP64", "PREC_BF16X3", "ND", "write_src_model",
+    "PREC_FP64", "PREC_BF16X3", "ND", "write_src_model", "read_irf_pars", "read_expcube",
 ]
 
 
@@ -75,6 +75,42 @@ def write_src_model(path, model_map, src_names, layout: int = LAYOUT_SHWE,
     capi.check(capi.lib().xts_write_src_model(os.fsencode(path), capi.dptr(m), S, H, W, E,
                                               names, flags,
                                               C.byref(wcs) if wcs is not None else None))
+
+
+def read_irf_pars(path, table: str):
+    """fits::read_irf_pars (fits/fits.read_irf_pars.cxx:12-126) -> (extents,
+    offsets, rowdata [nrows, row width] float32).  Host only."""
+    L = capi.lib()
+    nc, nr, rw = C.c_int64(), C.c_int64(), C.c_int64()
+    capi.check(L.xts_read_irf_pars(os.fsencode(path), table.encode(), C.byref(nc), None, 0,
+                                   C.byref(nr), C.byref(rw), None))
+    ext = (C.c_int64 * max(nc.value, 1))()
+    rows = np.zeros((nr.value, rw.value), np.float32)
+    capi.check(L.xts_read_irf_pars(os.fsencode(path), table.encode(), C.byref(nc), ext,
+                                   nc.value, C.byref(nr), C.byref(rw),
+                                   rows.ctypes.data_as(C.POINTER(C.c_float))))
+    extents = np.array(ext[:nc.value], np.int64)
+    offsets = np.concatenate([[0], np.cumsum(extents)[:-1]]).astype(np.int64)
+    return extents, offsets, rows
+
+
+def read_expcube(path, table: str = "EXPOSURE") -> dict:
+    """fits::read_expcube (fits/fits.read_expcube.cxx:8-116): the keywords and
+    cosbins [12 nside^2, nbrbins], ra, dec [12 nside^2] as float32."""
+    L = capi.lib()
+    info = capi.ExpcubeInfo()
+    capi.check(L.xts_read_expcube(os.fsencode(path), table.encode(), C.byref(info), None, None,
+                                  None))
+    npix = 12 * info.nside * info.nside
+    cos = np.zeros((npix, info.nbrbins), np.float32)
+    ra = np.zeros(npix, np.float32)
+    dec = np.zeros(npix, np.float32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    capi.check(L.xts_read_expcube(os.fsencode(path), table.encode(), C.byref(info), fp(cos),
+                                  fp(ra), fp(dec)))
+    return dict(cosbins=cos, ra=ra, dec=dec, nside=info.nside, nbrbins=info.nbrbins,
+                cosmin=info.cosmin, thetabin=bool(info.thetabin),
+                ordering=info.ordering.decode(), coordsys=info.coordsys.decode())
 
 
 class Context:

@@ -46,6 +46,13 @@ class Wcs(C.Structure):
                    float(crota2), p, int(bool(galactic)))
 
 
+class ExpcubeInfo(C.Structure):
+    """xts_expcube_info: the livetime table's keywords."""
+    _fields_ = [("nside", C.c_int64), ("nbrbins", C.c_int64), ("cosmin", C.c_double),
+                ("thetabin", C.c_int32), ("ordering", C.c_char * 16),
+                ("coordsys", C.c_char * 16)]
+
+
 class PsfGrid(C.Structure):
     _fields_ = [("params", _d), ("cosths", _d), ("logEs", _d), ("Mc", C.c_int64),
                 ("Me", C.c_int64)]
@@ -102,6 +109,11 @@ _SIGS = {
     "xts_write_src_model": [C.c_char_p, _d, _i64, _i64, _i64, _i64, C.POINTER(C.c_char_p),
                             C.c_int32, C.POINTER(Wcs)],
     "xts_sph2pix": [_vp, C.POINTER(Wcs), _d, _i64, _d],
+    "xts_read_irf_pars": [C.c_char_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                          _i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                          C.POINTER(C.c_float)],
+    "xts_read_expcube": [C.c_char_p, C.c_char_p, C.POINTER(ExpcubeInfo), C.POINTER(C.c_float),
+                         C.POINTER(C.c_float), C.POINTER(C.c_float)],
 }
 
 _lib = None

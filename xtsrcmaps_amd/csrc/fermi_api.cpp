@@ -11,6 +11,7 @@
 // to Fermi::set_xts_context.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
@@ -204,11 +205,18 @@ auto prepare_grid(std::vector<float> const& e_lo, std::vector<float> const& e_hi
                (double)c_lo[0]};
     g.cosths[0] = -1.0;
     g.cosths[Mc - 1] = 1.0;
-    for (size_t c = 0; c < Mc0; ++c) g.cosths[c + 1] = 0.5 * ((double)c_lo[c] + (double)c_hi[c]);
+    // the sum, product and log10 in float, as the reference forms them from
+    // its float columns
+    for (size_t c = 0; c < Mc0; ++c) {
+        const float sum = c_lo[c] + c_hi[c];
+        g.cosths[c + 1] = 0.5 * (double)sum;
+    }
     g.logEs[0] = 0.0;
     g.logEs[Me - 1] = 10.0;
-    for (size_t e = 0; e < Me0; ++e)
-        g.logEs[e + 1] = 0.5 * std::log10((double)e_lo[e] * (double)e_hi[e]);
+    for (size_t e = 0; e < Me0; ++e) {
+        const float prod = e_lo[e] * e_hi[e];
+        g.logEs[e + 1] = 0.5 * (double)std::log10(prod);
+    }
     for (size_t p = 0; p < n_grids; ++p)
         for (size_t c = 0; c < Mc; ++c) {
             const size_t cs = c == 0 ? 0 : (c >= Mc0 ? Mc0 - 1 : c - 1);
@@ -218,6 +226,66 @@ auto prepare_grid(std::vector<float> const& e_lo, std::vector<float> const& e_hi
             }
         }
     return g;
+}
+
+auto prepare_grid(fits::TablePars const& pars) -> IrfData3 {
+    const auto& x = pars.extents;
+    require(x.size() >= 5 && x[0] > 1 && x[1] == x[0] && x[2] > 1 && x[3] == x[2] &&
+                pars.rowdata.extent(0) == 1,
+            "prepare_grid: not an IRF grid table");
+    for (size_t k = 4; k < x.size(); ++k)
+        require(x[k] == x[0] * x[2], "prepare_grid: parameter column size");
+    auto col = [&](size_t k) {
+        const float* p = &pars.rowdata[0, pars.offsets[k]];
+        return std::vector<float>(p, p + x[k]);
+    };
+    std::vector<float> params;
+    for (size_t k = 4; k < x.size(); ++k) {
+        const auto c = col(k);
+        params.insert(params.end(), c.begin(), c.end());
+    }
+    return prepare_grid(col(0), col(1), col(2), col(3), params, x.size() - 4);
+}
+
+namespace {
+// irf.prepare_scale.cpp:7-12 and irf.prepare_effic.cpp:8-21
+IrfScale prepare_scale(fits::TablePars const& p) {
+    require(p.rowdata.extent(0) == 1 && p.rowdata.extent(1) == 3, "PSF scaling table shape");
+    return {p.rowdata[0, 0], p.rowdata[0, 1], p.rowdata[0, 2]};
+}
+IrfEffic prepare_effic(fits::TablePars const& p) {
+    require(p.rowdata.extent(0) == 2 && p.rowdata.extent(1) == 6, "efficiency table shape");
+    IrfEffic e{};
+    for (size_t k = 0; k < 6; ++k) {
+        e.p0[k] = p.rowdata[0, k];
+        e.p1[k] = p.rowdata[1, k];
+    }
+    return e;
+}
+}  // namespace
+
+auto load_aeff(std::string const& filename) -> std::optional<aeff::Pass8FB> {
+    const char* names[6] = {"EFFECTIVE AREA_FRONT", "PHI_DEPENDENCE_FRONT",
+                            "EFFICIENCY_PARAMS_FRONT", "EFFECTIVE AREA_BACK",
+                            "PHI_DEPENDENCE_BACK", "EFFICIENCY_PARAMS_BACK"};
+    std::optional<fits::TablePars> t[6];
+    for (int k = 0; k < 6; ++k)
+        if (!(t[k] = fits::read_irf_pars(filename, names[k]))) return std::nullopt;
+    return aeff::Pass8FB{{prepare_grid(*t[0]), prepare_grid(*t[1]), prepare_effic(*t[2])},
+                         {prepare_grid(*t[3]), prepare_grid(*t[4]), prepare_effic(*t[5])}};
+}
+
+auto load_psf(std::string const& filename) -> std::optional<psf::Pass8FB> {
+    const char* names[6] = {"RPSF_FRONT", "PSF_SCALING_PARAMS_FRONT", "FISHEYE_CORRECTION_FRONT",
+                            "RPSF_BACK",  "PSF_SCALING_PARAMS_BACK",  "FISHEYE_CORRECTION_BACK"};
+    std::optional<fits::TablePars> t[6];
+    for (int k = 0; k < 6; ++k)
+        if (!(t[k] = fits::read_irf_pars(filename, names[k]))) return std::nullopt;
+    psf::Pass8FB out{{prepare_grid(*t[0]), prepare_scale(*t[1]), prepare_grid(*t[2])},
+                     {prepare_grid(*t[3]), prepare_scale(*t[4]), prepare_grid(*t[5])}};
+    normalize_rpsf(out.front);
+    normalize_rpsf(out.back);
+    return out;
 }
 
 }  // namespace Irf
@@ -533,8 +601,59 @@ auto map_integral(Tensor<double, 4> const& model_map, Tensor<double, 2> const& s
 
 }  // namespace ModelMap
 
-// ---------------------------------------------------------------- output
+// ---------------------------------------------------------------- fits
 namespace fits {
+
+auto read_irf_pars(std::string const& filename, std::string tblname)
+    -> std::optional<TablePars> {
+    int64_t nc = 0, nr = 0, rw = 0;
+    if (xts_read_irf_pars(filename.c_str(), tblname.c_str(), &nc, nullptr, 0, &nr, &rw,
+                          nullptr) != XTS_OK) {
+        std::fprintf(stderr, "read_irf_pars: %s\n", xts_last_error());
+        return std::nullopt;
+    }
+    std::vector<int64_t> ext(nc > 0 ? nc : 1);
+    TablePars t{{}, {}, Tensor<float, 2>((size_t)nr, (size_t)rw)};
+    if (xts_read_irf_pars(filename.c_str(), tblname.c_str(), &nc, ext.data(), nc, &nr, &rw,
+                          t.rowdata.data()) != XTS_OK) {
+        std::fprintf(stderr, "read_irf_pars: %s\n", xts_last_error());
+        return std::nullopt;
+    }
+    size_t off = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+        t.extents.push_back((size_t)ext[c]);
+        t.offsets.push_back(off);
+        off += (size_t)ext[c];
+    }
+    return t;
+}
+
+auto read_expcube(std::string const& filename, std::string const& tblname)
+    -> std::optional<Obs::ExposureCubeData> {
+    xts_expcube_info info{};
+    if (xts_read_expcube(filename.c_str(), tblname.c_str(), &info, nullptr, nullptr, nullptr) !=
+        XTS_OK) {
+        std::fprintf(stderr, "read_expcube: %s\n", xts_last_error());
+        return std::nullopt;
+    }
+    const size_t npix = 12 * (size_t)info.nside * info.nside;
+    Obs::ExposureCubeData d;
+    d.cosbins.resize(npix * info.nbrbins);
+    d.ra.resize(npix);
+    d.dec.resize(npix);
+    if (xts_read_expcube(filename.c_str(), tblname.c_str(), &info, d.cosbins.data(),
+                         d.ra.data(), d.dec.data()) != XTS_OK) {
+        std::fprintf(stderr, "read_expcube: %s\n", xts_last_error());
+        return std::nullopt;
+    }
+    d.nside = (unsigned)info.nside;
+    d.nbrbins = (unsigned)info.nbrbins;
+    d.cosmin = info.cosmin;
+    d.ordering = info.ordering;
+    d.coordsys = info.coordsys;
+    d.thetabin = info.thetabin != 0;
+    return d;
+}
 
 auto write_src_model(std::string const& filename, Tensor<double, 4>& model_map,
                      std::vector<std::string> const& srcs) -> void {

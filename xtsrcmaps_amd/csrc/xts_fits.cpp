@@ -8,6 +8,13 @@
 // 32-bit ones'-complement sum of the data as big-endian words, and CHECKSUM
 // is the 16-character ASCII encoding of the complement of the header's sum
 // plus DATASUM, which makes the sum of the whole HDU -0.
+//
+// The readers of the IRF tables (fits/fits.read_irf_pars.cxx:12-126) and of
+// the livetime cube (fits/fits.read_expcube.cxx:8-116) parse binary-table
+// HDUs the same way: header cards to keywords, TFORMn to column types and
+// repeat counts, big-endian fields to host values.
+#include <algorithm>
+#include <cctype>
 #include <cinttypes>
 #include <cmath>
 #include <cstdarg>
@@ -15,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -194,6 +202,164 @@ void wcs_cards(std::vector<std::string>& c, const xts_wcs* w) {
     if (!w->is_galactic) c.push_back(card_real("EQUINOX", 2000.0));
 }
 
+// ------------------------------------------------------------------ reading
+struct Column {
+    char type;        // TFORM letter
+    int64_t repeat;   // elements per row
+    int64_t offset;   // byte offset in the row
+};
+
+struct TableHdu {
+    std::map<std::string, std::string> keys;   // card values, strings unquoted
+    std::vector<Column> cols;
+    int64_t row_bytes = 0, nrows = 0;
+    const unsigned char* data = nullptr;
+};
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(' '), b = s.find_last_not_of(' ');
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+std::string upper(std::string s) {
+    for (auto& c : s) c = (char)std::toupper((unsigned char)c);
+    return s;
+}
+
+// Value field of a card (columns 11-80): a quoted string (quotes doubled
+// inside, trailing blanks dropped) or the text before a '/' comment.
+std::string card_field(const std::string& v) {
+    std::string t = trim(v);
+    if (!t.empty() && t[0] == '\'') {
+        std::string out;
+        for (size_t i = 1; i < t.size(); ++i) {
+            if (t[i] == '\'') {
+                if (i + 1 < t.size() && t[i + 1] == '\'') {
+                    out += '\'';
+                    ++i;
+                    continue;
+                }
+                break;
+            }
+            out += t[i];
+        }
+        return trim(out);
+    }
+    return trim(t.substr(0, t.find('/')));
+}
+
+int64_t type_bytes(char t) {
+    switch (t) {
+        case 'L': case 'B': case 'A': return 1;
+        case 'I': return 2;
+        case 'J': case 'E': return 4;
+        case 'K': case 'D': case 'C': return 8;
+        case 'M': return 16;
+        default: return -1;
+    }
+}
+
+// The binary tables of a FITS file held in memory, by EXTNAME (upper case).
+int scan_tables(const std::vector<unsigned char>& buf, std::map<std::string, TableHdu>* out) {
+    size_t pos = 0;
+    bool first = true;
+    while (pos + kBlock <= buf.size()) {
+        std::map<std::string, std::string> keys;
+        bool end = false;
+        while (!end) {
+            if (pos + kBlock > buf.size()) return fail(XTS_ERR_INVALID, "FITS header truncated");
+            for (size_t i = 0; i < kBlock && !end; i += kCard) {
+                const std::string card(reinterpret_cast<const char*>(&buf[pos + i]), kCard);
+                const std::string key = trim(card.substr(0, 8));
+                if (key == "END") end = true;
+                else if (card.compare(8, 2, "= ") == 0) keys[key] = card_field(card.substr(10));
+            }
+            pos += kBlock;
+        }
+        if (first && keys.count("SIMPLE") == 0) return fail(XTS_ERR_INVALID, "not a FITS file");
+        first = false;
+        auto num = [&](const char* k, int64_t dflt) {
+            auto it = keys.find(k);
+            return it == keys.end() ? dflt : std::strtoll(it->second.c_str(), nullptr, 10);
+        };
+        const int64_t naxis = num("NAXIS", 0), bitpix = num("BITPIX", 8);
+        int64_t n = naxis > 0 ? (bitpix < 0 ? -bitpix : bitpix) / 8 : 0;
+        for (int64_t k = 1; k <= naxis; ++k) n *= num(("NAXIS" + std::to_string(k)).c_str(), 0);
+        n = (n + num("PCOUNT", 0)) * std::max<int64_t>(num("GCOUNT", 1), 1);
+        if (pos + (size_t)n > buf.size()) return fail(XTS_ERR_INVALID, "FITS data truncated");
+        if (keys.count("XTENSION") && upper(keys["XTENSION"]) == "BINTABLE") {
+            TableHdu t;
+            t.row_bytes = num("NAXIS1", 0);
+            t.nrows = num("NAXIS2", 0);
+            t.data = buf.data() + pos;
+            int64_t off = 0;
+            for (int64_t c = 1; c <= num("TFIELDS", 0); ++c) {
+                const std::string f = keys["TFORM" + std::to_string(c)];
+                size_t i = 0;
+                while (i < f.size() && std::isdigit((unsigned char)f[i])) ++i;
+                Column col{i < f.size() ? (char)std::toupper((unsigned char)f[i]) : '?',
+                           i > 0 ? std::strtoll(f.substr(0, i).c_str(), nullptr, 10) : 1, off};
+                const int64_t w = type_bytes(col.type);
+                if (w < 0) return fail(XTS_ERR_INVALID, "unsupported column format '%s'", f.c_str());
+                off += w * col.repeat;
+                t.cols.push_back(col);
+            }
+            if (off != t.row_bytes)
+                return fail(XTS_ERR_INVALID, "row of %lld bytes, columns of %lld",
+                            (long long)t.row_bytes, (long long)off);
+            t.keys = std::move(keys);
+            const std::string name = upper(t.keys.count("EXTNAME") ? t.keys["EXTNAME"] : "");
+            if (!out->count(name)) (*out)[name] = std::move(t);
+        }
+        pos += ((size_t)n + kBlock - 1) / kBlock * kBlock;
+    }
+    return XTS_OK;
+}
+
+int load_file(const char* path, std::vector<unsigned char>* buf) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(XTS_ERR_INVALID, "cannot open '%s'", path);
+    std::fseek(f, 0, SEEK_END);
+    const long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    buf->resize(n > 0 ? (size_t)n : 0);
+    const size_t got = buf->empty() ? 0 : std::fread(buf->data(), 1, buf->size(), f);
+    std::fclose(f);
+    if (got != buf->size()) return fail(XTS_ERR_INVALID, "cannot read '%s'", path);
+    return XTS_OK;
+}
+
+int find_table(const char* path, const char* name, std::vector<unsigned char>* buf,
+               TableHdu* t) {
+    if (load_file(path, buf) != XTS_OK) return XTS_ERR_INVALID;
+    std::map<std::string, TableHdu> tables;
+    if (scan_tables(*buf, &tables) != XTS_OK) return XTS_ERR_INVALID;
+    auto it = tables.find(upper(trim(name)));
+    if (it == tables.end())
+        return fail(XTS_ERR_INVALID, "no binary table '%s' in '%s'", name, path);
+    *t = it->second;
+    return XTS_OK;
+}
+
+uint32_t be32(const unsigned char* p) {
+    return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+
+// Element k of a column in a row, as float (from E or D fields).
+float field_float(const TableHdu& t, const Column& c, int64_t row, int64_t k) {
+    const unsigned char* p = t.data + row * t.row_bytes + c.offset;
+    if (c.type == 'E') {
+        const uint32_t u = be32(p + 4 * k);
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    }
+    const uint64_t u = ((uint64_t)be32(p + 8 * k) << 32) | be32(p + 8 * k + 4);
+    double d;
+    std::memcpy(&d, &u, 8);
+    return (float)d;
+}
+
 struct FileCloser {
     void operator()(FILE* f) const {
         if (f) std::fclose(f);
@@ -256,5 +422,79 @@ extern "C" int xts_write_src_model(const char* path, const double* map, int64_t 
             return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
     }
     if (std::fflush(f.get()) != 0) return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
+    return XTS_OK;
+}
+
+extern "C" int xts_read_irf_pars(const char* path, const char* tblname, int64_t* ncols,
+                                 int64_t* extents, int64_t cap_cols, int64_t* nrows,
+                                 int64_t* row_width, float* rowdata) {
+    if (!path || !tblname || !ncols || !nrows || !row_width)
+        return fail(XTS_ERR_INVALID, "xts_read_irf_pars: null pointer");
+    std::vector<unsigned char> buf;
+    TableHdu t;
+    if (find_table(path, tblname, &buf, &t) != XTS_OK) return XTS_ERR_INVALID;
+    int64_t width = 0;
+    for (size_t c = 0; c < t.cols.size(); ++c) {
+        if (t.cols[c].type != 'E')
+            return fail(XTS_ERR_INVALID, "table '%s' column %zu is not float32 ('%c')", tblname,
+                        c + 1, t.cols[c].type);
+        if (extents && (int64_t)c < cap_cols) extents[c] = t.cols[c].repeat;
+        width += t.cols[c].repeat;
+    }
+    *ncols = (int64_t)t.cols.size();
+    *nrows = t.nrows;
+    *row_width = width;
+    if (rowdata) {
+        // each row: the columns' elements in column order
+        for (int64_t r = 0; r < t.nrows; ++r) {
+            float* o = rowdata + r * width;
+            for (const Column& c : t.cols)
+                for (int64_t k = 0; k < c.repeat; ++k) *o++ = field_float(t, c, r, k);
+        }
+    }
+    return XTS_OK;
+}
+
+extern "C" int xts_read_expcube(const char* path, const char* tblname, xts_expcube_info* info,
+                                float* cosbins, float* ra, float* dec) {
+    if (!path || !tblname || !info) return fail(XTS_ERR_INVALID, "xts_read_expcube: null pointer");
+    std::vector<unsigned char> buf;
+    TableHdu t;
+    if (find_table(path, tblname, &buf, &t) != XTS_OK) return XTS_ERR_INVALID;
+    for (const char* k : {"NSIDE", "NBRBINS", "COSMIN", "ORDERING", "COORDSYS", "THETABIN"})
+        if (!t.keys.count(k))
+            return fail(XTS_ERR_INVALID, "xts_read_expcube: no %s keyword in '%s'", k, tblname);
+    const std::string ordering = t.keys["ORDERING"], coordsys = t.keys["COORDSYS"];
+    if (ordering != "NESTED")
+        return fail(XTS_ERR_INVALID, "livetime cube ordering %s is unsupported (NESTED only)",
+                    ordering.c_str());
+    if (coordsys != "EQU")
+        return fail(XTS_ERR_INVALID, "livetime cube COORDSYS %s is unsupported (EQU only)",
+                    coordsys.c_str());
+    const int64_t nside = std::strtoll(t.keys["NSIDE"].c_str(), nullptr, 10);
+    const int64_t nbr = std::strtoll(t.keys["NBRBINS"].c_str(), nullptr, 10);
+    const int64_t npix = 12 * nside * nside;
+    if (nside <= 0 || nbr <= 0 || t.cols.size() < 3 || t.cols[0].repeat * t.nrows < nbr * npix ||
+        t.cols[1].repeat * t.nrows < npix || t.cols[2].repeat * t.nrows < npix)
+        return fail(XTS_ERR_INVALID, "livetime cube '%s' does not hold 12 nside^2 x %lld bins",
+                    tblname, (long long)nbr);
+    for (int c = 0; c < 3; ++c)
+        if (t.cols[c].type != 'E' && t.cols[c].type != 'D')
+            return fail(XTS_ERR_INVALID, "livetime cube column %d is not real", c + 1);
+    info->nside = nside;
+    info->nbrbins = nbr;
+    info->cosmin = std::strtod(t.keys["COSMIN"].c_str(), nullptr);
+    info->thetabin = t.keys["THETABIN"] != "COSTHETA" ? 1 : 0;
+    std::snprintf(info->ordering, sizeof info->ordering, "%s", ordering.c_str());
+    std::snprintf(info->coordsys, sizeof info->coordsys, "%s", coordsys.c_str());
+    // column n read as one run of elements across rows (row-major), as
+    // fits_read_col does
+    auto column = [&](int c, int64_t n, float* out) {
+        const Column& col = t.cols[c];
+        for (int64_t i = 0; i < n; ++i) out[i] = field_float(t, col, i / col.repeat, i % col.repeat);
+    };
+    if (cosbins) column(0, nbr * npix, cosbins);
+    if (ra) column(1, npix, ra);
+    if (dec) column(2, npix, dec);
     return XTS_OK;
 }

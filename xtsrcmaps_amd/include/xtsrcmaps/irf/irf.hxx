@@ -2,8 +2,11 @@
 // irf/irf.normalize_rpsf.cpp:11-61, irf/irf.prepare_grid.cpp:26-102).
 #pragma once
 
+#include <optional>
+#include <string>
 #include <vector>
 
+#include "xtsrcmaps/fits/fits.hxx"
 #include "xtsrcmaps/irf/irf_types.hxx"
 #include "xtsrcmaps/tensor/tensor.hpp"
 
@@ -23,5 +26,14 @@ void normalize_rpsf(psf::Data& psfdata);
 auto prepare_grid(std::vector<float> const& e_lo, std::vector<float> const& e_hi,
                   std::vector<float> const& c_lo, std::vector<float> const& c_hi,
                   std::vector<float> const& params, std::size_t n_grids) -> IrfData3;
+
+// The same from the table of read_irf_pars (irf.prepare_grid.cpp:26-102).
+auto prepare_grid(fits::TablePars const& pars) -> IrfData3;
+
+// irf/irf.cxx:24-68: the six tables of a P8R3 front/back IRF file, gridded;
+// the PSF's King parameters normalised (irf.prepare_psf_data.cpp:6-25).
+// std::nullopt when a table cannot be read.
+auto load_aeff(std::string const& filename) -> std::optional<aeff::Pass8FB>;
+auto load_psf(std::string const& filename) -> std::optional<psf::Pass8FB>;
 
 }  // namespace Fermi::Irf
