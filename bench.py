@@ -179,8 +179,9 @@ def main():
     # VALU view of the row-weight pass.  Algorithmic: 4 flops per rule point
     # per eval, the work of a cubature that evaluates every point for every
     # energy; the pass evaluates each point once per pixel for all energies,
-    # so this rate can exceed the FP32 peak.  The issue fraction of the
-    # VALU comes from the committed rocprofv3 counter pass.
+    # so this rate can exceed the FP32 peak.  The issued VALU lane operations
+    # per second, as a fraction of the FP32 vector peak, come from the
+    # committed rocprofv3 counter pass.
     n_points = len(inp.rule)
     valu_tf = 4.0 * n_points * evals / (stage_ms["cubature"] / 1e3) / 1e12
     valu_prof = _profile(f"valu_{cfg.name}_rowweight.json")
@@ -212,8 +213,8 @@ def main():
         "valu": {"kernel": "rowweight_kernel<float>",
                  "algorithmic_tflops": valu_tf, "peak": FP32_VALU_PEAK_TFS, "unit": "TFLOP/s",
                  "algorithmic_flops_per_eval": 4 * n_points,
-                 "issue_frac": valu_prof["valu_issue_frac"] if valu_prof else None,
-                 "clock_ghz": valu_prof["clock_ghz"] if valu_prof else None},
+                 "issued_frac_of_fp32_peak":
+                     valu_prof["valu_frac_of_fp32_peak"] if valu_prof else None},
     }
     if not args.no_cpu_baseline:
         _log("bench: cpu baseline (oracle)")

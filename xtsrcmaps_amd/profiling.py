@@ -126,11 +126,10 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
 
 # VALU counters of one kernel (one rocprofv3 --pmc pass with SQ_WAVES,
 # SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_WAIT_INST_ANY, SQ_WAVE_CYCLES and
-# GRBM_GUI_ACTIVE).  GRBM_GUI_ACTIVE is summed over the 8 XCDs, so the
-# shader clock is GRBM_GUI_ACTIVE / 8 / duration; a wave64 VALU instruction
-# occupies one of the 1024 SIMDs for 4 cycles (transcendentals count once).
-N_XCD = 8
-N_SIMD = 1024
+# GRBM_GUI_ACTIVE).  The VALU rate is counted in lane operations: a wave64
+# VALU instruction is 64 lane operations, and MI355X's FP32 vector peak of
+# 157.3 TFLOP/s is 78.65e12 FMA lane operations per second.
+VALU_LANE_OPS_PEAK = 157.3e12 / 2
 
 
 def valu_json(run_dir: str, kernel_rx: str, cmd: str) -> dict:
@@ -148,15 +147,15 @@ def valu_json(run_dir: str, kernel_rx: str, cmd: str) -> dict:
     rows = list(disp.values())
     mean = lambda k: sum(d[k] for d in rows) / len(rows)
     ns = mean("ns")
-    clk_ghz = mean("GRBM_GUI_ACTIVE") / N_XCD / ns
     valu = mean("SQ_INSTS_VALU")
-    util = valu * 4 / (N_SIMD * mean("GRBM_GUI_ACTIVE") / N_XCD)
+    lane_ops = valu * 64 / (ns * 1e-9)
     return {"command": cmd, "kernel": rows[0]["name"], "dispatches": len(rows),
-            "duration_us": ns / 1e3, "clock_ghz": clk_ghz, "waves": mean("SQ_WAVES"),
+            "duration_us": ns / 1e3, "waves": mean("SQ_WAVES"),
             "valu_insts": valu, "lds_insts": mean("SQ_INSTS_LDS"),
             "valu_insts_per_wave": valu / mean("SQ_WAVES"),
             "wait_any_per_wave_cycles": mean("SQ_WAIT_INST_ANY") / mean("SQ_WAVE_CYCLES"),
-            "valu_issue_frac": util}
+            "valu_lane_ops_per_s": lane_ops,
+            "valu_frac_of_fp32_peak": lane_ops / VALU_LANE_OPS_PEAK}
 
 
 def main(argv=None) -> None:
