@@ -164,22 +164,45 @@ __global__ void bilerp_aeff_kernel(int C, int E, AeffSecs as) {
     }
 }
 
-// psf.corrected_exposure_psf.cxx:46-117 for both sections and
-// psf.mean_psf.cxx:20-30: for each source and column n = (d, e):
-//   front = (ECB . PAf) LTF0 + (WECB . PAf) LTF1, back likewise with PAb,
-//   mean  = (front + back) / exposure.
-// FP64 FMA; a block covers kSrcTile sources x 256 columns, the livetime rows
-// of its sources staged in LDS.
+// Both sections' PA summed, PA = PAf + PAb (the same livetime rows and, as
+// the reference codes it, the front LTF for both: exposure.cxx:136-150,
+// psf.psf_lookup_table_and_partial_integrals.cxx:42-47).
+__global__ void bilerp_aeff_sum_kernel(int C, int E, AeffSecs as, double* __restrict__ out) {
+    const size_t n = (size_t)C * kND * E;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int e = (int)(k % E);
+        const int d = (int)((k / E) % kND);
+        const int c = (int)(k / ((size_t)E * kND));
+        double sum = 0.0;
+#pragma unroll
+        for (int sec = 0; sec < 2; ++sec) {
+            const AeffSec& g = as.s[sec];
+            const BilerpPoint q = g.pts[c * E + e];
+            if (q.base < 0) continue;
+            const double* Kd = g.K + (size_t)d * g.Mc * g.Me + q.base;
+            const double v =
+                q.w00 * Kd[0] + q.w01 * Kd[1] + q.w10 * Kd[g.Me] + q.w11 * Kd[g.Me + 1];
+            sum += v * g.aeff[(size_t)c * E + e];
+        }
+        out[k] = sum;
+    }
+}
+
+// psf.corrected_exposure_psf.cxx:46-117 and psf.mean_psf.cxx:20-30: for each
+// source and column n = (d, e),
+//   (ECB . PA) LTF0 + (WECB . PA) LTF1   (/ exposure when kMean),
+// with PA = PAf + PAb for the mean PSF (front + back by linearity) or one
+// section's PA.  FP64 FMA; a block covers kSrcTile sources x 256 columns,
+// the livetime rows of its sources staged in LDS.
 constexpr int kSrcTile = 8;
 
-// kMean false: one section only, out = (ECB . PA) LTF0 + (WECB . PA) LTF1
-// (psf.corrected_exposure_psf.cxx:46-117 alone).
 template <bool kMean>
 __global__ void __launch_bounds__(256)
 mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
-                const double* __restrict__ wecb, const double* __restrict__ paf,
-                const double* __restrict__ pab, const double* __restrict__ ltf,
-                const double* __restrict__ expo, double* __restrict__ out) {
+                const double* __restrict__ wecb, const double* __restrict__ pa,
+                const double* __restrict__ ltf, const double* __restrict__ expo,
+                double* __restrict__ out) {
     extern __shared__ double sh[];  // [2][kSrcTile][C]
     const int N = kND * E;
     const int s0 = blockIdx.y * kSrcTile;
@@ -193,19 +216,16 @@ mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int e = n % E;
-    double ef[kSrcTile], wf[kSrcTile], eb[kSrcTile], wb[kSrcTile];
+    double ef[kSrcTile], wf[kSrcTile];
 #pragma unroll
-    for (int r = 0; r < kSrcTile; ++r) ef[r] = wf[r] = eb[r] = wb[r] = 0.0;
+    for (int r = 0; r < kSrcTile; ++r) ef[r] = wf[r] = 0.0;
 #pragma unroll 4
     for (int c = 0; c < C; ++c) {
-        const double f = paf[(size_t)c * N + n], b = kMean ? pab[(size_t)c * N + n] : 0.0;
+        const double f = pa[(size_t)c * N + n];
 #pragma unroll
         for (int r = 0; r < kSrcTile; ++r) {
-            const double x = sh[r * C + c], y = sh[(kSrcTile + r) * C + c];
-            ef[r] = fma(x, f, ef[r]);
-            wf[r] = fma(y, f, wf[r]);
-            eb[r] = fma(x, b, eb[r]);
-            wb[r] = fma(y, b, wb[r]);
+            ef[r] = fma(sh[r * C + c], f, ef[r]);
+            wf[r] = fma(sh[(kSrcTile + r) * C + c], f, wf[r]);
         }
     }
     const double l0 = ltf[e], l1 = ltf[E + e];
@@ -213,19 +233,14 @@ mean_psf_kernel(int S, int C, int E, const double* __restrict__ ecb,
     for (int r = 0; r < kSrcTile; ++r) {
         if (r < ns) {
             const int s = s0 + r;
-            const double front = ef[r] * l0 + wf[r] * l1;
-            if (kMean) {
-                const double back = eb[r] * l0 + wb[r] * l1;
-                out[(size_t)s * N + n] = (front + back) / expo[(size_t)s * E + e];
-            } else {
-                out[(size_t)s * N + n] = front;
-            }
+            const double v = ef[r] * l0 + wf[r] * l1;
+            out[(size_t)s * N + n] = kMean ? v / expo[(size_t)s * E + e] : v;
         }
     }
 }
 
-// BF16x3-MFMA variant: P = [ECB; WECB] (2S x C) . [PAf | PAb] (C x 2N)
-// computed by gemm_bf16x3, then this combine.
+// BF16x3-MFMA variant: P = [ECB; WECB] (2S x C) . PA (C x N) computed by
+// gemm_bf16x3, then this combine.
 __global__ void mean_psf_combine_kernel(int S, int E, const double* __restrict__ P,
                                         const double* __restrict__ ltf,
                                         const double* __restrict__ expo, double* __restrict__ out) {
@@ -234,21 +249,9 @@ __global__ void mean_psf_combine_kernel(int S, int E, const double* __restrict__
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_all;
          k += (size_t)gridDim.x * blockDim.x) {
         const int s = (int)(k / N), n = (int)(k % N), e = n % E;
-        const size_t ld = 2 * (size_t)N;
-        const double ef = P[(size_t)s * ld + n], eb = P[(size_t)s * ld + N + n];
-        const double wf = P[(size_t)(S + s) * ld + n], wb = P[(size_t)(S + s) * ld + N + n];
+        const double ef = P[(size_t)s * N + n], wf = P[(size_t)(S + s) * N + n];
         const double l0 = ltf[e], l1 = ltf[E + e];
-        out[k] = ((ef * l0 + wf * l1) + (eb * l0 + wb * l1)) / expo[(size_t)s * E + e];
-    }
-}
-
-__global__ void pack_pa_kernel(int C, int N, const double* __restrict__ a,
-                               const double* __restrict__ b, double* __restrict__ out) {
-    const size_t n = (size_t)C * 2 * N;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = k / (2 * (size_t)N), j = k % (2 * (size_t)N);
-        out[k] = j < (size_t)N ? a[c * N + j] : b[c * N + j - N];
+        out[k] = (ef * l0 + wf * l1) / expo[(size_t)s * E + e];
     }
 }
 
@@ -258,8 +261,9 @@ __global__ void pack_pa_kernel(int C, int N, const double* __restrict__ a,
 // is normalised by its last value, and so is uPsf.  A block takes kIntE
 // energies of one source; its threads split the 400 segments into kIntP
 // contiguous parts (consecutive threads: consecutive energies, so every
-// load is coalesced).  Pass 1 sums each part, the parts' exclusive prefix
-// comes from LDS, pass 2 writes the running sums, pass 3 normalises uPsf.
+// load is coalesced).  A thread loads its part's nodes into registers at
+// once and sums its segments; the parts' exclusive prefix comes from LDS;
+// then it writes the running sums and the normalised uPsf.
 // tab[d] = theta_d (rad), tab[401 + d] = sin theta_d.
 constexpr int kIntE = 16, kIntP = 32;
 
@@ -272,6 +276,8 @@ __device__ inline double segment_integral(const double* __restrict__ tab, int d,
     const double b = y1 - t1 * m;
     return 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
 }
+
+constexpr int kIntSeg = (kND - 1 + kIntP - 1) / kIntP;   // segments of a part, at most
 
 __global__ void __launch_bounds__(kIntE * kIntP)
 integral_kernel(int S, int E, double* __restrict__ upsf, double* __restrict__ parint,
@@ -286,36 +292,42 @@ integral_kernel(int S, int E, double* __restrict__ upsf, double* __restrict__ pa
     double* pi = parint + (size_t)s * kND * E + ec;
     constexpr int nseg = kND - 1;
     const int d_lo = 1 + part * nseg / kIntP, d_hi = 1 + (part + 1) * nseg / kIntP;
+    const int n = d_hi - d_lo;
+    // the part's nodes d_lo-1 .. d_hi-1 in registers, loaded together
+    double v[kIntSeg + 1], seg[kIntSeg];
+#pragma unroll
+    for (int i = 0; i <= kIntSeg; ++i) v[i] = i <= n ? u[(size_t)(d_lo - 1 + i) * E] : 0.0;
     double acc = 0.0;
-    double prev = u[(size_t)(d_lo - 1) * E];
-    for (int d = d_lo; d < d_hi; ++d) {
-        const double cur = u[(size_t)d * E];
-        acc += segment_integral(tab, d, prev, cur);
-        prev = cur;
+#pragma unroll
+    for (int i = 0; i < kIntSeg; ++i) {
+        seg[i] = i < n ? segment_integral(tab, d_lo + i, v[i], v[i + 1]) : 0.0;
+        acc += seg[i];
     }
     psum[part][el] = acc;
+    // after this barrier every thread holds its nodes: uPsf may be written
     __syncthreads();
     double carry = 0.0, tot = 0.0;
     for (int q = 0; q < kIntP; ++q) {
-        const double v = psum[q][el];
-        if (q < part) carry += v;
-        tot += v;
+        const double w = psum[q][el];
+        if (q < part) carry += w;
+        tot += w;
     }
-    // running sums, normalised by the total (node 0 stays 0)
-    double run = carry;
-    prev = u[(size_t)(d_lo - 1) * E];
-    if (valid && part == 0) pi[0] = 0.0;
-    for (int d = d_lo; d < d_hi; ++d) {
-        const double cur = u[(size_t)d * E];
-        run += segment_integral(tab, d, prev, cur);
-        prev = cur;
-        if (valid) pi[(size_t)d * E] = tot != 0.0 ? run / tot : run;
-    }
-    if (valid && part == 0 && total) total[(size_t)s * E + e] = tot;
-    __syncthreads();
     if (!valid) return;
-    if (part == 0) u[0] /= tot;
-    for (int d = d_lo; d < d_hi; ++d) u[(size_t)d * E] /= tot;
+    // running sums, normalised by the total (node 0 stays 0), and uPsf / total
+    double run = carry;
+    if (part == 0) {
+        pi[0] = 0.0;
+        u[0] = v[0] / tot;
+        if (total) total[(size_t)s * E + e] = tot;
+    }
+#pragma unroll
+    for (int i = 0; i < kIntSeg; ++i) {
+        if (i < n) {
+            run += seg[i];
+            pi[(size_t)(d_lo + i) * E] = tot != 0.0 ? run / tot : run;
+            u[(size_t)(d_lo + i) * E] = v[i + 1] / tot;
+        }
+    }
 }
 
 // Gauss-Legendre nodes and weights by Newton iteration on P_n.
@@ -398,11 +410,10 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
     const size_t kf = (size_t)kND * g[0].Mc * g[0].Me, kb = (size_t)kND * g[1].Mc * g[1].Me;
     const size_t npa = (size_t)C * N;
     void* w;
-    XTS_TRY(slot_reserve(ctx, kSlotWork0, (kf + kb + 2 * npa) * 8, &w));
+    XTS_TRY(slot_reserve(ctx, kSlotWork0, (kf + kb + npa) * 8, &w));
     double* Kf = (double*)w;
     double* Kb = Kf + kf;
-    double* PAf = Kb + kb;
-    double* PAb = PAf + npa;
+    double* PA = Kb + kb;   // front + back
     const KingSecs ks{{{g[0].params, Kf, g[0].Mc, g[0].Me}, {g[1].params, Kb, g[1].Mc, g[1].Me}}};
     king_kernel<<<dim3(std::min<unsigned>(div_up(std::max(kf, kb), 256), 4096), 2), 256, 0,
                   ctx->stream>>>(ks, dsep);
@@ -414,24 +425,22 @@ static int psf_lookup_device(xts_ctx* ctx, const PsfDevGrid g[2], const double* 
                          {g[1].cosths, g[1].logEs, bpb, g[1].Mc, g[1].Me}}};
     bilerp_points_kernel<<<dim3(div_up(C * E, 256), 2), 256, 0, ctx->stream>>>(C, E, costh, logE,
                                                                               bs, err_dev);
-    const AeffSecs as{{{bpf, Kf, af, PAf, g[0].Mc, g[0].Me}, {bpb, Kb, ab, PAb, g[1].Mc, g[1].Me}}};
-    bilerp_aeff_kernel<<<dim3(std::min<unsigned>(div_up(npa, 256), 16384), 2), 256, 0,
-                         ctx->stream>>>(C, E, as);
+    const AeffSecs as{{{bpf, Kf, af, nullptr, g[0].Mc, g[0].Me},
+                       {bpb, Kb, ab, nullptr, g[1].Mc, g[1].Me}}};
+    bilerp_aeff_sum_kernel<<<std::min<unsigned>(div_up(npa, 256), 16384), 256, 0, ctx->stream>>>(
+        C, E, as, PA);
     if (precision == XTS_PREC_FP64) {
         const size_t sm = 2 * kSrcTile * C * 8;
         mean_psf_kernel<true><<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
-            S, C, E, ecb, wecb, PAf, PAb, ltf, expo, upsf);
+            S, C, E, ecb, wecb, PA, ltf, expo, upsf);
     } else {
         void* w2;
-        const size_t nA = 2 * (size_t)S * C, nB = (size_t)C * 2 * N, nP = 2 * (size_t)S * 2 * N;
-        XTS_TRY(slot_reserve(ctx, kSlotWork2, (nA + nB + nP) * 8, &w2));
+        const size_t nA = 2 * (size_t)S * C, nP = 2 * (size_t)S * N;
+        XTS_TRY(slot_reserve(ctx, kSlotWork2, (nA + nP) * 8, &w2));
         double* A = (double*)w2;
-        double* B = A + nA;
-        double* P = B + nB;
+        double* P = A + nA;
         pack_rows_kernel<<<div_up(nA, 256), 256, 0, ctx->stream>>>(S, C, ecb, wecb, A);
-        pack_pa_kernel<<<std::min<unsigned>(div_up(nB, 256), 16384), 256, 0, ctx->stream>>>(
-            C, N, PAf, PAb, B);
-        XTS_TRY(gemm_bf16x3(ctx, A, B, P, 2 * S, C, 2 * N));
+        XTS_TRY(gemm_bf16x3(ctx, A, PA, P, 2 * S, C, N));
         mean_psf_combine_kernel<<<std::min<unsigned>(div_up((int64_t)S * N, 256), 16384), 256, 0,
                                   ctx->stream>>>(S, E, P, ltf, expo, upsf);
     }
@@ -644,8 +653,8 @@ int xts_corrected_exposure_psf(xts_ctx* ctx, const double* obs_psf, const double
     const int N = kND * (int)E;
     const size_t sm = 2 * kSrcTile * C * 8;
     mean_psf_kernel<false><<<dim3(div_up(N, 256), div_up(S, kSrcTile)), 256, sm, ctx->stream>>>(
-        (int)S, (int)C, (int)E, (double*)decb, (double*)dwecb, (double*)dpa, nullptr,
-        (double*)dltf, nullptr, (double*)dout);
+        (int)S, (int)C, (int)E, (double*)decb, (double*)dwecb, (double*)dpa, (double*)dltf,
+        nullptr, (double*)dout);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(out, dout, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipStreamSynchronize(ctx->stream));
