@@ -9,11 +9,12 @@
 //                        in-radius row histogram of the map integral
 //   4. near_rows_kernel  the same for the pixels whose rule points span more
 //                        rows (near the source), up to 48 rows per pixel
-//   5. hist_integral_kernel, factor_kernel
+//   5. hist_integral_kernel
 //                        map integral, correction factor per (source, energy)
 //   6. emit_*_kernel     every map value once: row weights . table rows
 //                        x exposure x correction
-//   7. near_emit_kernel  the same for the near pixels
+//   7. near_emit_kernel  the same for the near pixels (wide_pixel_kernel for
+//                        the few whose points touch more than 48 rows)
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -1124,41 +1125,6 @@ near_emit_kernel(CubArgs a) {
     }
 }
 
-// map_integral.cxx:21-44 through the row histogram: the in-radius sum of
-// the (solid-angle scaled) map is sum_d hist[s, d] uPsf[s, d, e] (FP64
-// table), listed pixels included.  One block per source: each thread takes one energy
-// and every P-th row (consecutive threads read consecutive energies), the
-// P partial sums are added in LDS.
-constexpr int kHistThreads = 1024;
-__global__ void __launch_bounds__(kHistThreads)
-hist_integral_kernel(int E, const double* __restrict__ hist, const double* __restrict__ upsf,
-                     double* __restrict__ mapint) {
-    __shared__ double h_s[kNDPad];
-    __shared__ double part[kHistThreads];
-    const int s = blockIdx.x, tid = threadIdx.x;
-    for (int d = tid; d < kND; d += kHistThreads) h_s[d] = hist[(size_t)s * kND + d];
-    const int Ec = min(E, kHistThreads);
-    const int P = kHistThreads / Ec;
-    const int el = tid % Ec, ph = tid / Ec;
-    const double* u = upsf + (size_t)s * kND * E;
-    for (int eb = 0; eb < E; eb += Ec) {
-        __syncthreads();
-        const int e = eb + el;
-        double acc = 0.0;
-        if (ph < P && e < E) {
-#pragma unroll 4
-            for (int d = ph; d < kND; d += P) acc = fma(h_s[d], u[(size_t)d * E + e], acc);
-        }
-        part[tid] = acc;
-        __syncthreads();
-        if (ph == 0 && e < E) {
-            double tot = 0.0;
-            for (int q = 0; q < P; ++q) tot += part[q * Ec + el];
-            mapint[(size_t)s * E + e] = tot;
-        }
-    }
-}
-
 // Pass 3: per (source, pixel range, energy chunk): map = sum_k W_k L[r0+k]
 // for pass-1 pixels, x exposure x correction when SCALED.  Listed pixels are
 // left to near_emit_kernel.  OT: output type.  Each thread first loads the
@@ -1371,6 +1337,69 @@ static int emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool scaled) {
 // ------------------------------------------------------------------ factors
 // model_map.map_integral.cxx:41-44 (reciprocal), map_correction_factor.cxx:
 // 24-50 and scale_map_by_exposure: f[s, e] = exposure x correction.
+// map_correction_factor.cxx:24-50 for one (source, energy): the PSF
+// integral to the node below the radius plus its whole next segment
+// (capped at 1), x 1 / map integral; 1 outside the field of view.
+__device__ inline double correction_factor(const SrcGeom& g, double invmi,
+                                           const double* __restrict__ upsf,
+                                           const double* __restrict__ parint,
+                                           const double* __restrict__ sep, int s, int e, int E) {
+    if (!g.in_fov) return 1.0;
+    const double r = g.radius_deg;
+    int d = r < 1.0 ? 1 : (int)floor(1.0 + log(1e4 * r) * kRecipStep);
+    if (d >= kND - 2) d = kND - 2;
+    const double t1 = sep[d] * kD2R, t2 = sep[d + 1] * kD2R;
+    const double y1 = kTwoPi * upsf[((size_t)s * kND + d) * E + e] * sin(t1);
+    const double y2 = kTwoPi * upsf[((size_t)s * kND + d + 1) * E + e] * sin(t2);
+    const double m = (y2 - y1) / (t2 - t1);
+    const double b = y1 - t1 * m;
+    const double v = 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
+    double integ = v + parint[((size_t)s * kND + d) * E + e];
+    integ = integ > 1.0 ? 1.0 : integ;
+    return integ * invmi;
+}
+
+// map_integral.cxx:21-44 through the row histogram: the in-radius sum of
+// the (solid-angle scaled) map is sum_d hist[s, d] uPsf[s, d, e] (FP64
+// table), listed pixels included; then the correction factor of each
+// (source, energy) from it.  One block per source: each thread takes one
+// energy and every P-th row (consecutive threads read consecutive
+// energies), the P partial sums are added in LDS.
+constexpr int kHistThreads = 1024;
+__global__ void __launch_bounds__(kHistThreads)
+hist_integral_kernel(int E, const double* __restrict__ hist, const double* __restrict__ upsf,
+                     double* __restrict__ mapint, const SrcGeom* __restrict__ geo,
+                     const double* __restrict__ parint, const double* __restrict__ sep,
+                     double* __restrict__ cf_out) {
+    __shared__ double h_s[kNDPad];
+    __shared__ double part[kHistThreads];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    for (int d = tid; d < kND; d += kHistThreads) h_s[d] = hist[(size_t)s * kND + d];
+    const int Ec = min(E, kHistThreads);
+    const int P = kHistThreads / Ec;
+    const int el = tid % Ec, ph = tid / Ec;
+    const double* u = upsf + (size_t)s * kND * E;
+    const SrcGeom g = geo[s];
+    for (int eb = 0; eb < E; eb += Ec) {
+        __syncthreads();
+        const int e = eb + el;
+        double acc = 0.0;
+        if (ph < P && e < E) {
+#pragma unroll 4
+            for (int d = ph; d < kND; d += P) acc = fma(h_s[d], u[(size_t)d * E + e], acc);
+        }
+        part[tid] = acc;
+        __syncthreads();
+        if (ph == 0 && e < E) {
+            double tot = 0.0;
+            for (int q = 0; q < P; ++q) tot += part[q * Ec + el];
+            mapint[(size_t)s * E + e] = tot;
+            const double invmi = tot != 0.0 ? 1.0 / tot : tot;
+            cf_out[(size_t)s * E + e] = correction_factor(g, invmi, upsf, parint, sep, s, e, E);
+        }
+    }
+}
+
 __global__ void factor_kernel(int S, int E, const SrcGeom* __restrict__ geo,
                               const double* __restrict__ mapint,
                               const double* __restrict__ invmi_in, const double* __restrict__ upsf,
@@ -1388,21 +1417,7 @@ __global__ void factor_kernel(int S, int E, const SrcGeom* __restrict__ geo,
     } else if (invmi_in) {
         invmi = invmi_in[k];
     }
-    double cf = 1.0;
-    if (g.in_fov) {
-        const double r = g.radius_deg;
-        int d = r < 1.0 ? 1 : (int)floor(1.0 + log(1e4 * r) * kRecipStep);
-        if (d >= kND - 2) d = kND - 2;
-        const double t1 = sep[d] * kD2R, t2 = sep[d + 1] * kD2R;
-        const double y1 = kTwoPi * upsf[((size_t)s * kND + d) * E + e] * sin(t1);
-        const double y2 = kTwoPi * upsf[((size_t)s * kND + d + 1) * E + e] * sin(t2);
-        const double m = (y2 - y1) / (t2 - t1);
-        const double b = y1 - t1 * m;
-        const double v = 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
-        double integ = v + parint[((size_t)s * kND + d) * E + e];
-        integ = integ > 1.0 ? 1.0 : integ;
-        cf = integ * invmi;
-    }
+    const double cf = correction_factor(g, invmi, upsf, parint, sep, s, e, E);
     if (cf_out) cf_out[k] = cf;
     if (invmi_out) invmi_out[k] = invmi;
 }
@@ -1444,12 +1459,6 @@ __global__ void map_integral_kernel(int H, int W, int E, const double* __restric
             }
         out[(size_t)s * E + e] = acc != 0.0 ? 1.0 / acc : acc;
     }
-}
-
-__global__ void zero_kernel(double* __restrict__ p, size_t n) {
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (size_t)gridDim.x * blockDim.x)
-        p[k] = 0.0;
 }
 
 __global__ void pix2sph_kernel(Wcs w, int64_t n, const double* __restrict__ in,
@@ -1512,17 +1521,22 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
 }
 
 // FP32 copy of the PSF table, energy axis padded to a multiple of 4.
+// The PSF table in the cubature's precision, energies padded to Ep; the
+// same grid clears the nzero accumulators of the model map.
 __global__ void lut_convert_kernel(const double* __restrict__ upsf, int S, int E, int Ep,
-                                   float* __restrict__ out32, double* __restrict__ out64) {
+                                   float* __restrict__ out32, double* __restrict__ out64,
+                                   double* __restrict__ zero, size_t nzero) {
     const size_t n = (size_t)S * kND * Ep;
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i0; k < n; k += step) {
         const int e = (int)(k % Ep);
         const size_t sd = k / Ep;
         const double v = e < E ? upsf[sd * E + e] : 0.0;
         if (out32) out32[k] = (float)v;
         if (out64) out64[k] = v;
     }
+    for (size_t k = i0; k < nzero; k += step) zero[k] = 0.0;
 }
 
 // The rule as per-point records on the device, with its bilinear-weight
@@ -1613,8 +1627,18 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     void* lut;
     const size_t lut_elems = (size_t)S * kND * Ep;
     XTS_TRY(slot_reserve(ctx, kSlotLut, lut_elems * (f64 ? 8 : 4), &lut));
+    // counters [2 S] (int, padded to doubles: near counts, pool entries taken
+    // per source) and FP64 accumulators: hist [S][401], mapint [S][E],
+    // factor [S][E]; the counters and hist start at zero
+    const size_t cnt_d = (size_t)S;
+    const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
+    void* accp;
+    XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
+    double* acc = (double*)accp;
+    const size_t nzero = full ? cnt_d + (size_t)S * kND : cnt_d;
     lut_convert_kernel<<<std::min<unsigned>(div_up(lut_elems, 256), 8192), 256, 0, ctx->stream>>>(
-        upsf, (int)S, (int)E, Ep, f64 ? nullptr : (float*)lut, f64 ? (double*)lut : nullptr);
+        upsf, (int)S, (int)E, Ep, f64 ? nullptr : (float*)lut, f64 ? (double*)lut : nullptr, acc,
+        nzero);
 
     // per-pixel row weights and first rows; listed (near) pixels, their pool
     // offsets and row ranges
@@ -1622,16 +1646,6 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const size_t rw_bytes = (size_t)S * plane * (f64 ? 32 : 16);
     void* rows;
     XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 16 + 512, &rows));
-    // counters [2 S] (int, padded to doubles: near counts, pool entries taken
-    // per source) and FP64 accumulators: hist [S][401], mapint [S][E],
-    // factor [S][E]
-    const size_t cnt_d = (size_t)S;
-    const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
-    void* accp;
-    XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
-    double* acc = (double*)accp;
-    const size_t nzero = full ? cnt_d + (size_t)S * kND : cnt_d;
-    zero_kernel<<<std::min<unsigned>(div_up(nzero, 256), 4096), 256, 0, ctx->stream>>>(acc, nzero);
     // pool of listed pixels' row weights, one segment per source with room
     // for 2048 pixels of 16 rows; a pixel that finds its segment full is
     // evaluated directly
@@ -1712,12 +1726,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     }
     if (full) {
         TimedRegion tr(ctx, "factor");
-        const int nse = (int)(S * E);
-        hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>((int)E, ca.hist, upsf,
-                                                                            ca.mapint);
-        factor_kernel<<<div_up(nse, 256), 256, 0, ctx->stream>>>((int)S, (int)E, ig.geo, ca.mapint,
-                                                                  nullptr, upsf, parint, ct->sep,
-                                                                  expo, fac, nullptr);
+        hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>(
+            (int)E, ca.hist, upsf, ca.mapint, ig.geo, parint, ct->sep, fac);
         XTS_HIP(hipGetLastError());
     }
     // Wide pixels depend only on the factors and the near lists, and write
