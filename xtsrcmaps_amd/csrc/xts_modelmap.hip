@@ -803,7 +803,8 @@ rowweight_kernel(CubArgs a) {
 // A pixel whose points touch at most kNearRows consecutive rows rbase ..
 // rbase+R-1 gathers its row weights and stores them, x solid angle, in a
 // pool (R values at an offset in its source's segment); its in-radius
-// weights go to the source's row histogram.  kNearSplit lanes of one wave
+// weights go to the block's row sums in LDS, which are added to the
+// source's row histogram once per block.  kNearSplit lanes of one wave
 // share a pixel, each taking every kNearSplit-th rule point into a
 // lane-private LDS histogram (rows x threads, no two lanes on one word); the
 // lanes then sum the kNearSplit partial rows.  Below 14 degrees the FP32 row
@@ -825,12 +826,15 @@ near_rows_kernel(CubArgs a) {
     const int N4 = (a.N + 3) & ~3;
     float4* tco_s = reinterpret_cast<float4*>(rule_s + N4);
     T* ph = reinterpret_cast<T*>(tco_s + N4);   // [kNearRows][kNearThreads]
+    // the block's in-radius row sums, added to the source's histogram once
+    __shared__ double hblk[kNDPad];
 
     const int s = blockIdx.y;
     const int count = a.near_count[s];
     if ((int)blockIdx.x * kNearPix >= count) return;
     const int tid = threadIdx.x;
     const int slot = tid / kNearSplit, part = tid % kNearSplit;
+    for (int k = tid; k < kND; k += kNearThreads) hblk[k] = 0.0;
     stage_rule<T>(a, isep_s, rule_s, N4, tid, kNearThreads);
     for (int p = tid; p < N4; p += kNearThreads) {
         const float wq = p < a.N ? (float)a.uvw[p].wq : 0.0f;
@@ -914,8 +918,8 @@ near_rows_kernel(CubArgs a) {
                     myh[l * kNearThreads] += wa;
                     myh[(l + 1) * kNearThreads] += wb;
                 } else if (inrad) {
-                    atomicAdd(&hist[i - 1], (double)(om * wa));
-                    atomicAdd(&hist[i], (double)(om * wb));
+                    atomicAdd(&hblk[i - 1], (double)(om * wa));
+                    atomicAdd(&hblk[i], (double)(om * wb));
                 }
             }
         }
@@ -929,13 +933,18 @@ near_rows_kernel(CubArgs a) {
                 for (int q = 0; q < kNearSplit; ++q) W += base[r * kNearThreads + q];
                 W *= om;
                 pool[off + r] = W;
-                if (inrad) atomicAdd(&hist[rbase + r], (double)W);
+                if (inrad) atomicAdd(&hblk[rbase + r], (double)W);
             }
         }
         if (live && part == 0) {
             meta_off[idx] = off;
             meta_rr[idx] = rbase * 256 + min(R, 255);
         }
+    }
+    if (do_hist) {
+        __syncthreads();
+        for (int k = tid; k < kND; k += kNearThreads)
+            if (hblk[k] != 0.0) atomicAdd(&hist[k], hblk[k]);
     }
 }
 
