@@ -8,13 +8,13 @@
 //                        PSF-table row weights (x solid angle) and the
 //                        in-radius row histogram of the map integral
 //   4. near_rows_kernel  the same for the pixels whose rule points span more
-//                        rows (near the source), up to 48 rows per pixel
+//                        rows (near the source), up to 24 rows per pixel
 //   5. hist_integral_kernel
 //                        map integral, correction factor per (source, energy)
 //   6. emit_*_kernel     every map value once: row weights . table rows
 //                        x exposure x correction
 //   7. near_emit_kernel  the same for the near pixels (wide_pixel_kernel for
-//                        the few whose points touch more than 48 rows)
+//                        the few whose points touch more than 24 rows)
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -813,7 +813,7 @@ rowweight_kernel(CubArgs a) {
 // "wide" (pool offset -1): only its in-radius weights are added here, point
 // by point, and wide_pixel_kernel evaluates it.  Grid: (blocks striding over
 // the list, S).
-constexpr int kNearRows = 48;
+constexpr int kNearRows = 24;
 constexpr int kNearSplit = 4;
 constexpr int kNearPix = kNearThreads / kNearSplit;   // pixels per block pass
 template <typename T>
