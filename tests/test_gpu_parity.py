@@ -279,6 +279,41 @@ def test_gpu_layouts_and_raw_cubature_agree(ctx, c1):
     np.testing.assert_allclose(m, 3.25, rtol=2e-6)
 
 
+@pytest.mark.parametrize("proj", ["CAR", "AIT"])
+def test_gpu_non_square_images(ctx, orc, c1, proj):
+    """H != W (the reference loops h < Nw, quirk Q3: this build loops h < Nh),
+    unequal CDELTs, an oblique field, a 3 x 4 tensor rule (tensor loop with
+    nu != nv) and the degree-7 list rule, both layouts, FP32 and FP64
+    cubature, against the FP64 oracle."""
+    import xtsrcmaps_amd as xa
+    from pipelines import DEG7, oracle_wcs
+    _log(f"test_gpu_non_square_images: 48x72 {proj}")
+    H, W = 48, 72
+    wcs = xa.Wcs.make((H, W), (20.5, 40.5), (210.0, 35.0), (-0.11, 0.08), 0.0, proj)
+    w = oracle_wcs(orc, wcs)
+    # sources inside, near an edge and outside the image
+    src = ctx.pix2sph(wcs, np.array([[24.0, 30.0], [2.0, 70.0], [60.0, 10.0]]))
+    rng = np.random.default_rng(21)
+    S, E = 3, 9
+    upsf = rng.uniform(0.1, 1.0, (S, 401, E)) * np.exp(-np.linspace(0, 8, 401))[None, :, None]
+    upsf, parint, _ = ctx.partial_total_integral(upsf)
+    expo = rng.uniform(1e8, 1e9, (S, E))
+    x3, w3 = np.polynomial.legendre.leggauss(3)
+    x4, w4 = np.polynomial.legendre.leggauss(4)
+    rule34 = np.array([[u, v, wu * wv] for u, wu in zip(x3, w3) for v, wv in zip(x4, w4)])
+    for rule in (rule34, DEG7):
+        ref = orc.point_src_model_map_wcs(H, W, w, src, expo, upsf, parint, rule)
+        for flags, tol in ((xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F64, 1e-10),
+                           (xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F32, 1e-5),
+                           (xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32, 1e-5)):
+            got = ctx.point_src_model_map_wcs(wcs, src, expo, upsf, parint, rule, flags=flags)
+            if flags & xa.LAYOUT_SEHW:
+                got = got.transpose(0, 2, 3, 1)
+            r = max_rel(got, ref)
+            _log(f"  rule {len(rule)} flags {flags:#x}: max rel {r:.2e}")
+            assert got.shape == (S, H, W, E) and r < tol
+
+
 def test_gpu_edge_cases(ctx, orc, c0):
     import xtsrcmaps_amd as xa
     from pipelines import oracle_wcs
