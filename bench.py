@@ -206,7 +206,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg.name),
                      "kernel": "model-map passes (rowweight_kernel, near_rows_kernel, "
-                               "hist_integral_kernel + factor_kernel, emit_sehw_kernel, "
+                               "hist_integral_kernel, emit_sehw_kernel, "
                                "near_emit_kernel + wide_pixel_kernel); algorithmic bytes = "
                                "4 B per FP32 map value; time = their summed device time",
                      "stage_ms_per_step": stage_ms},

@@ -67,7 +67,7 @@ def _one(pattern: str) -> str:
     return found[0]
 
 
-def stats_markdown(path: str, cmd: str, top: int = 16) -> str:
+def stats_markdown(path: str, cmd: str, top: int = 40) -> str:
     rows = list(csv.DictReader(open(path)))
     lines = [f"Command: `{cmd}`", "",
              "| kernel | calls | avg µs | total µs | % |", "|---|---|---|---|---|"]
