@@ -550,8 +550,10 @@ constexpr float kLinQTol = 1e-7f;
 // One point: wq = w/4; tA, tB, tC = wq (t1, t2, t3).  Accumulates the
 // cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1 likewise
 // of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
-// 2^24 (1 - Q) >= 1 whenever Q < 1.
-template <int NG>
+// 2^24 (1 - Q) >= 1 whenever Q < 1.  NM is the number of row boundaries
+// the pixel's points may cross (Q < NM + 1): with NM = 1 every point has
+// rel <= 1, so C1 and D1 are the totals and only the first mask is formed.
+template <int NG, int NM = 2>
 __device__ inline void lean_point(float z, float wq, float tA, float tB, float tC,
                                   const LeanPixel& lp, float& C0, float& D0, float& C1,
                                   float& D1, float& Tt) {
@@ -563,11 +565,14 @@ __device__ inline void lean_point(float z, float wq, float tA, float tB, float t
     const float f = __builtin_amdgcn_fractf(Q);
     const float wt = f * fmaf(f, fmaf(f, tC, tB), tA);
     const float m0 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 16777216.0f), 0.0f, 1.0f);
-    const float m1 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 33554432.0f), 0.0f, 1.0f);
     C0 = fmaf(wq, m0, C0);
     D0 = fmaf(wt, m0, D0);
-    C1 = fmaf(wq, m1, C1);
-    D1 = fmaf(wt, m1, D1);
+    if constexpr (NM >= 2) {
+        const float m1 =
+            __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 33554432.0f), 0.0f, 1.0f);
+        C1 = fmaf(wq, m1, C1);
+        D1 = fmaf(wt, m1, D1);
+    }
     Tt += wt;
 }
 
@@ -587,10 +592,11 @@ __device__ inline float lane(const float4& x, int k) {
     return k == 0 ? x.x : (k == 1 ? x.y : (k == 2 ? x.z : x.w));
 }
 
-template <int NG>
+// With NM = 1 the loops return C1 = wtot (the rule's sum of w/4) and D1 = Tt.
+template <int NG, int NM = 2>
 __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
-                                        const PixelGeom<float>& pg, int r0, float& C0, float& D0,
-                                        float& C1, float& D1, float& Tt) {
+                                        const PixelGeom<float>& pg, int r0, float wtot,
+                                        float& C0, float& D0, float& C1, float& D1, float& Tt) {
     const LeanPixel lp = lean_pixel<NG>(pg, r0);
     const int nv = a.tensor_nv;
     for (int i = 0; i < a.tensor_nu; ++i) {
@@ -607,17 +613,21 @@ __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const float v = lane(r[0], k);
-                lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), lane(r[1], k), lane(r[2], k),
-                               lane(r[3], k), lane(r[4], k), lp, C0, D0, C1, D1, Tt);
+                lean_point<NG, NM>(fmaf(fmaf(cu, v, bu), v, au), lane(r[1], k), lane(r[2], k),
+                                   lane(r[3], k), lane(r[4], k), lp, C0, D0, C1, D1, Tt);
             }
         }
     }
+    if constexpr (NM < 2) {
+        C1 = wtot;
+        D1 = Tt;
+    }
 }
 
-template <int NG>
+template <int NG, int NM = 2>
 __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeom<float>& pg,
-                                      int r0, float& C0, float& D0, float& C1, float& D1,
-                                      float& Tt) {
+                                      int r0, float wtot, float& C0, float& D0, float& C1,
+                                      float& D1, float& Tt) {
     const LeanPixel lp = lean_pixel<NG>(pg, r0);
     for (int p0 = 0; p0 < N4; p0 += 4) {
         float4 r[kLeanArrays];
@@ -628,9 +638,13 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
             const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
             const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
             const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
-            lean_point<NG>(fmaf(fmaf(cu, v, bu), v, au), lane(r[2], k), lane(r[3], k),
-                           lane(r[4], k), lane(r[5], k), lp, C0, D0, C1, D1, Tt);
+            lean_point<NG, NM>(fmaf(fmaf(cu, v, bu), v, au), lane(r[2], k), lane(r[3], k),
+                               lane(r[4], k), lane(r[5], k), lp, C0, D0, C1, D1, Tt);
         }
+    }
+    if constexpr (NM < 2) {
+        C1 = wtot;
+        D1 = Tt;
     }
 }
 
@@ -719,14 +733,21 @@ rowweight_kernel(CubArgs a) {
         if constexpr (sizeof(T) == 4) {
             if (lean6) {
                 const bool lin = __all(!fast || lin_q_error(pg) < kLinQTol);
+                // every fast pixel's points within three rows (one boundary)
+                const bool one = __all(!fast || ihi - ilo <= 1);
                 if (a.tensor_nu > 0) {
-                    if (lin) lean_tensor_loop<1>(a, lr, pg, r0, S0, T0, S01, T01, Tt);
-                    else if (lean4) lean_tensor_loop<3>(a, lr, pg, r0, S0, T0, S01, T01, Tt);
-                    else lean_tensor_loop<4>(a, lr, pg, r0, S0, T0, S01, T01, Tt);
+                    if (lin && one)
+                        lean_tensor_loop<1, 1>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    else if (lin) lean_tensor_loop<1>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    else if (lean4)
+                        lean_tensor_loop<3>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    else lean_tensor_loop<4>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
                 } else {
-                    if (lin) lean_list_loop<1>(N4, lr, pg, r0, S0, T0, S01, T01, Tt);
-                    else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, S0, T0, S01, T01, Tt);
-                    else lean_list_loop<4>(N4, lr, pg, r0, S0, T0, S01, T01, Tt);
+                    if (lin && one)
+                        lean_list_loop<1, 1>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    else if (lin) lean_list_loop<1>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    else lean_list_loop<4>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
                 }
             }
         } else if (lean6) {
