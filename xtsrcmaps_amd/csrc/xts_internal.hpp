@@ -84,7 +84,8 @@ struct xts_const_tables {
 // The last cubature rule uploaded, with the quantities derived from it.
 struct xts_rule_cache {
     std::vector<double> host;    // [N, 3] as given
-    void* dev = nullptr;         // per-point records on the device
+    void* dev = nullptr;         // per-point records on the device, then FP32 arrays
+    size_t lean_off = 0;         // byte offset of the FP32 arrays in dev
     double rho_scale = 0.0, wsum = 0.0;
     int nu = 0, nv = 0;
 };

@@ -40,6 +40,46 @@ __global__ void geometry_kernel(Wcs w, int H, int W, double* __restrict__ corner
     }
 }
 
+// The source-independent part of a pixel's point positions, four float4
+// per pixel (pixel index h W + w): the corners' centroid m (FP64, 3 x 8
+// bytes), hr = max_k |corner_k - m| / 2, and B, C, E of the bilinear blend
+// p(u, v) / 2 = A + u B + v (C + u E) (points_weights.hpp:37-51) -- the
+// corner differences that make B, C, E and hr do not involve the source, so
+// per source only A = (m - src) / 2 remains.  Packed as
+// [m0 m1] [m2 hr B0] [B1 B2 C0 C1] [C2 E0 E1 E2]; FP64 arithmetic, FP32 store.
+__global__ void pixel_frame_kernel(int H, int W, const double* __restrict__ corners,
+                                   float4* __restrict__ frame) {
+    const int n = H * W;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int h = k / W, w = k % W, Wp1 = W + 1;
+        const int c00 = h * Wp1 + w;
+        const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};  // LL LR UR UL
+        double P[4][3];
+        for (int q = 0; q < 4; ++q)
+            for (int c = 0; c < 3; ++c) P[q][c] = corners[3 * cidx[q] + c];
+        double m[3];
+        float B[3], C[3], E[3];
+        for (int c = 0; c < 3; ++c) {
+            m[c] = 0.25 * ((P[0][c] + P[1][c]) + (P[2][c] + P[3][c]));
+            B[c] = (float)(0.125 * ((P[1][c] - P[0][c]) + (P[2][c] - P[3][c])));
+            C[c] = (float)(0.125 * ((P[3][c] - P[0][c]) + (P[2][c] - P[1][c])));
+            E[c] = (float)(0.125 * ((P[0][c] - P[1][c]) + (P[2][c] - P[3][c])));
+        }
+        double hr = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            const double ex = P[q][0] - m[0], ey = P[q][1] - m[1], ez = P[q][2] - m[2];
+            hr = fmax(hr, 0.5 * sqrt(ex * ex + ey * ey + ez * ez));
+        }
+        auto lo = [](double x) { return __int_as_float((int)(__double_as_longlong(x) & 0xffffffffLL)); };
+        auto hi = [](double x) { return __int_as_float((int)(__double_as_longlong(x) >> 32)); };
+        float4* f = frame + 4 * (size_t)k;
+        f[0] = float4{lo(m[0]), hi(m[0]), lo(m[1]), hi(m[1])};
+        f[1] = float4{lo(m[2]), hi(m[2]), (float)hr, B[0]};
+        f[2] = float4{B[1], B[2], C[0], C[1]};
+        f[3] = float4{C[2], E[0], E[1], E[2]};
+    }
+}
+
 __device__ inline double vnorm3(const double* v) {
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 }
@@ -81,43 +121,52 @@ struct SrcGeom {
     int pad;
 };
 
-__global__ void radius_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
-                              const double* __restrict__ corners, SrcGeom* __restrict__ geo) {
+// Block s < S: the boundary minimum of source s, taken on squared chords
+// (the angle 2 asin(chord / 2) is increasing in the chord, so the minimum
+// angle is the angle of the minimum chord); block S: the field-of-view flag
+// of every source, one thread per source.
+__global__ void __launch_bounds__(256)
+radius_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
+              const double* __restrict__ corners, SrcGeom* __restrict__ geo) {
     const int s = blockIdx.x;
-    if (s >= S) return;
-    __shared__ double red[256];
+    if (s == S) {
+        const double buf = 3.5;
+        for (int t = threadIdx.x; t < S; t += blockDim.x) {
+            double p1, p2;
+            wcs_sph2pix(w, src_sph[2 * t], src_sph[2 * t + 1], &p1, &p2);
+            geo[t].in_fov = p1 > buf && p1 < (H - buf) && p2 > buf && p2 < (W - buf);
+        }
+        return;
+    }
+    __shared__ double red[4];
     double src[3];
     sph2dir(src_sph[2 * s], src_sph[2 * s + 1], src);
     const int nb = 2 * (H + 1) + 2 * (W + 1);
-    double mn = 360.0;
+    double mn = 16.0;   // above any squared chord of unit vectors
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         int ci, cj;
         if (k < H + 1) { ci = k; cj = 0; }
         else if (k < 2 * (H + 1)) { ci = k - (H + 1); cj = W; }
         else if (k < 2 * (H + 1) + W + 1) { ci = 0; cj = k - 2 * (H + 1); }
         else { ci = H; cj = k - 2 * (H + 1) - (W + 1); }
-        const double d = kR2D * dir_diff(src, corners + 3 * (ci * (W + 1) + cj));
-        mn = d < mn ? d : mn;
+        const double* c = corners + 3 * (ci * (W + 1) + cj);
+        const double x = src[0] - c[0], y = src[1] - c[1], z = src[2] - c[2];
+        mn = fmin(mn, x * x + y * y + z * z);
     }
-    red[threadIdx.x] = mn;
+    for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o));
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mn;
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        double p1, p2;
-        wcs_sph2pix(w, src_sph[2 * s], src_sph[2 * s + 1], &p1, &p2);
-        const double buf = 3.5;
-        SrcGeom g;
+        for (int q = 1; q < (int)blockDim.x / 64; ++q) mn = fmin(mn, red[q]);
+        // dir_diff of the nearest boundary point (sky_geom.hxx:305-310)
+        const double rad = kR2D * (2.0 * asin(0.5 * sqrt(mn)));
+        SrcGeom& g = geo[s];
         g.dir[0] = src[0]; g.dir[1] = src[1]; g.dir[2] = src[2];
-        g.radius_deg = red[0];
-        g.in_fov = p1 > buf && p1 < (H - buf) && p2 > buf && p2 < (W - buf);
-        const double c = 2.0 * sin(0.5 * kD2R * red[0]);
+        g.radius_deg = rad;
+        const double c = 2.0 * sin(0.5 * kD2R * rad);
         g.chord_lo = c * (1.0 - 1e-12);
         g.chord_hi = c * (1.0 + 1e-12);
         g.pad = 0;
-        geo[s] = g;
     }
 }
 
@@ -150,11 +199,13 @@ struct RuleUVW {
 
 struct CubArgs {
     const double* corners;
+    const float4* frame;       // [H W][4] pixel frames (pixel_frame_kernel)
     const double* centres;
     const double* omega;       // may be null (raw cubature)
     const SrcGeom* geo;
     const void* lut;           // [S][401][Ep] of T
     const RuleUVW* uvw;
+    const float* lean;         // [6][N4] FP32 rule arrays u, v, w/4, w/4 t1, t2, t3
     const float* inv_sep;      // [401] 2 R2D / sep[i] (sep in deg), FP32 table
     const double* inv_sep64;   // [401] FP64 table
     void* rw;                  // [S][H W] row weights (4 of T) x solid angle
@@ -331,6 +382,32 @@ __device__ inline PixelGeom<T> pixel_geom(const double* __restrict__ corners, in
     g.hr = hr;
     g.slo = T(2.0 * kR2D) * asin_pos(fmax(x0 - hr, T(0)));
     g.shi = T(2.0 * kR2D) * asin_pos(fmin(x0 + hr, T(1)));
+    return g;
+}
+
+// pixel_geom from the pixel's frame: the same quantities, with A formed from
+// the FP64 centroid and the rest read.
+__device__ inline PixelGeom<float> pixel_geom_frame(const float4* __restrict__ frame, int pix,
+                                                    const double* src, double rho_scale) {
+    const float4* f = frame + 4 * (size_t)pix;
+    const float4 q0 = f[0], q1 = f[1], q2 = f[2], q3 = f[3];
+    auto dbl = [](float lo, float hi) {
+        return __longlong_as_double(((long long)__float_as_int(hi) << 32) |
+                                    (unsigned int)__float_as_int(lo));
+    };
+    const double m[3] = {dbl(q0.x, q0.y), dbl(q0.z, q0.w), dbl(q1.x, q1.y)};
+    PixelGeom<float> g;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g.A[c] = (float)(0.5 * (m[c] - src[c]));
+    g.B[0] = q1.w; g.B[1] = q2.x; g.B[2] = q2.y;
+    g.C[0] = q2.z; g.C[1] = q2.w; g.C[2] = q3.x;
+    g.E[0] = q3.y; g.E[1] = q3.z; g.E[2] = q3.w;
+    const float x0 = fast_sqrt(g.A[0] * g.A[0] + g.A[1] * g.A[1] + g.A[2] * g.A[2]);
+    const float hr = q1.z * (float)rho_scale * 1.0001f + 1e-5f * x0;
+    g.x0 = x0;
+    g.hr = hr;
+    g.slo = float(2.0 * kR2D) * asin_pos(fmax(x0 - hr, 0.0f));
+    g.shi = float(2.0 * kR2D) * asin_pos(fmin(x0 + hr, 1.0f));
     return g;
 }
 
@@ -576,16 +653,21 @@ __device__ inline void lean_point(float z, float wq, float tA, float tB, float t
     Tt += wt;
 }
 
-// FP32 rule in LDS as arrays [N4] of u, v, w/4 and w/4 (t1, t2, t3), so that
-// a group of four points takes one 16-byte read per array.
+// FP32 rule as arrays [N4] of u, v, w/4 and w/4 (t1, t2, t3) in the
+// constant address space: every lane of a wave reads the same point, so the
+// reads are scalar loads (16 bytes per array for a group of four points)
+// and the rule needs no copy in LDS.
+using cfloat = const __attribute__((address_space(4))) float;
 struct LeanRule {
-    const float* p[6];   // u, v, wq, tA, tB, tC
+    cfloat* p[6];   // u, v, wq, tA, tB, tC
 };
 constexpr int kLeanArrays = 6;
 
+__device__ inline float4 lean_load4(cfloat* q) { return float4{q[0], q[1], q[2], q[3]}; }
+
 __device__ inline void lean_group(const LeanRule& lr, int p0, float4 (&r)[kLeanArrays]) {
 #pragma unroll
-    for (int j = 0; j < kLeanArrays; ++j) r[j] = *reinterpret_cast<const float4*>(lr.p[j] + p0);
+    for (int j = 0; j < kLeanArrays; ++j) r[j] = lean_load4(lr.p[j] + p0);
 }
 
 __device__ inline float lane(const float4& x, int k) {
@@ -608,8 +690,7 @@ __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
             const int p0 = i * nv + jj;
             float4 r[kLeanArrays - 1];
 #pragma unroll
-            for (int j = 1; j < kLeanArrays; ++j)
-                r[j - 1] = *reinterpret_cast<const float4*>(lr.p[j] + p0);
+            for (int j = 1; j < kLeanArrays; ++j) r[j - 1] = lean_load4(lr.p[j] + p0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const float v = lane(r[0], k);
@@ -649,9 +730,9 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
 }
 
 // Pass 1.  Grid: (tile blocks, S); LDS: the in-radius row histogram
-// [kNDPad] of double, 2 R2D / sep [kNDPad] of T, the rule (u, v, w/4,
-// redm1 w/4) and, for FP32, the arrays of LeanRule, both padded to a
-// multiple of 4 points.
+// [kNDPad] of double and, for FP64, 2 R2D / sep [kNDPad] and the rule (u,
+// v, w/4, redm1 w/4) padded to a multiple of 4 points.  FP32 reads its rule
+// arrays (LeanRule) and 2 R2D / sep from global memory.
 //
 // Every point of a pixel has its separation in [slo, shi], so the rows the
 // pixel touches are known before the point loop.  A pixel whose points touch
@@ -660,9 +741,10 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
 // Other pixels are listed for near_rows_kernel.  When every such pixel of a
 // wave lies below 28 degrees, the wave runs a branch-free FP32 loop over the
 // points in groups of four (grid-structured rules reuse the blend of each
-// rule row).
+// rule row).  Occupancy: 6 waves per SIMD for FP32 (80 VGPRs), 5 for FP64.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 5)))
 rowweight_kernel(CubArgs a) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -670,11 +752,9 @@ rowweight_kernel(CubArgs a) {
     T* isep_s = reinterpret_cast<T*>(hist_s + kNDPad);
     V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
     const int N4 = (a.N + 3) & ~3;
-    // FP32: the rule again as kLeanArrays arrays [N4]
-    float* lean_s = reinterpret_cast<float*>(rule_s + N4);
     LeanRule lr;
 #pragma unroll
-    for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = lean_s + j * N4;
+    for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = (cfloat*)(a.lean + j * N4);
 
     const int s = blockIdx.y;
     const int tid = threadIdx.x;
@@ -683,18 +763,7 @@ rowweight_kernel(CubArgs a) {
     const int t_end = min(ntiles, t_begin + a.tiles_per_block);
     if (t_begin >= ntiles) return;
 
-    stage_rule<T>(a, isep_s, rule_s, N4, tid, kThreads);
-    if constexpr (sizeof(T) == 4) {
-        for (int p = tid; p < N4; p += kThreads) {
-            const bool in = p < a.N;
-            const float wq = in ? (float)a.uvw[p].wq : 0.0f;
-            lean_s[p] = in ? (float)a.uvw[p].u : 0.0f;
-            lean_s[N4 + p] = in ? (float)a.uvw[p].v : 0.0f;
-            lean_s[2 * N4 + p] = wq;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) lean_s[(3 + k) * N4 + p] = wq * (float)kTPoly[k];
-        }
-    }
+    if constexpr (sizeof(T) == 8) stage_rule<T>(a, isep_s, rule_s, N4, tid, kThreads);
     for (int k = tid; k < kND; k += kThreads) hist_s[k] = 0.0;
     __syncthreads();
 
@@ -715,7 +784,9 @@ rowweight_kernel(CubArgs a) {
         // out-of-image lanes compute an in-image pixel and store nothing, so
         // the point loops run in uniform control flow
         const int hc = min(h, a.H - 1), wc = min(w, a.W - 1);
-        const PixelGeom<T> pg = pixel_geom<T>(a.corners, hc, wc, Wp1, g.dir, a.rho_scale);
+        PixelGeom<T> pg;
+        if constexpr (sizeof(T) == 4) pg = pixel_geom_frame(a.frame, hc * a.W + wc, g.dir, a.rho_scale);
+        else pg = pixel_geom<T>(a.corners, hc, wc, Wp1, g.dir, a.rho_scale);
         const int ilo = sep_index_fast(pg.slo), ihi = sep_index_fast(pg.shi);
         // rows touched: ilo-1 .. ihi; inside the table with t in [0, 1]
         const bool fast = (ihi - ilo) <= 2 && pg.slo >= T(1e-4) && pg.shi < T(69.9);
@@ -772,8 +843,17 @@ rowweight_kernel(CubArgs a) {
         if (!lean6 && __any(fast)) {
             // ---- far pixels beyond 28 degrees or sharing a wave with near ones
             for (int p = 0; p < a.N; ++p) {
-                const V4 r = rule_s[p];
-                const T u = r.x, v = r.y, wq = r.z;
+                T u, v, wq;
+                if constexpr (sizeof(T) == 4) {
+                    u = lr.p[0][p];
+                    v = lr.p[1][p];
+                    wq = lr.p[2][p];
+                } else {
+                    const V4 r = rule_s[p];
+                    u = r.x;
+                    v = r.y;
+                    wq = r.z;
+                }
                 const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
                 const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
                 const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
@@ -781,7 +861,8 @@ rowweight_kernel(CubArgs a) {
                 const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
                 const int i = sep_index_fast(sd);
                 // t = (s - sep[i-1]) / (sep[i] - sep[i-1]) = redm1 (s / sep[i-1] - 1)
-                const T tt = T(kRedm1) * (sd * (isep_s[i - 1] * T(0.5 / kR2D)) - T(1));
+                const T is = sizeof(T) == 4 ? (T)a.inv_sep[i - 1] : isep_s[i - 1];
+                const T tt = T(kRedm1) * (sd * (is * T(0.5 / kR2D)) - T(1));
                 const int rel = min(max(i - 1 - r0, 0), 2);
                 const T wt = wq * tt;
                 S0 += rel == 0 ? wq : T(0);
@@ -1503,6 +1584,7 @@ __global__ void pix2sph_kernel(Wcs w, int64_t n, const double* __restrict__ in,
 // ------------------------------------------------------------------ host side
 struct ImageGeom {
     double* corners = nullptr;
+    float4* frame = nullptr;
     double* centres = nullptr;
     double* omega = nullptr;
     SrcGeom* geo = nullptr;
@@ -1523,13 +1605,15 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
                           ImageGeom* ig) {
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     const size_t ncorn = (size_t)(H + 1) * (W + 1);
-    const size_t bytes = ncorn * 3 * 8 * 2 + (size_t)H * W * 8;
+    const size_t head = ((ncorn * 3 * 8 * 2 + (size_t)H * W * 8) + 255) & ~size_t(255);
+    const size_t bytes = head + (size_t)H * W * 64;
     void* base;
     XTS_TRY(slot_reserve(ctx, kSlotGeom, bytes, &base));
     char* p = (char*)base;
     ig->corners = (double*)p; p += ncorn * 3 * 8;
     ig->centres = (double*)p; p += ncorn * 3 * 8;
     ig->omega = (double*)p;
+    ig->frame = (float4*)((char*)base + head);
     void* g;
     XTS_TRY(slot_reserve(ctx, kSlotSrcGeo, (size_t)std::max(S, 1) * sizeof(SrcGeom), &g));
     ig->geo = (SrcGeom*)g;
@@ -1541,11 +1625,13 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
             w, H, W, ig->corners, ig->centres);
         solid_angle_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
                              ctx->stream>>>(H, W, ig->centres, ig->omega);
+        pixel_frame_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
+                             ctx->stream>>>(H, W, ig->corners, ig->frame);
         ctx->geom_wcs = *wcs;
         ctx->geom_valid = true;
     }
     if (S > 0 && src_sph_dev)
-        radius_kernel<<<S, 256, 0, ctx->stream>>>(w, H, W, S, src_sph_dev, ig->corners, ig->geo);
+        radius_kernel<<<S + 1, 256, 0, ctx->stream>>>(w, H, W, S, src_sph_dev, ig->corners, ig->geo);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -1573,7 +1659,7 @@ __global__ void lut_convert_kernel(const double* __restrict__ upsf, int S, int E
 // bound, weight sum and grid structure; re-uploaded only when its content
 // changes.
 static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** dev,
-                       double* rho_scale, double* wsum, int* tnu, int* tnv) {
+                       const float** lean, double* rho_scale, double* wsum, int* tnu, int* tnv) {
     const double* r = rule ? rule : kSquareDeg7;
     if (!rule) N = 12;
     XTS_CHECK(N > 0 && N <= 1024, "rule size %lld out of range (1..1024)", (long long)N);
@@ -1605,16 +1691,32 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
         } else {
             nv = 0;
         }
+        // the records, then the FP32 arrays of the lean point loop, formed
+        // in FP32 as the kernels use them, zero past the last point
+        const int64_t N4 = (N + 3) / 4 * 4;
+        const size_t rec = ((size_t)N * sizeof(RuleUVW) + 255) & ~size_t(255);
+        std::vector<unsigned char> blob(rec + (size_t)kLeanArrays * N4 * sizeof(float), 0);
+        std::memcpy(blob.data(), rp.data(), (size_t)N * sizeof(RuleUVW));
+        float* la = reinterpret_cast<float*>(blob.data() + rec);
+        for (int64_t p = 0; p < N; ++p) {
+            const float wq = (float)rp[p].wq;
+            la[p] = (float)rp[p].u;
+            la[N4 + p] = (float)rp[p].v;
+            la[2 * N4 + p] = wq;
+            for (int k = 0; k < 3; ++k) la[(3 + k) * N4 + p] = wq * (float)kTPoly[k];
+        }
         void* d;
-        XTS_TRY(to_device(ctx, kSlotRule, rp.data(), rp.size() * sizeof(RuleUVW), &d));
+        XTS_TRY(to_device(ctx, kSlotRule, blob.data(), blob.size(), &d));
         rc.host.assign(r, r + 3 * N);
         rc.dev = d;
+        rc.lean_off = rec;
         rc.rho_scale = rs;
         rc.wsum = ws;
         rc.nu = nu;
         rc.nv = nv;
     }
     *dev = (RuleUVW*)rc.dev;
+    *lean = reinterpret_cast<const float*>((const char*)rc.dev + rc.lean_off);
     *rho_scale = rc.rho_scale;
     *wsum = rc.wsum;
     *tnu = rc.nu;
@@ -1647,9 +1749,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const xts_const_tables* ct;
     XTS_TRY(const_tables(ctx, &ct));
     RuleUVW* rp;
+    const float* lean;
     double rho_scale, wsum;
     int tnu, tnv;
-    XTS_TRY(upload_rule(ctx, rule, N, &rp, &rho_scale, &wsum, &tnu, &tnv));
+    XTS_TRY(upload_rule(ctx, rule, N, &rp, &lean, &rho_scale, &wsum, &tnu, &tnv));
     if (!rule) N = 12;
 
     // PSF table in the cubature precision
@@ -1685,11 +1788,13 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
 
     CubArgs ca{};
     ca.corners = ig.corners;
+    ca.frame = ig.frame;
     ca.centres = ig.centres;
     ca.omega = full ? ig.omega : nullptr;
     ca.geo = ig.geo;
     ca.lut = lut;
     ca.uvw = rp;
+    ca.lean = lean;
     ca.inv_sep = ct->inv32;
     ca.inv_sep64 = ct->inv64;
     ca.rw = rows;
@@ -1729,8 +1834,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const int N4 = (int)((N + 3) / 4 * 4);
     const size_t tsz = f64 ? 8 : 4;
     {
-        const size_t sm = (size_t)kNDPad * 8 + (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz +
-                          (size_t)N4 * 4 * kLeanArrays;
+        const size_t sm = (size_t)kNDPad * 8 +
+                          (f64 ? (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz : 0);
         const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
         if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
