@@ -40,46 +40,6 @@ __global__ void geometry_kernel(Wcs w, int H, int W, double* __restrict__ corner
     }
 }
 
-// The source-independent part of a pixel's point positions, four float4
-// per pixel (pixel index h W + w): the corners' centroid m (FP64, 3 x 8
-// bytes), hr = max_k |corner_k - m| / 2, and B, C, E of the bilinear blend
-// p(u, v) / 2 = A + u B + v (C + u E) (points_weights.hpp:37-51) -- the
-// corner differences that make B, C, E and hr do not involve the source, so
-// per source only A = (m - src) / 2 remains.  Packed as
-// [m0 m1] [m2 hr B0] [B1 B2 C0 C1] [C2 E0 E1 E2]; FP64 arithmetic, FP32 store.
-__global__ void pixel_frame_kernel(int H, int W, const double* __restrict__ corners,
-                                   float4* __restrict__ frame) {
-    const int n = H * W;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int h = k / W, w = k % W, Wp1 = W + 1;
-        const int c00 = h * Wp1 + w;
-        const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};  // LL LR UR UL
-        double P[4][3];
-        for (int q = 0; q < 4; ++q)
-            for (int c = 0; c < 3; ++c) P[q][c] = corners[3 * cidx[q] + c];
-        double m[3];
-        float B[3], C[3], E[3];
-        for (int c = 0; c < 3; ++c) {
-            m[c] = 0.25 * ((P[0][c] + P[1][c]) + (P[2][c] + P[3][c]));
-            B[c] = (float)(0.125 * ((P[1][c] - P[0][c]) + (P[2][c] - P[3][c])));
-            C[c] = (float)(0.125 * ((P[3][c] - P[0][c]) + (P[2][c] - P[1][c])));
-            E[c] = (float)(0.125 * ((P[0][c] - P[1][c]) + (P[2][c] - P[3][c])));
-        }
-        double hr = 0.0;
-        for (int q = 0; q < 4; ++q) {
-            const double ex = P[q][0] - m[0], ey = P[q][1] - m[1], ez = P[q][2] - m[2];
-            hr = fmax(hr, 0.5 * sqrt(ex * ex + ey * ey + ez * ez));
-        }
-        auto lo = [](double x) { return __int_as_float((int)(__double_as_longlong(x) & 0xffffffffLL)); };
-        auto hi = [](double x) { return __int_as_float((int)(__double_as_longlong(x) >> 32)); };
-        float4* f = frame + 4 * (size_t)k;
-        f[0] = float4{lo(m[0]), hi(m[0]), lo(m[1]), hi(m[1])};
-        f[1] = float4{lo(m[2]), hi(m[2]), (float)hr, B[0]};
-        f[2] = float4{B[1], B[2], C[0], C[1]};
-        f[3] = float4{C[2], E[0], E[1], E[2]};
-    }
-}
-
 __device__ inline double vnorm3(const double* v) {
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 }
@@ -203,7 +163,6 @@ constexpr int kRowListed = -1, kRowWide = -2;
 
 struct CubArgs {
     const double* corners;
-    const float4* frame;       // [H W][4] pixel frames (pixel_frame_kernel)
     const double* centres;
     const double* omega;       // may be null (raw cubature)
     const SrcGeom* geo;
@@ -386,32 +345,6 @@ __device__ inline PixelGeom<T> pixel_geom(const double* __restrict__ corners, in
     g.hr = hr;
     g.slo = T(2.0 * kR2D) * asin_pos(fmax(x0 - hr, T(0)));
     g.shi = T(2.0 * kR2D) * asin_pos(fmin(x0 + hr, T(1)));
-    return g;
-}
-
-// pixel_geom from the pixel's frame: the same quantities, with A formed from
-// the FP64 centroid and the rest read.
-__device__ inline PixelGeom<float> pixel_geom_frame(const float4* __restrict__ frame, int pix,
-                                                    const double* src, double rho_scale) {
-    const float4* f = frame + 4 * (size_t)pix;
-    const float4 q0 = f[0], q1 = f[1], q2 = f[2], q3 = f[3];
-    auto dbl = [](float lo, float hi) {
-        return __longlong_as_double(((long long)__float_as_int(hi) << 32) |
-                                    (unsigned int)__float_as_int(lo));
-    };
-    const double m[3] = {dbl(q0.x, q0.y), dbl(q0.z, q0.w), dbl(q1.x, q1.y)};
-    PixelGeom<float> g;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) g.A[c] = (float)(0.5 * (m[c] - src[c]));
-    g.B[0] = q1.w; g.B[1] = q2.x; g.B[2] = q2.y;
-    g.C[0] = q2.z; g.C[1] = q2.w; g.C[2] = q3.x;
-    g.E[0] = q3.y; g.E[1] = q3.z; g.E[2] = q3.w;
-    const float x0 = fast_sqrt(g.A[0] * g.A[0] + g.A[1] * g.A[1] + g.A[2] * g.A[2]);
-    const float hr = q1.z * (float)rho_scale * 1.0001f + 1e-5f * x0;
-    g.x0 = x0;
-    g.hr = hr;
-    g.slo = float(2.0 * kR2D) * asin_pos(fmax(x0 - hr, 0.0f));
-    g.shi = float(2.0 * kR2D) * asin_pos(fmin(x0 + hr, 1.0f));
     return g;
 }
 
@@ -788,9 +721,7 @@ rowweight_kernel(CubArgs a) {
         // out-of-image lanes compute an in-image pixel and store nothing, so
         // the point loops run in uniform control flow
         const int hc = min(h, a.H - 1), wc = min(w, a.W - 1);
-        PixelGeom<T> pg;
-        if constexpr (sizeof(T) == 4) pg = pixel_geom_frame(a.frame, hc * a.W + wc, g.dir, a.rho_scale);
-        else pg = pixel_geom<T>(a.corners, hc, wc, Wp1, g.dir, a.rho_scale);
+        const PixelGeom<T> pg = pixel_geom<T>(a.corners, hc, wc, Wp1, g.dir, a.rho_scale);
         const int ilo = sep_index_fast(pg.slo), ihi = sep_index_fast(pg.shi);
         // rows touched: ilo-1 .. ihi; inside the table with t in [0, 1]
         const bool fast = (ihi - ilo) <= 2 && pg.slo >= T(1e-4) && pg.shi < T(69.9);
@@ -1598,7 +1529,6 @@ __global__ void pix2sph_kernel(Wcs w, int64_t n, const double* __restrict__ in,
 // ------------------------------------------------------------------ host side
 struct ImageGeom {
     double* corners = nullptr;
-    float4* frame = nullptr;
     double* centres = nullptr;
     double* omega = nullptr;
     SrcGeom* geo = nullptr;
@@ -1619,15 +1549,13 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
                           ImageGeom* ig) {
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     const size_t ncorn = (size_t)(H + 1) * (W + 1);
-    const size_t head = ((ncorn * 3 * 8 * 2 + (size_t)H * W * 8) + 255) & ~size_t(255);
-    const size_t bytes = head + (size_t)H * W * 64;
+    const size_t bytes = ncorn * 3 * 8 * 2 + (size_t)H * W * 8;
     void* base;
     XTS_TRY(slot_reserve(ctx, kSlotGeom, bytes, &base));
     char* p = (char*)base;
     ig->corners = (double*)p; p += ncorn * 3 * 8;
     ig->centres = (double*)p; p += ncorn * 3 * 8;
     ig->omega = (double*)p;
-    ig->frame = (float4*)((char*)base + head);
     void* g;
     XTS_TRY(slot_reserve(ctx, kSlotSrcGeo, (size_t)std::max(S, 1) * sizeof(SrcGeom), &g));
     ig->geo = (SrcGeom*)g;
@@ -1639,8 +1567,6 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
             w, H, W, ig->corners, ig->centres);
         solid_angle_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
                              ctx->stream>>>(H, W, ig->centres, ig->omega);
-        pixel_frame_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
-                             ctx->stream>>>(H, W, ig->corners, ig->frame);
         ctx->geom_wcs = *wcs;
         ctx->geom_valid = true;
     }
@@ -1802,7 +1728,6 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
 
     CubArgs ca{};
     ca.corners = ig.corners;
-    ca.frame = ig.frame;
     ca.centres = ig.centres;
     ca.omega = full ? ig.omega : nullptr;
     ca.geo = ig.geo;
