@@ -207,7 +207,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg.name),
                      "kernel": "model-map passes (rowweight_kernel, near_rows_kernel, "
                                "hist_integral_kernel, emit_sehw_kernel, "
-                               "near_emit_kernel + wide_pixel_kernel); algorithmic bytes = "
+                               "near_emit_kernel); algorithmic bytes = "
                                "4 B per FP32 map value; time = their summed device time",
                      "stage_ms_per_step": stage_ms},
         "valu": {"kernel": "rowweight_kernel<float>",

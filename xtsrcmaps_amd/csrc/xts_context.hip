@@ -157,12 +157,6 @@ int xts_ctx_destroy(xts_ctx* ctx) {
         hipEventDestroy(r.stop);
     }
     for (auto e : ctx->event_pool) hipEventDestroy(e);
-    if (ctx->aux_stream) {
-        hipStreamSynchronize(ctx->aux_stream);
-        hipStreamDestroy(ctx->aux_stream);
-    }
-    for (auto e : ctx->aux_events)
-        if (e) hipEventDestroy(e);
     for (auto& s : ctx->slots)
         if (s.ptr) hipFree(s.ptr);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);

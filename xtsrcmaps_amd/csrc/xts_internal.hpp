@@ -98,8 +98,6 @@ struct xts_ctx {
     int cu_count = 0;
     int timing = 0;                        // record events around kernels
     int tune_tiles_per_block = 0;          // 0: chosen per call (row-weight pass)
-    hipStream_t aux_stream = nullptr;      // second stream (default priority) and
-    hipEvent_t aux_events[2] = {nullptr, nullptr};   // its fork / join events
     std::vector<xts_timing_rec> timing_recs;
     std::vector<hipEvent_t> event_pool;
     bool consts_ready = false;

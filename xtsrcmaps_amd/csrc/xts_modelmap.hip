@@ -12,9 +12,10 @@
 //   5. hist_integral_kernel
 //                        map integral, correction factor per (source, energy)
 //   6. emit_*_kernel     every map value once: row weights . table rows
-//                        x exposure x correction
-//   7. near_emit_kernel  the same for the near pixels (wide_pixel_kernel for
-//                        the few whose points touch more than 24 rows)
+//                        x exposure x correction (the pooled near pixels as
+//                        zeros); extra blocks evaluate the few pixels whose
+//                        points touch more than 24 rows (wide_pixels)
+//   7. near_emit_kernel  the pooled near pixels' values, over the zeros
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -158,7 +159,7 @@ struct RuleUVW {
 };
 
 // rw_row of a listed pixel: kRowListed (set by the row-weight pass), turned
-// into kRowWide by the near pass when the pixel is left to wide_pixel_kernel.
+// into kRowWide by the near pass when the pixel is left to wide_pixels.
 constexpr int kRowListed = -1, kRowWide = -2;
 
 struct CubArgs {
@@ -187,6 +188,7 @@ struct CubArgs {
     void* out;
     int S, H, W, E, Ep, N;
     int tiles_w, tiles_h, tiles_per_block;
+    int emit_gx;               // emit blocks per source (x); the next kWideBlocks: wide pixels
     int layout_sehw, out_f32;
     int tensor_nu, tensor_nv;  // rule is an nu x nv grid (p = i nv + j), else 0
     double rho_scale;          // max over points of sum_k |b_k|
@@ -848,7 +850,7 @@ rowweight_kernel(CubArgs a) {
 // and fraction come from the relative log of lean_point.  A pixel touching
 // more rows (the one holding the source), or finding the pool full, is
 // "wide" (pool offset -1): only its in-radius weights are added here, point
-// by point, and wide_pixel_kernel evaluates it.  Grid: (blocks striding over
+// by point, and the emit's wide blocks evaluate it.  Grid: (blocks striding over
 // the list, S).
 constexpr int kNearRows = 24;
 constexpr int kNearSplit = 2;   // 31.8 us against 34.7 with 4 (c1)
@@ -986,24 +988,22 @@ near_rows_kernel(CubArgs a) {
     }
 }
 
-// Wide listed pixels (pool offset -1): one wave per pixel.  Every wave scans
-// the source's list; the k-th wide entry goes to wave k mod (waves per
-// source).  The wave's lanes take the rule points, adding each point's two
-// row weights to a 401-row LDS histogram; then each lane takes every 64th
-// table row, loads its EC energies as vectors and accumulates W_r L[r], and
-// the wave sums the 64 partial vectors.  x solid angle, x exposure x
-// correction when SCALED.  Grid: (kWideBlocks, S, energy chunks).
-constexpr int kWideBlocks = 4;
+// Wide listed pixels (pool offset -1): one wave per pixel, run by extra
+// blocks of the emit kernel (blockIdx.y = source, blockIdx.z = energy
+// chunk).  Every wave scans the source's list; the k-th wide entry goes to
+// wave gw = k mod nwaves.  The wave's lanes take the rule points, adding
+// each point's two row weights to a 401-row LDS histogram H (the wave's
+// own); then each lane takes every 64th table row, loads its EC energies as
+// vectors and accumulates W_r L[r], and the wave sums the 64 partial
+// vectors.  x solid angle, x exposure x correction when SCALED.
+constexpr int kWideBlocks = 2;   // extra emit blocks per source and energy chunk
 template <typename T, int EC, typename OT, bool SCALED>
-__global__ void __launch_bounds__(256)
-wide_pixel_kernel(CubArgs a) {
+__device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
     using V = typename Vec<T>::type;
     constexpr int VN = Vec<T>::n;
     constexpr int NV = EC / VN;
-    __shared__ T hw[4][kNDPad];
     const int s = blockIdx.y;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int gw = blockIdx.x * 4 + wave, nwaves = gridDim.x * 4;
+    const int lane = threadIdx.x % 64;
     const int count = a.near_count[s];
     const int e0 = blockIdx.z * EC;
     const int ne = min(EC, a.E - e0);
@@ -1012,7 +1012,6 @@ wide_pixel_kernel(CubArgs a) {
     const int* list = a.near_list + (size_t)s * plane;
     const int* meta_off = a.near_off + (size_t)s * plane;
     const T* lut_s0 = reinterpret_cast<const T*>(a.lut) + (size_t)s * kND * a.Ep + e0;
-    T* H = hw[wave];
     int rank = 0;   // wide entries seen before this chunk
     constexpr int kPre = 8;   // chunks whose entries are loaded together
     int pre[kPre];
@@ -1136,7 +1135,7 @@ near_emit_kernel(CubArgs a) {
         V acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = V{};
-        if (off < 0) continue;            // wide: wide_pixel_kernel
+        if (off < 0) continue;            // wide: the emit's wide blocks
         {
             const int rr = meta_rr[ic];
             const int rbase = rr >> 8, R = rr & 255;
@@ -1194,6 +1193,12 @@ emit_sehw_kernel(CubArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     __shared__ ChunkScale<EC> sc;
+    if ((int)blockIdx.x >= a.emit_gx) {
+        constexpr int kWaves = kEmitThreads / 64;
+        wide_pixels<T, EC, OT, SCALED>(a, (blockIdx.x - a.emit_gx) * kWaves + threadIdx.x / 64,
+                                       kWideBlocks * kWaves, lut_s + (threadIdx.x / 64) * kNDPad);
+        return;
+    }
 
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
@@ -1221,8 +1226,8 @@ emit_sehw_kernel(CubArgs a) {
     // Pixels of the near pool are written too, as zeros, and overwritten by
     // near_emit_kernel (later on the same stream): whole 128-byte lines
     // instead of lines with holes around each source, 145 -> 119 us for
-    // 100 sources x 37 planes of 200 x 200.  Wide pixels are left to
-    // wide_pixel_kernel, which may run beside this kernel.
+    // 100 sources x 37 planes of 200 x 200.  Wide pixels are left to the
+    // wide blocks of this kernel.
 #pragma unroll
     for (int k = 0; k < kEmitPix; ++k) {
         out_ok[k] = pix[k] < plane && r0[k] != kRowWide;
@@ -1268,6 +1273,12 @@ emit_shwe_kernel(CubArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     __shared__ ChunkScale<EC> sc;
+    if ((int)blockIdx.x >= a.emit_gx) {
+        constexpr int kWaves = kEmitThreads / 64;
+        wide_pixels<T, EC, OT, SCALED>(a, (blockIdx.x - a.emit_gx) * kWaves + threadIdx.x / 64,
+                                       kWideBlocks * kWaves, lut_s + (threadIdx.x / 64) * kNDPad);
+        return;
+    }
 
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
@@ -1349,35 +1360,22 @@ static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) 
 // Pooled near pixels (wide = false) or wide pixels (wide = true: a few
 // waves per source scan the list).
 template <typename T, int EC, typename OT, bool SCALED>
-static int launch_near_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool wide) {
-    if (wide) {
-        const dim3 wgrid(kWideBlocks, grid.y, grid.z);
-        wide_pixel_kernel<T, EC, OT, SCALED><<<wgrid, 256, 0, st>>>(ca);
-    } else {
-        const size_t sm = (size_t)kND * EC * sizeof(T);
-        XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED>, sm));
-        near_emit_kernel<T, EC, OT, SCALED><<<grid, kNearThreads, sm, st>>>(ca);
-    }
+static int launch_near_emit(hipStream_t st, const CubArgs& ca, dim3 grid) {
+    const size_t sm = (size_t)kND * EC * sizeof(T);
+    XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED>, sm));
+    near_emit_kernel<T, EC, OT, SCALED><<<grid, kNearThreads, sm, st>>>(ca);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
 
 template <typename T, int EC>
-static int near_emit(hipStream_t st, const CubArgs& ca, int64_t gx, bool scaled, bool wide) {
+static int near_emit(hipStream_t st, const CubArgs& ca, int64_t gx, bool scaled) {
     const dim3 grid((unsigned)gx, (unsigned)ca.S, (unsigned)div_up(ca.E, EC));
     if (ca.out_f32)
-        return scaled ? launch_near_emit<T, EC, float, true>(st, ca, grid, wide)
-                      : launch_near_emit<T, EC, float, false>(st, ca, grid, wide);
-    return scaled ? launch_near_emit<T, EC, double, true>(st, ca, grid, wide)
-                  : launch_near_emit<T, EC, double, false>(st, ca, grid, wide);
-}
-
-// The context's second stream and its two events (created on first use).
-static int aux_stream(xts_ctx* ctx) {
-    if (!ctx->aux_stream) XTS_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
-    for (auto& e : ctx->aux_events)
-        if (!e) XTS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    return XTS_OK;
+        return scaled ? launch_near_emit<T, EC, float, true>(st, ca, grid)
+                      : launch_near_emit<T, EC, float, false>(st, ca, grid);
+    return scaled ? launch_near_emit<T, EC, double, true>(st, ca, grid)
+                  : launch_near_emit<T, EC, double, false>(st, ca, grid);
 }
 
 template <typename T, int EC>
@@ -1804,27 +1802,15 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
             (int)E, ca.hist, upsf, ca.mapint, ig.geo, parint, ct->sep, fac);
         XTS_HIP(hipGetLastError());
     }
-    // Wide pixels depend only on the factors and the near lists, and write
-    // values no other pass writes: they run on the second stream while the
-    // far pixels are emitted (on the context stream when stage timing is
-    // on, whose events are on the context stream).
-    hipStream_t wide_st = ctx->stream;
-    if (!ctx->timing) {
-        XTS_TRY(aux_stream(ctx));
-        wide_st = ctx->aux_stream;
-        XTS_HIP(hipEventRecord(ctx->aux_events[0], ctx->stream));
-        XTS_HIP(hipStreamWaitEvent(wide_st, ctx->aux_events[0], 0));
-    }
-    if (wide_st != ctx->stream) {
-        if (f64) XTS_TRY((near_emit<double, 16>(wide_st, ca, near_gx, full, true)));
-        else XTS_TRY((near_emit<float, 40>(wide_st, ca, near_gx, full, true)));
-        XTS_HIP(hipEventRecord(ctx->aux_events[1], wide_st));
-    }
-    // pass 3: blocks of kEmitBlockPix pixels of one source
+    // pass 3: blocks of kEmitBlockPix pixels of one source, then kWideBlocks
+    // blocks per source for the wide pixels (they depend only on the factors
+    // and the near lists and write values no other pass writes, so they run
+    // beside the far pixels' blocks)
     {
-        const dim3 grid32((unsigned)div_up(plane, kEmitBlockPix), (unsigned)S,
+        ca.emit_gx = (int)div_up(plane, kEmitBlockPix);
+        const dim3 grid32((unsigned)(ca.emit_gx + kWideBlocks), (unsigned)S,
                           (unsigned)div_up(E, kEmitEC32));
-        const dim3 grid64((unsigned)div_up(plane, kEmitBlockPix), (unsigned)S,
+        const dim3 grid64((unsigned)(ca.emit_gx + kWideBlocks), (unsigned)S,
                           (unsigned)div_up(E, kEmitEC64));
         TimedRegion tr(ctx, "emit");
         if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx->stream, ca, grid64, full)));
@@ -1833,14 +1819,9 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     // pass 2b: listed pixels, scaled, written once
     {
         TimedRegion tr(ctx, "near_emit");
-        if (f64) XTS_TRY((near_emit<double, 16>(ctx->stream, ca, near_gx, full, false)));
-        else XTS_TRY((near_emit<float, 40>(ctx->stream, ca, near_gx, full, false)));
-        if (wide_st == ctx->stream) {
-            if (f64) XTS_TRY((near_emit<double, 16>(ctx->stream, ca, near_gx, full, true)));
-            else XTS_TRY((near_emit<float, 40>(ctx->stream, ca, near_gx, full, true)));
-        }
+        if (f64) XTS_TRY((near_emit<double, 16>(ctx->stream, ca, near_gx, full)));
+        else XTS_TRY((near_emit<float, 40>(ctx->stream, ca, near_gx, full)));
     }
-    if (wide_st != ctx->stream) XTS_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_events[1], 0));
     return XTS_OK;
 }
 

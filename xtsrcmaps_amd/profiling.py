@@ -25,7 +25,7 @@ import re
 import sys
 
 MM_KERNELS = (r"rowweight_kernel|near_rows_kernel|hist_integral_kernel|factor_kernel|emit_s|"
-              r"near_emit_kernel|wide_pixel_kernel")
+              r"near_emit_kernel")
 COPY_KERNELS = r"MulFunctor"          # torch.mul(src, 1.0, out=dst): reads and writes N bytes
 
 
@@ -111,8 +111,7 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
     per_kernel = {}
     for name, rx in (("rowweight", "rowweight_kernel"), ("near_rows", "near_rows_kernel"),
                      ("hist_factor", "hist_integral_kernel|factor_kernel"),
-                     ("emit", "emit_s"), ("near_emit", "near_emit_kernel"),
-                     ("wide_pixel", "wide_pixel_kernel")):
+                     ("emit", "emit_s"), ("near_emit", "near_emit_kernel")):
         f, _ = _counter_sum(mm_fetch, "FETCH_SIZE", rx)
         w, _ = _counter_sum(mm_write, "WRITE_SIZE", rx)
         per_kernel[name] = {"fetch_bytes": f * 1024 / calls / f_fetch,
