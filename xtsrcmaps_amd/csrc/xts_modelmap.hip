@@ -86,10 +86,10 @@ struct SrcGeom {
 // (the angle 2 asin(chord / 2) is increasing in the chord, so the minimum
 // angle is the angle of the minimum chord); block S: the field-of-view flag
 // of every source, one thread per source.
-__global__ void __launch_bounds__(256)
-radius_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
-              const double* __restrict__ corners, SrcGeom* __restrict__ geo) {
-    const int s = blockIdx.x;
+__device__ inline void radius_block(const Wcs& w, int H, int W, int S,
+                                    const double* __restrict__ src_sph,
+                                    const double* __restrict__ corners,
+                                    SrcGeom* __restrict__ geo, int s) {
     if (s == S) {
         const double buf = 3.5;
         for (int t = threadIdx.x; t < S; t += blockDim.x) {
@@ -129,6 +129,12 @@ radius_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
         g.chord_hi = c * (1.0 + 1e-12);
         g.pad = 0;
     }
+}
+
+__global__ void __launch_bounds__(256)
+radius_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
+              const double* __restrict__ corners, SrcGeom* __restrict__ geo) {
+    radius_block(w, H, W, S, src_sph, corners, geo, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ cubature
@@ -1544,7 +1550,7 @@ static int check_wcs(const xts_wcs* wcs) {
 
 // Geometry + (optionally) per-source radius into the kSlotGeom slot.
 static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph_dev, int S,
-                          ImageGeom* ig) {
+                          ImageGeom* ig, bool with_radius = true) {
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     const size_t ncorn = (size_t)(H + 1) * (W + 1);
     const size_t bytes = ncorn * 3 * 8 * 2 + (size_t)H * W * 8;
@@ -1568,21 +1574,20 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
         ctx->geom_wcs = *wcs;
         ctx->geom_valid = true;
     }
-    if (S > 0 && src_sph_dev)
+    if (S > 0 && src_sph_dev && with_radius)
         radius_kernel<<<S + 1, 256, 0, ctx->stream>>>(w, H, W, S, src_sph_dev, ig->corners, ig->geo);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
 
-// FP32 copy of the PSF table, energy axis padded to a multiple of 4.
-// The PSF table in the cubature's precision, energies padded to Ep; the
-// same grid clears the nzero accumulators of the model map.
-__global__ void lut_convert_kernel(const double* __restrict__ upsf, int S, int E, int Ep,
-                                   float* __restrict__ out32, double* __restrict__ out64,
-                                   double* __restrict__ zero, size_t nzero) {
+// The PSF table in the cubature's precision, energies padded to Ep (a
+// multiple of 4); the same threads clear the nzero accumulators of the
+// model map.
+__device__ inline void lut_convert_part(const double* __restrict__ upsf, int S, int E, int Ep,
+                                        float* __restrict__ out32, double* __restrict__ out64,
+                                        double* __restrict__ zero, size_t nzero, size_t i0,
+                                        size_t step) {
     const size_t n = (size_t)S * kND * Ep;
-    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t step = (size_t)gridDim.x * blockDim.x;
     for (size_t k = i0; k < n; k += step) {
         const int e = (int)(k % Ep);
         const size_t sd = k / Ep;
@@ -1591,6 +1596,22 @@ __global__ void lut_convert_kernel(const double* __restrict__ upsf, int S, int E
         if (out64) out64[k] = v;
     }
     for (size_t k = i0; k < nzero; k += step) zero[k] = 0.0;
+}
+
+// Both in one launch (they are independent): blocks 0 .. S take the
+// boundary radii and field-of-view flags, the rest the table conversion.
+__global__ void __launch_bounds__(256)
+radius_lut_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
+                  const double* __restrict__ corners, SrcGeom* __restrict__ geo,
+                  const double* __restrict__ upsf, int E, int Ep, float* __restrict__ out32,
+                  double* __restrict__ out64, double* __restrict__ zero, size_t nzero) {
+    if ((int)blockIdx.x <= S) {
+        radius_block(w, H, W, S, src_sph, corners, geo, blockIdx.x);
+        return;
+    }
+    lut_convert_part(upsf, S, E, Ep, out32, out64, zero, nzero,
+                     (size_t)(blockIdx.x - S - 1) * blockDim.x + threadIdx.x,
+                     (size_t)(gridDim.x - S - 1) * blockDim.x);
 }
 
 // The rule as per-point records on the device, with its bilinear-weight
@@ -1682,8 +1703,9 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const int sehw = (flags & XTS_LAYOUT_SEHW) ? 1 : 0;
     const int of32 = (flags & XTS_OUT_F32) ? 1 : 0;
 
+    // the radii are launched with the table conversion below
     ImageGeom ig;
-    XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig));
+    XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig, false));
     const xts_const_tables* ct;
     XTS_TRY(const_tables(ctx, &ct));
     RuleUVW* rp;
@@ -1707,9 +1729,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
     double* acc = (double*)accp;
     const size_t nzero = full ? cnt_d + (size_t)S * kND : cnt_d;
-    lut_convert_kernel<<<std::min<unsigned>(div_up(lut_elems, 256), 8192), 256, 0, ctx->stream>>>(
-        upsf, (int)S, (int)E, Ep, f64 ? nullptr : (float*)lut, f64 ? (double*)lut : nullptr, acc,
-        nzero);
+    radius_lut_kernel<<<(unsigned)(S + 1) + std::min<unsigned>(div_up(lut_elems, 256), 8192), 256,
+                        0, ctx->stream>>>(wcs_setup(*wcs), H, W, (int)S, src_sph, ig.corners, ig.geo,
+                                          upsf, (int)E, Ep, f64 ? nullptr : (float*)lut,
+                                          f64 ? (double*)lut : nullptr, acc, nzero);
 
     // per-pixel row weights and first rows; listed (near) pixels, their pool
     // offsets and row ranges
