@@ -3,6 +3,9 @@
 #include <cstring>
 
 #include <cmath>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 #include "xts_internal.hpp"
 
@@ -92,6 +95,17 @@ hipEvent_t pooled_event(xts_ctx* ctx) {
     hipEvent_t e = nullptr;
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
+}
+
+int allow_lds(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> done;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& d : done)
+        if (d.first == fn && d.second >= bytes) return XTS_OK;
+    XTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done.emplace_back(fn, bytes);
+    return XTS_OK;
 }
 
 int to_device(xts_ctx* ctx, int slot, const void* host, size_t bytes, void** dev) {

@@ -110,6 +110,9 @@ struct xts_ctx {
 namespace xts {
 
 int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out);
+// Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
+// and size: the attribute belongs to the function).
+int allow_lds(const void* fn, size_t bytes);
 
 // Constant tables on the device (uploaded on the first call).
 int const_tables(xts_ctx* ctx, const xts_const_tables** out);

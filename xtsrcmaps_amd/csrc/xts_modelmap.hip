@@ -1338,18 +1338,6 @@ emit_shwe_kernel(CubArgs a) {
     }
 }
 
-// Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
-// and size: the attribute belongs to the function).
-static int allow_lds(const void* fn, size_t bytes) {
-    static std::mutex mu;
-    static std::vector<std::pair<const void*, size_t>> done;
-    std::lock_guard<std::mutex> lock(mu);
-    for (auto& d : done)
-        if (d.first == fn && d.second >= bytes) return XTS_OK;
-    XTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    done.emplace_back(fn, bytes);
-    return XTS_OK;
-}
 
 template <typename T, int EC, typename OT, bool SCALED>
 static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) {
