@@ -40,6 +40,10 @@ CONFIGS = {
     "c0": Config("c0", 1, 100, 0.1, rule="deg7", seed=0, out_f32=False, src_mode="centre"),
     # 100 sources uniform over a 200x200 0.05 deg image, 8x8 Gauss-Legendre, FP32
     "c1": Config("c1", 100, 200, 0.05, rule="gl8", seed=1, out_f32=True, src_mode="uniform"),
+    # the 2,000-source 400x400 0.05 deg, 37-energy workload shape (the size of
+    # the larger configurations) through the plain cubature, FP32: 1.18e10
+    # evaluations, a 47 GB map
+    "c3p": Config("c3p", 2000, 400, 0.05, rule="gl8", seed=3, out_f32=True, src_mode="uniform"),
 }
 
 
