@@ -148,12 +148,13 @@ radius_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
 //                     solid angle, the in-radius row histogram of the map
 //                     integral, and a list of the pixels whose points span
 //                     more rows (near the source, or outside the table)
-//   near_rows_kernel  up to 48 row weights per listed pixel (pooled), into
+//   near_rows_kernel  up to 24 row weights per listed pixel (pooled), into
 //                     the same histogram
 //   emit_*_kernel     per (source, pixel, energy): the 4-row dot product x
-//                     exposure x correction factor, stored once
-//   near_emit_kernel  the listed pixels' values from their pooled rows (or
-//                     the direct cubature for the pixel holding the source)
+//                     exposure x correction factor, stored once (pooled
+//                     pixels as zeros); its extra blocks take the direct
+//                     cubature of the few wide pixels (the source's own)
+//   near_emit_kernel  the pooled pixels' values from their pool rows
 constexpr int kTileW = 16, kTileH = 16;   // pixels per block pass (256 threads; a wave: 16x4)
 constexpr int kThreads = kTileW * kTileH;
 constexpr int kNearThreads = 256;
