@@ -187,6 +187,8 @@ struct CubArgs {
     int* near_list;            // [S][H W] listed pixel indices
     int* near_off;             // [S][H W] pool offset per listed pixel (-1: wide)
     int* near_rr;              // [S][H W] rbase * 256 + R per listed pixel
+    int* wide_list;            // [S][H W] list positions of the wide pixels
+    int* wide_count;           // [S]
     void* near_pool;           // pooled row weights of listed pixels (T)
     int* near_pool_count;      // [S] pool entries taken in each source's segment
     int near_pool_cap;         // entries per source
@@ -985,7 +987,10 @@ near_rows_kernel(CubArgs a) {
         if (live && part == 0) {
             meta_off[idx] = off;
             meta_rr[idx] = rbase * 256 + min(R, 255);
-            if (off < 0) a.rw_row[(size_t)s * plane + pix] = kRowWide;
+            if (off < 0) {
+                a.rw_row[(size_t)s * plane + pix] = kRowWide;
+                a.wide_list[(size_t)s * plane + atomicAdd(&a.wide_count[s], 1)] = idx;
+            }
         }
     }
     if (do_hist) {
@@ -997,8 +1002,8 @@ near_rows_kernel(CubArgs a) {
 
 // Wide listed pixels (pool offset -1): one wave per pixel, run by extra
 // blocks of the emit kernel (blockIdx.y = source, blockIdx.z = energy
-// chunk).  Every wave scans the source's list; the k-th wide entry goes to
-// wave gw = k mod nwaves.  The wave's lanes take the rule points, adding
+// chunk).  The k-th entry of the source's wide list goes to wave
+// gw = k mod nwaves.  The wave's lanes take the rule points, adding
 // each point's two row weights to a 401-row LDS histogram H (the wave's
 // own); then each lane takes every 64th table row, loads its EC energies as
 // vectors and accumulates W_r L[r], and the wave sums the 64 partial
@@ -1011,95 +1016,72 @@ __device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
     constexpr int NV = EC / VN;
     const int s = blockIdx.y;
     const int lane = threadIdx.x % 64;
-    const int count = a.near_count[s];
     const int e0 = blockIdx.z * EC;
     const int ne = min(EC, a.E - e0);
     const SrcGeom g = a.geo[s];
     const size_t plane = (size_t)a.H * a.W;
     const int* list = a.near_list + (size_t)s * plane;
-    const int* meta_off = a.near_off + (size_t)s * plane;
     const T* lut_s0 = reinterpret_cast<const T*>(a.lut) + (size_t)s * kND * a.Ep + e0;
-    int rank = 0;   // wide entries seen before this chunk
-    constexpr int kPre = 8;   // chunks whose entries are loaded together
-    int pre[kPre];
-    for (int c = 0; c < count; c += 64) {
-        const int q = (c / 64) % kPre;
-        if (q == 0) {
+    // this wave's entries of the source's wide list (written by the near pass)
+    const int nw = a.wide_count[s];
+    for (int k = gw; k < nw; k += nwaves) {
+        const int pix = list[a.wide_list[(size_t)s * plane + k]];
+        const int h = pix / a.W, w = pix % a.W;
+        const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, a.W + 1, g.dir, a.rho_scale);
+        const T om = a.omega ? (T)a.omega[pix] : T(1);
+        for (int r = lane; r < kND; r += 64) H[r] = T(0);
+        for (int p = lane; p < a.N; p += 64) {
+            const RuleUVW q = a.uvw[p];
+            const T u = (T)q.u, v = (T)q.v, wq = (T)q.wq;
+            const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+            const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+            const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+            const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
+            const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
+            const int i = sep_index_fast(sd);
+            const T isep = sizeof(T) == 4 ? (T)a.inv_sep[i - 1] : (T)a.inv_sep64[i - 1];
+            const T tt = T(kRedm1) * (sd * (isep * T(0.5 / kR2D)) - T(1));
+            atomicAdd(&H[i - 1], wq * (T(1) - tt));
+            atomicAdd(&H[i], wq * tt);
+        }
+        V acc[NV];
 #pragma unroll
-            for (int j = 0; j < kPre; ++j) {
-                const int k = c + j * 64 + lane;
-                pre[j] = k < count ? meta_off[k] : 0;
+        for (int v = 0; v < NV; ++v) acc[v] = V{};
+        for (int r = lane; r < kND; r += 64) {
+            const T W = H[r];
+            if (W != T(0)) {
+                const V* L = reinterpret_cast<const V*>(lut_s0 + (size_t)r * a.Ep);
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    if (e0 + v * VN < a.Ep) vfma(acc[v], W, L[v]);
             }
         }
-        int mine = pre[0];
 #pragma unroll
-        for (int j = 1; j < kPre; ++j) mine = q == j ? pre[j] : mine;
-        const uint64_t wide = __ballot(mine < 0);
-        uint64_t todo = wide;
-        while (todo) {
-            const int b = __ffsll((unsigned long long)todo) - 1;
-            todo &= todo - 1;
-            const int my = rank + __popcll(wide & ((1ull << b) - 1));
-            if (my % nwaves != gw) continue;
-            const int pix = list[c + b];
-            const int h = pix / a.W, w = pix % a.W;
-            const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, a.W + 1, g.dir, a.rho_scale);
-            const T om = a.omega ? (T)a.omega[pix] : T(1);
-            for (int r = lane; r < kND; r += 64) H[r] = T(0);
-            for (int p = lane; p < a.N; p += 64) {
-                const RuleUVW q = a.uvw[p];
-                const T u = (T)q.u, v = (T)q.v, wq = (T)q.wq;
-                const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
-                const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
-                const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
-                const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
-                const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
-                const int i = sep_index_fast(sd);
-                const T isep = sizeof(T) == 4 ? (T)a.inv_sep[i - 1] : (T)a.inv_sep64[i - 1];
-                const T tt = T(kRedm1) * (sd * (isep * T(0.5 / kR2D)) - T(1));
-                atomicAdd(&H[i - 1], wq * (T(1) - tt));
-                atomicAdd(&H[i], wq * tt);
-            }
-            V acc[NV];
+        for (int v = 0; v < NV; ++v) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = V{};
-            for (int r = lane; r < kND; r += 64) {
-                const T W = H[r];
-                if (W != T(0)) {
-                    const V* L = reinterpret_cast<const V*>(lut_s0 + (size_t)r * a.Ep);
+            for (int j = 0; j < VN; ++j) {
+                T x = reinterpret_cast<const T*>(&acc[v])[j];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        if (e0 + v * VN < a.Ep) vfma(acc[v], W, L[v]);
-                }
-            }
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-#pragma unroll
-                for (int j = 0; j < VN; ++j) {
-                    T x = reinterpret_cast<const T*>(&acc[v])[j];
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-                    const int e = v * VN + j;
-                    if (lane == 0 && e < ne) {
-                        x *= om;
-                        OT* po = reinterpret_cast<OT*>(a.out) +
-                                 out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
-                        if (SCALED) {
-                            const int se = s * a.E + e0 + e;
-                            if constexpr (sizeof(OT) == 4) {
-                                *po = (float)x * (float)(a.expo[se] * a.cf[se]);
-                            } else {
-                                double y = (double)x * a.expo[se];
-                                *po = y * a.cf[se];
-                            }
+                for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+                const int e = v * VN + j;
+                if (lane == 0 && e < ne) {
+                    x *= om;
+                    OT* po = reinterpret_cast<OT*>(a.out) +
+                             out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
+                    if (SCALED) {
+                        const int se = s * a.E + e0 + e;
+                        if constexpr (sizeof(OT) == 4) {
+                            *po = (float)x * (float)(a.expo[se] * a.cf[se]);
                         } else {
-                            *po = (OT)x;
+                            double y = (double)x * a.expo[se];
+                            *po = y * a.cf[se];
                         }
+                    } else {
+                        *po = (OT)x;
                     }
                 }
             }
         }
-        rank += __popcll(wide);
     }
 }
 
@@ -1709,10 +1691,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     void* lut;
     const size_t lut_elems = (size_t)S * kND * Ep;
     XTS_TRY(slot_reserve(ctx, kSlotLut, lut_elems * (f64 ? 8 : 4), &lut));
-    // counters [2 S] (int, padded to doubles: near counts, pool entries taken
-    // per source) and FP64 accumulators: hist [S][401], mapint [S][E],
-    // factor [S][E]; the counters and hist start at zero
-    const size_t cnt_d = (size_t)S;
+    // counters [3 S] (int, padded to doubles: near counts, pool entries taken
+    // and wide pixels per source) and FP64 accumulators: hist [S][401],
+    // mapint [S][E], factor [S][E]; the counters and hist start at zero
+    const size_t cnt_d = div_up((size_t)3 * S, 2);
     const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
     void* accp;
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
@@ -1728,7 +1710,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const size_t plane = (size_t)H * W;
     const size_t rw_bytes = (size_t)S * plane * (f64 ? 32 : 16);
     void* rows;
-    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 16 + 512, &rows));
+    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 20 + 512, &rows));
     // pool of listed pixels' row weights, one segment per source with room
     // for 2048 pixels of 16 rows; a pixel that finds its segment full is
     // evaluated directly
@@ -1751,8 +1733,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.near_list = ca.rw_row + (size_t)S * plane;
     ca.near_off = ca.near_list + (size_t)S * plane;
     ca.near_rr = ca.near_off + (size_t)S * plane;
+    ca.wide_list = ca.near_rr + (size_t)S * plane;
     ca.near_count = (int*)acc;
     ca.near_pool_count = ca.near_count + S;
+    ca.wide_count = ca.near_pool_count + S;
     ca.near_pool = pool;
     ca.near_pool_cap = (int)pool_cap;
     ca.hist = full ? acc + cnt_d : nullptr;
