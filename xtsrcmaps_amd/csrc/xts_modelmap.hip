@@ -190,8 +190,7 @@ struct CubArgs {
     int* wide_list;            // [S][H W] list positions of the wide pixels
     int* wide_count;           // [S]
     void* near_pool;           // pooled row weights of listed pixels (T)
-    int* near_pool_count;      // [S] pool entries taken in each source's segment
-    int near_pool_cap;         // entries per source
+    int near_pool_cap;         // entries per source (kNearRows per list position)
     const double* expo;        // [S][E] or null
     const double* cf;          // [S][E] or null
     void* out;
@@ -900,7 +899,6 @@ near_rows_kernel(CubArgs a) {
     int* meta_rr = a.near_rr + (size_t)s * plane;
     T* pool = reinterpret_cast<T*>(a.near_pool);
     T* myh = ph + tid;                              // row r at myh[r * kNearThreads]
-    const int lane0 = (tid & 63) & ~(kNearSplit - 1);
     for (int first = blockIdx.x * kNearPix; first < count; first += gridDim.x * kNearPix) {
         const int idx = first + slot;
         const bool live = idx < count;
@@ -912,15 +910,11 @@ near_rows_kernel(CubArgs a) {
         // rows touched: rbase = lower row of slo .. the upper row of shi
         const int rbase = sep_index_fast(pg.slo) - 1;
         const int R = sep_index_fast(pg.shi) - rbase + 1;
-        int off = -1;
-        if (part == 0 && live) {
-            if (R <= kNearRows) {
-                // this source's segment of the pool
-                off = atomicAdd(&a.near_pool_count[s], R);
-                off = off + R > a.near_pool_cap ? -1 : off + s * a.near_pool_cap;
-            }
-        }
-        off = __shfl(off, lane0);
+        // a fixed slot of kNearRows entries per list position in this
+        // source's segment of the pool (no atomic in the pixel's chain)
+        const int off = live && R <= kNearRows && (idx + 1) * kNearRows <= a.near_pool_cap
+                            ? s * a.near_pool_cap + idx * kNearRows
+                            : -1;
         const bool narrow = off >= 0;
         if (narrow)
             for (int r = 0; r < R; ++r) myh[r * kNearThreads] = T(0);
@@ -973,7 +967,7 @@ near_rows_kernel(CubArgs a) {
         }
         if (narrow) {
             // the pixel's kNearSplit partial histograms are in this wave's
-            // lanes lane0 .. lane0+kNearSplit-1 (LDS order within a wave)
+            // adjacent lanes tid - part .. tid - part + kNearSplit - 1
             T* base = ph + (tid - part);
             for (int r = part; r < R; r += kNearSplit) {
                 T W = T(0);
@@ -1691,10 +1685,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     void* lut;
     const size_t lut_elems = (size_t)S * kND * Ep;
     XTS_TRY(slot_reserve(ctx, kSlotLut, lut_elems * (f64 ? 8 : 4), &lut));
-    // counters [3 S] (int, padded to doubles: near counts, pool entries taken
-    // and wide pixels per source) and FP64 accumulators: hist [S][401],
-    // mapint [S][E], factor [S][E]; the counters and hist start at zero
-    const size_t cnt_d = div_up((size_t)3 * S, 2);
+    // counters [2 S] (int, padded to doubles: near and wide pixels per
+    // source) and FP64 accumulators: hist [S][401], mapint [S][E], factor
+    // [S][E]; the counters and hist start at zero
+    const size_t cnt_d = (size_t)S;
     const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
     void* accp;
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
@@ -1711,10 +1705,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const size_t rw_bytes = (size_t)S * plane * (f64 ? 32 : 16);
     void* rows;
     XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 20 + 512, &rows));
-    // pool of listed pixels' row weights, one segment per source with room
-    // for 2048 pixels of 16 rows; a pixel that finds its segment full is
-    // evaluated directly
-    const int64_t pool_cap = std::min<int64_t>(2048 * 16, ((int64_t)1 << 30) / S);
+    // pool of listed pixels' row weights, one segment per source with a slot
+    // of kNearRows entries for each of the first 4096 list positions; a
+    // pixel beyond is evaluated directly (as a wide pixel)
+    const int64_t pool_cap = std::min<int64_t>(4096 * kNearRows, ((int64_t)1 << 30) / S);
     void* pool;
     XTS_TRY(slot_reserve(ctx, kSlotNearPool, (size_t)(pool_cap * S) * (f64 ? 8 : 4), &pool));
 
@@ -1735,8 +1729,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.near_rr = ca.near_off + (size_t)S * plane;
     ca.wide_list = ca.near_rr + (size_t)S * plane;
     ca.near_count = (int*)acc;
-    ca.near_pool_count = ca.near_count + S;
-    ca.wide_count = ca.near_pool_count + S;
+    ca.wide_count = ca.near_count + S;
     ca.near_pool = pool;
     ca.near_pool_cap = (int)pool_cap;
     ca.hist = full ? acc + cnt_d : nullptr;
