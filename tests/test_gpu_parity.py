@@ -352,3 +352,43 @@ def test_gpu_model_map_c1_full_size_properties(ctx, c1):
     rel = np.abs(m32[:10] - m64) / m64
     _log(f"  FP32 vs FP64 cubature: max rel {rel.max():.3e}")
     assert rel.max() < 1e-5
+
+
+_C3P_CHILD = r"""
+import sys
+import numpy as np
+import torch
+sys.path.insert(0, sys.argv[1])
+import bench
+import xtsrcmaps_amd as xa
+cfg, inp, ctx, d = bench.setup("c3p")
+bench.step(ctx, inp, d)
+torch.cuda.synchronize()
+out = d["out"]
+ok = bool(torch.isfinite(out).all()) and bool((out > 0).all())
+k = 8
+host = lambda x: x[:k].cpu().numpy()
+m64 = ctx.point_src_model_map_wcs(inp.wcs, inp.src_sph[:k], host(d["expo"]), host(d["upsf"]),
+                                  host(d["parint"]), inp.rule,
+                                  flags=xa.LAYOUT_SEHW | xa.OUT_F64 | xa.COMPUTE_F64)
+m32 = out[:k].double().cpu().numpy()
+rel = float((np.abs(m32 - m64) / m64).max())
+print(f"finite_positive={ok} max_rel={rel:.3e}", flush=True)
+sys.exit(0 if ok and rel < 1e-5 else 1)
+"""
+
+
+def test_gpu_model_map_c3p_full_size_properties():
+    """2,000 sources on a 400x400 image, 37 energies: 1.18e10 values, a 47 GB
+    FP32 map, through the device entry points as the bench runs them.  Every
+    value finite and positive; 8 sources within 1e-5 of the FP64 cubature of
+    the same stages.  (Large-S paths: pool segments, grid sizes, 64-bit
+    offsets.)  Run in a child process that imports torch before the library,
+    as bench.py does: the two must share torch's HIP runtime."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    _log("test_gpu_model_map_c3p_full: 2000 x 37 x 400 x 400 (child process)")
+    r = subprocess.run([sys.executable, "-c", _C3P_CHILD, root], capture_output=True, text=True,
+                       timeout=240)
+    _log("  " + r.stdout.strip()[-300:])
+    assert r.returncode == 0, r.stderr[-2000:]
