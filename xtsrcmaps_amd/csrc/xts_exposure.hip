@@ -245,7 +245,7 @@ int xts_aeff_value(xts_ctx* ctx, const double* costh, int64_t C, const double* l
 int xts_src_cosbins_dev(xts_ctx* ctx, const double* src_sph, int64_t S, const double* cube,
                         int64_t nside, int64_t C, double* out) {
     XTS_TRY(activate(ctx));
-    XTS_CHECK(src_sph && cube && out && S > 0 && C > 0, "bad arguments");
+    XTS_CHECK(src_sph && cube && out && S > 0 && C > 0 && S * C < (1LL << 31), "bad arguments");
     XTS_CHECK(nside > 0 && (nside & (nside - 1)) == 0 && nside <= (1LL << 29),
               "nside %lld is not a power of two", (long long)nside);
     src_cosbins_kernel<<<div_up(S * C, 256), 256, 0, ctx->stream>>>((int)S, (int)C, nside,

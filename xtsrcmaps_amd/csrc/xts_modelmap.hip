@@ -912,7 +912,9 @@ near_rows_kernel(CubArgs a) {
         const int R = sep_index_fast(pg.shi) - rbase + 1;
         // a fixed slot of kNearRows entries per list position in this
         // source's segment of the pool (no atomic in the pixel's chain)
-        const int off = live && R <= kNearRows && (idx + 1) * kNearRows <= a.near_pool_cap
+        // (idx < cap / kNearRows: no product that could overflow; s * cap and
+        // idx * kNearRows stay below 2^30)
+        const int off = live && R <= kNearRows && idx < a.near_pool_cap / kNearRows
                             ? s * a.near_pool_cap + idx * kNearRows
                             : -1;
         const bool narrow = off >= 0;
@@ -1656,8 +1658,9 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                             int64_t E, const double* rule, int64_t N, int32_t flags, void* out,
                             bool full) {
     XTS_TRY(check_wcs(wcs));
-    XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096, "bad S=%lld or E=%lld",
-              (long long)S, (long long)E);
+    // (S E < 2^31: source-energy indices are 32-bit in the kernels)
+    XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096 && S * E < (1LL << 31),
+              "bad S=%lld or E=%lld", (long long)S, (long long)E);
     XTS_CHECK(src_sph && upsf && out, "null device pointer");
     XTS_CHECK(!full || (expo && parint), "null exposure/parint");
     XTS_CHECK((flags & ~0x7) == 0, "unknown flags 0x%x", flags);

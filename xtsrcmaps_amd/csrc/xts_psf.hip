@@ -546,7 +546,9 @@ static int check_psf_args(const xts_psf_grid* f, const xts_psf_grid* b, int64_t 
     XTS_CHECK(f && b && f->params && b->params && f->cosths && b->cosths && f->logEs && b->logEs,
               "null PSF grid");
     XTS_CHECK(f->Mc > 1 && f->Me > 1 && b->Mc > 1 && b->Me > 1, "bad PSF grid size");
-    XTS_CHECK(C > 0 && E > 0 && S > 0 && (int64_t)kND * E < (1LL << 30), "bad sizes");
+    XTS_CHECK(C > 0 && E > 0 && S > 0 && (int64_t)kND * E < (1LL << 30) && S * E < (1LL << 31) &&
+                  S * C < (1LL << 31),
+              "bad sizes");
     return XTS_OK;
 }
 
