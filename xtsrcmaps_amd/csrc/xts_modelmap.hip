@@ -176,7 +176,7 @@ struct CubArgs {
     const SrcGeom* geo;
     const void* lut;           // [S][401][Ep] of T
     const RuleUVW* uvw;
-    const float* lean;         // [6][N4] FP32 rule arrays u, v, w/4, w/4 t1, t2, t3
+    const float* lean;         // [3][N4] FP32 rule arrays u, v, w/4
     const float* inv_sep;      // [401] 2 R2D / sep[i] (sep in deg), FP32 table
     const double* inv_sep64;   // [401] FP64 table
     void* rw;                  // [S][H W] row weights (4 of T) x solid angle
@@ -571,63 +571,81 @@ __device__ inline float lin_q_error(const PixelGeom<float>& pg) {
 }
 constexpr float kLinQTol = 1e-7f;
 
-// One point: wq = w/4; tA, tB, tC = wq (t1, t2, t3).  Accumulates the
-// cumulative sums C0 = sum_{rel=0} w, C1 = sum_{rel<=1} w, D0, D1 likewise
-// of w t, and Tt = sum w t.  The masks are exact: for Q in [0, 3),
-// 2^24 (1 - Q) >= 1 whenever Q < 1.  NM is the number of row boundaries
-// the pixel's points may cross (Q < NM + 1): with NM = 1 every point has
-// rel <= 1, so C1 and D1 are the totals and only the first mask is formed.
+// One point.  Each point spreads its weight wq = w/4 over the rows on
+// either side of its separation: (1 - t) to the lower row, t to the upper.
+// With h = rel + t(f) its continuous row coordinate relative to r0 (rel =
+// floor(Q), f = frac(Q)), that is wq hat(h - k) for row r0 + k, hat(x) =
+// max(0, 1 - |x|).  On h in [0, NM + 1) the hats are linear combinations of
+// 1, h, and the ramps max(h - 1, 0), max(h - 2, 0), so a pixel needs only
+// the sums of wq h and of wq times the ramps (lean_rows).  The loop works
+// with h' = h - 1 = Q' + f (t(f) / f - 1), Q' = Q - 1 (the q0 of a
+// LeanPixel formed at r0 + 1), and accumulates A = sum wq h',
+// B1 = sum wq clamp(h', 0, 1), B2 = sum wq clamp(h' - 1, 0, 1).  The hats
+// are continuous in h, so rounding of Q near a row moves weight by a
+// rounding-sized amount only.  NM is the number of row boundaries the
+// pixel's points may cross: NM = 1 keeps h' < 1 and needs no B2.
 template <int NG, int NM = 2>
-__device__ inline void lean_point(float z, float wq, float tA, float tB, float tC,
-                                  const LeanPixel& lp, float& C0, float& D0, float& C1,
-                                  float& D1, float& Tt) {
+__device__ inline void lean_point(float z, float wq, const LeanPixel& lp, float& A, float& B1,
+                                  float& B2) {
     const float L = __builtin_amdgcn_logf(z);
     float g = lp.g[NG - 1];
 #pragma unroll
     for (int k = NG - 2; k >= 0; --k) g = fmaf(g, z, lp.g[k]);
     const float Q = fmaf(g, z, fmaf((float)(0.5 * kRecipStep * 0.69314718055994530942), L, lp.q0));
     const float f = __builtin_amdgcn_fractf(Q);
-    const float wt = f * fmaf(f, fmaf(f, tC, tB), tA);
-    const float m0 = __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 16777216.0f), 0.0f, 1.0f);
-    C0 = fmaf(wq, m0, C0);
-    D0 = fmaf(wt, m0, D0);
+    const float h = fmaf(f, fmaf(f, fmaf(f, (float)kTPoly[2], (float)kTPoly[1]),
+                                 (float)(kTPoly[0] - 1.0)), Q);
+    A = fmaf(wq, h, A);
     if constexpr (NM >= 2) {
-        const float m1 =
-            __builtin_amdgcn_fmed3f(fmaf(Q, -16777216.0f, 33554432.0f), 0.0f, 1.0f);
-        C1 = fmaf(wq, m1, C1);
-        D1 = fmaf(wt, m1, D1);
+        B1 = fmaf(wq, __builtin_amdgcn_fmed3f(h, 0.0f, 1.0f), B1);
+        B2 = fmaf(wq, __builtin_amdgcn_fmed3f(h - 1.0f, 0.0f, 1.0f), B2);
+    } else {
+        B1 = fmaf(wq, fmaxf(h, 0.0f), B1);
     }
-    Tt += wt;
 }
 
-// FP32 rule as arrays [N4] of u, v, w/4 and w/4 (t1, t2, t3) in the
-// constant address space: every lane of a wave reads the same point, so the
-// reads are scalar loads (16 bytes per array for a group of four points)
-// and the rule needs no copy in LDS.
+// Row weights W_k = sum wq hat(h - k) of rows r0 .. r0+3 from the sums of
+// lean_point (wtot = sum wq over the rule): with c1 = clamp(h', 0, 1) and
+// c2 = clamp(h' - 1, 0, 1), max(h', 0) = c1 + c2 and max(h' - 1, 0) = c2,
+// and on h in [0, 3)
+//   hat(h)     = max(h', 0) - h'
+//   hat(h - 1) = 1 + h' - 2 max(h', 0) + max(h' - 1, 0)
+//   hat(h - 2) = max(h', 0) - 2 max(h' - 1, 0)
+//   hat(h - 3) = max(h' - 1, 0) ... = 0 below h = 3; W3 = B2 is row r0+3.
+template <int NM>
+__device__ inline void lean_rows(float wtot, float A, float B1, float B2, float (&W)[4]) {
+    if constexpr (NM < 2) B2 = 0.0f;
+    W[0] = (B1 + B2) - A;
+    W[1] = (wtot + A) - fmaf(2.0f, B1, B2);
+    W[2] = B1 - B2;
+    W[3] = B2;
+}
+
+// FP32 rule as arrays [N4] of u, v and w/4 in the constant address space:
+// every lane of a wave reads the same point, so the reads are scalar loads
+// (16 bytes per array for a group of four points) and the rule needs no
+// copy in LDS.
 using cfloat = const __attribute__((address_space(4))) float;
 struct LeanRule {
-    cfloat* p[6];   // u, v, wq, tA, tB, tC
+    cfloat* p[3];   // u, v, wq
 };
-constexpr int kLeanArrays = 6;
+constexpr int kLeanArrays = 3;
 
 __device__ inline float4 lean_load4(cfloat* q) { return float4{q[0], q[1], q[2], q[3]}; }
-
-__device__ inline void lean_group(const LeanRule& lr, int p0, float4 (&r)[kLeanArrays]) {
-#pragma unroll
-    for (int j = 0; j < kLeanArrays; ++j) r[j] = lean_load4(lr.p[j] + p0);
-}
 
 __device__ inline float lane(const float4& x, int k) {
     return k == 0 ? x.x : (k == 1 ? x.y : (k == 2 ? x.z : x.w));
 }
 
-// With NM = 1 the loops return C1 = wtot (the rule's sum of w/4) and D1 = Tt.
+// The loops return the row weights W of rows r0 .. r0+3 (before the solid
+// angle); wtot = the rule's sum of w/4.
 template <int NG, int NM = 2>
 __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
                                         const PixelGeom<float>& pg, int r0, float wtot,
-                                        float& C0, float& D0, float& C1, float& D1, float& Tt) {
-    const LeanPixel lp = lean_pixel<NG>(pg, r0);
+                                        float (&W)[4]) {
+    const LeanPixel lp = lean_pixel<NG>(pg, r0 + 1);
     const int nv = a.tensor_nv;
+    float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
     for (int i = 0; i < a.tensor_nu; ++i) {
         const float u = lr.p[0][i * nv];
         const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
@@ -635,45 +653,48 @@ __device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
         const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
         for (int jj = 0; jj < nv; jj += 4) {
             const int p0 = i * nv + jj;
-            float4 r[kLeanArrays - 1];
-#pragma unroll
-            for (int j = 1; j < kLeanArrays; ++j) r[j - 1] = lean_load4(lr.p[j] + p0);
+            const float4 v = lean_load4(lr.p[1] + p0), wq = lean_load4(lr.p[2] + p0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const float v = lane(r[0], k);
-                lean_point<NG, NM>(fmaf(fmaf(cu, v, bu), v, au), lane(r[1], k), lane(r[2], k),
-                                   lane(r[3], k), lane(r[4], k), lp, C0, D0, C1, D1, Tt);
+                const float vk = lane(v, k);
+                lean_point<NG, NM>(fmaf(fmaf(cu, vk, bu), vk, au), lane(wq, k), lp, A, B1, B2);
             }
         }
     }
-    if constexpr (NM < 2) {
-        C1 = wtot;
-        D1 = Tt;
-    }
+    lean_rows<NM>(wtot, A, B1, B2, W);
 }
 
 template <int NG, int NM = 2>
 __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeom<float>& pg,
-                                      int r0, float wtot, float& C0, float& D0, float& C1,
-                                      float& D1, float& Tt) {
-    const LeanPixel lp = lean_pixel<NG>(pg, r0);
+                                      int r0, float wtot, float (&W)[4]) {
+    const LeanPixel lp = lean_pixel<NG>(pg, r0 + 1);
+    float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
     for (int p0 = 0; p0 < N4; p0 += 4) {
-        float4 r[kLeanArrays];
-        lean_group(lr, p0, r);
+        const float4 u = lean_load4(lr.p[0] + p0), v = lean_load4(lr.p[1] + p0),
+                     wq = lean_load4(lr.p[2] + p0);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float u = lane(r[0], k), v = lane(r[1], k);
-            const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
-            const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
-            const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
-            lean_point<NG, NM>(fmaf(fmaf(cu, v, bu), v, au), lane(r[2], k), lane(r[3], k),
-                               lane(r[4], k), lane(r[5], k), lp, C0, D0, C1, D1, Tt);
+            const float uk = lane(u, k), vk = lane(v, k);
+            const float au = fmaf(fmaf(lp.a2, uk, lp.a1), uk, lp.a0);
+            const float bu = fmaf(fmaf(lp.b2, uk, lp.b1), uk, lp.b0);
+            const float cu = fmaf(fmaf(lp.c2, uk, lp.c1), uk, lp.c0);
+            lean_point<NG, NM>(fmaf(fmaf(cu, vk, bu), vk, au), lane(wq, k), lp, A, B1, B2);
         }
     }
-    if constexpr (NM < 2) {
-        C1 = wtot;
-        D1 = Tt;
-    }
+    lean_rows<NM>(wtot, A, B1, B2, W);
+}
+
+// Row weights of rows r0 .. r0+3 from the per-row sums S_k of w/4 and T_k
+// of w t / 4 given as S0, T0, the cumulative S01, T01 and the totals wsum,
+// Tt: row k receives S_k - T_k from its own points and T_{k-1} from those
+// below it.
+template <typename T>
+__device__ inline void rows_from_sums(T wsum, T S0, T T0, T S01, T T01, T Tt, T (&W)[4]) {
+    const T S1 = S01 - S0, T1 = T01 - T0, S2 = wsum - S01, T2 = Tt - T01;
+    W[0] = S0 - T0;
+    W[1] = T0 + (S1 - T1);
+    W[2] = T1 + (S2 - T2);
+    W[3] = T2;
 }
 
 // Pass 1.  Grid: (tile blocks, S); LDS: the in-radius row histogram
@@ -737,7 +758,9 @@ rowweight_kernel(CubArgs a) {
         const bool fast = (ihi - ilo) <= 2 && pg.slo >= T(1e-4) && pg.shi < T(69.9);
         const int r0 = min(max(ilo - 1, 0), kND - 4);
 
-        // S0 = sum_{rel=0} w, S01 = sum_{rel<=1} w, likewise T0, T01 of w t
+        // row weights of r0 .. r0+3; the FP64 and per-point paths form them
+        // from S0 = sum_{rel=0} w, S01 = sum_{rel<=1} w, likewise T0, T01 of w t
+        T W[4] = {T(0), T(0), T(0), T(0)};
         T S0 = T(0), T0 = T(0), S01 = T(0), T01 = T(0), Tt = T(0);
         // FP32: lanes that are not fast are listed for the near pass and
         // their sums are discarded, so they may run the lean loop (which
@@ -752,18 +775,15 @@ rowweight_kernel(CubArgs a) {
                 // every fast pixel's points within three rows (one boundary)
                 const bool one = __all(!fast || ihi - ilo <= 1);
                 if (a.tensor_nu > 0) {
-                    if (lin && one)
-                        lean_tensor_loop<1, 1>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
-                    else if (lin) lean_tensor_loop<1>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
-                    else if (lean4)
-                        lean_tensor_loop<3>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
-                    else lean_tensor_loop<4>(a, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    if (lin && one) lean_tensor_loop<1, 1>(a, lr, pg, r0, wsum, W);
+                    else if (lin) lean_tensor_loop<1>(a, lr, pg, r0, wsum, W);
+                    else if (lean4) lean_tensor_loop<3>(a, lr, pg, r0, wsum, W);
+                    else lean_tensor_loop<4>(a, lr, pg, r0, wsum, W);
                 } else {
-                    if (lin && one)
-                        lean_list_loop<1, 1>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
-                    else if (lin) lean_list_loop<1>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
-                    else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
-                    else lean_list_loop<4>(N4, lr, pg, r0, wsum, S0, T0, S01, T01, Tt);
+                    if (lin && one) lean_list_loop<1, 1>(N4, lr, pg, r0, wsum, W);
+                    else if (lin) lean_list_loop<1>(N4, lr, pg, r0, wsum, W);
+                    else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, wsum, W);
+                    else lean_list_loop<4>(N4, lr, pg, r0, wsum, W);
                 }
             }
         } else if (lean6) {
@@ -784,6 +804,7 @@ rowweight_kernel(CubArgs a) {
             }
             S01 += S0;
             T01 += T0;
+            rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
         }
         if (!lean6 && __any(fast)) {
             // ---- far pixels beyond 28 degrees or sharing a wave with near ones
@@ -816,15 +837,14 @@ rowweight_kernel(CubArgs a) {
                 T01 += rel <= 1 ? wt : T(0);
                 Tt += wt;
             }
+            rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
         }
         if (valid) {
             const int pix = h * a.W + w;
             if (fast) {
-                // row weights from the cumulative sums, x solid angle
-                const T S1 = S01 - S0, T1 = T01 - T0, S2 = wsum - S01, T2 = Tt - T01;
+                // row weights x solid angle
                 const T om = a.omega ? (T)a.omega[pix] : T(1);
-                const V4 wv = V4{om * (S0 - T0), om * (T0 + (S1 - T1)), om * (T1 + (S2 - T2)),
-                                 om * T2};
+                const V4 wv = V4{om * W[0], om * W[1], om * W[2], om * W[3]};
                 rw[pix] = wv;
                 rw_row[pix] = r0;
                 if (do_hist && in_radius(g, a.centres, h * Wp1 + w)) {
@@ -1629,7 +1649,6 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
             la[p] = (float)rp[p].u;
             la[N4 + p] = (float)rp[p].v;
             la[2 * N4 + p] = wq;
-            for (int k = 0; k < 3; ++k) la[(3 + k) * N4 + p] = wq * (float)kTPoly[k];
         }
         void* d;
         XTS_TRY(to_device(ctx, kSlotRule, blob.data(), blob.size(), &d));
