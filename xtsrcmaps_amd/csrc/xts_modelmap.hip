@@ -684,6 +684,193 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
     lean_rows<NM>(wtot, A, B1, B2, W);
 }
 
+// ---- FP32 point loop without a logarithm, for pixels whose points lie
+// close to the pixel centre relative to their separation.
+// The row weights are linear in the separation s between rows (weight
+// (s - s_{k-1}) / (s_k - s_{k-1}) to row k from below), so with rows r0,
+// r0+1, r0+2, r0+3 at s0, s1 = s0 e^step, ... and
+//   sigma = (s - s1) / (s1 - s0)
+// they are linear combinations of 1, sigma, |sigma| and |sigma - G|,
+// G = e^step = (s2 - s1) / (s1 - s0) (poly_rows).  sigma is a polynomial in
+// x = z - 1 (z = y / y_c as in LeanPixel, x quadratic in u and v with no
+// constant term beyond the rounding of y_c iy):
+//   s / s1 = e^(step q_c) sqrt(1 + x) exp(g(y_c (1 + x)) - g(y_c)),
+// q_c = Q - 1 at the pixel centre, g as above; the Taylor series of the
+// product to degree D times e^(step q_c) / (1 - e^-step), plus the constant
+// expm1(step q_c) / (1 - e^-step), is sigma(x).  A point costs the two FMAs
+// of x, D of the polynomial and two (three with NM = 2) sums.
+// Remainder: the binomial series of sqrt(1 + x) bounds it on |x| <= r by
+// 1.1 K |binom(1/2, D + 1)| r^(D+1) / (1 - r), K = e^(2 step) /
+// (1 - e^-step) = 32.25 (the exp factor's own terms add under 5%);
+// kPolyReach[D - 2] is the r at which that bound reaches 1e-7 (in units of
+// a row spacing, where the FP32 rounding of sigma is ~2e-7).
+constexpr int kPolyMaxD = 6;
+constexpr float kPolyReach[kPolyMaxD - 1] = {0.00355f, 0.0163f, 0.0397f, 0.0709f, 0.1065f};
+constexpr double kRowG = 1.0 + 1.0 / kRedm1;           // e^step
+constexpr double kInvD0 = kRedm1 + 1.0;                // 1 / (1 - e^-step)
+
+struct PolyPixel {
+    float x0, x1, x2;   // x(u, v) = x0 + u (x1 + u x2) + v (b(u) + v c(u))
+    float b0, b1, b2, c0, c1, c2;
+    float c[kPolyMaxD + 1];
+};
+
+// Bound r on |x| over the pixel's points: |p / 2| lies in [x0 - hr, x0 + hr],
+// so z lies in [(1 - rho)^2, (1 + rho)^2] with rho = hr / x0.
+__device__ inline float poly_reach(const PixelGeom<float>& pg) {
+    const float rho = pg.hr / pg.x0;
+    return rho * (2.0f + rho);
+}
+
+template <int D>
+__device__ inline PolyPixel poly_pixel(const PixelGeom<float>& pg, int r0) {
+    const float* A = pg.A; const float* B = pg.B; const float* C = pg.C; const float* E = pg.E;
+    auto dot = [](const float* x, const float* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const float yc = dot(A, A);
+    const float iy = 1.0f / yc;
+    PolyPixel pp;
+    pp.x0 = fmaf(yc, iy, -1.0f);
+    pp.x1 = 2.0f * dot(A, B) * iy;
+    pp.x2 = dot(B, B) * iy;
+    pp.b0 = 2.0f * dot(A, C) * iy;
+    pp.b1 = 2.0f * (dot(A, E) + dot(B, C)) * iy;
+    pp.b2 = 2.0f * dot(B, E) * iy;
+    pp.c0 = dot(C, C) * iy;
+    pp.c1 = 2.0f * dot(C, E) * iy;
+    pp.c2 = dot(E, E) * iy;
+    // q_c = Q - 1 at z = 1 (y = 1 / iy), in FP64; |step q_c| < 0.07
+    const double ycd = 1.0 / (double)iy;
+    const double* gc = kGQuartic;
+    const double gv = ycd * (gc[0] + ycd * (gc[1] + ycd * (gc[2] + ycd * gc[3])));
+    const double qc = (kLnTwoR2D + kLn1e4 + gv) * kRecipStep -
+                      0.5 * kRecipStep * ln_pos((double)iy) - (double)r0;
+    const double e = qc * kSepStep;
+    // expm1(e) = e (1 + e/2 (1 + e/3 (1 + ... (1 + e/8)))), remainder < 1e-16
+    double em = 1.0;
+#pragma unroll
+    for (int k = 8; k >= 2; --k) em = fma(em, e * (1.0 / k), 1.0);
+    em *= e;
+    pp.c[0] = (float)(em * kInvD0);
+    const float K = (float)((1.0 + em) * kInvD0);
+    // exp(g(y_c (1 + x)) - g(y_c)) = 1 + e1 x + e2 x^2 + e3 x^3 + ...
+    const float yf = (float)ycd;
+    const float g1 = (float)gc[0], g2 = (float)gc[1], g3 = (float)gc[2], g4 = (float)gc[3];
+    const float ga = yf * (g1 + yf * (2.0f * g2 + yf * (3.0f * g3 + yf * 4.0f * g4)));
+    const float gb = yf * yf * (g2 + yf * (3.0f * g3 + yf * 6.0f * g4));
+    const float gcb = yf * yf * yf * (g3 + yf * 4.0f * g4);
+    const float ex[4] = {1.0f, ga, gb + 0.5f * ga * ga, gcb + ga * gb + ga * ga * ga / 6.0f};
+    // binomial series of sqrt(1 + x)
+    constexpr float bs[kPolyMaxD + 1] = {1.0f, 0.5f, -0.125f, 0.0625f, -0.0390625f,
+                                         0.02734375f, -0.0205078125f};
+#pragma unroll
+    for (int k = 1; k <= kPolyMaxD; ++k) {
+        float r = 0.0f;
+#pragma unroll
+        for (int j = 0; j <= 3 && j <= k; ++j) r = fmaf(bs[k - j], ex[j], r);
+        pp.c[k] = k <= D ? K * r : 0.0f;
+    }
+    return pp;
+}
+
+// Row weights from A = sum wq sigma, B1 = sum wq |sigma|, B2 = sum wq
+// |sigma - G| (wtot = sum wq): with P1 = sum wq max(sigma, 0) and P2 = sum
+// wq max(sigma - G, 0), row r0 gets max(-sigma, 0), row r0+1 ... the hat
+// of slopes 1 and 1 / G, row r0+2 max(sigma, 0) / G - max(sigma - G, 0)
+// (1 / G + 1 / G^2), row r0+3 max(sigma - G, 0) / G^2.
+template <int NM>
+__device__ inline void poly_rows(float wtot, float A, float B1, float B2, float (&W)[4]) {
+    constexpr float iG = (float)(1.0 / kRowG), iG2 = (float)(1.0 / (kRowG * kRowG));
+    const float P1 = 0.5f * (B1 + A);
+    W[0] = 0.5f * (B1 - A);
+    if constexpr (NM < 2) {
+        W[2] = P1 * iG;
+        W[3] = 0.0f;
+    } else {
+        const float P2 = 0.5f * (B2 + fmaf(-(float)kRowG, wtot, A));
+        W[2] = fmaf(P1, iG, -P2 * (iG + iG2));
+        W[3] = P2 * iG2;
+    }
+    W[1] = wtot - W[0] - W[2] - W[3];
+}
+
+template <int D, int NM>
+__device__ inline void poly_point(float x, float wq, const PolyPixel& pp, float& A, float& B1,
+                                  float& B2) {
+    float sg = pp.c[D];
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) sg = fmaf(sg, x, pp.c[k]);
+    A = fmaf(wq, sg, A);
+    B1 = fmaf(wq, fabsf(sg), B1);
+    if constexpr (NM >= 2) B2 = fmaf(wq, fabsf(sg - (float)kRowG), B2);
+}
+
+template <int D, int NM>
+__device__ inline void poly_tensor_loop(const CubArgs& a, const LeanRule& lr,
+                                        const PixelGeom<float>& pg, int r0, float wtot,
+                                        float (&W)[4]) {
+    const PolyPixel pp = poly_pixel<D>(pg, r0);
+    const int nv = a.tensor_nv;
+    float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
+    for (int i = 0; i < a.tensor_nu; ++i) {
+        const float u = lr.p[0][i * nv];
+        const float xu = fmaf(fmaf(pp.x2, u, pp.x1), u, pp.x0);
+        const float bu = fmaf(fmaf(pp.b2, u, pp.b1), u, pp.b0);
+        const float cu = fmaf(fmaf(pp.c2, u, pp.c1), u, pp.c0);
+        for (int jj = 0; jj < nv; jj += 4) {
+            const int p0 = i * nv + jj;
+            const float4 v = lean_load4(lr.p[1] + p0), wq = lean_load4(lr.p[2] + p0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float vk = lane(v, k);
+                poly_point<D, NM>(fmaf(fmaf(cu, vk, bu), vk, xu), lane(wq, k), pp, A, B1, B2);
+            }
+        }
+    }
+    poly_rows<NM>(wtot, A, B1, B2, W);
+}
+
+template <int D, int NM>
+__device__ inline void poly_list_loop(int N4, const LeanRule& lr, const PixelGeom<float>& pg,
+                                      int r0, float wtot, float (&W)[4]) {
+    const PolyPixel pp = poly_pixel<D>(pg, r0);
+    float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
+    for (int p0 = 0; p0 < N4; p0 += 4) {
+        const float4 u = lean_load4(lr.p[0] + p0), v = lean_load4(lr.p[1] + p0),
+                     wq = lean_load4(lr.p[2] + p0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float uk = lane(u, k), vk = lane(v, k);
+            const float xu = fmaf(fmaf(pp.x2, uk, pp.x1), uk, pp.x0);
+            const float bu = fmaf(fmaf(pp.b2, uk, pp.b1), uk, pp.b0);
+            const float cu = fmaf(fmaf(pp.c2, uk, pp.c1), uk, pp.c0);
+            poly_point<D, NM>(fmaf(fmaf(cu, vk, bu), vk, xu), lane(wq, k), pp, A, B1, B2);
+        }
+    }
+    poly_rows<NM>(wtot, A, B1, B2, W);
+}
+
+template <int D, int NM>
+__device__ inline void poly_loop(const CubArgs& a, const LeanRule& lr, const PixelGeom<float>& pg,
+                                 int r0, int N4, float wtot, float (&W)[4]) {
+    if (a.tensor_nu > 0) poly_tensor_loop<D, NM>(a, lr, pg, r0, wtot, W);
+    else poly_list_loop<D, NM>(N4, lr, pg, r0, wtot, W);
+}
+
+// Runs the polynomial loop of the least degree whose reach covers every
+// lane's rx; false when none does.
+template <int NM>
+__device__ inline bool poly_dispatch(const CubArgs& a, const LeanRule& lr,
+                                     const PixelGeom<float>& pg, int r0, int N4, float wtot,
+                                     float rx, float (&W)[4]) {
+    if (__all(rx <= kPolyReach[0])) poly_loop<2, NM>(a, lr, pg, r0, N4, wtot, W);
+    else if (__all(rx <= kPolyReach[1])) poly_loop<3, NM>(a, lr, pg, r0, N4, wtot, W);
+    else if (__all(rx <= kPolyReach[2])) poly_loop<4, NM>(a, lr, pg, r0, N4, wtot, W);
+    else if (__all(rx <= kPolyReach[3])) poly_loop<5, NM>(a, lr, pg, r0, N4, wtot, W);
+    else if (__all(rx <= kPolyReach[4])) poly_loop<6, NM>(a, lr, pg, r0, N4, wtot, W);
+    else return false;
+    return true;
+}
+
 // Row weights of rows r0 .. r0+3 from the per-row sums S_k of w/4 and T_k
 // of w t / 4 given as S0, T0, the cumulative S01, T01 and the totals wsum,
 // Tt: row k receives S_k - T_k from its own points and T_{k-1} from those
@@ -771,19 +958,24 @@ rowweight_kernel(CubArgs a) {
         const bool lean4 = lean6 && __all(!fast || pg.shi < T(14.0));
         if constexpr (sizeof(T) == 4) {
             if (lean6) {
-                const bool lin = __all(!fast || lin_q_error(pg) < kLinQTol);
                 // every fast pixel's points within three rows (one boundary)
                 const bool one = __all(!fast || ihi - ilo <= 1);
-                if (a.tensor_nu > 0) {
-                    if (lin && one) lean_tensor_loop<1, 1>(a, lr, pg, r0, wsum, W);
-                    else if (lin) lean_tensor_loop<1>(a, lr, pg, r0, wsum, W);
-                    else if (lean4) lean_tensor_loop<3>(a, lr, pg, r0, wsum, W);
-                    else lean_tensor_loop<4>(a, lr, pg, r0, wsum, W);
-                } else {
-                    if (lin && one) lean_list_loop<1, 1>(N4, lr, pg, r0, wsum, W);
-                    else if (lin) lean_list_loop<1>(N4, lr, pg, r0, wsum, W);
-                    else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, wsum, W);
-                    else lean_list_loop<4>(N4, lr, pg, r0, wsum, W);
+                const float rx = fast ? poly_reach(pg) : 0.0f;
+                const bool poly = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
+                                      : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
+                if (!poly) {
+                    const bool lin = __all(!fast || lin_q_error(pg) < kLinQTol);
+                    if (a.tensor_nu > 0) {
+                        if (lin && one) lean_tensor_loop<1, 1>(a, lr, pg, r0, wsum, W);
+                        else if (lin) lean_tensor_loop<1>(a, lr, pg, r0, wsum, W);
+                        else if (lean4) lean_tensor_loop<3>(a, lr, pg, r0, wsum, W);
+                        else lean_tensor_loop<4>(a, lr, pg, r0, wsum, W);
+                    } else {
+                        if (lin && one) lean_list_loop<1, 1>(N4, lr, pg, r0, wsum, W);
+                        else if (lin) lean_list_loop<1>(N4, lr, pg, r0, wsum, W);
+                        else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, wsum, W);
+                        else lean_list_loop<4>(N4, lr, pg, r0, wsum, W);
+                    }
                 }
             }
         } else if (lean6) {
