@@ -11,7 +11,7 @@ evaluates sigma = (s - s1) / (s1 - s0) as a polynomial in x = y / y_c - 1
 w |sigma - e^step|, from which poly_rows recovers the row weights.  These
 tests restate poly_pixel's coefficients and the remainder bound behind
 kPolyReach in float64 and compare them with s computed by arcsin, and
-check poly_rows and lean_rows against per-point hat weights.
+check poly_rows against per-point hat weights.
 """
 import math
 import os
@@ -32,7 +32,7 @@ def _source_constants():
     def arr(name):
         m = re.search(name + r"\[[^\]]*\]\s*=\s*\{([^}]*)\}", text)
         return [float(v.rstrip("f")) for v in m.group(1).replace("\n", " ").split(",") if v.strip()]
-    return {"gq": arr("kGQuartic"), "tp": arr("kTPoly"), "reach": arr("kPolyReach")}
+    return {"gq": arr("kGQuartic"), "reach": arr("kPolyReach")}
 
 
 C = _source_constants()
@@ -74,10 +74,23 @@ def geometry(D):
     return A, B, Cv, E, x0, hr
 
 
-def poly_coeffs(yc, qc, D):
-    """poly_pixel's sigma(x) coefficients c_0 .. c_D (float64)."""
+def asin_series():
+    """kAsinSeries as written in the source (fractions)."""
+    text = open(SRC).read()
+    m = re.search(r"kAsinSeries\[[^\]]*\]\s*=\s*\{([^}]*)\}", text)
+    vals = []
+    for v in m.group(1).replace("\n", " ").split(","):
+        num, _, den = v.strip().partition("/")
+        vals.append(float(num) / (float(den) if den else 1.0))
+    return vals
+
+
+def poly_coeffs(yc, s1, D):
+    """poly_pixel's sigma(x) coefficients c_0 .. c_D (float64), expanded
+    about y = yc, for row r0+1 at separation s1 (deg)."""
     g1, g2, g3, g4 = C["gq"]
-    em = math.expm1(qc * STEP)
+    G = sum(a * yc ** n for n, a in enumerate(asin_series()))
+    em = math.sqrt(yc) * G * (2 * R2D / s1) - 1.0
     inv_d0 = 1.0 / (1.0 - math.exp(-STEP))
     K = (1.0 + em) * inv_d0
     ga = yc * (g1 + yc * (2 * g2 + yc * (3 * g3 + yc * 4 * g4)))
@@ -131,18 +144,35 @@ def test_sigma_series_matches_arcsin(sc, size):
         s = 2 * R2D * np.arcsin(np.sqrt(y))
         yc = float(A @ A)
         assert np.all(np.abs(y / yc - 1) <= reach(x0, hr))
-        sc_exact = 2 * R2D * math.asin(math.sqrt(yc))
         # the pixel's first row as the kernel picks it: ilo - 1 for the
         # interval of the lowest separation
         slo = 2 * R2D * math.asin(x0 - hr)
         r0 = int(math.floor(math.log(slo / 1e-4) / STEP)) + 1
         s0, s1 = sep_node(r0), sep_node(r0 + 1)
-        qc = math.log(sc_exact / s1) / STEP
-        c = poly_coeffs(yc, qc, d)
+        c = poly_coeffs(yc, s1, d)
         x = y / yc - 1
         sig = np.polyval(c[::-1], x)
         exact = (s - s1) / (s1 - s0)
         assert np.max(np.abs(sig - exact)) < 1.5e-7, (sc, size, d)
+
+
+def test_asin_series_table():
+    """kAsinSeries holds a_n = C(2n, n) / (4^n (2n + 1)) and reaches 2e-13 at
+    the largest separation the series path takes."""
+    a = asin_series()
+    ref = [math.comb(2 * n, n) / (4 ** n * (2 * n + 1)) for n in range(len(a))]
+    np.testing.assert_allclose(a, ref, rtol=1e-15)
+    y = math.sin(math.radians(14.25)) ** 2
+    G = sum(c * y ** n for n, c in enumerate(a))
+    assert abs(G - math.asin(math.sqrt(y)) / math.sqrt(y)) < 2e-13
+
+
+def test_fast_pixels_within_reach():
+    """A pixel the pass keeps in registers spans at most three row
+    intervals (shi / slo < e^(3 step)), so its x range lies within the
+    reach of the degree-6 series."""
+    rho = (math.exp(3 * STEP) - 1) / (math.exp(3 * STEP) + 1)   # (1+rho)/(1-rho) = e^(3 step)
+    assert rho * (2 + rho) < C["reach"][-1]
 
 
 def test_reach_table_is_the_remainder_bound():
@@ -194,28 +224,3 @@ def test_poly_rows_recover_hat_weights(nm):
         A, B1, B2 = np.sum(w * sig), np.sum(w * np.abs(sig)), np.sum(w * np.abs(sig - G))
         got = poly_rows(w.sum(), A, B1, B2, G, nm)
         np.testing.assert_allclose(got, hat_rows(sig, w, G), rtol=0, atol=1e-12)
-
-
-@pytest.mark.parametrize("nm", [1, 2])
-def test_lean_rows_recover_hat_weights(nm):
-    """lean_rows in the log-separation coordinate: h = rel + t(f) with the
-    cubic t of kTPoly, rows at integer h, sums of w h', w clamp(h', 0, 1)
-    and w clamp(h' - 1, 0, 1) with h' = h - 1."""
-    t1, t2, t3 = C["tp"]
-    rng = np.random.default_rng(11 + nm)
-    for _ in range(20):
-        n = int(rng.integers(1, 80))
-        Q = rng.uniform(0, nm + 1, n)
-        w = rng.uniform(0.01, 1, n)
-        rel, f = np.floor(Q), Q - np.floor(Q)
-        t = f * (t1 + f * (t2 + f * t3))
-        W = np.zeros(4)
-        for r, tt, wq in zip(rel.astype(int), t, w):
-            W[r] += wq * (1 - tt)
-            W[r + 1] += wq * tt
-        hp = (Q - 1) + f * ((t1 - 1) + f * (t2 + f * t3))
-        A = np.sum(w * hp)
-        B1 = np.sum(w * np.clip(hp, 0, 1))
-        B2 = np.sum(w * np.clip(hp - 1, 0, 1)) if nm == 2 else 0.0
-        got = [(B1 + B2) - A, (w.sum() + A) - (2 * B1 + B2), B1 - B2, B2]
-        np.testing.assert_allclose(got, W, rtol=0, atol=1e-12)

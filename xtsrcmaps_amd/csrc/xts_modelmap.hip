@@ -77,7 +77,7 @@ __global__ void solid_angle_kernel(int H, int W, const double* __restrict__ cent
 struct SrcGeom {
     double dir[3];
     double radius_deg;
-    double chord_lo, chord_hi;  // bracket of 2 sin(radius / 2)
+    double chord2_lo, chord2_hi;  // bracket of (2 sin(radius / 2))^2
     int in_fov;
     int pad;
 };
@@ -125,8 +125,8 @@ __device__ inline void radius_block(const Wcs& w, int H, int W, int S,
         g.dir[0] = src[0]; g.dir[1] = src[1]; g.dir[2] = src[2];
         g.radius_deg = rad;
         const double c = 2.0 * sin(0.5 * kD2R * rad);
-        g.chord_lo = c * (1.0 - 1e-12);
-        g.chord_hi = c * (1.0 + 1e-12);
+        g.chord2_lo = (c * (1.0 - 1e-12)) * (c * (1.0 - 1e-12));
+        g.chord2_hi = (c * (1.0 + 1e-12)) * (c * (1.0 + 1e-12));
         g.pad = 0;
     }
 }
@@ -372,8 +372,8 @@ __device__ inline bool in_radius(const SrcGeom& g, const double* __restrict__ ce
     const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
     const double c2 = ex * ex + ey * ey + ez * ez;
     // the bracket's 1e-12 margins dwarf the rounding of the squares
-    if (c2 < g.chord_lo * g.chord_lo) return true;
-    if (c2 > g.chord_hi * g.chord_hi) return false;
+    if (c2 < g.chord2_lo) return true;
+    if (c2 > g.chord2_hi) return false;
     return in_radius_exact(c2, g.radius_deg);
 }
 
@@ -469,7 +469,8 @@ __device__ inline void stage_lut(const CubArgs& a, int s, int e0, T* lut_s, int 
     }
 }
 
-// ---- FP32 point loop for pixels away from the source.
+// ---- FP32 row and fraction of a point from the log of its separation (the
+// near pass below 14 deg).
 // With J(s) = ln(s / 1e-4 deg) / step the reference's lower row is
 // 1 + floor(J) and t = (e^(step frac(J)) - 1) / (e^step - 1)
 // (convolve_pixel_psf_logsep.hpp:46-54).  Per pixel r0 is its first row, so
@@ -481,7 +482,8 @@ __device__ inline void stage_lut(const CubArgs& a, int s, int e0, T* lut_s, int 
 // Q_c = 1 + (ln(2 R2D) + ln 1e4 + ln(y_c) / 2) / step - r0 (per pixel, FP64).
 // The relative log keeps Q accurate to ~1e-6 in FP32.  g is a least-squares
 // fit on separations below 14.5 deg (cubic, max error 5e-11) or 28.5 deg
-// (quartic, 1.2e-10); t(f) = f (t1 + f (t2 + f t3)) (max error 5e-8).
+// (quartic, 1.2e-10; its derivatives feed the series of the row-weight
+// pass); t(f) = f (t1 + f (t2 + f t3)) (max error 5e-8).
 constexpr double kLnTwoR2D = 4.741374145600756;   // ln(2 R2D)
 constexpr double kGCubic[3] = {0.16666666949401615, 0.06110791481077321, 0.03421911418748551};
 constexpr double kGQuartic[4] = {0.1666666648047744, 0.06111208423547483, 0.03360670491188797,
@@ -489,14 +491,12 @@ constexpr double kGQuartic[4] = {0.1666666648047744, 0.06111208423547483, 0.0336
 constexpr double kTPoly[3] = {0.9832291565474904, 0.016581961801669792, 0.00018883180009917163};
 
 // z(u, v) = a(u) + v (b(u) + v c(u)), each of a, b, c quadratic in u, plus
-// the Q offset and the g coefficients scaled to z: g_k y^k = g_k' z^k.
-// NG = 1 is g linearised about the pixel centre,
-//   g(y) ~ g(y_c) + g'(y_c) y_c (z - 1),
-// whose error g''(y) (y - y_c)^2 / 2 is bounded by lin_q_error.
+// the Q offset and the cubic g's coefficients scaled to z: g_k y^k =
+// g_k' z^k.
 struct LeanPixel {
     float a0, a1, a2, b0, b1, b2, c0, c1, c2;
     float q0;
-    float g[4];
+    float g[3];
 };
 
 // Natural log of a positive normal double: x = m 2^k with m in
@@ -520,11 +520,6 @@ __device__ inline double ln_pos(double x) {
     return fma((double)k, 0.69314718055994530942, 2.0 * t * p);
 }
 
-// Upper bound of g'' on separations below 28.5 deg (g'' rises from 0.1206
-// at y = 0 to 0.1366 at sin^2(14.25 deg)).
-constexpr double kG2Max = 0.14;
-
-template <int NG>
 __device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
     const float* A = pg.A; const float* B = pg.B; const float* C = pg.C; const float* E = pg.E;
     auto dot = [](const float* x, const float* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
@@ -544,81 +539,13 @@ __device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
     double q0 = 1.0 + (kLnTwoR2D + kLn1e4) * kRecipStep -
                 0.5 * kRecipStep * ln_pos((double)iy) - (double)r0;
     const double ycd = 1.0 / (double)iy;
-    if constexpr (NG == 1) {
-        const double* gc = kGQuartic;
-        const double gv = ycd * (gc[0] + ycd * (gc[1] + ycd * (gc[2] + ycd * gc[3])));
-        const double gd = gc[0] + ycd * (2.0 * gc[1] + ycd * (3.0 * gc[2] + ycd * 4.0 * gc[3]));
-        q0 += kRecipStep * (gv - ycd * gd);
-        lp.g[0] = (float)(kRecipStep * ycd * gd);
-        lp.g[1] = lp.g[2] = lp.g[3] = 0.0f;
-    } else {
-        const double* gc = NG == 3 ? kGCubic : kGQuartic;
-        double ycp = ycd;
-        for (int k = 0; k < 4; ++k) {
-            lp.g[k] = k < NG ? (float)(gc[k] * kRecipStep * ycp) : 0.0f;
-            ycp *= ycd;
-        }
+    double ycp = ycd;
+    for (int k = 0; k < 3; ++k) {
+        lp.g[k] = (float)(kGCubic[k] * kRecipStep * ycp);
+        ycp *= ycd;
     }
     lp.q0 = (float)q0;
     return lp;
-}
-
-// Bound on the error in Q of the linearised g: the points' y lie within
-// (x0 + hr)^2 - x0^2 of y_c = x0^2.
-__device__ inline float lin_q_error(const PixelGeom<float>& pg) {
-    const float dy = pg.hr * (2.0f * pg.x0 + pg.hr);
-    return (float)(0.5 * kG2Max * kRecipStep) * dy * dy;
-}
-constexpr float kLinQTol = 1e-7f;
-
-// One point.  Each point spreads its weight wq = w/4 over the rows on
-// either side of its separation: (1 - t) to the lower row, t to the upper.
-// With h = rel + t(f) its continuous row coordinate relative to r0 (rel =
-// floor(Q), f = frac(Q)), that is wq hat(h - k) for row r0 + k, hat(x) =
-// max(0, 1 - |x|).  On h in [0, NM + 1) the hats are linear combinations of
-// 1, h, and the ramps max(h - 1, 0), max(h - 2, 0), so a pixel needs only
-// the sums of wq h and of wq times the ramps (lean_rows).  The loop works
-// with h' = h - 1 = Q' + f (t(f) / f - 1), Q' = Q - 1 (the q0 of a
-// LeanPixel formed at r0 + 1), and accumulates A = sum wq h',
-// B1 = sum wq clamp(h', 0, 1), B2 = sum wq clamp(h' - 1, 0, 1).  The hats
-// are continuous in h, so rounding of Q near a row moves weight by a
-// rounding-sized amount only.  NM is the number of row boundaries the
-// pixel's points may cross: NM = 1 keeps h' < 1 and needs no B2.
-template <int NG, int NM = 2>
-__device__ inline void lean_point(float z, float wq, const LeanPixel& lp, float& A, float& B1,
-                                  float& B2) {
-    const float L = __builtin_amdgcn_logf(z);
-    float g = lp.g[NG - 1];
-#pragma unroll
-    for (int k = NG - 2; k >= 0; --k) g = fmaf(g, z, lp.g[k]);
-    const float Q = fmaf(g, z, fmaf((float)(0.5 * kRecipStep * 0.69314718055994530942), L, lp.q0));
-    const float f = __builtin_amdgcn_fractf(Q);
-    const float h = fmaf(f, fmaf(f, fmaf(f, (float)kTPoly[2], (float)kTPoly[1]),
-                                 (float)(kTPoly[0] - 1.0)), Q);
-    A = fmaf(wq, h, A);
-    if constexpr (NM >= 2) {
-        B1 = fmaf(wq, __builtin_amdgcn_fmed3f(h, 0.0f, 1.0f), B1);
-        B2 = fmaf(wq, __builtin_amdgcn_fmed3f(h - 1.0f, 0.0f, 1.0f), B2);
-    } else {
-        B1 = fmaf(wq, fmaxf(h, 0.0f), B1);
-    }
-}
-
-// Row weights W_k = sum wq hat(h - k) of rows r0 .. r0+3 from the sums of
-// lean_point (wtot = sum wq over the rule): with c1 = clamp(h', 0, 1) and
-// c2 = clamp(h' - 1, 0, 1), max(h', 0) = c1 + c2 and max(h' - 1, 0) = c2,
-// and on h in [0, 3)
-//   hat(h)     = max(h', 0) - h'
-//   hat(h - 1) = 1 + h' - 2 max(h', 0) + max(h' - 1, 0)
-//   hat(h - 2) = max(h', 0) - 2 max(h' - 1, 0)
-//   hat(h - 3) = max(h' - 1, 0) ... = 0 below h = 3; W3 = B2 is row r0+3.
-template <int NM>
-__device__ inline void lean_rows(float wtot, float A, float B1, float B2, float (&W)[4]) {
-    if constexpr (NM < 2) B2 = 0.0f;
-    W[0] = (B1 + B2) - A;
-    W[1] = (wtot + A) - fmaf(2.0f, B1, B2);
-    W[2] = B1 - B2;
-    W[3] = B2;
 }
 
 // FP32 rule as arrays [N4] of u, v and w/4 in the constant address space:
@@ -637,53 +564,6 @@ __device__ inline float lane(const float4& x, int k) {
     return k == 0 ? x.x : (k == 1 ? x.y : (k == 2 ? x.z : x.w));
 }
 
-// The loops return the row weights W of rows r0 .. r0+3 (before the solid
-// angle); wtot = the rule's sum of w/4.
-template <int NG, int NM = 2>
-__device__ inline void lean_tensor_loop(const CubArgs& a, const LeanRule& lr,
-                                        const PixelGeom<float>& pg, int r0, float wtot,
-                                        float (&W)[4]) {
-    const LeanPixel lp = lean_pixel<NG>(pg, r0 + 1);
-    const int nv = a.tensor_nv;
-    float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
-    for (int i = 0; i < a.tensor_nu; ++i) {
-        const float u = lr.p[0][i * nv];
-        const float au = fmaf(fmaf(lp.a2, u, lp.a1), u, lp.a0);
-        const float bu = fmaf(fmaf(lp.b2, u, lp.b1), u, lp.b0);
-        const float cu = fmaf(fmaf(lp.c2, u, lp.c1), u, lp.c0);
-        for (int jj = 0; jj < nv; jj += 4) {
-            const int p0 = i * nv + jj;
-            const float4 v = lean_load4(lr.p[1] + p0), wq = lean_load4(lr.p[2] + p0);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float vk = lane(v, k);
-                lean_point<NG, NM>(fmaf(fmaf(cu, vk, bu), vk, au), lane(wq, k), lp, A, B1, B2);
-            }
-        }
-    }
-    lean_rows<NM>(wtot, A, B1, B2, W);
-}
-
-template <int NG, int NM = 2>
-__device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeom<float>& pg,
-                                      int r0, float wtot, float (&W)[4]) {
-    const LeanPixel lp = lean_pixel<NG>(pg, r0 + 1);
-    float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
-    for (int p0 = 0; p0 < N4; p0 += 4) {
-        const float4 u = lean_load4(lr.p[0] + p0), v = lean_load4(lr.p[1] + p0),
-                     wq = lean_load4(lr.p[2] + p0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float uk = lane(u, k), vk = lane(v, k);
-            const float au = fmaf(fmaf(lp.a2, uk, lp.a1), uk, lp.a0);
-            const float bu = fmaf(fmaf(lp.b2, uk, lp.b1), uk, lp.b0);
-            const float cu = fmaf(fmaf(lp.c2, uk, lp.c1), uk, lp.c0);
-            lean_point<NG, NM>(fmaf(fmaf(cu, vk, bu), vk, au), lane(wq, k), lp, A, B1, B2);
-        }
-    }
-    lean_rows<NM>(wtot, A, B1, B2, W);
-}
-
 // ---- FP32 point loop without a logarithm, for pixels whose points lie
 // close to the pixel centre relative to their separation.
 // The row weights are linear in the separation s between rows (weight
@@ -692,12 +572,13 @@ __device__ inline void lean_list_loop(int N4, const LeanRule& lr, const PixelGeo
 //   sigma = (s - s1) / (s1 - s0)
 // they are linear combinations of 1, sigma, |sigma| and |sigma - G|,
 // G = e^step = (s2 - s1) / (s1 - s0) (poly_rows).  sigma is a polynomial in
-// x = z - 1 (z = y / y_c as in LeanPixel, x quadratic in u and v with no
-// constant term beyond the rounding of y_c iy):
-//   s / s1 = e^(step q_c) sqrt(1 + x) exp(g(y_c (1 + x)) - g(y_c)),
-// q_c = Q - 1 at the pixel centre, g as above; the Taylor series of the
-// product to degree D times e^(step q_c) / (1 - e^-step), plus the constant
-// expm1(step q_c) / (1 - e^-step), is sigma(x).  A point costs the two FMAs
+// x = y iy - 1 (iy ~ 1 / y_c, x quadratic in u and v with no constant term
+// beyond the rounding of y_c iy).  With y_e = 1 / iy and s_e its separation,
+//   s / s1 = (s_e / s1) sqrt(1 + x) exp(g(y_e (1 + x)) - g(y_e)),
+// g as above; the Taylor series of the product to degree D times
+// (s_e / s1) / (1 - e^-step), plus the constant (s_e / s1 - 1) /
+// (1 - e^-step), is sigma(x); s_e / s1 - 1 is formed in FP64 from the
+// arcsine series and the table of 2 R2D / sep.  A point costs the two FMAs
 // of x, D of the polynomial and two (three with NM = 2) sums.
 // Remainder: the binomial series of sqrt(1 + x) bounds it on |x| <= r by
 // 1.1 K |binom(1/2, D + 1)| r^(D+1) / (1 - r), K = e^(2 step) /
@@ -718,16 +599,22 @@ struct PolyPixel {
 // Bound r on |x| over the pixel's points: |p / 2| lies in [x0 - hr, x0 + hr],
 // so z lies in [(1 - rho)^2, (1 + rho)^2] with rho = hr / x0.
 __device__ inline float poly_reach(const PixelGeom<float>& pg) {
-    const float rho = pg.hr / pg.x0;
+    const float rho = pg.hr * __builtin_amdgcn_rcpf(pg.x0) * 1.000001f;
     return rho * (2.0f + rho);
 }
 
+// asin(sqrt y) / sqrt y = sum a_n y^n, a_n = C(2n, n) / (4^n (2n + 1)); to
+// y^8 the remainder is below 2e-13 on y <= sin^2(14.25 deg).
+constexpr double kAsinSeries[9] = {1.0, 1.0 / 6, 3.0 / 40, 5.0 / 112, 35.0 / 1152, 63.0 / 2816,
+                                   231.0 / 13312, 143.0 / 10240, 6435.0 / 557056};
+
 template <int D>
-__device__ inline PolyPixel poly_pixel(const PixelGeom<float>& pg, int r0) {
+__device__ inline PolyPixel poly_pixel(const CubArgs& a, const PixelGeom<float>& pg, int r0) {
     const float* A = pg.A; const float* B = pg.B; const float* C = pg.C; const float* E = pg.E;
     auto dot = [](const float* x, const float* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
     const float yc = dot(A, A);
-    const float iy = 1.0f / yc;
+    // any iy near 1 / yc serves: the series is expanded about y_e = 1 / iy
+    const float iy = __builtin_amdgcn_rcpf(yc);
     PolyPixel pp;
     pp.x0 = fmaf(yc, iy, -1.0f);
     pp.x1 = 2.0f * dot(A, B) * iy;
@@ -738,36 +625,39 @@ __device__ inline PolyPixel poly_pixel(const PixelGeom<float>& pg, int r0) {
     pp.c0 = dot(C, C) * iy;
     pp.c1 = 2.0f * dot(C, E) * iy;
     pp.c2 = dot(E, E) * iy;
-    // q_c = Q - 1 at z = 1 (y = 1 / iy), in FP64; |step q_c| < 0.07
-    const double ycd = 1.0 / (double)iy;
-    const double* gc = kGQuartic;
-    const double gv = ycd * (gc[0] + ycd * (gc[1] + ycd * (gc[2] + ycd * gc[3])));
-    const double qc = (kLnTwoR2D + kLn1e4 + gv) * kRecipStep -
-                      0.5 * kRecipStep * ln_pos((double)iy) - (double)r0;
-    const double e = qc * kSepStep;
-    // expm1(e) = e (1 + e/2 (1 + e/3 (1 + ... (1 + e/8)))), remainder < 1e-16
-    double em = 1.0;
+    // s_e / s1 - 1 in FP64, s_e = 2 asin(sqrt y_e): y_e by one Newton step
+    // on yc, sqrt y_e by one on the FP32 reciprocal square root
+    const double yd = (double)yc;
+    const double ye = yd * fma(-(double)iy, yd, 2.0);
+    const double rs = (double)__builtin_amdgcn_rsqf(yc);
+    double sq = ye * rs;
+    sq = fma(fma(-sq, sq, ye), 0.5 * rs, sq);
+    double G = kAsinSeries[8];
 #pragma unroll
-    for (int k = 8; k >= 2; --k) em = fma(em, e * (1.0 / k), 1.0);
-    em *= e;
+    for (int k = 7; k >= 0; --k) G = fma(G, ye, kAsinSeries[k]);
+    // inv_sep64[k] = 2 R2D / sep[k]
+    const double em = fma(sq * G, a.inv_sep64[r0 + 1], -1.0);
     pp.c[0] = (float)(em * kInvD0);
+    // exp(g(y_e (1 + x)) - g(y_e)) = 1 + e1 x + e2 x^2 + e3 x^3 + ..., times
+    // K = (s_e / s1) / (1 - e^-step)
     const float K = (float)((1.0 + em) * kInvD0);
-    // exp(g(y_c (1 + x)) - g(y_c)) = 1 + e1 x + e2 x^2 + e3 x^3 + ...
-    const float yf = (float)ycd;
-    const float g1 = (float)gc[0], g2 = (float)gc[1], g3 = (float)gc[2], g4 = (float)gc[3];
+    const float yf = (float)ye;
+    const float g1 = (float)kGQuartic[0], g2 = (float)kGQuartic[1], g3 = (float)kGQuartic[2],
+                g4 = (float)kGQuartic[3];
     const float ga = yf * (g1 + yf * (2.0f * g2 + yf * (3.0f * g3 + yf * 4.0f * g4)));
     const float gb = yf * yf * (g2 + yf * (3.0f * g3 + yf * 6.0f * g4));
     const float gcb = yf * yf * yf * (g3 + yf * 4.0f * g4);
-    const float ex[4] = {1.0f, ga, gb + 0.5f * ga * ga, gcb + ga * gb + ga * ga * ga / 6.0f};
+    const float ex[4] = {K, K * ga, K * (gb + 0.5f * ga * ga),
+                         K * (gcb + ga * gb + ga * ga * ga * (1.0f / 6.0f))};
     // binomial series of sqrt(1 + x)
     constexpr float bs[kPolyMaxD + 1] = {1.0f, 0.5f, -0.125f, 0.0625f, -0.0390625f,
                                          0.02734375f, -0.0205078125f};
 #pragma unroll
-    for (int k = 1; k <= kPolyMaxD; ++k) {
+    for (int k = 1; k <= D; ++k) {
         float r = 0.0f;
 #pragma unroll
         for (int j = 0; j <= 3 && j <= k; ++j) r = fmaf(bs[k - j], ex[j], r);
-        pp.c[k] = k <= D ? K * r : 0.0f;
+        pp.c[k] = r;
     }
     return pp;
 }
@@ -808,7 +698,7 @@ template <int D, int NM>
 __device__ inline void poly_tensor_loop(const CubArgs& a, const LeanRule& lr,
                                         const PixelGeom<float>& pg, int r0, float wtot,
                                         float (&W)[4]) {
-    const PolyPixel pp = poly_pixel<D>(pg, r0);
+    const PolyPixel pp = poly_pixel<D>(a, pg, r0);
     const int nv = a.tensor_nv;
     float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
     for (int i = 0; i < a.tensor_nu; ++i) {
@@ -830,9 +720,10 @@ __device__ inline void poly_tensor_loop(const CubArgs& a, const LeanRule& lr,
 }
 
 template <int D, int NM>
-__device__ inline void poly_list_loop(int N4, const LeanRule& lr, const PixelGeom<float>& pg,
-                                      int r0, float wtot, float (&W)[4]) {
-    const PolyPixel pp = poly_pixel<D>(pg, r0);
+__device__ inline void poly_list_loop(const CubArgs& a, int N4, const LeanRule& lr,
+                                      const PixelGeom<float>& pg, int r0, float wtot,
+                                      float (&W)[4]) {
+    const PolyPixel pp = poly_pixel<D>(a, pg, r0);
     float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
     for (int p0 = 0; p0 < N4; p0 += 4) {
         const float4 u = lean_load4(lr.p[0] + p0), v = lean_load4(lr.p[1] + p0),
@@ -853,7 +744,7 @@ template <int D, int NM>
 __device__ inline void poly_loop(const CubArgs& a, const LeanRule& lr, const PixelGeom<float>& pg,
                                  int r0, int N4, float wtot, float (&W)[4]) {
     if (a.tensor_nu > 0) poly_tensor_loop<D, NM>(a, lr, pg, r0, wtot, W);
-    else poly_list_loop<D, NM>(N4, lr, pg, r0, wtot, W);
+    else poly_list_loop<D, NM>(a, N4, lr, pg, r0, wtot, W);
 }
 
 // Runs the polynomial loop of the least degree whose reach covers every
@@ -955,30 +846,20 @@ rowweight_kernel(CubArgs a) {
         // FP64 indexes the table per point and keeps every lane in range.
         const bool lean6 = sizeof(T) == 4 ? __any(fast) && __all(!fast || pg.shi < T(28.0))
                                           : __all(fast && pg.shi < T(28.0));
-        const bool lean4 = lean6 && __all(!fast || pg.shi < T(14.0));
+        bool series = false;   // the wave's row weights came from a point loop below
         if constexpr (sizeof(T) == 4) {
             if (lean6) {
                 // every fast pixel's points within three rows (one boundary)
                 const bool one = __all(!fast || ihi - ilo <= 1);
                 const float rx = fast ? poly_reach(pg) : 0.0f;
-                const bool poly = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
-                                      : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
-                if (!poly) {
-                    const bool lin = __all(!fast || lin_q_error(pg) < kLinQTol);
-                    if (a.tensor_nu > 0) {
-                        if (lin && one) lean_tensor_loop<1, 1>(a, lr, pg, r0, wsum, W);
-                        else if (lin) lean_tensor_loop<1>(a, lr, pg, r0, wsum, W);
-                        else if (lean4) lean_tensor_loop<3>(a, lr, pg, r0, wsum, W);
-                        else lean_tensor_loop<4>(a, lr, pg, r0, wsum, W);
-                    } else {
-                        if (lin && one) lean_list_loop<1, 1>(N4, lr, pg, r0, wsum, W);
-                        else if (lin) lean_list_loop<1>(N4, lr, pg, r0, wsum, W);
-                        else if (lean4) lean_list_loop<3>(N4, lr, pg, r0, wsum, W);
-                        else lean_list_loop<4>(N4, lr, pg, r0, wsum, W);
-                    }
-                }
+                // a fast pixel spans at most three row intervals, shi / slo <
+                // e^(3 step), so rho < 0.051 and r < 0.104: degree 6 always
+                // reaches; a wave it did not would take the per-point path
+                series = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
+                             : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
             }
         } else if (lean6) {
+            series = true;
             // FP64: exact asin, row from the log of the separation
             for (int p0 = 0; p0 < N4; p0 += 4) {
                 V4 r[4];
@@ -998,7 +879,7 @@ rowweight_kernel(CubArgs a) {
             T01 += T0;
             rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
         }
-        if (!lean6 && __any(fast)) {
+        if (!series && __any(fast)) {
             // ---- far pixels beyond 28 degrees or sharing a wave with near ones
             for (int p = 0; p < a.N; ++p) {
                 T u, v, wq;
@@ -1136,7 +1017,7 @@ near_rows_kernel(CubArgs a) {
         if constexpr (sizeof(T) == 4) lean = narrow && pg.slo >= T(1e-4) && pg.shi < T(14.0);
         if (lean) {
             if constexpr (sizeof(T) == 4) {
-                const LeanPixel lp = lean_pixel<3>(pg, rbase);
+                const LeanPixel lp = lean_pixel(pg, rbase);
 #pragma unroll 4
                 for (int p = part; p < a.N; p += kNearSplit) {
                     const V4 rr = rule_s[p];
