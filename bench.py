@@ -65,6 +65,13 @@ def setup(cfg_name: str):
         parint=torch.empty((S, 401, E), dtype=torch.float64, device=dev),
         out=torch.empty((S, E, H, W), dtype=torch.float32, device=dev),
     )
+    # the PSF stages on a second context and stream, beside the model map's
+    # row-weight half (which needs no PSF); both streams at default priority
+    d["stream"] = stream
+    d["psf_stream"] = torch.cuda.Stream()
+    d["psf_ctx"] = xa.Context(0)
+    d["psf_ctx"].set_stream(d["psf_stream"].cuda_stream)
+    d["serial"] = False
     torch.cuda.synchronize()
     return cfg, inp, ctx, d
 
@@ -82,22 +89,42 @@ def measured_traffic(config: str):
     return p["traffic_bytes_per_call"] if p else None
 
 
-def step(ctx, inp, d, precision=0):
-    import xtsrcmaps_amd as xa
+def psf_stages(pctx, inp, d, precision=0):
+    """Cosine bins, exposure and mean PSF on context pctx (its stream)."""
     from xtsrcmaps_amd import capi
     S = d["src"].shape[0]
     L = capi.lib()
-    capi.check(L.xts_src_cosbins_dev(ctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["cube"]),
+    capi.check(L.xts_src_cosbins_dev(pctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["cube"]),
                                      inp.nside, d["cube"].shape[1], capi.tptr(d["ecb"])))
-    capi.check(L.xts_src_cosbins_dev(ctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["wcube"]),
+    capi.check(L.xts_src_cosbins_dev(pctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["wcube"]),
                                      inp.nside, d["wcube"].shape[1], capi.tptr(d["wecb"])))
-    ctx.exposure_dev(d["ecb"], d["wecb"], d["af"], d["ab"], d["ltf"], d["expo"], precision)
-    ctx.psf_lookup_dev(d["king"]["front"], d["king"]["back"], d["costh"], d["logE"], d["af"],
-                       d["ab"], d["ecb"], d["wecb"], d["ltf"], d["expo"], d["upsf"], d["parint"],
-                       precision)
-    ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
-                                    d["out"], inp.rule,
-                                    flags=xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32)
+    pctx.exposure_dev(d["ecb"], d["wecb"], d["af"], d["ab"], d["ltf"], d["expo"], precision)
+    pctx.psf_lookup_dev(d["king"]["front"], d["king"]["back"], d["costh"], d["logE"], d["af"],
+                        d["ab"], d["ecb"], d["wecb"], d["ltf"], d["expo"], d["upsf"], d["parint"],
+                        precision)
+
+
+def step(ctx, inp, d, precision=0):
+    """One pass of the hot path: exposure, mean PSF and the model map.  The
+    model map's row-weight half runs on ctx's stream beside the PSF stages on
+    the second context's stream; its second half waits for them.  With
+    d["serial"], everything runs on ctx's stream, the model map in one call."""
+    import xtsrcmaps_amd as xa
+    flags = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32
+    if d["serial"]:
+        psf_stages(ctx, inp, d, precision)
+        ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                        d["out"], inp.rule, flags=flags)
+        return
+    sa, sb = d["stream"], d["psf_stream"]
+    # the previous step's second half has read expo, upsf and parint
+    sb.wait_stream(sa)
+    ctx.point_src_model_map_rows_dev(inp.wcs, d["src"], d["expo"].shape[1], inp.rule,
+                                     flags=flags)
+    psf_stages(d["psf_ctx"], inp, d, precision)
+    sa.wait_stream(sb)
+    ctx.point_src_model_map_finish_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                       d["out"], inp.rule, flags=flags)
 
 
 def cpu_baseline(cfg, inp, gpu_map):
@@ -129,6 +156,8 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 BF16x3 contractions")
+    ap.add_argument("--serial", action="store_true",
+                    help="one stream, the model map in one call (no overlap with the PSF)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -138,6 +167,7 @@ def main():
     import torch
     _log(f"bench: setup {args.config}")
     cfg, inp, ctx, d = setup(args.config)
+    d["serial"] = args.serial
     _log("bench: warmup")
     for _ in range(args.warmup):
         step(ctx, inp, d, args.precision)
@@ -202,7 +232,8 @@ def main():
                                f"CAR {cfg.cdelt} deg, {cfg.n_energy} energies, 8x8 Gauss-"
                                f"Legendre pixel cubature, FP32 [S,E,H,W] map",
                    "sources": cfg.n_src, "npix": cfg.npix, "energies": cfg.n_energy,
-                   "rule": cfg.rule, "contraction": "bf16x3" if args.precision else "fp64"},
+                   "rule": cfg.rule, "contraction": "bf16x3" if args.precision else "fp64",
+                   "streams": 1 if args.serial else 2},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg.name),
                      "kernel": "model-map passes (rowweight_kernel, near_rows_kernel, "

@@ -223,6 +223,26 @@ int xts_point_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* wcs,
                                     const double* parint, int64_t E,
                                     const double* rule, int64_t N,
                                     int32_t flags, void* out);
+/* xts_point_src_model_map_wcs_dev in two calls on the context's stream.
+ * The first needs only the image, the sources and the rule: pixel geometry,
+ * boundary radii, the per-pixel row weights of the PSF table and the
+ * near-pixel rows.  The second (same arguments plus the PSF table, exposure
+ * and output) converts the table, forms the map integral and correction
+ * factor, and writes the map.  A caller that computes the PSF on another
+ * stream (another context) can run it beside the first half, then make this
+ * context's stream wait for it before the second.  Between the two calls the
+ * context runs no other model map.  The result equals the one-call entry
+ * point's.  No counterpart in the reference, whose model map is one function
+ * (model_map/model_map.point_src_model_map_wcs.cxx:6-45). */
+int xts_point_src_model_map_rows_dev(xts_ctx* ctx, const xts_wcs* wcs,
+                                     const double* src_sph, int64_t S, int64_t E,
+                                     const double* rule, int64_t N, int32_t flags);
+int xts_point_src_model_map_finish_dev(xts_ctx* ctx, const xts_wcs* wcs,
+                                       const double* src_sph, int64_t S,
+                                       const double* exposure, const double* upsf,
+                                       const double* parint, int64_t E,
+                                       const double* rule, int64_t N,
+                                       int32_t flags, void* out);
 
 /* cubature_convolve_map_psf (mm_cubature/cubature.implementation.cxx:21-55):
  * the raw cubature only. */

@@ -378,6 +378,66 @@ sys.exit(0 if ok and rel < 1e-5 else 1)
 """
 
 
+_SPLIT_CHILD = r"""
+import sys
+import torch
+sys.path.insert(0, sys.argv[1])
+import bench
+import xtsrcmaps_amd as xa
+cfg, inp, ctx, d = bench.setup("c1")
+flags = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32
+E = cfg.n_energy
+for _ in range(2):
+    bench.step(ctx, inp, d)          # row weights beside the PSF on a second stream
+torch.cuda.synchronize()
+a = d["out"].clone()
+ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"], d["out"],
+                                inp.rule, flags=flags)
+torch.cuda.synchronize()
+b = d["out"].clone()
+ctx.point_src_model_map_rows_dev(inp.wcs, d["src"], E, inp.rule, flags=flags)
+ctx.point_src_model_map_finish_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                   d["out"], inp.rule, flags=flags)
+torch.cuda.synchronize()
+c = d["out"].clone()
+rel = lambda x, y: float(((x - y).abs() / y.abs()).max())
+print(f"two-stream vs one call {rel(a, b):.2e}, split vs one call {rel(c, b):.2e}", flush=True)
+# the second half without a first half with the same arguments is refused
+# before any launch
+refused = 0
+try:
+    ctx.point_src_model_map_finish_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                       d["out"], inp.rule, flags=flags)
+except Exception:
+    refused += 1
+ctx.point_src_model_map_rows_dev(inp.wcs, d["src"], E, inp.rule, flags=flags)
+try:
+    ctx.point_src_model_map_finish_dev(inp.wcs, d["src"][:50], d["expo"][:50], d["upsf"][:50],
+                                       d["parint"][:50], d["out"], inp.rule, flags=flags)
+except Exception:
+    refused += 1
+print(f"refused {refused}", flush=True)
+ok = rel(a, b) < 2e-7 and rel(c, b) < 2e-7 and refused == 2
+sys.exit(0 if ok else 1)
+"""
+
+
+def test_gpu_model_map_split_halves():
+    """The model map in two calls (row weights, then table, factor and emit)
+    equals the one-call entry point, also with the PSF stages computed on a
+    second context and stream between the halves, as the bench runs them;
+    atomically accumulated map integrals may differ in the last bit, hence
+    2e-7.  A second half without a matching first half is refused.  (Child
+    process: torch before the library.)"""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    _log("test_gpu_model_map_split_halves: c1, two streams (child process)")
+    r = subprocess.run([sys.executable, "-c", _SPLIT_CHILD, root], capture_output=True,
+                       text=True, timeout=240)
+    _log("  " + r.stdout.strip()[-300:])
+    assert r.returncode == 0, r.stdout[-1000:] + r.stderr[-2000:]
+
+
 def test_gpu_model_map_c3p_full_size_properties():
     """2,000 sources on a 400x400 image, 37 energies: 1.18e10 values, a 47 GB
     FP32 map, through the device entry points as the bench runs them.  Every

@@ -414,3 +414,23 @@ class Context:
         capi.check(capi.lib().xts_point_src_model_map_wcs_dev(
             self._h, C.byref(wcs), capi.tptr(src_sph), S, capi.tptr(exposure), capi.tptr(upsf),
             capi.tptr(parint), E, rp, N, int(flags), C.c_void_p(out.data_ptr())))
+
+    def point_src_model_map_rows_dev(self, wcs: Wcs, src_sph, E, rule=None,
+                                     flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
+        """First half of point_src_model_map_wcs_dev: the work that needs only
+        the image, the sources and the rule (row weights), enqueued on the
+        context stream.  Keep `rule` alive and unchanged until the second half."""
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_point_src_model_map_rows_dev(
+            self._h, C.byref(wcs), capi.tptr(src_sph), src_sph.shape[0], int(E), rp, N,
+            int(flags)))
+
+    def point_src_model_map_finish_dev(self, wcs: Wcs, src_sph, exposure, upsf, parint, out,
+                                       rule=None, flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
+        """Second half: table conversion, map integral, correction factor and
+        the map, after point_src_model_map_rows_dev with the same arguments."""
+        S, E = exposure.shape
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_point_src_model_map_finish_dev(
+            self._h, C.byref(wcs), capi.tptr(src_sph), S, capi.tptr(exposure), capi.tptr(upsf),
+            capi.tptr(parint), E, rp, N, int(flags), C.c_void_p(out.data_ptr())))

@@ -98,6 +98,10 @@ _SIGS = {
                                     C.c_int32, _vp],
     "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
                                         _d, _i64, C.c_int32, _vp],
+    "xts_point_src_model_map_rows_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _i64, _d, _i64,
+                                         C.c_int32],
+    "xts_point_src_model_map_finish_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
+                                           _d, _i64, C.c_int32, _vp],
     "xts_cubature_convolve_map_psf": [_vp, C.POINTER(Wcs), _d, _i64, _d, _i64, _d, _i64,
                                       C.c_int32, _vp],
     "xts_solid_angle": [_vp, C.POINTER(Wcs), _d],

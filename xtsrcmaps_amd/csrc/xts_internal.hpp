@@ -105,6 +105,16 @@ struct xts_ctx {
     xts_rule_cache rule;
     bool geom_valid = false;               // pixel geometry in kSlotGeom is for geom_wcs
     xts_wcs geom_wcs{};
+    // arguments of a model map whose row-weight half has been enqueued and
+    // whose second half has not
+    struct {
+        bool valid = false;
+        xts_wcs wcs{};
+        const double* src = nullptr;
+        const double* rule = nullptr;
+        int64_t S = 0, E = 0, N = 0;
+        int32_t flags = 0;
+    } pending_rows;
 };
 
 namespace xts {

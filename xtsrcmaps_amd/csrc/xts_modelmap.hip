@@ -1647,7 +1647,7 @@ __device__ inline void lut_convert_part(const double* __restrict__ upsf, int S, 
                                         float* __restrict__ out32, double* __restrict__ out64,
                                         double* __restrict__ zero, size_t nzero, size_t i0,
                                         size_t step) {
-    const size_t n = (size_t)S * kND * Ep;
+    const size_t n = (out32 || out64) ? (size_t)S * kND * Ep : 0;
     for (size_t k = i0; k < n; k += step) {
         const int e = (int)(k % Ep);
         const size_t sd = k / Ep;
@@ -1661,17 +1661,18 @@ __device__ inline void lut_convert_part(const double* __restrict__ upsf, int S, 
 // Both in one launch (they are independent): blocks 0 .. S take the
 // boundary radii and field-of-view flags, the rest the table conversion.
 __global__ void __launch_bounds__(256)
-radius_lut_kernel(Wcs w, int H, int W, int S, const double* __restrict__ src_sph,
+radius_lut_kernel(Wcs w, int H, int W, int S, int nrad, const double* __restrict__ src_sph,
                   const double* __restrict__ corners, SrcGeom* __restrict__ geo,
                   const double* __restrict__ upsf, int E, int Ep, float* __restrict__ out32,
                   double* __restrict__ out64, double* __restrict__ zero, size_t nzero) {
-    if ((int)blockIdx.x <= S) {
+    // blocks 0 .. nrad-1 (nrad = S + 1 or 0): radii and field-of-view flags
+    if ((int)blockIdx.x < nrad) {
         radius_block(w, H, W, S, src_sph, corners, geo, blockIdx.x);
         return;
     }
     lut_convert_part(upsf, S, E, Ep, out32, out64, zero, nzero,
-                     (size_t)(blockIdx.x - S - 1) * blockDim.x + threadIdx.x,
-                     (size_t)(gridDim.x - S - 1) * blockDim.x);
+                     (size_t)(blockIdx.x - nrad) * blockDim.x + threadIdx.x,
+                     (size_t)(gridDim.x - nrad) * blockDim.x);
 }
 
 // The rule as per-point records on the device, with its bilinear-weight
@@ -1745,16 +1746,23 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
 // The device pipeline shared by the host and _dev entry points.
 //   full = true : model map (x omega, x exposure x correction)
 //   full = false: raw cubature (cubature_convolve_map_psf)
+// Phases of one model map: kRows needs only the image, the sources and the
+// rule (geometry, radii, row weights, near-pixel rows); kFinish the PSF
+// table, exposure and output (table conversion, map integral and factor,
+// emit).  kAll runs both in one call.
+enum { kPhaseRows = 1, kPhaseFinish = 2, kPhaseAll = 3 };
+
 static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                             const double* expo, const double* upsf, const double* parint,
                             int64_t E, const double* rule, int64_t N, int32_t flags, void* out,
-                            bool full) {
+                            bool full, int phase = kPhaseAll) {
     XTS_TRY(check_wcs(wcs));
     // (S E < 2^31: source-energy indices are 32-bit in the kernels)
     XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096 && S * E < (1LL << 31),
               "bad S=%lld or E=%lld", (long long)S, (long long)E);
-    XTS_CHECK(src_sph && upsf && out, "null device pointer");
-    XTS_CHECK(!full || (expo && parint), "null exposure/parint");
+    XTS_CHECK(src_sph, "null device pointer");
+    XTS_CHECK(!(phase & kPhaseFinish) || (upsf && out), "null device pointer");
+    XTS_CHECK(!(phase & kPhaseFinish) || !full || (expo && parint), "null exposure/parint");
     XTS_CHECK((flags & ~0x7) == 0, "unknown flags 0x%x", flags);
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     XTS_CHECK((int64_t)H * W <= (int64_t)1 << 30, "image of %lld pixels exceeds 2^30",
@@ -1789,10 +1797,18 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
     double* acc = (double*)accp;
     const size_t nzero = full ? cnt_d + (size_t)S * kND : cnt_d;
-    radius_lut_kernel<<<(unsigned)(S + 1) + std::min<unsigned>(div_up(lut_elems, 256), 8192), 256,
-                        0, ctx->stream>>>(wcs_setup(*wcs), H, W, (int)S, src_sph, ig.corners, ig.geo,
-                                          upsf, (int)E, Ep, f64 ? nullptr : (float*)lut,
-                                          f64 ? (double*)lut : nullptr, acc, nzero);
+    {
+        // radii and cleared accumulators before the row weights; the table
+        // once the PSF is known (one launch when both are)
+        const bool rad = phase & kPhaseRows, conv = phase & kPhaseFinish;
+        const unsigned nrad = rad ? (unsigned)(S + 1) : 0u;
+        const unsigned nconv = std::min<unsigned>(div_up(conv ? lut_elems : nzero, 256), 8192);
+        radius_lut_kernel<<<nrad + std::max(nconv, 1u), 256, 0, ctx->stream>>>(
+            wcs_setup(*wcs), H, W, (int)S, (int)nrad, src_sph, ig.corners, ig.geo,
+            conv ? upsf : nullptr, (int)E, Ep, conv && !f64 ? (float*)lut : nullptr,
+            conv && f64 ? (double*)lut : nullptr, acc, rad ? nzero : 0);
+        XTS_HIP(hipGetLastError());
+    }
 
     // per-pixel row weights and first rows; listed (near) pixels, their pool
     // offsets and row ranges
@@ -1854,7 +1870,9 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.tiles_per_block = (int)std::min<int64_t>(std::max<int64_t>(tpb, 1), ntiles);
     const int N4 = (int)((N + 3) / 4 * 4);
     const size_t tsz = f64 ? 8 : 4;
-    {
+    const int64_t near_gx = std::min<int64_t>(std::max<int64_t>(1, 2 * want_blocks / S),
+                                              div_up(plane, kNearPix));
+    if (phase & kPhaseRows) {
         const size_t sm = (size_t)kNDPad * 8 +
                           (f64 ? (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz : 0);
         const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
@@ -1864,9 +1882,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         XTS_HIP(hipGetLastError());
     }
     // pass 2a: blocks stride over each source's list
-    const int64_t near_gx = std::min<int64_t>(std::max<int64_t>(1, 2 * want_blocks / S),
-                                              div_up(plane, kNearPix));
-    {
+    if (phase & kPhaseRows) {
         const size_t sm = (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz + (size_t)N4 * 16 +
                           (size_t)kNearRows * kNearThreads * tsz;
         const dim3 grid((unsigned)near_gx, (unsigned)S);
@@ -1880,6 +1896,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         }
         XTS_HIP(hipGetLastError());
     }
+    if (!(phase & kPhaseFinish)) return XTS_OK;
     if (full) {
         TimedRegion tr(ctx, "factor");
         hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>(
@@ -1925,6 +1942,45 @@ int xts_point_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* wcs, const doub
                                     int64_t N, int32_t flags, void* out) {
     XTS_TRY(activate(ctx));
     return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out, true);
+}
+
+int xts_point_src_model_map_rows_dev(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph,
+                                     int64_t S, int64_t E, const double* rule, int64_t N,
+                                     int32_t flags) {
+    XTS_TRY(activate(ctx));
+    ctx->pending_rows.valid = false;
+    XTS_TRY(model_map_device(ctx, wcs, src_sph, S, nullptr, nullptr, nullptr, E, rule, N, flags,
+                             nullptr, true, kPhaseRows));
+    auto& p = ctx->pending_rows;
+    p.wcs = *wcs;
+    p.src = src_sph;
+    p.rule = rule;
+    p.S = S; p.E = E; p.N = N;
+    p.flags = flags;
+    p.valid = true;
+    return XTS_OK;
+}
+
+int xts_point_src_model_map_finish_dev(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph,
+                                       int64_t S, const double* expo, const double* upsf,
+                                       const double* parint, int64_t E, const double* rule,
+                                       int64_t N, int32_t flags, void* out) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(wcs != nullptr, "null wcs");
+    auto& p = ctx->pending_rows;
+    // the rule by content: the row-weight half uploaded it, and a wrapper may
+    // pass an equal copy
+    const bool same_rule =
+        rule ? (p.rule != nullptr && ctx->rule.host.size() == (size_t)(3 * N) &&
+                std::equal(rule, rule + 3 * N, ctx->rule.host.begin()))
+             : p.rule == nullptr;
+    XTS_CHECK(p.valid && std::memcmp(&p.wcs, wcs, sizeof(xts_wcs)) == 0 && p.src == src_sph &&
+                  same_rule && p.S == S && p.E == E && p.N == N && p.flags == flags,
+              "xts_point_src_model_map_finish_dev: no row-weight half with these arguments "
+              "(xts_point_src_model_map_rows_dev)");
+    p.valid = false;
+    return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out,
+                            true, kPhaseFinish);
 }
 
 int xts_point_src_model_map_wcs(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph,
