@@ -187,14 +187,19 @@ def main():
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / args.steps
     # per-stage device time from HIP events on the library's stream (the
-    # stream every kernel of the step is launched on), in a second pass so
-    # that the events do not perturb the headline timing
-    _log(f"bench: stage timing, {args.steps} steps")
+    # stream the model-map kernels are launched on), in a second pass so that
+    # the events do not perturb the headline timing, and with the step on one
+    # stream: beside the PSF stages the row-weight pass shares the CUs, and
+    # its events would time the sharing rather than the kernel
+    _log(f"bench: stage timing, {args.steps} steps (one stream)")
+    serial = d["serial"]
+    d["serial"] = True
     ctx.enable_timing(True)
     for name in STAGES:
         ctx.kernel_times(name)
     for _ in range(args.steps):
         step(ctx, inp, d, args.precision)
+    d["serial"] = serial
     stage_ms = {}
     for name in STAGES:
         t, n = ctx.kernel_times(name)
