@@ -91,13 +91,8 @@ def measured_traffic(config: str):
 
 def psf_stages(pctx, inp, d, precision=0):
     """Cosine bins, exposure and mean PSF on context pctx (its stream)."""
-    from xtsrcmaps_amd import capi
-    S = d["src"].shape[0]
-    L = capi.lib()
-    capi.check(L.xts_src_cosbins_dev(pctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["cube"]),
-                                     inp.nside, d["cube"].shape[1], capi.tptr(d["ecb"])))
-    capi.check(L.xts_src_cosbins_dev(pctx.handle, capi.tptr(d["src"]), S, capi.tptr(d["wcube"]),
-                                     inp.nside, d["wcube"].shape[1], capi.tptr(d["wecb"])))
+    pctx.src_cosbins_dev(d["src"], d["cube"], inp.nside, d["ecb"])
+    pctx.src_cosbins_dev(d["src"], d["wcube"], inp.nside, d["wecb"])
     pctx.exposure_dev(d["ecb"], d["wecb"], d["af"], d["ab"], d["ltf"], d["expo"], precision)
     pctx.psf_lookup_dev(d["king"]["front"], d["king"]["back"], d["costh"], d["logE"], d["af"],
                         d["ab"], d["ecb"], d["wecb"], d["ltf"], d["expo"], d["upsf"], d["parint"],

@@ -145,12 +145,20 @@ int main(int argc, char** argv) {
                 c->thetabin = true;
             }
         }
-        Fermi::Source::SourceData<Fermi::Source::PointSource> src;
+        // the sources as the reference's source-model reader leaves them
+        // (Source::XtSrc; main.cxx passes src.point into the path)
+        Fermi::Config::XtCfg cfg;
+        Fermi::Source::XtSrc xsrc;
+        auto& src = xsrc.point;
         src.sph_locs = tensor<2>(in.at("src_sph"));
         for (size_t s = 0; s < src.sph_locs.extent(0); ++s) {
-            src.srcs.push_back({"src" + std::to_string(s), 0.0f});
-            src.names.push_back("src" + std::to_string(s));
+            Fermi::Source::PointSource ps;
+            ps.name = "src" + std::to_string(s);
+            ps.spectrum = Fermi::Source::NormalSpectrum{{}, Fermi::Source::SpectrumType::PowerLaw};
+            ps.spatial_model = Fermi::Source::SkyDirFunctionSpatialModel{};
+            src.srcs.push_back(ps);
         }
+        src.names = Fermi::Source::source_names(src.srcs);
 
         // IRFs: King sections normalised in place as the reference's loader does
         Fermi::Irf::XtIrf irf;
@@ -180,10 +188,10 @@ int main(int argc, char** argv) {
         irf.front_LTF = Fermi::Irf::livetime_efficiency_factors(
             obs.logEs, irf.aeff_irf.front.efficiency_params);
 
-        // the pipeline of main.cxx
-        const auto exp = Fermi::Exposure::compute_exposure(obs, src, irf);
-        const auto psf = Fermi::Psf::compute_psf_data(obs, irf, exp);
-        auto map = Fermi::ModelMap::compute_srcmaps(obs, src, exp, psf);
+        // the pipeline of main.cxx:26-30
+        auto exp = Fermi::Exposure::compute_exposure(cfg, obs, xsrc.point, irf);
+        auto psf = Fermi::Psf::compute_psf_data(obs, irf, exp);
+        auto map = Fermi::ModelMap::compute_srcmaps(obs, xsrc.point, exp, psf);
         // the output step main.cxx:40 leaves commented out
         if (fits_out) Fermi::fits::write_src_model(fits_out, map, src.names);
 

@@ -1,0 +1,139 @@
+"""The hand-written FITS parser against malformed headers, built with
+AddressSanitizer and UndefinedBehaviorSanitizer (host code only: the FITS
+entry points and a small driver, tests/cpp_asan/fits_check.cpp).  Every
+crafted file must be refused with an error code; the reference's IRF files
+must still read; no sanitizer may report."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    exe = str(tmp_path_factory.mktemp("asan") / "fits_check")
+    cmd = [cxx, "-std=c++20", "-O1", "-g", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=all", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "xtsrcmaps_amd", "csrc", "xts_fits.cpp"),
+           os.path.join(ROOT, "tests", "cpp_asan", "fits_check.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def _run(driver, *args):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([driver, *args], capture_output=True, text=True, env=env, timeout=60)
+    assert r.returncode == 0, f"sanitizer or crash ({r.returncode}):\n{r.stdout}\n{r.stderr}"
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr
+    return r.stdout
+
+
+def _card(k, v):
+    if isinstance(v, str) and not v.lstrip("-").isdigit() and v not in ("T", "F"):
+        v = "'" + v.ljust(8) + "'"
+    return (k.ljust(8) + "= " + str(v).rjust(20)).ljust(80).encode()
+
+
+def _hdu(cards, data=b""):
+    h = b"".join(_card(k, v) for k, v in cards) + b"END".ljust(80)
+    h += b" " * (-len(h) % 2880)
+    return h + data + b"\0" * (-len(data) % 2880)
+
+
+PRIMARY = [("SIMPLE", "T"), ("BITPIX", 8), ("NAXIS", 0)]
+
+
+def _table(nrows=2, naxis1=8, pcount=0, tfields=2, tform=("1E", "1E"), extra=(), data=None,
+           naxis2=None, gcount=1, naxis=2):
+    cards = [("XTENSION", "BINTABLE"), ("BITPIX", 8), ("NAXIS", naxis), ("NAXIS1", naxis1),
+             ("NAXIS2", nrows if naxis2 is None else naxis2), ("PCOUNT", pcount),
+             ("GCOUNT", gcount), ("TFIELDS", tfields)]
+    cards += [(f"TFORM{i + 1}", f) for i, f in enumerate(tform)]
+    cards += [("EXTNAME", "T")] + list(extra)
+    if data is None:
+        data = struct.pack(">" + "f" * (2 * nrows), *np.arange(2 * nrows, dtype=float))
+    return _hdu(PRIMARY) + _hdu(cards, data)
+
+
+CRAFTED = {
+    "negative_naxis2": _table(naxis2=-5),
+    "negative_pcount_hides_rows": _table(pcount=-16, nrows=2),
+    "pcount_cancels_rows": _table(nrows=1000, pcount=-8000, data=b""),
+    "huge_naxis": _table(naxis2=2 ** 61),
+    "overflowing_product": _table(naxis1=2 ** 40, naxis2=2 ** 40),
+    "truncated_data": _table(nrows=10000, data=b"\0" * 100),
+    "row_width_mismatch": _table(naxis1=12),
+    "bad_tform": _table(tform=("1E", "1Q")),
+    "huge_repeat": _table(tform=("99999999999999999999E", "1E")),
+    "negative_tfields": _table(tfields=-3),
+    "negative_gcount": _table(gcount=-1),
+    "naxis_3_table": _table(naxis=3),
+    "malformed_integer": _table(extra=()).replace(b"NAXIS2  =                    2",
+                                                   b"NAXIS2  =                  2x2"),
+    "negative_naxis": _table(naxis=-1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CRAFTED))
+def test_crafted_table_is_refused(driver, tmp_path, name):
+    p = tmp_path / f"{name}.fits"
+    p.write_bytes(CRAFTED[name])
+    out = _run(driver, "irf", str(p), "T")
+    assert out.splitlines()[-1].startswith("rc 1"), out
+
+
+def test_well_formed_table_reads(driver, tmp_path):
+    p = tmp_path / "ok.fits"
+    p.write_bytes(_table())
+    out = _run(driver, "irf", str(p), "T")
+    assert "rc 0 rows 2 width 2 sum 6" in out, out
+
+
+def _expcube(nside, nbr, repeat0=None):
+    npix = 12 * nside * nside
+    rep = nbr if repeat0 is None else repeat0
+    cards = [("XTENSION", "BINTABLE"), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", 4 * rep + 8),
+             ("NAXIS2", npix), ("PCOUNT", 0), ("GCOUNT", 1), ("TFIELDS", 3),
+             ("TFORM1", f"{rep}E"), ("TFORM2", "1E"), ("TFORM3", "1E"), ("EXTNAME", "EXPOSURE"),
+             ("NSIDE", nside), ("NBRBINS", nbr), ("COSMIN", 0), ("ORDERING", "NESTED"),
+             ("COORDSYS", "EQU"), ("THETABIN", "SQRT(1-COSTHETA)")]
+    data = b"\0" * ((4 * rep + 8) * npix)
+    return _hdu(PRIMARY) + _hdu(cards, data)
+
+
+def test_expcube_bounds(driver, tmp_path):
+    ok = tmp_path / "ok.fits"
+    ok.write_bytes(_expcube(2, 3))
+    assert "rc 0 nside 2" in _run(driver, "expcube", str(ok), "EXPOSURE")
+    for name, blob in {"zero_repeat": _expcube(2, 3, repeat0=0),
+                       "short_rows": _expcube(2, 3, repeat0=2)}.items():
+        p = tmp_path / f"{name}.fits"
+        p.write_bytes(blob)
+        assert _run(driver, "expcube", str(p), "EXPOSURE").splitlines()[-1] == "rc 1", name
+    big = _expcube(2, 3).replace(b"NSIDE   =                    2",
+                                 b"NSIDE   =           1000000000")
+    p = tmp_path / "huge_nside.fits"
+    p.write_bytes(big)
+    assert _run(driver, "expcube", str(p), "EXPOSURE").splitlines()[-1] == "rc 1"
+
+
+def test_reference_irf_files_and_writer_under_sanitizers(driver, tmp_path):
+    for f, tables in (("psf_P8R3_SOURCE_V2_FB.fits", ["RPSF_FRONT", "PSF_SCALING_PARAMS_BACK"]),
+                      ("aeff_P8R3_SOURCE_V2_FB.fits", ["EFFECTIVE AREA_FRONT",
+                                                       "EFFICIENCY_PARAMS_BACK"])):
+        for t in tables:
+            out = _run(driver, "irf", os.path.join(GOLD, f), t)
+            assert "rc 0 rows" in out, out
+    out = _run(driver, "write", str(tmp_path / "maps.fits"))
+    assert out.splitlines()[0] == "rc 0", out

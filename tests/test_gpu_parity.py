@@ -416,8 +416,26 @@ try:
                                        d["parint"][:50], d["out"], inp.rule, flags=flags)
 except Exception:
     refused += 1
+# any other model map between the halves, or a change of stream, cancels the
+# pending first half: its second half is refused before any launch
+ctx.point_src_model_map_rows_dev(inp.wcs, d["src"], E, inp.rule, flags=flags)
+ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"], d["out"],
+                                inp.rule, flags=flags)
+try:
+    ctx.point_src_model_map_finish_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                       d["out"], inp.rule, flags=flags)
+except Exception:
+    refused += 1
+ctx.point_src_model_map_rows_dev(inp.wcs, d["src"], E, inp.rule, flags=flags)
+ctx.set_stream(d["stream"].cuda_stream)
+try:
+    ctx.point_src_model_map_finish_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                       d["out"], inp.rule, flags=flags)
+except Exception:
+    refused += 1
+torch.cuda.synchronize()
 print(f"refused {refused}", flush=True)
-ok = rel(a, b) < 2e-7 and rel(c, b) < 2e-7 and refused == 2
+ok = rel(a, b) < 2e-7 and rel(c, b) < 2e-7 and refused == 4
 sys.exit(0 if ok else 1)
 """
 

@@ -386,51 +386,77 @@ class Context:
         return out
 
     # ------------------------------------------------------------ device variants
+    def src_cosbins_dev(self, src_sph, cube, nside, out):
+        """Device-pointer Exposure::src_cosbins: out [S, C] from cube
+        [12 nside^2, C] (async)."""
+        S, Cn = src_sph.shape[0], cube.shape[1]
+        p = capi.devs([("src_sph", src_sph, "float64", (S, 2)),
+                       ("cube", cube, "float64", (12 * int(nside) ** 2, Cn)),
+                       ("out", out, "float64", (S, Cn))])
+        capi.check(capi.lib().xts_src_cosbins_dev(self._h, p[0], S, p[1], int(nside), Cn, p[2]))
+
     def exposure_dev(self, ecb, wecb, af, ab, ltf, out, precision=PREC_FP64):
         """Device-pointer Exposure::exposure on torch CUDA tensors (async)."""
         S, Cn = ecb.shape
         E = af.shape[1]
-        capi.check(capi.lib().xts_exposure_dev(self._h, *(capi.tptr(t) for t in
-                                                            (ecb, wecb, af, ab, ltf)),
-                                               S, Cn, E, capi.tptr(out), int(precision)))
+        p = capi.devs([("ecb", ecb, "float64", (S, Cn)), ("wecb", wecb, "float64", (S, Cn)),
+                       ("aeff_front", af, "float64", (Cn, E)),
+                       ("aeff_back", ab, "float64", (Cn, E)),
+                       ("front_ltf", ltf, "float64", (2, E)), ("exposure", out, "float64", (S, E))])
+        capi.check(capi.lib().xts_exposure_dev(self._h, *p[:5], S, Cn, E, p[5], int(precision)))
 
     def psf_lookup_dev(self, front, back, costh, logE, af, ab, ecb, wecb, ltf, expo, upsf,
                        parint, precision=PREC_FP64):
         """Device-pointer mean-PSF build; front/back are tuples of CUDA tensors."""
-        gf = capi.psf_grid_dev(*front)
-        gb = capi.psf_grid_dev(*back)
         S, E = expo.shape
+        Cn = costh.shape[0]
+        p = capi.devs(capi.psf_grid_specs(*front) + capi.psf_grid_specs(*back) + [
+            ("costh", costh, "float64", (Cn,)), ("logE", logE, "float64", (E,)),
+            ("aeff_front", af, "float64", (Cn, E)), ("aeff_back", ab, "float64", (Cn, E)),
+            ("ecb", ecb, "float64", (S, Cn)), ("wecb", wecb, "float64", (S, Cn)),
+            ("front_ltf", ltf, "float64", (2, E)), ("exposure", expo, "float64", (S, E)),
+            ("upsf", upsf, "float64", (S, 401, E)), ("parint", parint, "float64", (S, 401, E))])
+        gf = capi.psf_grid_dev(p[0:3], front[1].shape[0], front[2].shape[0])
+        gb = capi.psf_grid_dev(p[3:6], back[1].shape[0], back[2].shape[0])
+        q = p[6:]
         capi.check(capi.lib().xts_psf_lookup_table_and_partial_integrals_dev(
-            self._h, C.byref(gf), C.byref(gb), capi.tptr(costh), costh.shape[0],
-            capi.tptr(logE), E, capi.tptr(af), capi.tptr(ab), capi.tptr(ecb), capi.tptr(wecb),
-            capi.tptr(ltf), capi.tptr(expo), S, capi.tptr(upsf), capi.tptr(parint),
-            int(precision)))
+            self._h, C.byref(gf), C.byref(gb), q[0], Cn, q[1], E, q[2], q[3], q[4], q[5], q[6],
+            q[7], S, q[8], q[9], int(precision)))
+
+    @staticmethod
+    def _map_specs(wcs: Wcs, src_sph, exposure, upsf, parint, out, flags):
+        S, E = exposure.shape
+        dt = "float32" if flags & OUT_F32 else "float64"
+        return [("src_sph", src_sph, "float64", (S, 2)), ("exposure", exposure, "float64", (S, E)),
+                ("upsf", upsf, "float64", (S, 401, E)), ("parint", parint, "float64", (S, 401, E)),
+                ("out", out, dt, None, S * E * wcs.naxis1 * wcs.naxis2)]
 
     def point_src_model_map_wcs_dev(self, wcs: Wcs, src_sph, exposure, upsf, parint, out,
                                     rule=None, flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
         """Device-pointer model map on torch CUDA tensors (async on the context stream)."""
         S, E = exposure.shape
+        p = capi.devs(self._map_specs(wcs, src_sph, exposure, upsf, parint, out, int(flags)))
         rp, N = capi.rule_ptr(rule)
         capi.check(capi.lib().xts_point_src_model_map_wcs_dev(
-            self._h, C.byref(wcs), capi.tptr(src_sph), S, capi.tptr(exposure), capi.tptr(upsf),
-            capi.tptr(parint), E, rp, N, int(flags), C.c_void_p(out.data_ptr())))
+            self._h, C.byref(wcs), p[0], S, p[1], p[2], p[3], E, rp, N, int(flags), p[4]))
 
     def point_src_model_map_rows_dev(self, wcs: Wcs, src_sph, E, rule=None,
                                      flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
         """First half of point_src_model_map_wcs_dev: the work that needs only
         the image, the sources and the rule (row weights), enqueued on the
         context stream.  Keep `rule` alive and unchanged until the second half."""
+        S = src_sph.shape[0]
+        p = capi.devs([("src_sph", src_sph, "float64", (S, 2))])
         rp, N = capi.rule_ptr(rule)
         capi.check(capi.lib().xts_point_src_model_map_rows_dev(
-            self._h, C.byref(wcs), capi.tptr(src_sph), src_sph.shape[0], int(E), rp, N,
-            int(flags)))
+            self._h, C.byref(wcs), p[0], S, int(E), rp, N, int(flags)))
 
     def point_src_model_map_finish_dev(self, wcs: Wcs, src_sph, exposure, upsf, parint, out,
                                        rule=None, flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
         """Second half: table conversion, map integral, correction factor and
         the map, after point_src_model_map_rows_dev with the same arguments."""
         S, E = exposure.shape
+        p = capi.devs(self._map_specs(wcs, src_sph, exposure, upsf, parint, out, int(flags)))
         rp, N = capi.rule_ptr(rule)
         capi.check(capi.lib().xts_point_src_model_map_finish_dev(
-            self._h, C.byref(wcs), capi.tptr(src_sph), S, capi.tptr(exposure), capi.tptr(upsf),
-            capi.tptr(parint), E, rp, N, int(flags), C.c_void_p(out.data_ptr())))
+            self._h, C.byref(wcs), p[0], S, p[1], p[2], p[3], E, rp, N, int(flags), p[4]))

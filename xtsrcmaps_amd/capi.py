@@ -175,6 +175,38 @@ def tptr(t) -> _vp:
     return _vp(t.data_ptr())
 
 
+def _meta(name, t, dtype, shape=None, numel=None):
+    got = str(t.dtype).replace("torch.", "")
+    if got != dtype:
+        raise XtsError(1, f"{name}: dtype {got}, expected {dtype}")
+    if shape is not None and tuple(t.shape) != tuple(int(x) for x in shape):
+        raise XtsError(1, f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if numel is not None and t.numel() != int(numel):
+        raise XtsError(1, f"{name}: {t.numel()} elements, expected {int(numel)}")
+    if not t.is_contiguous():
+        raise XtsError(1, f"{name}: not contiguous")
+
+
+def devs(specs, device=None) -> list:
+    """Device pointers of torch tensors checked against what an entry point
+    will read or write through them.  specs: (name, tensor, dtype, shape) or
+    (name, tensor, dtype, None, numel).  Every tensor's dtype, shape or
+    element count and contiguity are checked first, then that all are on one
+    GPU (device, if given, or the first tensor's).  A mismatch raises
+    XtsError before any launch."""
+    for sp in specs:
+        _meta(*sp)
+    if device is None and specs:
+        device = specs[0][1].device
+    for sp in specs:
+        name, t = sp[0], sp[1]
+        if not t.is_cuda:
+            raise XtsError(1, f"{name}: device entry points need CUDA tensors")
+        if t.device != device:
+            raise XtsError(1, f"{name}: on {t.device}, the other arguments on {device}")
+    return [_vp(sp[1].data_ptr()) for sp in specs]
+
+
 _keep = []
 
 
@@ -184,10 +216,16 @@ def psf_grid(params, cosths, logEs):
     return g, (p, c, e)
 
 
-def psf_grid_dev(params, cosths, logEs):
-    g = PsfGrid(C.cast(_vp(params.data_ptr()), _d), C.cast(_vp(cosths.data_ptr()), _d),
-                C.cast(_vp(logEs.data_ptr()), _d), cosths.shape[0], logEs.shape[0])
-    return g
+def psf_grid_specs(params, cosths, logEs):
+    """Argument specs (for devs) of a device-resident King PSF grid."""
+    Mc, Me = cosths.shape[0], logEs.shape[0]
+    return [("psf params", params, "float64", (Mc, Me, 6)),
+            ("psf cosths", cosths, "float64", (Mc,)), ("psf logEs", logEs, "float64", (Me,))]
+
+
+def psf_grid_dev(ptrs, Mc, Me):
+    """PsfGrid of three device pointers (params, cosths, logEs)."""
+    return PsfGrid(C.cast(ptrs[0], _d), C.cast(ptrs[1], _d), C.cast(ptrs[2], _d), Mc, Me)
 
 
 def aeff_grid(cosths, logEs, params, min_costheta):

@@ -187,6 +187,9 @@ int xts_ctx_set_stream(xts_ctx* ctx, void* stream) {
         ctx->geom_valid = false;
         ctx->rule.dev = nullptr;
     }
+    // a pending first half of a model map was enqueued in the old stream's
+    // order; its second half is refused from now on
+    ctx->pending_rows.valid = false;
     ctx->stream = next;
     return XTS_OK;
 }

@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cerrno>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -278,30 +279,70 @@ int scan_tables(const std::vector<unsigned char>& buf, std::map<std::string, Tab
         }
         if (first && keys.count("SIMPLE") == 0) return fail(XTS_ERR_INVALID, "not a FITS file");
         first = false;
-        auto num = [&](const char* k, int64_t dflt) {
+        // integer keywords: digits only (optional sign), in range
+        bool bad_num = false;
+        auto num = [&](const char* k, int64_t dflt) -> int64_t {
             auto it = keys.find(k);
-            return it == keys.end() ? dflt : std::strtoll(it->second.c_str(), nullptr, 10);
+            if (it == keys.end()) return dflt;
+            const char* c = it->second.c_str();
+            char* e = nullptr;
+            errno = 0;
+            const long long v = std::strtoll(c, &e, 10);
+            if (e == c || *e != '\0' || errno == ERANGE) bad_num = true;
+            return v;
         };
         const int64_t naxis = num("NAXIS", 0), bitpix = num("BITPIX", 8);
+        const int64_t pcount = num("PCOUNT", 0), gcount = num("GCOUNT", 1);
+        if (bad_num) return fail(XTS_ERR_INVALID, "FITS header: malformed integer keyword");
+        if (naxis < 0 || naxis > 999)
+            return fail(XTS_ERR_INVALID, "FITS header: NAXIS = %lld", (long long)naxis);
+        if (bitpix != 8 && bitpix != 16 && bitpix != 32 && bitpix != 64 && bitpix != -32 &&
+            bitpix != -64)
+            return fail(XTS_ERR_INVALID, "FITS header: BITPIX = %lld", (long long)bitpix);
+        if (pcount < 0 || gcount < 0)
+            return fail(XTS_ERR_INVALID, "FITS header: PCOUNT = %lld, GCOUNT = %lld",
+                        (long long)pcount, (long long)gcount);
+        // data bytes = |BITPIX|/8 * prod(NAXISn), + PCOUNT, x GCOUNT, checked
+        // for overflow and against what the file holds after this header
         int64_t n = naxis > 0 ? (bitpix < 0 ? -bitpix : bitpix) / 8 : 0;
-        for (int64_t k = 1; k <= naxis; ++k) n *= num(("NAXIS" + std::to_string(k)).c_str(), 0);
-        n = (n + num("PCOUNT", 0)) * std::max<int64_t>(num("GCOUNT", 1), 1);
-        if (pos + (size_t)n > buf.size()) return fail(XTS_ERR_INVALID, "FITS data truncated");
+        for (int64_t k = 1; k <= naxis; ++k) {
+            const int64_t ax = num(("NAXIS" + std::to_string(k)).c_str(), 0);
+            if (bad_num || ax < 0)
+                return fail(XTS_ERR_INVALID, "FITS header: NAXIS%lld = %lld", (long long)k,
+                            (long long)ax);
+            if (__builtin_mul_overflow(n, ax, &n))
+                return fail(XTS_ERR_INVALID, "FITS header: data size overflows");
+        }
+        if (__builtin_add_overflow(n, pcount, &n) ||
+            __builtin_mul_overflow(n, std::max<int64_t>(gcount, 1), &n))
+            return fail(XTS_ERR_INVALID, "FITS header: data size overflows");
+        if ((uint64_t)n > buf.size() - pos) return fail(XTS_ERR_INVALID, "FITS data truncated");
         if (keys.count("XTENSION") && upper(keys["XTENSION"]) == "BINTABLE") {
             TableHdu t;
             t.row_bytes = num("NAXIS1", 0);
             t.nrows = num("NAXIS2", 0);
+            const int64_t tfields = num("TFIELDS", 0);
+            if (naxis != 2 || t.row_bytes < 0 || t.nrows < 0 || tfields < 0 || tfields > 999)
+                return fail(XTS_ERR_INVALID, "binary table: NAXIS = %lld, TFIELDS = %lld",
+                            (long long)naxis, (long long)tfields);
+            // the rows lie inside the HDU's data (the heap follows them)
+            int64_t table_bytes;
+            if (__builtin_mul_overflow(t.row_bytes, t.nrows, &table_bytes) || table_bytes > n)
+                return fail(XTS_ERR_INVALID, "binary table: rows exceed the data");
             t.data = buf.data() + pos;
             int64_t off = 0;
-            for (int64_t c = 1; c <= num("TFIELDS", 0); ++c) {
+            for (int64_t c = 1; c <= tfields; ++c) {
                 const std::string f = keys["TFORM" + std::to_string(c)];
                 size_t i = 0;
                 while (i < f.size() && std::isdigit((unsigned char)f[i])) ++i;
+                if (i > 18) return fail(XTS_ERR_INVALID, "column format '%s' too long", f.c_str());
                 Column col{i < f.size() ? (char)std::toupper((unsigned char)f[i]) : '?',
                            i > 0 ? std::strtoll(f.substr(0, i).c_str(), nullptr, 10) : 1, off};
                 const int64_t w = type_bytes(col.type);
                 if (w < 0) return fail(XTS_ERR_INVALID, "unsupported column format '%s'", f.c_str());
-                off += w * col.repeat;
+                int64_t cb;
+                if (__builtin_mul_overflow(w, col.repeat, &cb) || __builtin_add_overflow(off, cb, &off))
+                    return fail(XTS_ERR_INVALID, "column format '%s' overflows", f.c_str());
                 t.cols.push_back(col);
             }
             if (off != t.row_bytes)
@@ -322,6 +363,10 @@ int load_file(const char* path, std::vector<unsigned char>* buf) {
     std::fseek(f, 0, SEEK_END);
     const long n = std::ftell(f);
     std::fseek(f, 0, SEEK_SET);
+    if (n < 0) {
+        std::fclose(f);
+        return fail(XTS_ERR_INVALID, "cannot size '%s'", path);
+    }
     buf->resize(n > 0 ? (size_t)n : 0);
     const size_t got = buf->empty() ? 0 : std::fread(buf->data(), 1, buf->size(), f);
     std::fclose(f);
@@ -473,9 +518,15 @@ extern "C" int xts_read_expcube(const char* path, const char* tblname, xts_expcu
                     coordsys.c_str());
     const int64_t nside = std::strtoll(t.keys["NSIDE"].c_str(), nullptr, 10);
     const int64_t nbr = std::strtoll(t.keys["NBRBINS"].c_str(), nullptr, 10);
+    // bounds keep every product below 2^62 (row counts and repeats are
+    // bounded by the file's size, checked when the table was scanned)
+    if (nside <= 0 || nside > (1 << 13) || nbr <= 0 || nbr > (1 << 16) || t.cols.size() < 3 ||
+        t.cols[0].repeat <= 0 || t.cols[1].repeat <= 0 || t.cols[2].repeat <= 0)
+        return fail(XTS_ERR_INVALID, "livetime cube '%s': NSIDE %lld, NBRBINS %lld or its "
+                    "columns out of range", tblname, (long long)nside, (long long)nbr);
     const int64_t npix = 12 * nside * nside;
-    if (nside <= 0 || nbr <= 0 || t.cols.size() < 3 || t.cols[0].repeat * t.nrows < nbr * npix ||
-        t.cols[1].repeat * t.nrows < npix || t.cols[2].repeat * t.nrows < npix)
+    if (t.cols[0].repeat * t.nrows < nbr * npix || t.cols[1].repeat * t.nrows < npix ||
+        t.cols[2].repeat * t.nrows < npix)
         return fail(XTS_ERR_INVALID, "livetime cube '%s' does not hold 12 nside^2 x %lld bins",
                     tblname, (long long)nbr);
     for (int c = 0; c < 3; ++c)

@@ -1756,6 +1756,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                             const double* expo, const double* upsf, const double* parint,
                             int64_t E, const double* rule, int64_t N, int32_t flags, void* out,
                             bool full, int phase = kPhaseAll) {
+    // any model map other than the second half of a split one overwrites the
+    // near lists, counters and radii that a pending first half left: its
+    // second half is refused from now on
+    if (phase != kPhaseFinish) ctx->pending_rows.valid = false;
     XTS_TRY(check_wcs(wcs));
     // (S E < 2^31: source-energy indices are 32-bit in the kernels)
     XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096 && S * E < (1LL << 31),

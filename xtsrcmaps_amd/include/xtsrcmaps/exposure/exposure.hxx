@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "xtsrcmaps/config/config.hxx"
 #include "xtsrcmaps/irf/irf.hxx"
 #include "xtsrcmaps/observation/obs_types.hxx"
 #include "xtsrcmaps/source/source.hxx"
@@ -34,9 +35,11 @@ auto exposure(Tensor<double, 2> const& src_exposure_cosbins,
               Tensor<double, 2> const& front_aeff, Tensor<double, 2> const& back_aeff,
               Tensor<double, 2> const& front_ltfs) -> Tensor<double, 2>;
 
+// The configuration is not read (as in the reference); it is kept so that
+// main.cxx:28's call compiles unchanged.
 template <Source::SourceConcept T>
-auto compute_exposure(Obs::XtObs const& obs, Source::SourceData<T> const& src,
-                      Irf::XtIrf const& irf) -> XtExp {
+auto compute_exposure(Config::XtCfg const& /*cfg*/, Obs::XtObs const& obs,
+                      Source::SourceData<T> const& src, Irf::XtIrf const& irf) -> XtExp {
     auto const exp_costhetas = costhetas(obs.exp_cube);
     auto const exp_map = map(obs.exp_cube);
     auto const wexp_map = map(obs.weighted_exp_cube);

@@ -6,8 +6,10 @@
 #include <string>
 #include <vector>
 
+#include "xtsrcmaps/config/config.hxx"
 #include "xtsrcmaps/fits/fits.hxx"
 #include "xtsrcmaps/irf/irf_types.hxx"
+#include "xtsrcmaps/observation/obs_types.hxx"
 #include "xtsrcmaps/tensor/tensor.hpp"
 
 namespace Fermi::Irf {
@@ -35,5 +37,11 @@ auto prepare_grid(fits::TablePars const& pars) -> IrfData3;
 // std::nullopt when a table cannot be read.
 auto load_aeff(std::string const& filename) -> std::optional<aeff::Pass8FB>;
 auto load_psf(std::string const& filename) -> std::optional<psf::Pass8FB>;
+
+// irf/irf.hxx:19: both IRF files of the configuration and the front LTF.
+// Declared for main.cxx:23; its definition is the reference's own
+// irf/irf.collect_irf_data.cpp, which calls only load_aeff, load_psf and
+// livetime_efficiency_factors above and is kept as it is.
+auto collect_irf_data(Config::XtCfg const& cfg, Obs::XtObs const& obs) -> XtIrf;
 
 }  // namespace Fermi::Irf
