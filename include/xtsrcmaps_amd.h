@@ -41,6 +41,11 @@ extern "C" {
 #define XTS_OUT_F32 0x2       /* float output                                  */
 #define XTS_COMPUTE_F32 0x0   /* FP32 cubature (FP64 geometry and scaling)     */
 #define XTS_COMPUTE_F64 0x4   /* FP64 cubature throughout                      */
+/* Correction factor with the PSF integrated to exactly the boundary radius
+ * from the node below it, the FermiTools-consistent variant; without it, as
+ * model_map.map_correction_factor.cxx:24-50 codes it (node index 1 below one
+ * degree, the whole next segment). */
+#define XTS_CF_EXACT_RADIUS 0x8
 
 /* contraction precision for the exposure and mean-PSF builds */
 #define XTS_PREC_FP64 0       /* FP64 FMA                                      */

@@ -338,13 +338,25 @@ def psf_boundary_radius(Nh, Nw, src_sph, w):
     return rad, fov
 
 
-def map_correction_factor(invmi, radius, in_fov, upsf, parint):
+def map_correction_factor(invmi, radius, in_fov, upsf, parint, exact=False):
+    """As coded (exact=False) or the FermiTools-consistent exact-radius variant."""
     invmi, radius, upsf, parint = map(_c, (invmi, radius, upsf, parint))
     fov = _c(in_fov, np.uint8)
     S, D, E = upsf.shape
     out = np.zeros((S, E))
-    lib().or_map_correction_factor(_p(invmi), _p(radius), _p(fov, _u8), _p(upsf),
-                                   _p(parint), _l(S), _l(D), _l(E), _p(out))
+    lib().or_map_correction_factor_mode(_p(invmi), _p(radius), _p(fov, _u8), _p(upsf),
+                                        _p(parint), _l(S), _l(D), _l(E),
+                                        C.c_int(1 if exact else 0), _p(out))
+    return out
+
+
+def psf_radius_integral(radius, upsf, parint, exact=False):
+    """PSF integral [S, E] to each source's radius (uncapped)."""
+    radius, upsf, parint = map(_c, (radius, upsf, parint))
+    S, D, E = upsf.shape
+    out = np.zeros((S, E))
+    lib().or_psf_radius_integral(_p(radius), _p(upsf), _p(parint), _l(S), _l(D), _l(E),
+                                 C.c_int(1 if exact else 0), _p(out))
     return out
 
 

@@ -900,34 +900,65 @@ void or_map_integral(const double* map, long S, long Nh, long Nw, long E,
     for (long i = 0; i < S * E; ++i) out[i] = out[i] != 0.0 ? 1.0 / out[i] : out[i];
 }
 
-/* model_map/model_map.map_correction_factor.cxx:3-53 as coded: the PSF
- * integral to the start of the segment after the radius node (whole
- * segment), capped at 1, times the inverse map integral; 1 outside FOV. */
-void or_map_correction_factor(const double* invmi, const double* radius,
-                              const uint8_t* in_fov, const double* upsf,
-                              const double* parint, long S, long D, long E,
-                              double* out) {
+/* The PSF integral to the correction radius of each (source, energy).
+ * mode 0: model_map/model_map.map_correction_factor.cxx:31-47 as coded: the
+ *   node index d of psf/psf.fast_separation_lower_index.cxx:5-22 (1 for
+ *   radii below 1 degree, Q5), clamped to 399, and the whole segment
+ *   d -> d+1 of the linear interpolant of 2 pi sin(theta) psf (Q6).
+ * mode 1: the FermiTools-consistent variant: d the node below the radius
+ *   (sep[d] <= R < sep[d+1]) and the segment integrated to exactly R.
+ * Not capped (the caller caps at 1). */
+void or_psf_radius_integral(const double* radius, const double* upsf,
+                            const double* parint, long S, long D, long E,
+                            int mode, double* out) {
     double sep[OR_ND];
     or_separations(sep);
     for (long s = 0; s < S; ++s) {
-        if (!in_fov[s]) {
-            for (long e = 0; e < E; ++e) out[s * E + e] = 1.0;
-            continue;
+        const double R = radius[s];
+        long d;
+        if (mode == 0) {
+            d = or_fast_separation_lower_index(R);
+        } else {
+            d = R < 1e-4 ? 0 : (long)floor(1.0 + log(1e4 * R) * RECIPSTEP);
+            while (d + 1 < OR_ND && sep[d + 1] <= R) ++d;   /* rounding at a node */
+            while (d > 0 && sep[d] > R) --d;
         }
-        long d = or_fast_separation_lower_index(radius[s]);
         if (d >= OR_ND - 2) d = OR_ND - 2;
         const double t1 = sep[d] * D2R, t2 = sep[d + 1] * D2R;
+        const double te = mode == 0 ? t2 : R * D2R;
         for (long e = 0; e < E; ++e) {
             const double y1 = TWOPI * upsf[((size_t)s * D + d) * E + e] * sin(t1);
             const double y2 = TWOPI * upsf[((size_t)s * D + d + 1) * E + e] * sin(t2);
             const double m = (y2 - y1) / (t2 - t1);
             const double b = y1 - t1 * m;
-            const double v = 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
-            double integ = v + parint[((size_t)s * D + d) * E + e];
-            integ = integ > 1.0 ? 1.0 : integ;
-            out[s * E + e] = integ * invmi[s * E + e];
+            const double v = 0.5 * m * (te * te - t1 * t1) + b * (te - t1);
+            out[s * E + e] = v + parint[((size_t)s * D + d) * E + e];
         }
     }
+}
+
+/* model_map/model_map.map_correction_factor.cxx:3-53 (mode 0, as coded) or
+ * its FermiTools-consistent variant (mode 1): the PSF integral to the
+ * radius, capped at 1, times the inverse map integral; 1 outside FOV. */
+void or_map_correction_factor_mode(const double* invmi, const double* radius,
+                                   const uint8_t* in_fov, const double* upsf,
+                                   const double* parint, long S, long D, long E,
+                                   int mode, double* out) {
+    or_psf_radius_integral(radius, upsf, parint, S, D, E, mode, out);
+    for (long s = 0; s < S; ++s)
+        for (long e = 0; e < E; ++e) {
+            double integ = out[s * E + e];
+            integ = integ > 1.0 ? 1.0 : integ;
+            out[s * E + e] = in_fov[s] ? integ * invmi[s * E + e] : 1.0;
+        }
+}
+
+/* model_map/model_map.map_correction_factor.cxx:3-53 as coded. */
+void or_map_correction_factor(const double* invmi, const double* radius,
+                              const uint8_t* in_fov, const double* upsf,
+                              const double* parint, long S, long D, long E,
+                              double* out) {
+    or_map_correction_factor_mode(invmi, radius, in_fov, upsf, parint, S, D, E, 0, out);
 }
 
 /* model_map.scale_map_by_solid_angle.cxx:6-26 */

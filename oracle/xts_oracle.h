@@ -21,14 +21,17 @@
  * integral, and map_integral / map_correction_factor through the
  * constant-LUT pipeline identity at a 1-degree radius.
  *
- * Parity unpinned (restated from the cited lines, no independent check):
- *   - or_solid_angle: the neighbouring-centre quad of math/solid_angle.hpp:
- *     50-75 is only checked against the exact CAR pixel area to O(pixel^2),
- *     which a corner quad would pass too;
- *   - or_map_correction_factor for radii below 1 degree (the index-1 branch
- *     of psf.fast_separation_lower_index.cxx:15-18);
- *   - or_pixel_logsep for separations outside [1e-4, 70] degrees (the
- *     extrapolation of convolve_pixel_psf_logsep.hpp:46-54).
+ * Pinned by mathematics the tests compute themselves
+ * (tests/test_oracle_pins.py): or_solid_angle as the spherical area of the
+ * neighbouring-centre quad (oblique rotated CAR, sheared AIT, and AIT's
+ * equal area), where a corner quad, one triangle or a shifted quad misses;
+ * or_map_correction_factor's node index and segment for every radius
+ * (including the index-1 branch below 1 degree) against the exact integral
+ * of a piecewise-linear integrand; or_pixel_logsep outside [1e-4, 70]
+ * degrees against the edge segments' lines; or_psf_radius_integral's
+ * exact-radius variant against FermiTools' ft_mm_upsf_radius_integ.bin.
+ *
+ * Parity unpinned: none.
  */
 #ifndef XTS_ORACLE_H
 #define XTS_ORACLE_H
@@ -144,6 +147,13 @@ void or_psf_boundary_radius(long Nh, long Nw, const double* src_sph, long S,
 void or_map_integral(const double* map, long S, long Nh, long Nw, long E,
                      const double* src_sph, const or_wcs* w,
                      const double* radius, const uint8_t* in_fov, double* out);
+void or_psf_radius_integral(const double* radius, const double* upsf,
+                            const double* parint, long S, long D, long E,
+                            int mode, double* out);
+void or_map_correction_factor_mode(const double* invmi, const double* radius,
+                                   const uint8_t* in_fov, const double* upsf,
+                                   const double* parint, long S, long D, long E,
+                                   int mode, double* out);
 void or_map_correction_factor(const double* inv_mapinteg, const double* radius,
                               const uint8_t* in_fov, const double* upsf,
                               const double* parint, long S, long D, long E,

@@ -11,41 +11,6 @@ import pytest
 from conftest import max_rel, near_relative
 
 
-@pytest.fixture(scope="module")
-def psf_grids(orc, golden):
-    P = golden("irf_psf")
-    out = {}
-    for k in ("front", "back"):
-        cos, loge, par, _ = orc.prepare_grid(P[k + "_e_lo"], P[k + "_e_hi"],
-                                             P[k + "_c_lo"], P[k + "_c_hi"], P[k + "_raw"])
-        out[k] = dict(raw=par, norm=orc.normalize_rpsf(par, loge, P[k + "_scale"]),
-                      cos=cos, loge=loge)
-    return out
-
-
-@pytest.fixture(scope="module")
-def chain(orc, golden, psf_grids):
-    """The 3C279 exposure + mean-PSF chain for all 263 sources."""
-    A, X = golden("irf_aeff"), golden("exposure_psf")
-    logE = np.log10(X["energies"])
-    costh = orc.costhetas(40, 0.0, True)
-    ltf = orc.ltf(logE, A["front_effic_p0"], A["front_effic_p1"])
-    aeff = {}
-    for k in ("front", "back"):
-        cos, loge, par, minc = orc.prepare_grid(A[k + "_e_lo"], A[k + "_e_hi"],
-                                                A[k + "_c_lo"], A[k + "_c_hi"], A[k + "_raw"])
-        aeff[k] = orc.aeff_value(costh, logE, cos, loge, par[..., 0], minc)
-    ecb, wecb = X["ecb"].astype(float), X["wecb"].astype(float)
-    expo, parts = orc.exposure(ecb, wecb, aeff["front"], aeff["back"], ltf, parts=True)
-    g = psf_grids
-    upsf, parint = orc.psf_lookup_table_and_partial_integrals(
-        (g["front"]["norm"], g["front"]["cos"], g["front"]["loge"]),
-        (g["back"]["norm"], g["back"]["cos"], g["back"]["loge"]),
-        costh, logE, aeff["front"], aeff["back"], ecb, wecb, ltf, expo)
-    return dict(logE=logE, costh=costh, ltf=ltf, aeff=aeff, expo=expo, parts=parts,
-                upsf=upsf, parint=parint, ecb=ecb, wecb=wecb)
-
-
 def test_separations_match_reference_table(orc, golden):
     # psf/psf.hxx:24-31 vs tests/test_king.cxx:26-108 (rtol 1e-5 there)
     sep = orc.separations()

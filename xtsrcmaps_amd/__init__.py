@@ -15,14 +15,15 @@ import numpy as np
 
 from . import capi
 from .capi import (XtsError, Wcs, PROJ_CAR, PROJ_AIT, LAYOUT_SHWE, LAYOUT_SEHW,
-                   OUT_F64, OUT_F32, COMPUTE_F32, COMPUTE_F64, PREC_FP64, PREC_BF16X3)
+                   OUT_F64, OUT_F32, COMPUTE_F32, COMPUTE_F64, CF_EXACT_RADIUS, PREC_FP64,
+                   PREC_BF16X3)
 
 ND = 401
 
 __all__ = [
     "Context", "Wcs", "XtsError", "separations", "costhetas",
     "livetime_efficiency_factors", "PROJ_CAR", "PROJ_AIT", "LAYOUT_SHWE",
-    "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64",
+    "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64", "CF_EXACT_RADIUS",
     "PREC_FP64", "PREC_BF16X3", "ND", "write_src_model", "read_irf_pars", "read_expcube",
 ]
 

@@ -14,6 +14,7 @@ PROJ_CAR, PROJ_AIT = 0, 1
 LAYOUT_SHWE, LAYOUT_SEHW = 0x0, 0x1
 OUT_F64, OUT_F32 = 0x0, 0x2
 COMPUTE_F32, COMPUTE_F64 = 0x0, 0x4
+CF_EXACT_RADIUS = 0x8
 PREC_FP64, PREC_BF16X3 = 0, 1
 SCALE_PIXEL, SCALE_SRC_ENERGY = 0, 1
 

@@ -1460,20 +1460,32 @@ static int emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool scaled) {
 // map_correction_factor.cxx:24-50 for one (source, energy): the PSF
 // integral to the node below the radius plus its whole next segment
 // (capped at 1), x 1 / map integral; 1 outside the field of view.
+// With exact (XTS_CF_EXACT_RADIUS) the FermiTools-consistent variant: the
+// node below the radius for every radius, and the segment integrated to the
+// radius itself.
 __device__ inline double correction_factor(const SrcGeom& g, double invmi,
                                            const double* __restrict__ upsf,
                                            const double* __restrict__ parint,
-                                           const double* __restrict__ sep, int s, int e, int E) {
+                                           const double* __restrict__ sep, int s, int e, int E,
+                                           int exact = 0) {
     if (!g.in_fov) return 1.0;
     const double r = g.radius_deg;
-    int d = r < 1.0 ? 1 : (int)floor(1.0 + log(1e4 * r) * kRecipStep);
+    int d;
+    if (!exact) {
+        d = r < 1.0 ? 1 : (int)floor(1.0 + log(1e4 * r) * kRecipStep);
+    } else {
+        d = r < 1e-4 ? 0 : (int)floor(1.0 + log(1e4 * r) * kRecipStep);
+        while (d + 1 < kND && sep[d + 1] <= r) ++d;   // rounding at a node
+        while (d > 0 && sep[d] > r) --d;
+    }
     if (d >= kND - 2) d = kND - 2;
     const double t1 = sep[d] * kD2R, t2 = sep[d + 1] * kD2R;
+    const double te = exact ? r * kD2R : t2;
     const double y1 = kTwoPi * upsf[((size_t)s * kND + d) * E + e] * sin(t1);
     const double y2 = kTwoPi * upsf[((size_t)s * kND + d + 1) * E + e] * sin(t2);
     const double m = (y2 - y1) / (t2 - t1);
     const double b = y1 - t1 * m;
-    const double v = 0.5 * m * (t2 * t2 - t1 * t1) + b * (t2 - t1);
+    const double v = 0.5 * m * (te * te - t1 * t1) + b * (te - t1);
     double integ = v + parint[((size_t)s * kND + d) * E + e];
     integ = integ > 1.0 ? 1.0 : integ;
     return integ * invmi;
@@ -1490,7 +1502,7 @@ __global__ void __launch_bounds__(kHistThreads)
 hist_integral_kernel(int E, const double* __restrict__ hist, const double* __restrict__ upsf,
                      double* __restrict__ mapint, const SrcGeom* __restrict__ geo,
                      const double* __restrict__ parint, const double* __restrict__ sep,
-                     double* __restrict__ cf_out) {
+                     double* __restrict__ cf_out, int exact) {
     __shared__ double h_s[kNDPad];
     __shared__ double part[kHistThreads];
     const int s = blockIdx.x, tid = threadIdx.x;
@@ -1515,7 +1527,8 @@ hist_integral_kernel(int E, const double* __restrict__ hist, const double* __res
             for (int q = 0; q < P; ++q) tot += part[q * Ec + el];
             mapint[(size_t)s * E + e] = tot;
             const double invmi = tot != 0.0 ? 1.0 / tot : tot;
-            cf_out[(size_t)s * E + e] = correction_factor(g, invmi, upsf, parint, sep, s, e, E);
+            cf_out[(size_t)s * E + e] =
+                correction_factor(g, invmi, upsf, parint, sep, s, e, E, exact);
         }
     }
 }
@@ -1767,7 +1780,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_CHECK(src_sph, "null device pointer");
     XTS_CHECK(!(phase & kPhaseFinish) || (upsf && out), "null device pointer");
     XTS_CHECK(!(phase & kPhaseFinish) || !full || (expo && parint), "null exposure/parint");
-    XTS_CHECK((flags & ~0x7) == 0, "unknown flags 0x%x", flags);
+    XTS_CHECK((flags & ~0xF) == 0, "unknown flags 0x%x", flags);
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     XTS_CHECK((int64_t)H * W <= (int64_t)1 << 30, "image of %lld pixels exceeds 2^30",
               (long long)H * W);
@@ -1904,7 +1917,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     if (full) {
         TimedRegion tr(ctx, "factor");
         hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>(
-            (int)E, ca.hist, upsf, ca.mapint, ig.geo, parint, ct->sep, fac);
+            (int)E, ca.hist, upsf, ca.mapint, ig.geo, parint, ct->sep, fac,
+            (flags & XTS_CF_EXACT_RADIUS) ? 1 : 0);
         XTS_HIP(hipGetLastError());
     }
     // pass 3: blocks of kEmitBlockPix pixels of one source, then kWideBlocks
