@@ -228,6 +228,24 @@ int xts_point_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* wcs,
                                     const double* parint, int64_t E,
                                     const double* rule, int64_t N,
                                     int32_t flags, void* out);
+/* xts_point_src_model_map_wcs_dev for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * result; it calls no hipMalloc, hipFree or allocating routine: every buffer
+ * besides the inputs and out comes from a fixed 256 MiB device scratch that
+ * each device holds from the library's load until the process ends.  A call
+ * needing more (about 36 bytes per source and pixel, 400 KB per source of
+ * near-pixel pool, plus the image's geometry and the PSF table copy: the
+ * 100-source 200x200 37-energy map fits) fails with XTS_ERR_INVALID before
+ * any launch.  Calls that overlap on one device (two contexts, two streams)
+ * share the scratch and race; the scratch keeps the last call's contents,
+ * and everything that must start at zero is cleared by the call's kernels. */
+int xts_point_src_model_map_wcs_dev_noalloc(xts_ctx* ctx, const xts_wcs* wcs,
+                                            const double* src_sph, int64_t S,
+                                            const double* exposure, const double* upsf,
+                                            const double* parint, int64_t E,
+                                            const double* rule, int64_t N,
+                                            int32_t flags, void* out);
 /* xts_point_src_model_map_wcs_dev in two calls on the context's stream.
  * The first needs only the image, the sources and the rule: pixel geometry,
  * boundary radii, the per-pixel row weights of the PSF table and the

@@ -99,6 +99,8 @@ _SIGS = {
                                     C.c_int32, _vp],
     "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
                                         _d, _i64, C.c_int32, _vp],
+    "xts_point_src_model_map_wcs_dev_noalloc": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp,
+                                                _i64, _d, _i64, C.c_int32, _vp],
     "xts_point_src_model_map_rows_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _i64, _d, _i64,
                                          C.c_int32],
     "xts_point_src_model_map_finish_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,

@@ -35,6 +35,17 @@ int activate(xts_ctx* ctx) {
 }
 
 int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out) {
+    if (ctx->arena.base) {
+        // no allocation: the next aligned piece of the arena
+        const size_t off = ctx->arena.used;
+        const size_t n = arena_bytes(bytes);
+        if (off > ctx->arena.cap || n > ctx->arena.cap - off)
+            return fail(XTS_ERR_INVALID, "call needs more than the %zu MiB no-allocation scratch",
+                        ctx->arena.cap >> 20);
+        *out = ctx->arena.base + off;
+        ctx->arena.used = off + n;
+        return XTS_OK;
+    }
     Slot& s = ctx->slots[slot];
     if (bytes == 0) bytes = 256;
     if (s.bytes < bytes) {
@@ -54,11 +65,12 @@ int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out) {
     return XTS_OK;
 }
 
-int const_tables(xts_ctx* ctx, const xts_const_tables** out) {
-    if (!ctx->consts_ready) {
-        // separations (psf/psf.hxx:24-31), 2 R2D / sep, theta and sin theta
-        std::vector<double> h(kND * 4 + kND / 2 + 1);
-        double* sep = h.data();
+// separations (psf/psf.hxx:24-31), 2 R2D / sep, theta and sin theta, then
+// 2 R2D / sep in FP32 (half the doubles)
+static const std::vector<double>& host_const_tables() {
+    static const std::vector<double> h = [] {
+        std::vector<double> v(kND * 4 + kND / 2 + 1);
+        double* sep = v.data();
         double* inv = sep + kND;
         double* th = inv + kND;
         float* inv32 = reinterpret_cast<float*>(th + 2 * kND);
@@ -71,15 +83,37 @@ int const_tables(xts_ctx* ctx, const xts_const_tables** out) {
             th[i] = kD2R * sep[i];
             th[kND + i] = std::sin(th[i]);
         }
+        return v;
+    }();
+    return h;
+}
+
+size_t const_tables_bytes() { return host_const_tables().size() * 8; }
+
+static void set_const_ptrs(xts_const_tables* c, double* dd) {
+    c->sep = dd;
+    c->inv64 = dd + kND;
+    c->theta = dd + 2 * kND;
+    c->inv32 = reinterpret_cast<float*>(dd + 4 * kND);
+}
+
+int const_tables(xts_ctx* ctx, const xts_const_tables** out) {
+    const std::vector<double>& h = host_const_tables();
+    if (ctx->arena.base) {
+        // inside the arena, uploaded on the context stream for this call only
+        void* d;
+        XTS_TRY(slot_reserve(ctx, kSlotConst, h.size() * 8, &d));
+        XTS_HIP(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        set_const_ptrs(&ctx->arena_consts, (double*)d);
+        *out = &ctx->arena_consts;
+        return XTS_OK;
+    }
+    if (!ctx->consts_ready) {
         void* d;
         XTS_TRY(slot_reserve(ctx, kSlotConst, h.size() * 8, &d));
         // a blocking copy: done once, before any kernel reads the tables
         XTS_HIP(hipMemcpy(d, h.data(), h.size() * 8, hipMemcpyHostToDevice));
-        double* dd = (double*)d;
-        ctx->consts.sep = dd;
-        ctx->consts.inv64 = dd + kND;
-        ctx->consts.theta = dd + 2 * kND;
-        ctx->consts.inv32 = reinterpret_cast<float*>(dd + 4 * kND);
+        set_const_ptrs(&ctx->consts, (double*)d);
         ctx->consts_ready = true;
     }
     *out = &ctx->consts;
@@ -158,6 +192,9 @@ int xts_ctx_create(int device, xts_ctx** out) {
         return fail(XTS_ERR_DEVICE, "xts_ctx_create: %s", hipGetErrorString(e));
     }
     c->stream = c->own_stream;
+    // the no-allocation entry point's scratch address on this device (its
+    // status is kept and reported by that entry point)
+    noalloc_scratch_init(device);
     *out = c;
     return XTS_OK;
 }

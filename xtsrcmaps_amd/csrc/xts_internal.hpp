@@ -105,6 +105,15 @@ struct xts_ctx {
     xts_rule_cache rule;
     bool geom_valid = false;               // pixel geometry in kSlotGeom is for geom_wcs
     xts_wcs geom_wcs{};
+    // A caller-provided device region that slot_reserve bump-allocates from
+    // instead of the slots while base is set (the no-allocation entry
+    // point); nothing taken from it is cached past the call.
+    struct {
+        char* base = nullptr;
+        size_t cap = 0, used = 0;
+    } arena;
+    xts_const_tables arena_consts;         // the constant tables inside the arena
+    std::vector<unsigned char> arena_rule_blob;  // host copy of the rule uploaded into it
     // arguments of a model map whose row-weight half has been enqueued and
     // whose second half has not
     struct {
@@ -120,12 +129,25 @@ struct xts_ctx {
 namespace xts {
 
 int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out);
+// Bytes slot_reserve takes from an arena for a request of `bytes`, and the
+// arena alignment: every request starts on a 256-byte boundary.
+constexpr size_t kArenaAlign = 256;
+inline size_t arena_bytes(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    return (bytes + kArenaAlign - 1) & ~(kArenaAlign - 1);
+}
+// Device address of the no-allocation scratch on the current device, taken
+// once per device at context creation (xts_modelmap.hip).
+int noalloc_scratch_init(int device);
+int noalloc_scratch(int device, void** ptr, size_t* bytes);
 // Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
 // and size: the attribute belongs to the function).
 int allow_lds(const void* fn, size_t bytes);
 
-// Constant tables on the device (uploaded on the first call).
+// Constant tables on the device (uploaded on the first call; into the arena
+// on every call while one is set).
 int const_tables(xts_ctx* ctx, const xts_const_tables** out);
+size_t const_tables_bytes();
 
 // Brackets kernel launches with events on the context stream when timing is
 // enabled (xts_ctx_enable_timing); no effect otherwise.

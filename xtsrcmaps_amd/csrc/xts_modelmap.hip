@@ -1638,14 +1638,18 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
     ig->geo = (SrcGeom*)g;
     const Wcs w = wcs_setup(*wcs);
     // the pixel geometry depends on the WCS alone: recomputed only when the
-    // image changes (the slot keeps the last image's)
-    if (!ctx->geom_valid || std::memcmp(&ctx->geom_wcs, wcs, sizeof(xts_wcs)) != 0) {
+    // image changes (the slot keeps the last image's); always inside an
+    // arena, whose contents do not outlive the call
+    const bool cached = ctx->arena.base == nullptr;
+    if (!cached || !ctx->geom_valid || std::memcmp(&ctx->geom_wcs, wcs, sizeof(xts_wcs)) != 0) {
         geometry_kernel<<<std::min<unsigned>(div_up(ncorn, 256), 4096), 256, 0, ctx->stream>>>(
             w, H, W, ig->corners, ig->centres);
         solid_angle_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
                              ctx->stream>>>(H, W, ig->centres, ig->omega);
-        ctx->geom_wcs = *wcs;
-        ctx->geom_valid = true;
+        if (cached) {
+            ctx->geom_wcs = *wcs;
+            ctx->geom_valid = true;
+        }
     }
     if (S > 0 && src_sph_dev && with_radius)
         radius_kernel<<<S + 1, 256, 0, ctx->stream>>>(w, H, W, S, src_sph_dev, ig->corners, ig->geo);
@@ -1696,9 +1700,13 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
     const double* r = rule ? rule : kSquareDeg7;
     if (!rule) N = 12;
     XTS_CHECK(N > 0 && N <= 1024, "rule size %lld out of range (1..1024)", (long long)N);
-    xts_rule_cache& rc = ctx->rule;
-    if (!(rc.dev && rc.host.size() == (size_t)(3 * N) &&
-          std::memcmp(rc.host.data(), r, (size_t)(3 * N) * sizeof(double)) == 0)) {
+    // inside an arena the rule is uploaded for this call only: the context's
+    // cache is neither used nor changed
+    const bool arena = ctx->arena.base != nullptr;
+    xts_rule_cache in_arena;
+    xts_rule_cache& rc = arena ? in_arena : ctx->rule;
+    if (arena || !(rc.dev && rc.host.size() == (size_t)(3 * N) &&
+                   std::memcmp(rc.host.data(), r, (size_t)(3 * N) * sizeof(double)) == 0)) {
         std::vector<RuleUVW> rp((size_t)N);
         double rs = 1.0;
         for (int64_t p = 0; p < N; ++p) {
@@ -1738,7 +1746,13 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
             la[2 * N4 + p] = wq;
         }
         void* d;
-        XTS_TRY(to_device(ctx, kSlotRule, blob.data(), blob.size(), &d));
+        if (arena) {
+            ctx->arena_rule_blob = std::move(blob);
+            XTS_TRY(to_device(ctx, kSlotRule, ctx->arena_rule_blob.data(),
+                              ctx->arena_rule_blob.size(), &d));
+        } else {
+            XTS_TRY(to_device(ctx, kSlotRule, blob.data(), blob.size(), &d));
+        }
         rc.host.assign(r, r + 3 * N);
         rc.dev = d;
         rc.lean_off = rec;
@@ -1944,6 +1958,72 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     return XTS_OK;
 }
 
+// ------------------------------------------------------- no-allocation path
+// The device memory of xts_point_src_model_map_wcs_dev_noalloc: every buffer
+// a model map uses besides its inputs and output (pixel geometry, radii,
+// constant tables, rule, PSF table copy, accumulators, row weights, near
+// lists and pool) is carved from this array.  Each device holds its own
+// copy from the library's load until the process ends.  Calls that overlap
+// on one device share it and race.
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+static int g_scratch_rc[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+
+// Once per device, with that device current: the array's device address (the
+// name is a host-side stand-in and must never be passed to a kernel).
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return XTS_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return g_scratch_rc[device];
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_rc[device] = e == hipSuccess && p ? XTS_OK : XTS_ERR_DEVICE;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_done[device] = true;
+    if (e != hipSuccess) (void)hipGetLastError();   // leave no sticky error behind
+    return g_scratch_rc[device];
+}
+
+int noalloc_scratch(int device, void** ptr, size_t* bytes) {
+    XTS_CHECK(device >= 0 && device < kMaxDevices, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    XTS_CHECK(g_scratch_done[device], "no-allocation scratch not set up on device %d", device);
+    if (g_scratch_rc[device] != XTS_OK)
+        return fail(XTS_ERR_DEVICE, "hipGetSymbolAddress of the no-allocation scratch failed "
+                    "on device %d", device);
+    *ptr = g_scratch_ptr[device];
+    *bytes = kNoAllocScratchBytes;
+    return XTS_OK;
+}
+
+// Arena bytes model_map_device takes for one full model map: the requests it
+// makes, in order, each rounded as slot_reserve rounds them.
+static size_t model_map_arena_bytes(int64_t S, const xts_wcs* wcs, int64_t E, int64_t N,
+                                    int32_t flags) {
+    const bool f64 = (flags & XTS_COMPUTE_F64) != 0;
+    const size_t H = (size_t)wcs->naxis1, W = (size_t)wcs->naxis2, plane = H * W;
+    const size_t ncorn = (H + 1) * (W + 1), tsz = f64 ? 8 : 4;
+    const size_t Ep = (size_t)((E + 3) / 4 * 4);
+    const int64_t N4 = (N + 3) / 4 * 4;
+    const size_t rec = ((size_t)N * sizeof(RuleUVW) + 255) & ~size_t(255);
+    const int64_t pool_cap = std::min<int64_t>(4096 * kNearRows, ((int64_t)1 << 30) / S);
+    size_t b = 0;
+    b += arena_bytes(ncorn * 3 * 8 * 2 + plane * 8);                          // geometry
+    b += arena_bytes((size_t)std::max<int64_t>(S, 1) * sizeof(SrcGeom));      // radii
+    b += arena_bytes(const_tables_bytes());                                   // tables
+    b += arena_bytes(rec + (size_t)kLeanArrays * N4 * sizeof(float));        // rule
+    b += arena_bytes((size_t)S * kND * Ep * tsz);                             // PSF table
+    b += arena_bytes(((size_t)S + (size_t)S * kND + 2 * (size_t)S * E) * 8);  // accumulators
+    b += arena_bytes((size_t)S * plane * (f64 ? 32 : 16) + (size_t)S * plane * 20 + 512);
+    b += arena_bytes((size_t)(pool_cap * S) * tsz);                          // near pool
+    return b;
+}
+
 static size_t out_bytes(int64_t S, const xts_wcs* wcs, int64_t E, int32_t flags) {
     return (size_t)S * wcs->naxis1 * wcs->naxis2 * E * ((flags & XTS_OUT_F32) ? 4 : 8);
 }
@@ -1959,6 +2039,37 @@ int xts_point_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* wcs, const doub
                                     const double* parint, int64_t E, const double* rule,
                                     int64_t N, int32_t flags, void* out) {
     XTS_TRY(activate(ctx));
+    return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out, true);
+}
+
+int xts_point_src_model_map_wcs_dev_noalloc(xts_ctx* ctx, const xts_wcs* wcs,
+                                            const double* src_sph, int64_t S,
+                                            const double* expo, const double* upsf,
+                                            const double* parint, int64_t E, const double* rule,
+                                            int64_t N, int32_t flags, void* out) {
+    XTS_TRY(activate(ctx));
+    XTS_TRY(check_wcs(wcs));
+    XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096 && S * E < (1LL << 31),
+              "bad S=%lld or E=%lld", (long long)S, (long long)E);
+    XTS_CHECK(rule == nullptr || (N > 0 && N <= 1024), "rule size %lld out of range (1..1024)",
+              (long long)N);
+    XTS_CHECK((int64_t)wcs->naxis1 * wcs->naxis2 <= (int64_t)1 << 30, "image too large");
+    void* base;
+    size_t cap;
+    XTS_TRY(noalloc_scratch(ctx->device, &base, &cap));
+    const size_t need = model_map_arena_bytes(S, wcs, E, rule ? N : 12, flags);
+    XTS_CHECK(need <= cap,
+              "xts_point_src_model_map_wcs_dev_noalloc: %lld sources x %lld x %lld pixels x "
+              "%lld energies need %zu MiB of scratch, more than its %zu MiB",
+              (long long)S, (long long)wcs->naxis1, (long long)wcs->naxis2, (long long)E,
+              (need + (1 << 20) - 1) >> 20, cap >> 20);
+    struct ArenaScope {
+        xts_ctx* c;
+        ~ArenaScope() { c->arena.base = nullptr; c->arena.cap = c->arena.used = 0; }
+    } scope{ctx};
+    ctx->arena.base = (char*)base;
+    ctx->arena.cap = cap;
+    ctx->arena.used = 0;
     return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out, true);
 }
 
