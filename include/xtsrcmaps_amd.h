@@ -46,6 +46,14 @@ extern "C" {
  * model_map.map_correction_factor.cxx:24-50 codes it (node index 1 below one
  * degree, the whole next segment). */
 #define XTS_CF_EXACT_RADIUS 0x8
+/* Far pixels whose rule points span one table node take the learned
+ * surrogate (xts_ctx_set_surrogate) instead of the point loop; FP32 cubature
+ * only.  No counterpart in the reference. */
+#define XTS_SURROGATE 0x10
+/* With XTS_SURROGATE: the same pixels without the network, their absolute
+ * first moment taken as |A| (the value at the points' mean separation): the
+ * method without the network, for comparison. */
+#define XTS_SURROGATE_NO_NET 0x20
 
 /* contraction precision for the exposure and mean-PSF builds */
 #define XTS_PREC_FP64 0       /* FP64 FMA                                      */
@@ -310,6 +318,56 @@ int xts_pix2sph(xts_ctx* ctx, const xts_wcs* wcs, const double* pix /*[n,2]*/,
                 int64_t n, double* sph /* [n, 2] */);
 int xts_sph2pix(xts_ctx* ctx, const xts_wcs* wcs, const double* sph, int64_t n,
                 double* pix);
+
+/* ------------------------------------------------------ learned surrogate */
+/* The surrogate of the far-pixel cubature: a 4 -> 64 -> 64 -> 64 -> 1 SiLU
+ * network that predicts, from four features of a pixel's point separations
+ * about the one table node they straddle, the excess B1 - |A| of their
+ * absolute first moment over the first moment's absolute value (the part of
+ * the pixel's row weights that is not smooth in its geometry).  No
+ * counterpart in the reference, which runs the point loop of
+ * mm_cubature/convolve_pixel_psf_logsep.hpp:23-96 for every pixel.
+ * params: FP32 [XTS_MLP_PARAMS] = W1 [64][4], b1 [64], W2 [64][64], b2 [64],
+ * W3 [64][64], b3 [64], w4 [64], b4; norm: FP32 [XTS_MLP_NORM] = mu [4],
+ * 1/sd [4], y scale, y offset (x' = (x - mu)/sd, y = y' scale + offset).
+ * All pointers are device pointers; calls enqueue on the context stream. */
+#define XTS_MLP_PARAMS 8705
+#define XTS_MLP_NORM 10
+
+/* Training samples from the FP64 cubature: n_candidates random (source,
+ * pixel) pairs of the image (sources over the image widened by a quarter on
+ * each side); those the model map's row-weight pass would give the network
+ * are written compactly as rows of 8 floats (sigma_c, g_u, g_v, kappa,
+ * target B1 - |A|, A, centre separation in degrees, weight total), at most
+ * capacity rows; *count (device uint64, cleared by the caller) receives the
+ * number found.  rule [N, 3] device doubles, as for the model map. */
+int xts_surrogate_samples_dev(xts_ctx* ctx, const xts_wcs* wcs, const double* rule, int64_t N,
+                              uint64_t seed, int64_t n_candidates, int64_t capacity, float* out,
+                              uint64_t* count);
+/* FP32 forward of B samples (x rows of ldx floats, features first) -> y [B]. */
+int xts_mlp_forward_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,
+                        int64_t ldx, int64_t B, float* y);
+/* BF16 inference on the matrix cores (high and low BF16 weight parts,
+ * FP32 accumulation) -> y [B]. */
+int xts_mlp_infer_bf16_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,
+                           int64_t ldx, int64_t B, float* y);
+/* Gradient of the mean squared error of the normalised outputs over B
+ * samples (targets y_target, stride ldy) -> grad [XTS_MLP_PARAMS], loss [1]. */
+int xts_mlp_grad_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,
+                     int64_t ldx, const float* y_target, int64_t ldy, int64_t B, float* grad,
+                     float* loss);
+/* One Adam step (step >= 1) on params [XTS_MLP_PARAMS]; grad is cleared. */
+int xts_mlp_adam_dev(xts_ctx* ctx, float* params, float* grad, float* m, float* v, int64_t step,
+                     float lr, float beta1, float beta2, float eps);
+/* steps Adam steps of minibatches of `batch` rows drawn from the n samples,
+ * learning rate lr with cosine decay to lr/20; loss_hist [steps] (or NULL)
+ * receives each step's mean squared error. */
+int xts_surrogate_train_dev(xts_ctx* ctx, float* params, const float* norm, const float* x,
+                            int64_t ldx, const float* y, int64_t ldy, int64_t n, int64_t steps,
+                            int64_t batch, float lr, uint64_t seed, float* loss_hist);
+/* The network used by model maps with XTS_SURROGATE (NULL params clears it);
+ * packed into the context on its stream and synchronised. */
+int xts_ctx_set_surrogate(xts_ctx* ctx, const float* params, const float* norm);
 
 /* ------------------------------------------------------------- output */
 /* Fermi::fits::write_src_model (fits/fits.write_src_model.cxx:10-66): a FITS

@@ -1,0 +1,141 @@
+"""GPU tests of the learned surrogate of the far-pixel cubature:
+- the HIP training kernels' gradient of the mean squared error against the
+  FP64 backpropagation of tests/mlp_ref.py on a fixed minibatch (relative
+  to each parameter block's largest entry, 1e-4: FP32 sums of 200 terms);
+- one Adam step against the update rule (Kingma & Ba 2015);
+- BF16 matrix-core inference against the FP32 forward pass;
+- the network trained on the device from the device's FP64 cubature of the
+  2,000-source 400x400 0.05-degree image, then the FP32 model map with the
+  surrogate against the FP64 oracle's full cubature for 16 sources at full
+  size and 37 energies: at most 1e-4 relative per map value (the metric's
+  bound), and the same pipeline without the network (the first moment's
+  absolute value, i.e. the value at the points' mean separation) at least
+  ten times worse.
+Child process: torch (device buffers) before the library."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+_CHILD = r"""
+import sys, time
+import numpy as np
+import torch
+root = sys.argv[1]
+sys.path[:0] = [root, root + "/tests"]
+import xtsrcmaps_amd as xa
+from xtsrcmaps_amd import surrogate as sg, synthetic
+import mlp_ref as mr
+ok = True
+def check(name, cond, msg):
+    global ok
+    print(f"{name}: {msg} {'ok' if cond else 'FAILED'}", flush=True)
+    ok = ok and bool(cond)
+
+ctx = xa.Context(0)
+dev = torch.device("cuda:0")
+# one stream for torch's buffers and the library's kernels
+stream = torch.cuda.Stream()
+torch.cuda.set_stream(stream)
+ctx.set_stream(stream.cuda_stream)
+f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+rng = np.random.default_rng(11)
+# -- gradient and forward against FP64 (inputs rounded to FP32 first)
+p = (mr.init_params(5) + rng.normal(0, 0.02, mr.NPARAM)).astype(np.float32)
+norm = np.array([0.12, 0.3, 0.1, 0.002, 4.0, 3.0, 5.0, 90.0, 0.03, 0.0], np.float32)
+B = 200
+x = np.zeros((B, 8), np.float32)
+x[:, :4] = rng.normal(size=(B, 4)) * [0.25, 0.3, 0.2, 0.01] + [0.12, 0.3, 0.1, 0.002]
+x[:, 4:] = rng.normal(size=(B, 4))          # other columns of the rows: not read
+y = rng.uniform(0, 0.08, B).astype(np.float32)
+P, N, X, Y = f32(p), f32(norm), f32(x), f32(y)
+grad = torch.zeros(mr.NPARAM, dtype=torch.float32, device=dev)
+loss = torch.zeros(1, dtype=torch.float32, device=dev)
+ctx.mlp_grad_dev(P, N, X, Y, grad, loss)
+ctx.synchronize()
+lref, gref = mr.loss_and_grad(p.astype(np.float64), norm.astype(np.float64), x.astype(np.float64),
+                              y.astype(np.float64))
+g = grad.cpu().numpy().astype(np.float64)
+worst = 0.0
+for name, (a, b) in {"W1": (0, 256), "b1": (256, 320), "W2": (320, 4416), "b2": (4416, 4480),
+                     "W3": (4480, 8576), "b3": (8576, 8640), "w4": (8640, 8704),
+                     "b4": (8704, 8705)}.items():
+    worst = max(worst, np.abs(g[a:b] - gref[a:b]).max() / np.abs(gref[a:b]).max())
+check("gradient", worst < 1e-4, f"max block-relative error vs FP64 backprop {worst:.2e}")
+check("loss", abs(loss.item() - lref) / lref < 1e-5, f"{loss.item():.6e} vs {lref:.6e}")
+yf = torch.empty(B, dtype=torch.float32, device=dev)
+ctx.mlp_forward_dev(P, N, X, yf)
+ctx.synchronize()
+yr = mr.forward(p.astype(np.float64), norm.astype(np.float64), x.astype(np.float64))
+e = np.abs(yf.cpu().numpy() - yr).max() / np.abs(yr).max()
+check("forward", e < 1e-5, f"FP32 forward vs FP64 {e:.2e}")
+# -- one Adam step
+m = f32(rng.normal(0, 1e-3, mr.NPARAM)); v = f32(rng.uniform(0, 1e-6, mr.NPARAM))
+m0, v0, g0, p0 = m.cpu().numpy().astype(np.float64), v.cpu().numpy().astype(np.float64), g, p.astype(np.float64)
+ctx.mlp_adam_dev(P, grad, m, v, 3, 1e-3)
+ctx.synchronize()
+b1, b2 = 0.9, 0.999
+me, ve = b1 * m0 + (1 - b1) * g0, b2 * v0 + (1 - b2) * g0 ** 2
+pe = p0 - 1e-3 * (me / (1 - b1 ** 3)) / (np.sqrt(ve / (1 - b2 ** 3)) + 1e-8)
+ea = np.abs(P.cpu().numpy() - pe).max() / 1e-3
+check("adam", ea < 1e-3 and float(grad.abs().max()) == 0.0,
+      f"step error {ea:.1e} of the rate, gradient cleared")
+
+# -- training on the device from the FP64 cubature of the image
+cfg = synthetic.CONFIGS["c3"]
+inp = synthetic.make_inputs(cfg)
+t0 = time.perf_counter()
+net = sg.fit(ctx, inp.wcs, inp.rule, n_samples=4_000_000, steps=6000, batch=8192, seed=1)
+print(f"trained: {net.n_samples} samples in {net.seconds['samples']:.2f} s, {net.steps} steps "
+      f"in {net.seconds['train']:.2f} s, final loss {net.final_loss:.3e}", flush=True)
+hold = sg.samples(ctx, inp.wcs, inp.rule, 200_000, seed=99)
+yb = torch.empty(hold.shape[0], dtype=torch.float32, device=dev)
+y32 = torch.empty_like(yb)
+ctx.mlp_forward_dev(net.params, net.norm, hold, yb, bf16=True)
+ctx.mlp_forward_dev(net.params, net.norm, hold, y32)
+ctx.synchronize()
+tb = hold[:, 4]
+eb = (yb - tb).abs()
+db = (yb - y32).abs()
+print(f"held-out: |B1 excess| max {float(tb.max()):.3f}; network error max {float(eb.max()):.2e} "
+      f"p99 {float(eb.quantile(0.99)):.2e} mean {float(eb.mean()):.2e}; BF16 vs FP32 "
+      f"max {float(db.max()):.2e}", flush=True)
+check("bf16", float(db.max()) < 2e-3, "BF16 inference within 2e-3 of FP32 on the moment")
+
+# -- model map with the surrogate against the oracle's full cubature
+from oracle import oracle as orc
+from pipelines import oracle_stages, oracle_rule, oracle_wcs
+pick = np.linspace(0, cfg.n_src - 1, 16).round().astype(int)
+sub = type(inp)(**{**inp.__dict__, "src_sph": inp.src_sph[pick]})
+o = oracle_stages(orc, sub)
+ref = orc.point_src_model_map_wcs(cfg.npix, cfg.npix, oracle_wcs(orc, inp.wcs), o["src"],
+                                  o["expo"], o["upsf"], o["parint"], oracle_rule(inp))
+base = xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F32
+res = {}
+for name, fl in (("cubature", base), ("surrogate", base | xa.SURROGATE),
+                 ("no network", base | xa.SURROGATE | xa.SURROGATE_NO_NET)):
+    got = ctx.point_src_model_map_wcs(inp.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
+                                      inp.rule, flags=fl)
+    rel = np.abs(got - ref) / np.abs(ref)
+    res[name] = rel
+    print(f"{name}: max rel {rel.max():.3e} median {np.median(rel):.3e} "
+          f"p99.99 {np.quantile(rel, 0.9999):.3e}", flush=True)
+check("cubature", res["cubature"].max() < 1e-5, "plain FP32 cubature within 1e-5")
+check("surrogate", res["surrogate"].max() <= 1e-4, "surrogate within 1e-4")
+check("ablation", res["no network"].max() > 10 * res["surrogate"].max(),
+      "without the network at least ten times worse")
+sys.exit(0 if ok else 1)
+"""
+
+
+def test_gpu_surrogate_training_inference_and_map():
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("test_gpu_surrogate: gradient, Adam, training, BF16 inference, c3 map (child)",
+          flush=True)
+    r = subprocess.run([sys.executable, "-c", _CHILD, root], capture_output=True, text=True,
+                       timeout=280)
+    print(r.stdout[-3000:], flush=True)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]

@@ -15,8 +15,8 @@ import numpy as np
 
 from . import capi
 from .capi import (XtsError, Wcs, PROJ_CAR, PROJ_AIT, LAYOUT_SHWE, LAYOUT_SEHW,
-                   OUT_F64, OUT_F32, COMPUTE_F32, COMPUTE_F64, CF_EXACT_RADIUS, PREC_FP64,
-                   PREC_BF16X3)
+                   OUT_F64, OUT_F32, COMPUTE_F32, COMPUTE_F64, CF_EXACT_RADIUS, SURROGATE,
+                   SURROGATE_NO_NET, PREC_FP64, PREC_BF16X3)
 
 ND = 401
 
@@ -24,6 +24,7 @@ __all__ = [
     "Context", "Wcs", "XtsError", "separations", "costhetas",
     "livetime_efficiency_factors", "PROJ_CAR", "PROJ_AIT", "LAYOUT_SHWE",
     "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64", "CF_EXACT_RADIUS",
+    "SURROGATE", "SURROGATE_NO_NET",
     "PREC_FP64", "PREC_BF16X3", "ND", "write_src_model", "read_irf_pars", "read_expcube",
 ]
 
@@ -385,6 +386,78 @@ class Context:
         capi.check(capi.lib().xts_sph2pix(self._h, C.byref(wcs), capi.dptr(sph), sph.shape[0],
                                           capi.dptr(out)))
         return out
+
+    # ------------------------------------------------------- learned surrogate
+    def surrogate_samples_dev(self, wcs: Wcs, rule_dev, seed, n_candidates, out, count):
+        """Training samples of the image (xts_surrogate_samples_dev): rows of
+        8 floats into out [capacity, 8]; count: device int64 [1], zero before
+        the first call, advanced by the number of samples found."""
+        N = rule_dev.shape[0]
+        p = capi.devs([("rule", rule_dev, "float64", (N, 3)),
+                       ("out", out, "float32", (out.shape[0], 8)),
+                       ("count", count, "int64", (1,))])
+        capi.check(capi.lib().xts_surrogate_samples_dev(
+            self._h, C.byref(wcs), C.cast(p[0], capi._d), N, int(seed) & (2**64 - 1),
+            int(n_candidates), out.shape[0], p[1], p[2]))
+
+    @staticmethod
+    def _mlp_specs(params, norm, x):
+        return [("params", params, "float32", (capi.MLP_PARAMS,)),
+                ("norm", norm, "float32", (capi.MLP_NORM,)),
+                ("x", x, "float32", (x.shape[0], x.shape[1]))]
+
+    def mlp_forward_dev(self, params, norm, x, y, bf16=False):
+        """FP32 forward (or BF16 matrix-core inference) of rows x [B, >=4] -> y [B]."""
+        if x.dim() != 2 or x.shape[1] < 4:
+            raise capi.XtsError(1, "x: expected [B, >= 4]")
+        B = x.shape[0]
+        p = capi.devs(self._mlp_specs(params, norm, x) + [("y", y, "float32", (B,))])
+        fn = capi.lib().xts_mlp_infer_bf16_dev if bf16 else capi.lib().xts_mlp_forward_dev
+        capi.check(fn(self._h, p[0], p[1], p[2], x.shape[1], B, p[3]))
+
+    def mlp_grad_dev(self, params, norm, x, y_target, grad, loss):
+        """Mean-squared-error gradient over rows x [B, >=4] with targets
+        y_target [B] -> grad [8705], loss [1]."""
+        if x.dim() != 2 or x.shape[1] < 4:
+            raise capi.XtsError(1, "x: expected [B, >= 4]")
+        B = x.shape[0]
+        p = capi.devs(self._mlp_specs(params, norm, x) +
+                      [("y_target", y_target, "float32", (B,)),
+                       ("grad", grad, "float32", (capi.MLP_PARAMS,)),
+                       ("loss", loss, "float32", (1,))])
+        capi.check(capi.lib().xts_mlp_grad_dev(self._h, p[0], p[1], p[2], x.shape[1], p[3], 1,
+                                               B, p[4], p[5]))
+
+    def mlp_adam_dev(self, params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        sh = (capi.MLP_PARAMS,)
+        p = capi.devs([("params", params, "float32", sh), ("grad", grad, "float32", sh),
+                       ("m", m, "float32", sh), ("v", v, "float32", sh)])
+        capi.check(capi.lib().xts_mlp_adam_dev(self._h, *p, int(step), float(lr), float(beta1),
+                                               float(beta2), float(eps)))
+
+    def surrogate_train_dev(self, params, norm, samples, steps, batch, lr, seed, loss_hist=None):
+        """Adam training on the rows of samples [n, 8] (features 0..3, target
+        4), in place on params; loss_hist [steps] (optional) per step."""
+        n = samples.shape[0]
+        specs = [("params", params, "float32", (capi.MLP_PARAMS,)),
+                 ("norm", norm, "float32", (capi.MLP_NORM,)),
+                 ("samples", samples, "float32", (n, 8))]
+        if loss_hist is not None:
+            specs.append(("loss_hist", loss_hist, "float32", (int(steps),)))
+        p = capi.devs(specs)
+        ys = capi._vp(samples.data_ptr() + 16)   # column 4
+        capi.check(capi.lib().xts_surrogate_train_dev(
+            self._h, p[0], p[1], p[2], 8, ys, 8, n, int(steps), int(batch), float(lr),
+            int(seed) & (2**64 - 1), p[3] if loss_hist is not None else None))
+
+    def set_surrogate(self, params=None, norm=None):
+        """The network of XTS_SURROGATE model maps (None clears it)."""
+        if params is None:
+            capi.check(capi.lib().xts_ctx_set_surrogate(self._h, None, None))
+            return
+        p = capi.devs([("params", params, "float32", (capi.MLP_PARAMS,)),
+                       ("norm", norm, "float32", (capi.MLP_NORM,))])
+        capi.check(capi.lib().xts_ctx_set_surrogate(self._h, p[0], p[1]))
 
     # ------------------------------------------------------------ device variants
     def src_cosbins_dev(self, src_sph, cube, nside, out):

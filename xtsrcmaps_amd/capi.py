@@ -15,6 +15,8 @@ LAYOUT_SHWE, LAYOUT_SEHW = 0x0, 0x1
 OUT_F64, OUT_F32 = 0x0, 0x2
 COMPUTE_F32, COMPUTE_F64 = 0x0, 0x4
 CF_EXACT_RADIUS = 0x8
+SURROGATE, SURROGATE_NO_NET = 0x10, 0x20
+MLP_PARAMS, MLP_NORM = 8705, 10
 PREC_FP64, PREC_BF16X3 = 0, 1
 SCALE_PIXEL, SCALE_SRC_ENERGY = 0, 1
 
@@ -99,6 +101,16 @@ _SIGS = {
                                     C.c_int32, _vp],
     "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
                                         _d, _i64, C.c_int32, _vp],
+    "xts_surrogate_samples_dev": [_vp, C.POINTER(Wcs), _d, _i64, C.c_uint64, _i64, _i64, _vp,
+                                  _vp],
+    "xts_mlp_forward_dev": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
+    "xts_mlp_infer_bf16_dev": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
+    "xts_mlp_grad_dev": [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp],
+    "xts_mlp_adam_dev": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float,
+                         C.c_float],
+    "xts_surrogate_train_dev": [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64,
+                                C.c_float, C.c_uint64, _vp],
+    "xts_ctx_set_surrogate": [_vp, _vp, _vp],
     "xts_point_src_model_map_wcs_dev_noalloc": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp,
                                                 _i64, _d, _i64, C.c_int32, _vp],
     "xts_point_src_model_map_rows_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _i64, _d, _i64,

@@ -63,7 +63,8 @@ enum SlotId {
     kSlotIn0 = 0, kSlotIn1, kSlotIn2, kSlotIn3, kSlotIn4, kSlotIn5,
     kSlotIn6, kSlotIn7, kSlotOut0, kSlotOut1, kSlotOut2, kSlotOut3,
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
-    kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotBilerp, kSlotCount
+    kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotBilerp,
+    kSlotMlpState, kSlotMlpPacked, kSlotSurrogate, kSlotSurList, kSlotCount
 };
 
 }  // namespace xts
@@ -113,6 +114,9 @@ struct xts_ctx {
         size_t cap = 0, used = 0;
     } arena;
     xts_const_tables arena_consts;         // the constant tables inside the arena
+    // the surrogate network in its matrix-core record (xts_ctx_set_surrogate)
+    void* surrogate = nullptr;
+    bool surrogate_ready = false;
     std::vector<unsigned char> arena_rule_blob;  // host copy of the rule uploaded into it
     // arguments of a model map whose row-weight half has been enqueued and
     // whose second half has not

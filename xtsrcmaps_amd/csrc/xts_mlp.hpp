@@ -1,0 +1,191 @@
+// xts_mlp.hpp -- the surrogate network of the far-pixel cubature: a
+// 4 -> 64 -> 64 -> 64 -> 1 multilayer perceptron with SiLU activations, its
+// parameter layout, and its BF16 inference on the matrix cores.
+//
+// What it predicts.  For a pixel whose rule points all lie within two
+// consecutive intervals of the PSF table's separation grid (one node, at
+// separation s1, inside the points' range), the cubature reduces to three
+// row weights, which are linear in three sums over the rule points: the
+// weight total, A = sum w sigma and B1 = sum w |sigma|, with sigma = (s -
+// s1) / (s1 - s0) the point's separation in units of the interval below the
+// node.  A is smooth in the pixel's geometry and a 2x2 Gauss-Legendre rule
+// of sigma gives it; B1 is not (it has a kink wherever a point crosses the
+// node), and is what the network predicts, as the excess y = B1 - |A| >= 0,
+// from four features of sigma over the pixel's bilinear parameter square
+// (u, v in [-1, 1]): its centre value, its half-differences across the
+// pixel along u and along v, and the sum of its four edge-midpoint second
+// differences (the curvature, which for a pixel far from the source lies
+// across the radial direction).  B1 is energy independent, so one network
+// evaluation serves every energy plane; the pixel-integrated PSF of each
+// energy follows exactly from the three row weights and the table.
+//
+// Parameters (FP32, 8705): W1 [64][4], b1 [64], W2 [64][64], b2 [64],
+// W3 [64][64], b3 [64], w4 [64], b4; plus a normalisation record
+// (mu [4], inv_sd [4], y_scale, y_offset): x' = (x - mu) inv_sd, y = y'
+// y_scale + y_offset.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+namespace xts {
+
+constexpr int kMlpIn = 4, kMlpH = 64;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpH * kMlpIn;
+constexpr int kOffW2 = kOffB1 + kMlpH, kOffB2 = kOffW2 + kMlpH * kMlpH;
+constexpr int kOffW3 = kOffB2 + kMlpH, kOffB3 = kOffW3 + kMlpH * kMlpH;
+constexpr int kOffW4 = kOffB3 + kMlpH, kOffB4 = kOffW4 + kMlpH;
+constexpr int kMlpParams = kOffB4 + 1;   // 8705
+constexpr int kMlpNorm = 10;             // mu[4], inv_sd[4], y_scale, y_offset
+
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t mlp_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float mlp_from_bf16(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline float silu(float x) { return x * __frcp_rn(1.0f + __expf(-x)); }
+
+// Inference record in the matrix cores' operand order.  A 16x16 weight tile
+// (outputs 16 To .., inputs 16 Ti ..) is 64 lanes of four BF16 values: lane
+// l holds W[16 To + l % 16][16 Ti + 4 (l / 16) + j], j = 0..3, the A operand
+// of v_mfma_f32_16x16x16_bf16 for a product W . H^T whose B operand, lane l,
+// is sample l % 16's inputs 16 Ti + 4 (l / 16) + j -- exactly how the
+// previous layer's result D (lane l: outputs 16 To + 4 (l / 16) + i of
+// sample l % 16) lies in registers, so layers chain without a transpose.
+// The first layer's 16-wide K holds the inputs' high and low BF16 parts
+// against the weights' high and low parts (x_hi w_hi + x_lo w_hi + x_hi
+// w_lo); the hidden layers use a high and a low weight tile each.
+struct MlpPacked {
+    mlp_bf16x4 w1[4][64];          // [To][lane]
+    mlp_bf16x4 w2[2][16][64];      // [hi/lo][To * 4 + Ti][lane]
+    mlp_bf16x4 w3[2][16][64];
+    float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
+    float b4, norm[kMlpNorm];
+    float pad[5];
+};
+static_assert(sizeof(MlpPacked) % 16 == 0, "MlpPacked size");
+
+// Pack FP32 parameters into the inference record (one block of 256 threads).
+__device__ inline void mlp_pack(const float* __restrict__ p, const float* __restrict__ norm,
+                                MlpPacked* __restrict__ out, int tid, int nthr) {
+    for (int k = tid; k < 4 * 64; k += nthr) {
+        const int To = k / 64, l = k % 64, g = l / 16, o = 16 * To + l % 16;
+        mlp_bf16x4 v;
+        for (int j = 0; j < 4; ++j) {
+            const float w = p[kOffW1 + o * kMlpIn + j];
+            const uint16_t hi = mlp_bf16(w), lo = mlp_bf16(w - mlp_from_bf16(hi));
+            v[j] = (short)(g <= 1 ? hi : (g == 2 ? lo : 0));
+        }
+        out->w1[To][l] = v;
+    }
+    for (int k = tid; k < 2 * 16 * 64; k += nthr) {
+        const int layer = k / (16 * 64), r = k % (16 * 64);
+        const int t = r / 64, l = r % 64, To = t / 4, Ti = t % 4;
+        const float* W = p + (layer == 0 ? kOffW2 : kOffW3);
+        mlp_bf16x4 hi4, lo4;
+        for (int j = 0; j < 4; ++j) {
+            const float w = W[(16 * To + l % 16) * kMlpH + 16 * Ti + 4 * (l / 16) + j];
+            const uint16_t hi = mlp_bf16(w);
+            hi4[j] = (short)hi;
+            lo4[j] = (short)mlp_bf16(w - mlp_from_bf16(hi));
+        }
+        mlp_bf16x4(*dst)[16][64] = layer == 0 ? out->w2 : out->w3;
+        dst[0][t][l] = hi4;
+        dst[1][t][l] = lo4;
+    }
+    for (int k = tid; k < kMlpH; k += nthr) {
+        out->b1[k] = p[kOffB1 + k];
+        out->b2[k] = p[kOffB2 + k];
+        out->b3[k] = p[kOffB3 + k];
+        out->w4[k] = p[kOffW4 + k];
+    }
+    if (tid == 0) {
+        out->b4 = p[kOffB4];
+        for (int k = 0; k < kMlpNorm; ++k) out->norm[k] = norm[k];
+        for (int k = 0; k < 5; ++k) out->pad[k] = 0.0f;
+    }
+}
+
+__device__ inline mlp_bf16x4 mlp_silu_bf16(mlp_f32x4& acc) {
+    mlp_bf16x4 h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[i] = (short)mlp_bf16(silu(acc[i]));
+    return h;
+}
+
+// One wave evaluates 16 samples: lane l carries sample l % 16's raw
+// features x[4] (every lane of the sample the same values); the result, for
+// sample l % 16, is returned in every lane.  m: the record (LDS or global).
+__device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float (&x)[4]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    // normalised inputs, split into BF16 high and low parts by lane group
+    mlp_bf16x4 in;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float xn = (x[j] - m->norm[j]) * m->norm[4 + j];
+        const uint16_t hi = mlp_bf16(xn);
+        const uint16_t lo = mlp_bf16(xn - mlp_from_bf16(hi));
+        in[j] = (short)(g == 1 ? lo : (g == 3 ? 0 : hi));
+    }
+    mlp_bf16x4 H[4];
+#pragma unroll
+    for (int To = 0; To < 4; ++To) {
+        mlp_f32x4 acc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = m->b1[16 * To + 4 * g + i];
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w1[To][lane], in, acc, 0, 0, 0);
+        H[To] = mlp_silu_bf16(acc);
+    }
+    // hidden layer 2
+    mlp_f32x4 acc2[4];
+#pragma unroll
+    for (int To = 0; To < 4; ++To) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc2[To][i] = m->b2[16 * To + 4 * g + i];
+#pragma unroll
+        for (int Ti = 0; Ti < 4; ++Ti) {
+            acc2[To] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w2[1][To * 4 + Ti][lane],
+                                                                 H[Ti], acc2[To], 0, 0, 0);
+            acc2[To] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w2[0][To * 4 + Ti][lane],
+                                                                 H[Ti], acc2[To], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int To = 0; To < 4; ++To) H[To] = mlp_silu_bf16(acc2[To]);
+    // hidden layer 3, its activations kept in FP32 for the output dot product
+    float y = 0.0f;
+#pragma unroll
+    for (int To = 0; To < 4; ++To) {
+        mlp_f32x4 acc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = m->b3[16 * To + 4 * g + i];
+#pragma unroll
+        for (int Ti = 0; Ti < 4; ++Ti) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w3[1][To * 4 + Ti][lane], H[Ti],
+                                                            acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w3[0][To * 4 + Ti][lane], H[Ti],
+                                                            acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], silu(acc[i]), y);
+    }
+    // the four lane groups hold disjoint outputs of the same sample
+    y += __shfl_xor(y, 16);
+    y += __shfl_xor(y, 32);
+    return fmaf(y + m->b4, m->norm[8], m->norm[9]);
+}
+
+}  // namespace xts

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "xts_internal.hpp"
+#include "xts_mlp.hpp"
 
 namespace xts {
 
@@ -202,6 +203,12 @@ struct CubArgs {
     double rho_scale;          // max over points of sum_k |b_k|
     double wsum;               // sum over points of w/4
     double j0, jk;             // j = j0 + jk log2(asin(chord / 2))
+    // learned surrogate of the one-node far pixels: 0 off, 1 the network,
+    // 2 without it (|A| for the absolute moment)
+    int sur;
+    float4* sur_list;          // [S][H W][2]: features; A, pixel index bits
+    int* sur_count;            // [S]
+    const MlpPacked* sur_net;
 };
 
 __device__ inline float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994530942f; }
@@ -762,6 +769,45 @@ __device__ inline bool poly_dispatch(const CubArgs& a, const LeanRule& lr,
     return true;
 }
 
+// ---- the surrogate's inputs for a one-node pixel (xts_mlp.hpp): sigma of
+// the degree-D series at the centre and the four edge midpoints of the
+// pixel's (u, v) square, and A from the 2x2 Gauss-Legendre rule of sigma
+// (times the rule's weight total) in place of the point loop.
+template <int D>
+__device__ inline void sur_pixel(const CubArgs& a, const PixelGeom<float>& pg, int r0,
+                                 float wsum, float (&f)[4], float& A) {
+    const PolyPixel pp = poly_pixel<D>(a, pg, r0);
+    auto sig = [&](float u, float v) {
+        const float xu = fmaf(fmaf(pp.x2, u, pp.x1), u, pp.x0);
+        const float bu = fmaf(fmaf(pp.b2, u, pp.b1), u, pp.b0);
+        const float cu = fmaf(fmaf(pp.c2, u, pp.c1), u, pp.c0);
+        const float x = fmaf(fmaf(cu, v, bu), v, xu);
+        float sg = pp.c[D];
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) sg = fmaf(sg, x, pp.c[k]);
+        return sg;
+    };
+    const float sc = sig(0.f, 0.f);
+    const float su = sig(1.f, 0.f), sd = sig(-1.f, 0.f), sv = sig(0.f, 1.f), sw = sig(0.f, -1.f);
+    f[0] = sc;
+    f[1] = 0.5f * (su - sd);
+    f[2] = 0.5f * (sv - sw);
+    f[3] = (su + sd) + (sv + sw) - 4.0f * sc;
+    constexpr float g = 0.57735026918962576f;   // 1 / sqrt(3)
+    A = 0.25f * wsum * ((sig(-g, -g) + sig(g, -g)) + (sig(-g, g) + sig(g, g)));
+}
+
+__device__ inline bool sur_dispatch(const CubArgs& a, const PixelGeom<float>& pg, int r0,
+                                    float wsum, float rx, float (&f)[4], float& A) {
+    if (__all(rx <= kPolyReach[0])) sur_pixel<2>(a, pg, r0, wsum, f, A);
+    else if (__all(rx <= kPolyReach[1])) sur_pixel<3>(a, pg, r0, wsum, f, A);
+    else if (__all(rx <= kPolyReach[2])) sur_pixel<4>(a, pg, r0, wsum, f, A);
+    else if (__all(rx <= kPolyReach[3])) sur_pixel<5>(a, pg, r0, wsum, f, A);
+    else if (__all(rx <= kPolyReach[4])) sur_pixel<6>(a, pg, r0, wsum, f, A);
+    else return false;
+    return true;
+}
+
 // Row weights of rows r0 .. r0+3 from the per-row sums S_k of w/4 and T_k
 // of w t / 4 given as S0, T0, the cumulative S01, T01 and the totals wsum,
 // Tt: row k receives S_k - T_k from its own points and T_{k-1} from those
@@ -788,7 +834,7 @@ __device__ inline void rows_from_sums(T wsum, T S0, T T0, T S01, T T01, T Tt, T 
 // wave lies below 28 degrees, the wave runs a branch-free FP32 loop over the
 // points in groups of four (grid-structured rules reuse the blend of each
 // rule row).  Occupancy: 6 waves per SIMD for FP32 (80 VGPRs), 5 for FP64.
-template <typename T>
+template <typename T, bool SUR = false>
 __global__ void __launch_bounds__(kThreads)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 5)))
 rowweight_kernel(CubArgs a) {
@@ -847,6 +893,8 @@ rowweight_kernel(CubArgs a) {
         const bool lean6 = sizeof(T) == 4 ? __any(fast) && __all(!fast || pg.shi < T(28.0))
                                           : __all(fast && pg.shi < T(28.0));
         bool series = false;   // the wave's row weights came from a point loop below
+        bool listed = false;   // handed to the surrogate pass (sur_rows_kernel)
+        float sf[4] = {0.f, 0.f, 0.f, 0.f}, sA = 0.f;
         if constexpr (sizeof(T) == 4) {
             if (lean6) {
                 // every fast pixel's points within three rows (one boundary)
@@ -855,8 +903,19 @@ rowweight_kernel(CubArgs a) {
                 // a fast pixel spans at most three row intervals, shi / slo <
                 // e^(3 step), so rho < 0.051 and r < 0.104: degree 6 always
                 // reaches; a wave it did not would take the per-point path
-                series = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
-                             : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
+                if (SUR && one) {
+                    // surrogate: A from four points; pixels whose bounds hold
+                    // no node have B1 = |A| exactly, the others go to the
+                    // network
+                    series = sur_dispatch(a, pg, r0, wsum, rx, sf, sA);
+                    if (series) {
+                        listed = fast && ihi == ilo + 1;
+                        poly_rows<1>(wsum, sA, fabsf(sA), 0.0f, W);
+                    }
+                } else {
+                    series = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
+                                 : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
+                }
             }
         } else if (lean6) {
             series = true;
@@ -912,7 +971,15 @@ rowweight_kernel(CubArgs a) {
             }
             rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
         }
-        if (valid) {
+        if (valid && listed) {
+            // the surrogate pass writes its row weights and histogram rows
+            const int pix = h * a.W + w;
+            rw_row[pix] = r0;
+            const int k = atomicAdd(&a.sur_count[s], 1);
+            float4* e = a.sur_list + 2 * ((size_t)s * plane + k);
+            e[0] = float4{sf[0], sf[1], sf[2], sf[3]};
+            e[1] = float4{sA, __int_as_float(pix), 0.0f, 0.0f};
+        } else if (valid) {
             const int pix = h * a.W + w;
             if (fast) {
                 // row weights x solid angle
@@ -936,6 +1003,74 @@ rowweight_kernel(CubArgs a) {
         __syncthreads();
         for (int k = tid; k < kND; k += kThreads)
             if (hist_s[k] != 0.0) atomicAdd(&a.hist[(size_t)s * kND + k], hist_s[k]);
+    }
+}
+
+// Pass 1b (surrogate mode): the listed one-node pixels of each source.  Each
+// wave evaluates the network for 16 pixels at a time (xts_mlp.hpp), forms
+// B1 = |A| + max(y, 0) (y = 0 without the network), the three row weights
+// (poly_rows), stores them x solid angle and adds the in-radius ones to the
+// block's LDS histogram, added to the source's once.  Grid: (blocks
+// striding over the list, S).
+constexpr int kSurThreads = 256;
+__global__ void __launch_bounds__(kSurThreads)
+sur_rows_kernel(CubArgs a) {
+    __shared__ MlpPacked net;
+    __shared__ double hblk[kNDPad];
+    const int s = blockIdx.y;
+    const int count = a.sur_count[s];
+    const int waves = kSurThreads / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+    if ((int)blockIdx.x * waves * 16 >= count) return;
+    const bool use_net = a.sur == 1;
+    if (use_net) {
+        const float4* src = reinterpret_cast<const float4*>(a.sur_net);
+        float4* dst = reinterpret_cast<float4*>(&net);
+        for (int k = threadIdx.x; k < (int)(sizeof(MlpPacked) / 16); k += kSurThreads)
+            dst[k] = src[k];
+    }
+    for (int k = threadIdx.x; k < kND; k += kSurThreads) hblk[k] = 0.0;
+    __syncthreads();
+    const SrcGeom g = a.geo[s];
+    const bool do_hist = a.hist && g.in_fov;
+    const size_t plane = (size_t)a.H * a.W;
+    const int Wp1 = a.W + 1;
+    const float4* list = a.sur_list + 2 * (size_t)s * plane;
+    float4* rw = reinterpret_cast<float4*>(a.rw) + (size_t)s * plane;
+    const int* rw_row = a.rw_row + (size_t)s * plane;
+    const float wsum = (float)a.wsum;
+    // every wave runs the same number of groups (uniform exit), lanes past
+    // the end evaluate the last entry and store nothing
+    const int ngroups = (count + 15) / 16;
+    for (int grp = blockIdx.x * waves + wave; grp < ngroups; grp += gridDim.x * waves) {
+        const int i = grp * 16 + (lane & 15);
+        const int ic = min(i, count - 1);
+        const float4 f = list[2 * ic], e = list[2 * ic + 1];
+        float y = 0.0f;
+        if (use_net) {
+            const float x[4] = {f.x, f.y, f.z, f.w};
+            y = mlp_infer16(&net, x);
+        }
+        if (lane < 16 && i < count) {
+            const float A = e.x;
+            const int pix = __float_as_int(e.y);
+            float W[4];
+            poly_rows<1>(wsum, A, fabsf(A) + fmaxf(y, 0.0f), 0.0f, W);
+            const float om = a.omega ? (float)a.omega[pix] : 1.0f;
+            const float4 wv = float4{om * W[0], om * W[1], om * W[2], om * W[3]};
+            rw[pix] = wv;
+            const int h = pix / a.W, w = pix % a.W;
+            if (do_hist && in_radius(g, a.centres, h * Wp1 + w)) {
+                const int r0 = rw_row[pix];
+                atomicAdd(&hblk[r0], (double)wv.x);
+                atomicAdd(&hblk[r0 + 1], (double)wv.y);
+                atomicAdd(&hblk[r0 + 2], (double)wv.z);
+            }
+        }
+    }
+    if (do_hist) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < kND; k += kSurThreads)
+            if (hblk[k] != 0.0) atomicAdd(&a.hist[(size_t)s * kND + k], hblk[k]);
     }
 }
 
@@ -1779,6 +1914,10 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
 // emit).  kAll runs both in one call.
 enum { kPhaseRows = 1, kPhaseFinish = 2, kPhaseAll = 3 };
 
+// Per-source int counters (near, wide and surrogate pixels), in doubles of
+// the accumulator slot.
+static size_t model_map_counter_doubles(int64_t S) { return (size_t)(3 * S + 1) / 2; }
+
 static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                             const double* expo, const double* upsf, const double* parint,
                             int64_t E, const double* rule, int64_t N, int32_t flags, void* out,
@@ -1794,11 +1933,16 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_CHECK(src_sph, "null device pointer");
     XTS_CHECK(!(phase & kPhaseFinish) || (upsf && out), "null device pointer");
     XTS_CHECK(!(phase & kPhaseFinish) || !full || (expo && parint), "null exposure/parint");
-    XTS_CHECK((flags & ~0xF) == 0, "unknown flags 0x%x", flags);
+    XTS_CHECK((flags & ~0x3F) == 0, "unknown flags 0x%x", flags);
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     XTS_CHECK((int64_t)H * W <= (int64_t)1 << 30, "image of %lld pixels exceeds 2^30",
               (long long)H * W);
     const bool f64 = (flags & XTS_COMPUTE_F64) != 0;
+    const int sur = (flags & XTS_SURROGATE) ? ((flags & XTS_SURROGATE_NO_NET) ? 2 : 1) : 0;
+    XTS_CHECK(!(flags & XTS_SURROGATE_NO_NET) || sur, "XTS_SURROGATE_NO_NET needs XTS_SURROGATE");
+    XTS_CHECK(!sur || (!f64 && full), "the surrogate serves the FP32 model map only");
+    XTS_CHECK(sur != 1 || ctx->surrogate_ready,
+              "XTS_SURROGATE: no network set (xts_ctx_set_surrogate)");
     const int sehw = (flags & XTS_LAYOUT_SEHW) ? 1 : 0;
     const int of32 = (flags & XTS_OUT_F32) ? 1 : 0;
 
@@ -1822,7 +1966,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     // counters [2 S] (int, padded to doubles: near and wide pixels per
     // source) and FP64 accumulators: hist [S][401], mapint [S][E], factor
     // [S][E]; the counters and hist start at zero
-    const size_t cnt_d = (size_t)S;
+    const size_t cnt_d = model_map_counter_doubles(S);
     const size_t nacc = cnt_d + (full ? (size_t)S * kND + 2 * (size_t)S * E : 0);
     void* accp;
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
@@ -1872,6 +2016,15 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.wide_list = ca.near_rr + (size_t)S * plane;
     ca.near_count = (int*)acc;
     ca.wide_count = ca.near_count + S;
+    ca.sur_count = ca.wide_count + S;
+    ca.sur = sur;
+    ca.sur_net = sur == 1 ? (const MlpPacked*)ctx->surrogate : nullptr;
+    ca.sur_list = nullptr;
+    if (sur) {
+        void* sl;
+        XTS_TRY(slot_reserve(ctx, kSlotSurList, (size_t)S * plane * 32, &sl));
+        ca.sur_list = (float4*)sl;
+    }
     ca.near_pool = pool;
     ca.near_pool_cap = (int)pool_cap;
     ca.hist = full ? acc + cnt_d : nullptr;
@@ -1909,7 +2062,15 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
         if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        else if (sur) rowweight_kernel<float, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
         else rowweight_kernel<float><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        XTS_HIP(hipGetLastError());
+    }
+    // pass 1b: the surrogate's pixels, blocks stride over each source's list
+    if (sur && (phase & kPhaseRows)) {
+        const dim3 grid((unsigned)std::max<int64_t>(1, 4 * want_blocks / S), (unsigned)S);
+        TimedRegion tr(ctx, "surrogate");
+        sur_rows_kernel<<<grid, kSurThreads, 0, ctx->stream>>>(ca);
         XTS_HIP(hipGetLastError());
     }
     // pass 2a: blocks stride over each source's list
@@ -2018,7 +2179,8 @@ static size_t model_map_arena_bytes(int64_t S, const xts_wcs* wcs, int64_t E, in
     b += arena_bytes(const_tables_bytes());                                   // tables
     b += arena_bytes(rec + (size_t)kLeanArrays * N4 * sizeof(float));        // rule
     b += arena_bytes((size_t)S * kND * Ep * tsz);                             // PSF table
-    b += arena_bytes(((size_t)S + (size_t)S * kND + 2 * (size_t)S * E) * 8);  // accumulators
+    b += arena_bytes((model_map_counter_doubles(S) + (size_t)S * kND + 2 * (size_t)S * E) *
+                     8);                                                      // accumulators
     b += arena_bytes((size_t)S * plane * (f64 ? 32 : 16) + (size_t)S * plane * 20 + 512);
     b += arena_bytes((size_t)(pool_cap * S) * tsz);                          // near pool
     return b;
@@ -2054,6 +2216,8 @@ int xts_point_src_model_map_wcs_dev_noalloc(xts_ctx* ctx, const xts_wcs* wcs,
     XTS_CHECK(rule == nullptr || (N > 0 && N <= 1024), "rule size %lld out of range (1..1024)",
               (long long)N);
     XTS_CHECK((int64_t)wcs->naxis1 * wcs->naxis2 <= (int64_t)1 << 30, "image too large");
+    XTS_CHECK(!(flags & (XTS_SURROGATE | XTS_SURROGATE_NO_NET)),
+              "xts_point_src_model_map_wcs_dev_noalloc: the surrogate is not offered here");
     void* base;
     size_t cap;
     XTS_TRY(noalloc_scratch(ctx->device, &base, &cap));

@@ -44,6 +44,9 @@ CONFIGS = {
     # the larger configurations) through the plain cubature, FP32: 1.18e10
     # evaluations, a 47 GB map
     "c3p": Config("c3p", 2000, 400, 0.05, rule="gl8", seed=3, out_f32=True, src_mode="uniform"),
+    # the surrogate workload: the same 2,000 sources on 400x400 at 0.05 deg,
+    # 37 energies, the 8x8 Gauss-Legendre rule, FP32 map
+    "c3": Config("c3", 2000, 400, 0.05, rule="gl8", seed=3, out_f32=True, src_mode="uniform"),
 }
 
 
