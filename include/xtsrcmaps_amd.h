@@ -111,11 +111,19 @@ int xts_abi_version(void);
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
 /* Launch-shape override for the model map (no reference counterpart):
- * "tiles_per_block" (16x16-pixel tiles per block of the row-weight pass);
- * 0 restores the default choice.  Results do not depend on it. */
+ * "tiles_per_block" (16x16-pixel tiles per block of the row-weight pass;
+ * 0 restores the default choice); "geometry_cache" (1, the default: the
+ * pixel geometry and solid angles of the last image are kept between model
+ * maps of that image; 0: recomputed by every call).  Results do not depend
+ * on either. */
 int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value);
 int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms,
                          int64_t* count);
+/* The shape the last batched model map ran with: "batch_sources" (0: the
+ * default of about six batches), "rowweight_blocks" (0: the full grid),
+ * "batches"; and the counts "graph_captures", "graph_replays" of the
+ * context's batched calls.  No reference counterpart. */
+int xts_ctx_get_tuning(xts_ctx* ctx, const char* key, int64_t* value);
 
 /* ------------------------------------------------------------- (a) exposure */
 /* Exposure::costhetas (exposure/exposure.cxx:29-40). */
@@ -131,6 +139,13 @@ int xts_livetime_efficiency_factors(const double* logEs, int64_t E,
 int xts_aeff_value(xts_ctx* ctx, const double* costh, int64_t C,
                    const double* logE, int64_t E, const xts_aeff_grid* grid,
                    double* out /* [C, E] */);
+
+/* Device variant: grid arrays, costh, logE and out are device pointers.
+ * Samples outside the grid set *range_error (a device int32 the caller
+ * clears; NULL: not reported) and give 0. */
+int xts_aeff_value_dev(xts_ctx* ctx, const double* costh, int64_t C,
+                       const double* logE, int64_t E, const xts_aeff_grid* grid,
+                       double* out, int32_t* range_error);
 
 /* Exposure::src_cosbins (exposure/exposure.cxx:42-63) with
  * Healpix::ang2pix NESTED (healpix/healpix.cxx:182-229). */
@@ -274,6 +289,51 @@ int xts_point_src_model_map_finish_dev(xts_ctx* ctx, const xts_wcs* wcs,
                                        const double* parint, int64_t E,
                                        const double* rule, int64_t N,
                                        int32_t flags, void* out);
+
+/* ------------------------------------------------- batched model maps */
+/* Many sources over several images in one call (no counterpart in the
+ * reference, which maps one image's sources in one dense loop,
+ * mm_cubature/cubature.implementation.cxx:33-53).  The sources of image i
+ * are [src_begin[i], src_begin[i+1]) of the source arrays (src_begin[0] = 0,
+ * n_images + 1 entries, host memory); wcs [n_images] host.  Each source's
+ * values are written for the pixels of its footprint box only, as an FP32
+ * [E][h1 - h0][w1 - w0] block at `offset` elements of the output. */
+typedef struct {
+    int64_t h0, h1, w0, w1; /* pixel indices [h0, h1) x [w0, w1) of its image */
+    int64_t offset;         /* element offset of its block in the output */
+} xts_footprint;
+
+/* Footprint boxes: per source the largest over energies of the separation
+ * at which the normalised cumulative PSF integral (parint, device [S, 401,
+ * E]) reaches `containment` (e.g. 0.995; psf.partial_total_integral.cxx:
+ * 23-48), plus the pixel half-diagonal (radius [S] deg, host, may be NULL),
+ * and the pixel box of that disk about the source; consecutive offsets;
+ * *total = the output's element count.  src_sph device [S, 2].  Host
+ * results: synchronises the context stream. */
+int xts_model_map_footprints(xts_ctx* ctx, int64_t n_images, const xts_wcs* wcs,
+                             const int64_t* src_begin, const double* src_sph,
+                             const double* parint, int64_t E, double containment,
+                             xts_footprint* boxes, double* radius, int64_t* total);
+
+/* The batched model map: for each source, xts_point_src_model_map_wcs_dev's
+ * values (its own image, exposure, PSF table, correction factor) inside its
+ * box.  Device pointers as for that entry point; boxes host [S] (from
+ * xts_model_map_footprints); out device float [out_elems]; flags must hold
+ * XTS_LAYOUT_SEHW | XTS_OUT_F32 and FP32 cubature (XTS_SURROGATE and
+ * XTS_CF_EXACT_RADIUS apply).  The work runs as a cost-ordered list of
+ * source batches alternating between the context's stream and a second
+ * internal stream (row weights of one batch beside the emit of the other);
+ * it is ordered after the work already on the context stream and the
+ * context stream waits for all of it.  Tuning keys: "batch_sources",
+ * "rowweight_blocks", "autotune" (time candidate shapes on the first call
+ * of a workload shape, keep the fastest), "graph" (capture the launches in a
+ * HIP graph on the first call with given arguments, replay afterwards). */
+int xts_point_src_model_maps_batched_dev(xts_ctx* ctx, int64_t n_images, const xts_wcs* wcs,
+                                         const int64_t* src_begin, const double* src_sph,
+                                         const double* exposure, const double* upsf,
+                                         const double* parint, int64_t E, const double* rule,
+                                         int64_t N, int32_t flags, const xts_footprint* boxes,
+                                         float* out, int64_t out_elems);
 
 /* cubature_convolve_map_psf (mm_cubature/cubature.implementation.cxx:21-55):
  * the raw cubature only. */

@@ -397,7 +397,7 @@ import torch
 sys.path.insert(0, sys.argv[1])
 import bench
 import xtsrcmaps_amd as xa
-cfg, inp, ctx, d = bench.setup("c3p")
+cfg, inp, ctx, d = bench.setup("c3p", surrogate=False)
 bench.step(ctx, inp, d)
 torch.cuda.synchronize()
 out = d["out"]
@@ -426,7 +426,7 @@ import torch
 sys.path.insert(0, sys.argv[1])
 import bench
 import xtsrcmaps_amd as xa
-cfg, inp, ctx, d = bench.setup("c1")
+cfg, inp, ctx, d = bench.setup("c1", surrogate=False)
 flags = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32
 E = cfg.n_energy
 for _ in range(2):

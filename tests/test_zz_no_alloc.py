@@ -23,7 +23,7 @@ sys.path.insert(0, sys.argv[1])
 import bench
 import xtsrcmaps_amd as xa
 from xtsrcmaps_amd import capi
-cfg, inp, ctx, d = bench.setup("c1")
+cfg, inp, ctx, d = bench.setup("c1", surrogate=False)
 d["serial"] = True
 bench.step(ctx, inp, d)                 # exposure and PSF on the device
 torch.cuda.synchronize()

@@ -150,6 +150,12 @@ class Context:
     def synchronize(self):
         capi.check(capi.lib().xts_ctx_synchronize(self._h))
 
+    def get_tuning(self, key: str) -> int:
+        """The last batched model map's shape: "batch_sources", "rowweight_blocks", "batches"."""
+        v = C.c_int64()
+        capi.check(capi.lib().xts_ctx_get_tuning(self._h, key.encode(), C.byref(v)))
+        return int(v.value)
+
     def set_tuning(self, key: str, value: int):
         """Launch-shape override ("tiles_per_block"; 0 = default)."""
         capi.check(capi.lib().xts_ctx_set_tuning(self._h, key.encode(), int(value)))
@@ -387,6 +393,46 @@ class Context:
                                           capi.dptr(out)))
         return out
 
+    # ---------------------------------------------------- batched model maps
+    def model_map_footprints(self, wcs_list, src_begin, src_sph, parint, containment=0.995):
+        """Footprint boxes of every source (xts_model_map_footprints): returns
+        (boxes [S] capi.Footprint array, radius [S] deg, total output elements)."""
+        n = len(wcs_list)
+        begin = np.ascontiguousarray(src_begin, np.int64)
+        S = int(begin[-1])
+        E = parint.shape[2]
+        p = capi.devs([("src_sph", src_sph, "float64", (S, 2)),
+                       ("parint", parint, "float64", (S, 401, E))])
+        W = (capi.Wcs * n)(*wcs_list)
+        boxes = (capi.Footprint * S)()
+        radius = np.zeros(S)
+        total = C.c_int64()
+        capi.check(capi.lib().xts_model_map_footprints(
+            self._h, n, W, begin.ctypes.data_as(C.POINTER(C.c_int64)), p[0], p[1], E,
+            float(containment), boxes, capi.dptr(radius), C.byref(total)))
+        return boxes, radius, int(total.value)
+
+    def point_src_model_maps_batched_dev(self, wcs_list, src_begin, src_sph, exposure, upsf,
+                                         parint, boxes, out, rule=None,
+                                         flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):
+        """Batched model maps of several images into the footprint blocks of
+        out (float32 device tensor of the footprints' total size)."""
+        n = len(wcs_list)
+        begin = np.ascontiguousarray(src_begin, np.int64)
+        S, E = exposure.shape
+        if int(begin[-1]) != S or len(boxes) != S:
+            raise capi.XtsError(1, "src_begin, boxes and the source arrays disagree")
+        p = capi.devs([("src_sph", src_sph, "float64", (S, 2)),
+                       ("exposure", exposure, "float64", (S, E)),
+                       ("upsf", upsf, "float64", (S, 401, E)),
+                       ("parint", parint, "float64", (S, 401, E)),
+                       ("out", out, "float32", None, out.numel())])
+        W = (capi.Wcs * n)(*wcs_list)
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_point_src_model_maps_batched_dev(
+            self._h, n, W, begin.ctypes.data_as(C.POINTER(C.c_int64)), p[0], p[1], p[2], p[3],
+            E, rp, N, int(flags), boxes, p[4], out.numel()))
+
     # ------------------------------------------------------- learned surrogate
     def surrogate_samples_dev(self, wcs: Wcs, rule_dev, seed, n_candidates, out, count):
         """Training samples of the image (xts_surrogate_samples_dev): rows of
@@ -460,6 +506,23 @@ class Context:
         capi.check(capi.lib().xts_ctx_set_surrogate(self._h, p[0], p[1]))
 
     # ------------------------------------------------------------ device variants
+    def aeff_value_dev(self, costh, logE, grid, out, range_error=None):
+        """Device Irf::aeff_value: grid = (cosths [Mc], logEs [Me], params
+        [Mc, Me], min_costheta) with device tensors -> out [C, E]."""
+        gc, ge, gp, minc = grid
+        Cn, E, Mc, Me = costh.shape[0], logE.shape[0], gc.shape[0], ge.shape[0]
+        specs = [("costh", costh, "float64", (Cn,)), ("logE", logE, "float64", (E,)),
+                 ("aeff cosths", gc, "float64", (Mc,)), ("aeff logEs", ge, "float64", (Me,)),
+                 ("aeff params", gp, "float64", (Mc, Me)), ("out", out, "float64", (Cn, E))]
+        if range_error is not None:
+            specs.append(("range_error", range_error, "int32", (1,)))
+        p = capi.devs(specs)
+        g = capi.AeffGrid(C.cast(p[4], capi._d), C.cast(p[2], capi._d), C.cast(p[3], capi._d),
+                          Mc, Me, float(minc))
+        capi.check(capi.lib().xts_aeff_value_dev(
+            self._h, C.cast(p[0], capi._d), Cn, C.cast(p[1], capi._d), E, C.byref(g),
+            C.cast(p[5], capi._d), p[6] if range_error is not None else None))
+
     def src_cosbins_dev(self, src_sph, cube, nside, out):
         """Device-pointer Exposure::src_cosbins: out [S, C] from cube
         [12 nside^2, C] (async)."""

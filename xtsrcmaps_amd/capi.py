@@ -61,6 +61,11 @@ class PsfGrid(C.Structure):
                 ("Me", C.c_int64)]
 
 
+class Footprint(C.Structure):
+    _fields_ = [("h0", C.c_int64), ("h1", C.c_int64), ("w0", C.c_int64), ("w1", C.c_int64),
+                ("offset", C.c_int64)]
+
+
 class AeffGrid(C.Structure):
     _fields_ = [("params", _d), ("cosths", _d), ("logEs", _d), ("Mc", C.c_int64),
                 ("Me", C.c_int64), ("min_costheta", C.c_double)]
@@ -101,6 +106,13 @@ _SIGS = {
                                     C.c_int32, _vp],
     "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
                                         _d, _i64, C.c_int32, _vp],
+    "xts_ctx_get_tuning": [_vp, C.c_char_p, C.POINTER(_i64)],
+    "xts_model_map_footprints": [_vp, _i64, C.POINTER(Wcs), C.POINTER(_i64), _vp, _vp, _i64,
+                                 C.c_double, C.POINTER(Footprint), _d, C.POINTER(_i64)],
+    "xts_point_src_model_maps_batched_dev": [_vp, _i64, C.POINTER(Wcs), C.POINTER(_i64), _vp,
+                                             _vp, _vp, _vp, _i64, _d, _i64, C.c_int32,
+                                             C.POINTER(Footprint), _vp, _i64],
+    "xts_aeff_value_dev": [_vp, _d, _i64, _d, _i64, C.POINTER(AeffGrid), _d, _vp],
     "xts_surrogate_samples_dev": [_vp, C.POINTER(Wcs), _d, _i64, C.c_uint64, _i64, _i64, _vp,
                                   _vp],
     "xts_mlp_forward_dev": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],

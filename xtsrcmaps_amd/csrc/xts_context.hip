@@ -60,6 +60,7 @@ int slot_reserve(xts_ctx* ctx, int slot, size_t bytes, void** out) {
         want = (want + 4095) & ~size_t(4095);
         XTS_HIP(hipMalloc(&s.ptr, want));
         s.bytes = want;
+        ++ctx->generation;
     }
     *out = s.ptr;
     return XTS_OK;
@@ -208,6 +209,11 @@ int xts_ctx_destroy(xts_ctx* ctx) {
         hipEventDestroy(r.stop);
     }
     for (auto e : ctx->event_pool) hipEventDestroy(e);
+    if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) hipEventDestroy(ctx->join_ev);
+    if (ctx->aux) xts_ctx_destroy(ctx->aux);
+    hipSetDevice(ctx->device);
     for (auto& s : ctx->slots)
         if (s.ptr) hipFree(s.ptr);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -234,7 +240,42 @@ int xts_ctx_set_stream(xts_ctx* ctx, void* stream) {
 int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
     XTS_CHECK(ctx != nullptr && key != nullptr, "null argument");
     XTS_CHECK(value >= 0 && value <= 1 << 20, "tuning value %lld out of range", (long long)value);
+    if (strcmp(key, "geometry_cache") == 0) {
+        XTS_CHECK(value == 0 || value == 1, "geometry_cache is 0 or 1");
+        ctx->geom_cache = value != 0;
+        ctx->geom_valid = false;
+        return XTS_OK;
+    }
+    if (strcmp(key, "rowweight_blocks") == 0) {
+        ctx->tune_rw_blocks = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "batch_sources") == 0) {
+        ctx->tune_batch_sources = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "autotune") == 0) {
+        XTS_CHECK(value == 0 || value == 1, "autotune is 0 or 1");
+        ctx->autotune = (int)value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "graph") == 0) {
+        XTS_CHECK(value == 0 || value == 1, "graph is 0 or 1");
+        ctx->use_graph = (int)value;
+        return XTS_OK;
+    }
     if (strcmp(key, "tiles_per_block") == 0) ctx->tune_tiles_per_block = (int)value;
+    else return fail(XTS_ERR_INVALID, "unknown tuning key '%s'", key);
+    return XTS_OK;
+}
+
+int xts_ctx_get_tuning(xts_ctx* ctx, const char* key, int64_t* value) {
+    XTS_CHECK(ctx != nullptr && key != nullptr && value != nullptr, "null argument");
+    if (strcmp(key, "batch_sources") == 0) *value = ctx->last_batch_sources;
+    else if (strcmp(key, "rowweight_blocks") == 0) *value = ctx->last_rw_blocks;
+    else if (strcmp(key, "batches") == 0) *value = ctx->last_batches;
+    else if (strcmp(key, "graph_replays") == 0) *value = ctx->graph_replays;
+    else if (strcmp(key, "graph_captures") == 0) *value = ctx->graph_captures;
     else return fail(XTS_ERR_INVALID, "unknown tuning key '%s'", key);
     return XTS_OK;
 }

@@ -242,6 +242,24 @@ int xts_aeff_value(xts_ctx* ctx, const double* costh, int64_t C, const double* l
     return XTS_OK;
 }
 
+int xts_aeff_value_dev(xts_ctx* ctx, const double* costh, int64_t C, const double* logE,
+                       int64_t E, const xts_aeff_grid* g, double* out, int32_t* range_error) {
+    XTS_TRY(activate(ctx));
+    XTS_CHECK(costh && logE && g && g->params && g->cosths && g->logEs && out, "null pointer");
+    XTS_CHECK(C > 0 && E > 0 && g->Mc > 1 && g->Me > 1 && C * E < (1LL << 31), "bad sizes");
+    int* derr = (int*)range_error;
+    if (!derr) {
+        void* d;
+        XTS_TRY(slot_reserve(ctx, kSlotOut1, sizeof(int), &d));
+        derr = (int*)d;
+    }
+    aeff_value_kernel<<<div_up(C * E, 256), 256, 0, ctx->stream>>>(
+        (int)C, (int)E, costh, logE, g->cosths, (int)g->Mc, g->logEs, (int)g->Me, g->params,
+        g->min_costheta, out, derr);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
 int xts_src_cosbins_dev(xts_ctx* ctx, const double* src_sph, int64_t S, const double* cube,
                         int64_t nside, int64_t C, double* out) {
     XTS_TRY(activate(ctx));

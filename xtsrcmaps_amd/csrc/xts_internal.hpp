@@ -99,11 +99,13 @@ struct xts_ctx {
     int cu_count = 0;
     int timing = 0;                        // record events around kernels
     int tune_tiles_per_block = 0;          // 0: chosen per call (row-weight pass)
+    int64_t tune_rw_blocks = 0;            // >0: the row-weight pass on this many resident blocks
     std::vector<xts_timing_rec> timing_recs;
     std::vector<hipEvent_t> event_pool;
     bool consts_ready = false;
     xts_const_tables consts;
     xts_rule_cache rule;
+    bool geom_cache = true;                // keep the last image's geometry (tuning key)
     bool geom_valid = false;               // pixel geometry in kSlotGeom is for geom_wcs
     xts_wcs geom_wcs{};
     // A caller-provided device region that slot_reserve bump-allocates from
@@ -117,6 +119,26 @@ struct xts_ctx {
     // the surrogate network in its matrix-core record (xts_ctx_set_surrogate)
     void* surrogate = nullptr;
     bool surrogate_ready = false;
+    // batched engine: the second context its odd batches run on (own slots,
+    // own stream), fork/join events, the tuned launch shapes per workload
+    // and the captured graph of the last batched call
+    xts_ctx* aux = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    struct Tuned {
+        std::string key;
+        int64_t batch_sources = 0, rw_blocks = 0;
+        double ms = 0.0;
+    };
+    std::vector<Tuned> tuned;
+    int autotune = 0, use_graph = 0;
+    int64_t tune_batch_sources = 0;        // >0: sources per batch of the batched engine
+    int64_t last_batch_sources = 0, last_rw_blocks = 0, last_batches = 0;  // last batched call
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+    // advanced by every buffer (re)allocation and every upload of a rule or
+    // network: a captured graph is replayed only while it has not changed
+    uint64_t generation = 0;
+    int64_t graph_replays = 0, graph_captures = 0;
     std::vector<unsigned char> arena_rule_blob;  // host copy of the rule uploaded into it
     // arguments of a model map whose row-weight half has been enqueued and
     // whose second half has not
@@ -143,6 +165,11 @@ inline size_t arena_bytes(size_t bytes) {
 // Device address of the no-allocation scratch on the current device, taken
 // once per device at context creation (xts_modelmap.hip).
 int noalloc_scratch_init(int device);
+// The device model map of xts_point_src_model_map_wcs_dev, for the batched
+// engine's batches (xts_batch.hip).
+int model_map_dev_call(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
+                       const double* expo, const double* upsf, const double* parint, int64_t E,
+                       const double* rule, int64_t N, int32_t flags, void* out);
 int noalloc_scratch(int device, void** ptr, size_t* bytes);
 // Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
 // and size: the attribute belongs to the function).

@@ -197,6 +197,7 @@ struct CubArgs {
     void* out;
     int S, H, W, E, Ep, N;
     int tiles_w, tiles_h, tiles_per_block;
+    int rw_persist;            // row-weight pass on a 1-D grid of resident blocks
     int emit_gx;               // emit blocks per source (x); the next kWideBlocks: wide pixels
     int layout_sehw, out_f32;
     int tensor_nu, tensor_nv;  // rule is an nu x nv grid (p = i nv + j), else 0
@@ -848,17 +849,35 @@ rowweight_kernel(CubArgs a) {
 #pragma unroll
     for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = (cfloat*)(a.lean + j * N4);
 
-    const int s = blockIdx.y;
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_w * a.tiles_h;
-    const int t_begin = blockIdx.x * a.tiles_per_block;
-    const int t_end = min(ntiles, t_begin + a.tiles_per_block);
-    if (t_begin >= ntiles) return;
+    // The block's items in the flattened (source, tile) space: a run of
+    // tiles_per_block tiles of source blockIdx.y on the (tile runs, sources)
+    // grid; with rw_persist (a 1-D grid of fewer, resident blocks) the
+    // block's contiguous share of all S x ntiles items, taken as segments
+    // split at source boundaries.
+    long long i0, i1;
+    if (a.rw_persist) {
+        const long long n = (long long)a.S * ntiles;
+        i0 = n * blockIdx.x / gridDim.x;
+        i1 = n * (blockIdx.x + 1) / gridDim.x;
+    } else {
+        const int tb = blockIdx.x * a.tiles_per_block;
+        if (tb >= ntiles) return;
+        i0 = (long long)blockIdx.y * ntiles + tb;
+        i1 = (long long)blockIdx.y * ntiles + min(ntiles, tb + a.tiles_per_block);
+    }
+    if (i0 >= i1) return;
 
     if constexpr (sizeof(T) == 8) stage_rule<T>(a, isep_s, rule_s, N4, tid, kThreads);
     for (int k = tid; k < kND; k += kThreads) hist_s[k] = 0.0;
     __syncthreads();
 
+  for (long long seg = i0; seg < i1;) {
+    const int s = (int)(seg / ntiles);
+    const int t_begin = (int)(seg % ntiles);
+    const int t_end = (int)min((long long)ntiles, t_begin + (i1 - seg));
+    seg += t_end - t_begin;
     const SrcGeom g = a.geo[s];
     const bool do_hist = a.hist && g.in_fov;
     const int lx = tid % kTileW, ly = tid / kTileW;
@@ -1000,10 +1019,16 @@ rowweight_kernel(CubArgs a) {
         }
     }
     if (do_hist) {
+        // the segment's rows into the source's histogram, cleared for the next
         __syncthreads();
         for (int k = tid; k < kND; k += kThreads)
-            if (hist_s[k] != 0.0) atomicAdd(&a.hist[(size_t)s * kND + k], hist_s[k]);
+            if (hist_s[k] != 0.0) {
+                atomicAdd(&a.hist[(size_t)s * kND + k], hist_s[k]);
+                hist_s[k] = 0.0;
+            }
+        __syncthreads();
     }
+  }
 }
 
 // Pass 1b (surrogate mode): the listed one-node pixels of each source.  Each
@@ -1775,7 +1800,7 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
     // the pixel geometry depends on the WCS alone: recomputed only when the
     // image changes (the slot keeps the last image's); always inside an
     // arena, whose contents do not outlive the call
-    const bool cached = ctx->arena.base == nullptr;
+    const bool cached = ctx->arena.base == nullptr && ctx->geom_cache;
     if (!cached || !ctx->geom_valid || std::memcmp(&ctx->geom_wcs, wcs, sizeof(xts_wcs)) != 0) {
         geometry_kernel<<<std::min<unsigned>(div_up(ncorn, 256), 4096), 256, 0, ctx->stream>>>(
             w, H, W, ig->corners, ig->centres);
@@ -1890,6 +1915,7 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
         }
         rc.host.assign(r, r + 3 * N);
         rc.dev = d;
+        ++ctx->generation;
         rc.lean_off = rec;
         rc.rho_scale = rs;
         rc.wsum = ws;
@@ -2059,7 +2085,11 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     if (phase & kPhaseRows) {
         const size_t sm = (size_t)kNDPad * 8 +
                           (f64 ? (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz : 0);
-        const dim3 grid((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
+        ca.rw_persist = ctx->tune_rw_blocks > 0 ? 1 : 0;
+        const dim3 grid = ca.rw_persist
+                              ? dim3((unsigned)std::min<int64_t>(ctx->tune_rw_blocks,
+                                                                 (int64_t)S * ntiles))
+                              : dim3((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
         if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
         else if (sur) rowweight_kernel<float, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
@@ -2184,6 +2214,14 @@ static size_t model_map_arena_bytes(int64_t S, const xts_wcs* wcs, int64_t E, in
     b += arena_bytes((size_t)S * plane * (f64 ? 32 : 16) + (size_t)S * plane * 20 + 512);
     b += arena_bytes((size_t)(pool_cap * S) * tsz);                          // near pool
     return b;
+}
+
+// The one-call device model map for the batched engine (xts_batch.hip).
+int model_map_dev_call(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
+                       const double* expo, const double* upsf, const double* parint, int64_t E,
+                       const double* rule, int64_t N, int32_t flags, void* out) {
+    return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out,
+                            true);
 }
 
 static size_t out_bytes(int64_t S, const xts_wcs* wcs, int64_t E, int32_t flags) {

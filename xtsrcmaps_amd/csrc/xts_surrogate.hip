@@ -571,6 +571,7 @@ int xts_ctx_set_surrogate(xts_ctx* ctx, const float* params, const float* norm) 
     XTS_HIP(hipStreamSynchronize(ctx->stream));
     ctx->surrogate = mp;
     ctx->surrogate_ready = true;
+    ++ctx->generation;
     return XTS_OK;
 }
 

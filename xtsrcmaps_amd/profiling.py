@@ -24,27 +24,28 @@ import os
 import re
 import sys
 
-MM_KERNELS = (r"rowweight_kernel|near_rows_kernel|hist_integral_kernel|factor_kernel|emit_s|"
-              r"near_emit_kernel")
+MM_KERNELS = (r"rowweight_kernel|sur_rows_kernel|near_rows_kernel|hist_integral_kernel|"
+              r"factor_kernel|emit_s|near_emit_kernel")
 COPY_KERNELS = r"MulFunctor"          # torch.mul(src, 1.0, out=dst): reads and writes N bytes
 
 
 # ---------------------------------------------------------------- workloads
-def _modelmap(config: str, calls: int) -> None:
+def _modelmap(config: str, calls: int, surrogate: bool = False) -> None:
     import torch
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     sys.path.insert(0, root)
     import bench
     import xtsrcmaps_amd as xa
-    cfg, inp, ctx, d = bench.setup(config)
-    bench.step(ctx, inp, d)
+    cfg, inp, ctx, d = bench.setup(config, surrogate=surrogate)
+    flags = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32 | (xa.SURROGATE if surrogate else 0)
+    bench.step(ctx, inp, d, flags)
     torch.cuda.synchronize()
     for _ in range(calls):
         ctx.point_src_model_map_wcs_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
-                                        d["out"], inp.rule,
-                                        flags=xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32)
+                                        d["out"], inp.rule, flags=flags)
     torch.cuda.synchronize()
-    print(f"modelmap: {calls} calls of {config}", flush=True)
+    print(f"modelmap: {calls} calls of {config}" + (" with the surrogate" if surrogate else ""),
+          flush=True)
 
 
 def _copy(mib: int, calls: int) -> None:
@@ -109,7 +110,8 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
     fetch_b = mf * 1024 / calls / f_fetch
     write_b = mw * 1024 / calls / f_write
     per_kernel = {}
-    for name, rx in (("rowweight", "rowweight_kernel"), ("near_rows", "near_rows_kernel"),
+    for name, rx in (("rowweight", "rowweight_kernel"), ("surrogate", "sur_rows_kernel"),
+                     ("near_rows", "near_rows_kernel"),
                      ("hist_factor", "hist_integral_kernel|factor_kernel"),
                      ("emit", "emit_s"), ("near_emit", "near_emit_kernel")):
         f, _ = _counter_sum(mm_fetch, "FETCH_SIZE", rx)
@@ -163,6 +165,7 @@ def main(argv=None) -> None:
     p = sub.add_parser("modelmap")
     p.add_argument("--config", default="c1")
     p.add_argument("--calls", type=int, default=3)
+    p.add_argument("--surrogate", action="store_true")
     p = sub.add_parser("copy")
     p.add_argument("--mib", type=int, default=592)
     p.add_argument("--calls", type=int, default=3)
@@ -174,13 +177,15 @@ def main(argv=None) -> None:
         p.add_argument("--" + k, required=True)
     p.add_argument("--copy-mib", type=int, required=True)
     p.add_argument("--cmd", required=True)
+    p.add_argument("--kernel-only", default="",
+                   help="report this kernel alone (e.g. emit) as traffic_bytes_per_call")
     p = sub.add_parser("valu")
     p.add_argument("run_dir")
     p.add_argument("--kernel", default="rowweight_kernel<float>")
     p.add_argument("--cmd", required=True)
     a = ap.parse_args(argv)
     if a.what == "modelmap":
-        _modelmap(a.config, a.calls)
+        _modelmap(a.config, a.calls, a.surrogate)
     elif a.what == "copy":
         _copy(a.mib, a.calls)
     elif a.what == "stats":
@@ -188,8 +193,16 @@ def main(argv=None) -> None:
     elif a.what == "valu":
         print(json.dumps(valu_json(a.run_dir, re.escape(a.kernel), a.cmd), indent=1))
     else:
-        print(json.dumps(traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write,
-                                      a.copy_mib, a.cmd), indent=1))
+        t = traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write, a.copy_mib, a.cmd)
+        if a.kernel_only:
+            k = t["per_kernel"][a.kernel_only]
+            t = {"command": a.cmd, "kernel": a.kernel_only, "fetch_factor": t["fetch_factor"],
+                 "write_factor": t["write_factor"], "model_map_calls": t["model_map_calls"],
+                 "fetch_bytes_per_call": k["fetch_bytes"], "write_bytes_per_call": k["write_bytes"],
+                 "traffic_bytes_per_call": k["fetch_bytes"] + k["write_bytes"],
+                 "model_map": {kk: t[kk] for kk in ("fetch_bytes_per_call", "write_bytes_per_call",
+                                                    "traffic_bytes_per_call", "per_kernel")}}
+        print(json.dumps(t, indent=1))
 
 
 if __name__ == "__main__":

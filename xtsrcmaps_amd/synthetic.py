@@ -159,3 +159,58 @@ def make_inputs(cfg: Config) -> Inputs:
     costh = 1.0 - ((np.arange(40) + 0.5) / 40) ** 2      # cosmin 0, sqrt binning
     return Inputs(cfg, wcs, src, logE, costh, king, (0.0, 1.0, 0.0), aeff, (p0, p1), cube,
                   wcube, nside, rule_for(cfg))
+
+
+# ------------------------------------------------------------- several images
+@dataclass
+class BatchConfig:
+    """Sources over several CAR images (plate carree, CRVAL2 = 0, centred at
+    different longitudes), one shared instrument and livetime cube."""
+    name: str
+    rois: tuple              # ((npix, n_src), ...)
+    cdelt: float = 0.05
+    n_energy: int = 37
+    rule: str = "gl8"
+    seed: int = 2
+    lon0: tuple = (40.0, 110.0, 200.0, 300.0)
+
+    @property
+    def n_src(self) -> int:
+        return sum(n for _, n in self.rois)
+
+    def dense_evals(self) -> int:
+        return sum(n * p * p for p, n in self.rois) * self.n_energy
+
+
+BATCH_CONFIGS = {
+    # 2,000 sources across four images of 64^2, 128^2, 256^2 and 400^2 pixels
+    # at 0.05 deg, in proportion to their areas
+    "c2": BatchConfig("c2", ((64, 33), (128, 133), (256, 532), (400, 1302))),
+}
+
+
+@dataclass
+class BatchInputs:
+    cfg: BatchConfig
+    wcs: list                # [n_images] Wcs
+    src_begin: np.ndarray    # [n_images + 1] int64
+    phys: Inputs             # src_sph = every source, ordered by image
+
+
+def make_batch_inputs(cfg: BatchConfig) -> BatchInputs:
+    rng = np.random.default_rng(cfg.seed + 1000)
+    wcs, srcs, begin = [], [], [0]
+    for (n, ns), lon0 in zip(cfg.rois, cfg.lon0):
+        w = Wcs.make((n, n), (n / 2 + 0.5, n / 2 + 0.5), (lon0, 0.0), (-cfg.cdelt, cfg.cdelt),
+                     0.0, "CAR")
+        p = rng.uniform(0.5, n + 0.5, (ns, 2))
+        lon, lat = pix2sph_car(w, p[:, 0], p[:, 1])
+        srcs.append(np.stack([lon, lat], axis=1))
+        wcs.append(w)
+        begin.append(begin[-1] + ns)
+    # the instrument and livetime of a single-image configuration with the
+    # same seed and energies; its sources replaced by every image's
+    base = make_inputs(Config(cfg.name, 1, 16, cfg.cdelt, n_energy=cfg.n_energy, rule=cfg.rule,
+                              seed=cfg.seed, src_mode="uniform"))
+    phys = Inputs(**{**base.__dict__, "src_sph": np.concatenate(srcs)})
+    return BatchInputs(cfg, wcs, np.array(begin, np.int64), phys)
