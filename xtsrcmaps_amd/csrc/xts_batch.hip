@@ -173,6 +173,10 @@ static int ensure_aux(xts_ctx* ctx) {
     if (!ctx->fork_ev) XTS_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     if (!ctx->join_ev) XTS_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
     xts_ctx* a = ctx->aux;
+    for (xts_ctx* x : {ctx, a}) {
+        if (!x->rows_ev) XTS_HIP(hipEventCreateWithFlags(&x->rows_ev, hipEventDisableTiming));
+        if (!x->fin_ev) XTS_HIP(hipEventCreateWithFlags(&x->fin_ev, hipEventDisableTiming));
+    }
     // batches switch images on both contexts, and a captured graph must not
     // depend on which image a context's geometry buffer held before: the
     // geometry of every batch is recomputed (the second context's setting
@@ -224,8 +228,15 @@ static std::vector<Batch> plan(const BatchCall& c, int64_t batch_sources) {
     return out;
 }
 
-// Enqueue every batch: even batches on ctx's stream, odd ones on the second
-// context's, which forks from and joins back into ctx's stream.
+// Enqueue every batch as a two-stage pipeline over two contexts: even
+// batches on ctx's stream, odd ones on the second context's, which forks
+// from and joins back into ctx's stream.  Each batch's row-weight half waits
+// for the previous batch's (the PSF table and the factors follow the row
+// weights), and its emit for the previous batch's emit, so the compute-bound
+// row weights of batch k + 1 run beside the store-bound emit of batch k.
+// A context's
+// batches are in its stream's order, so batch k + 2 reuses batch k's
+// buffers only after batch k is done.
 static int run(xts_ctx* ctx, const BatchCall& c, int64_t batch_sources, int64_t rw_blocks,
                const xts_footprint* boxes_dev) {
     const std::vector<Batch> bs = plan(c, batch_sources);
@@ -240,26 +251,30 @@ static int run(xts_ctx* ctx, const BatchCall& c, int64_t batch_sources, int64_t 
     for (size_t k = 0; k < bs.size(); ++k) {
         const Batch& b = bs[k];
         xts_ctx* x = ctxs[k & 1];
+        xts_ctx* y = ctxs[(k + 1) & 1];
         const xts_wcs& w = c.wcs[b.img];
         const size_t plane = (size_t)w.naxis1 * w.naxis2;
         const double* src = c.src + 2 * b.b0;
         const double* ex = c.expo + b.b0 * c.E;
         const double* up = c.upsf + (size_t)b.b0 * kND * c.E;
         const double* pa = c.parint + (size_t)b.b0 * kND * c.E;
-        if (b.direct) {
-            XTS_TRY(model_map_dev_call(x, &w, src, b.n, ex, up, pa, c.E, c.rule, c.N, c.flags,
-                                       c.out + c.boxes[b.b0].offset));
-        } else {
-            void* dense;
-            XTS_TRY(slot_reserve(x, kSlotOut3, (size_t)b.n * c.E * plane * 4, &dense));
-            XTS_TRY(model_map_dev_call(x, &w, src, b.n, ex, up, pa, c.E, c.rule, c.N, c.flags,
-                                       dense));
+        void* dst = c.out + c.boxes[b.b0].offset;
+        if (!b.direct) XTS_TRY(slot_reserve(x, kSlotOut3, (size_t)b.n * c.E * plane * 4, &dst));
+        if (k > 0) XTS_HIP(hipStreamWaitEvent(x->stream, y->rows_ev, 0));
+        XTS_TRY(model_map_dev_call(x, &w, src, b.n, ex, up, pa, c.E, c.rule, c.N, c.flags, dst,
+                                   3));
+        XTS_HIP(hipEventRecord(x->rows_ev, x->stream));
+        if (k > 0) XTS_HIP(hipStreamWaitEvent(x->stream, y->fin_ev, 0));
+        XTS_TRY(model_map_dev_call(x, &w, src, b.n, ex, up, pa, c.E, c.rule, c.N, c.flags, dst,
+                                   4));
+        if (!b.direct) {
             XTS_TRY(activate(ctx));
             box_copy_kernel<<<dim3(256, (unsigned)b.n), 256, 0, x->stream>>>(
-                (const float*)dense, (int)b.n, (int)c.E, (int)w.naxis1, (int)w.naxis2,
+                (const float*)dst, (int)b.n, (int)c.E, (int)w.naxis1, (int)w.naxis2,
                 boxes_dev + b.b0, c.out);
             XTS_HIP(hipGetLastError());
         }
+        XTS_HIP(hipEventRecord(x->fin_ev, x->stream));
     }
     XTS_TRY(activate(ctx));
     XTS_HIP(hipEventRecord(ctx->join_ev, ctx->aux->stream));

@@ -212,6 +212,8 @@ int xts_ctx_destroy(xts_ctx* ctx) {
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->join_ev) hipEventDestroy(ctx->join_ev);
+    if (ctx->rows_ev) hipEventDestroy(ctx->rows_ev);
+    if (ctx->fin_ev) hipEventDestroy(ctx->fin_ev);
     if (ctx->aux) xts_ctx_destroy(ctx->aux);
     hipSetDevice(ctx->device);
     for (auto& s : ctx->slots)

@@ -124,6 +124,9 @@ struct xts_ctx {
     // and the captured graph of the last batched call
     xts_ctx* aux = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // the batched engine's pipeline: this context's last row-weight half and
+    // last second half done
+    hipEvent_t rows_ev = nullptr, fin_ev = nullptr;
     struct Tuned {
         std::string key;
         int64_t batch_sources = 0, rw_blocks = 0;
@@ -167,9 +170,12 @@ inline size_t arena_bytes(size_t bytes) {
 int noalloc_scratch_init(int device);
 // The device model map of xts_point_src_model_map_wcs_dev, for the batched
 // engine's batches (xts_batch.hip).
+// phase: bits 1 the row-weight half (geometry, radii, row weights, near
+// rows), 2 the PSF table and the factors, 4 the emit (after calls with bits 1
+// and 2 and the same arguments on the same context); 7 the whole map.
 int model_map_dev_call(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                        const double* expo, const double* upsf, const double* parint, int64_t E,
-                       const double* rule, int64_t N, int32_t flags, void* out);
+                       const double* rule, int64_t N, int32_t flags, void* out, int phase);
 int noalloc_scratch(int device, void** ptr, size_t* bytes);
 // Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
 // and size: the attribute belongs to the function).

@@ -41,6 +41,8 @@ constexpr int kMlpParams = kOffB4 + 1;   // 8705
 constexpr int kMlpNorm = 10;             // mu[4], inv_sd[4], y_scale, y_offset
 
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef short mlp_s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t mlp_bf16(float f) {
@@ -56,7 +58,11 @@ __host__ __device__ inline float mlp_from_bf16(uint16_t h) {
     return f;
 }
 
-__device__ inline float silu(float x) { return x * __frcp_rn(1.0f + __expf(-x)); }
+// x / (1 + e^-x) from the hardware base-2 exponential and reciprocal
+// (about 1 ulp each; the hidden activations are rounded to BF16 anyway)
+__device__ inline float silu(float x) {
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+}
 
 // Inference record in the matrix cores' operand order.  A 16x16 weight tile
 // (outputs 16 To .., inputs 16 Ti ..) is 64 lanes of four BF16 values: lane
@@ -67,11 +73,19 @@ __device__ inline float silu(float x) { return x * __frcp_rn(1.0f + __expf(-x));
 // sample l % 16) lies in registers, so layers chain without a transpose.
 // The first layer's 16-wide K holds the inputs' high and low BF16 parts
 // against the weights' high and low parts (x_hi w_hi + x_lo w_hi + x_hi
-// w_lo); the hidden layers use a high and a low weight tile each.
+// w_lo).  The hidden layers run on v_mfma_f32_16x16x32_bf16, a high and a
+// low weight tile each: a 32-wide input chunk Ti is two of the previous
+// layer's 16-output tiles side by side, lane l carrying in its eight k-slots
+// inputs 32 Ti + 4 (l / 16) + j and 32 Ti + 16 + 4 (l / 16) + j, j = 0..3 --
+// the A operand's slots hold the weights of the same inputs, so the pair
+// multiplies as W . H^T without moving any activation between lanes.
+__host__ __device__ inline int mlp_kslot(int Ti, int g, int j) {
+    return 32 * Ti + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
 struct MlpPacked {
     mlp_bf16x4 w1[4][64];          // [To][lane]
-    mlp_bf16x4 w2[2][16][64];      // [hi/lo][To * 4 + Ti][lane]
-    mlp_bf16x4 w3[2][16][64];
+    mlp_s16x8 w2[2][8][64];        // [hi/lo][To * 2 + Ti][lane]
+    mlp_s16x8 w3[2][8][64];
     float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
     float b4, norm[kMlpNorm];
     float pad[5];
@@ -91,20 +105,20 @@ __device__ inline void mlp_pack(const float* __restrict__ p, const float* __rest
         }
         out->w1[To][l] = v;
     }
-    for (int k = tid; k < 2 * 16 * 64; k += nthr) {
-        const int layer = k / (16 * 64), r = k % (16 * 64);
-        const int t = r / 64, l = r % 64, To = t / 4, Ti = t % 4;
+    for (int k = tid; k < 2 * 8 * 64; k += nthr) {
+        const int layer = k / (8 * 64), r = k % (8 * 64);
+        const int t = r / 64, l = r % 64, To = t / 2, Ti = t % 2;
         const float* W = p + (layer == 0 ? kOffW2 : kOffW3);
-        mlp_bf16x4 hi4, lo4;
-        for (int j = 0; j < 4; ++j) {
-            const float w = W[(16 * To + l % 16) * kMlpH + 16 * Ti + 4 * (l / 16) + j];
+        mlp_s16x8 hi8, lo8;
+        for (int j = 0; j < 8; ++j) {
+            const float w = W[(16 * To + l % 16) * kMlpH + mlp_kslot(Ti, l / 16, j)];
             const uint16_t hi = mlp_bf16(w);
-            hi4[j] = (short)hi;
-            lo4[j] = (short)mlp_bf16(w - mlp_from_bf16(hi));
+            hi8[j] = (short)hi;
+            lo8[j] = (short)mlp_bf16(w - mlp_from_bf16(hi));
         }
-        mlp_bf16x4(*dst)[16][64] = layer == 0 ? out->w2 : out->w3;
-        dst[0][t][l] = hi4;
-        dst[1][t][l] = lo4;
+        mlp_s16x8(*dst)[8][64] = layer == 0 ? out->w2 : out->w3;
+        dst[0][t][l] = hi8;
+        dst[1][t][l] = lo8;
     }
     for (int k = tid; k < kMlpH; k += nthr) {
         out->b1[k] = p[kOffB1 + k];
@@ -124,6 +138,18 @@ __device__ inline mlp_bf16x4 mlp_silu_bf16(mlp_f32x4& acc) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) h[i] = (short)mlp_bf16(silu(acc[i]));
     return h;
+}
+
+// Two of the previous layer's 16-output tiles as one 32-wide input chunk.
+__device__ inline mlp_bf16x8 mlp_chunk(const mlp_bf16x4& a, const mlp_bf16x4& b) {
+    const mlp_s16x8 v = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+    return __builtin_bit_cast(mlp_bf16x8, v);
+}
+
+__device__ inline mlp_f32x4 mlp_mfma32(const mlp_s16x8& w, const mlp_bf16x8& h,
+                                       const mlp_f32x4& acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_bf16x8, w), h, acc,
+                                                   0, 0, 0);
 }
 
 // One wave evaluates 16 samples: lane l carries sample l % 16's raw
@@ -150,21 +176,22 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
         H[To] = mlp_silu_bf16(acc);
     }
     // hidden layer 2
+    mlp_bf16x8 X[2] = {mlp_chunk(H[0], H[1]), mlp_chunk(H[2], H[3])};
     mlp_f32x4 acc2[4];
 #pragma unroll
     for (int To = 0; To < 4; ++To) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc2[To][i] = m->b2[16 * To + 4 * g + i];
 #pragma unroll
-        for (int Ti = 0; Ti < 4; ++Ti) {
-            acc2[To] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w2[1][To * 4 + Ti][lane],
-                                                                 H[Ti], acc2[To], 0, 0, 0);
-            acc2[To] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w2[0][To * 4 + Ti][lane],
-                                                                 H[Ti], acc2[To], 0, 0, 0);
+        for (int Ti = 0; Ti < 2; ++Ti) {
+            acc2[To] = mlp_mfma32(m->w2[1][To * 2 + Ti][lane], X[Ti], acc2[To]);
+            acc2[To] = mlp_mfma32(m->w2[0][To * 2 + Ti][lane], X[Ti], acc2[To]);
         }
     }
 #pragma unroll
     for (int To = 0; To < 4; ++To) H[To] = mlp_silu_bf16(acc2[To]);
+    X[0] = mlp_chunk(H[0], H[1]);
+    X[1] = mlp_chunk(H[2], H[3]);
     // hidden layer 3, its activations kept in FP32 for the output dot product
     float y = 0.0f;
 #pragma unroll
@@ -173,11 +200,9 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = m->b3[16 * To + 4 * g + i];
 #pragma unroll
-        for (int Ti = 0; Ti < 4; ++Ti) {
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w3[1][To * 4 + Ti][lane], H[Ti],
-                                                            acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w3[0][To * 4 + Ti][lane], H[Ti],
-                                                            acc, 0, 0, 0);
+        for (int Ti = 0; Ti < 2; ++Ti) {
+            acc = mlp_mfma32(m->w3[1][To * 2 + Ti][lane], X[Ti], acc);
+            acc = mlp_mfma32(m->w3[0][To * 2 + Ti][lane], X[Ti], acc);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], silu(acc[i]), y);

@@ -207,8 +207,8 @@ struct CubArgs {
     // learned surrogate of the one-node far pixels: 0 off, 1 the network,
     // 2 without it (|A| for the absolute moment)
     int sur;
-    float4* sur_list;          // [S][H W][2]: features; A, pixel index bits
-    int* sur_count;            // [S]
+    float4* sur_list;          // [S][tiles][256][2]: features; A, pixel, solid angle, 2 r0 + in-radius
+    int* sur_count;            // [S][tiles] entries of each tile's 256 list slots
     const MlpPacked* sur_net;
 };
 
@@ -798,14 +798,12 @@ __device__ inline void sur_pixel(const CubArgs& a, const PixelGeom<float>& pg, i
     A = 0.25f * wsum * ((sig(-g, -g) + sig(g, -g)) + (sig(-g, g) + sig(g, g)));
 }
 
+// Nine series evaluations per pixel: the highest degree for every wave (it
+// reaches every one-node pixel) keeps a single instance of the set-up.
 __device__ inline bool sur_dispatch(const CubArgs& a, const PixelGeom<float>& pg, int r0,
                                     float wsum, float rx, float (&f)[4], float& A) {
-    if (__all(rx <= kPolyReach[0])) sur_pixel<2>(a, pg, r0, wsum, f, A);
-    else if (__all(rx <= kPolyReach[1])) sur_pixel<3>(a, pg, r0, wsum, f, A);
-    else if (__all(rx <= kPolyReach[2])) sur_pixel<4>(a, pg, r0, wsum, f, A);
-    else if (__all(rx <= kPolyReach[3])) sur_pixel<5>(a, pg, r0, wsum, f, A);
-    else if (__all(rx <= kPolyReach[4])) sur_pixel<6>(a, pg, r0, wsum, f, A);
-    else return false;
+    if (!__all(rx <= kPolyReach[kPolyMaxD - 2])) return false;
+    sur_pixel<kPolyMaxD>(a, pg, r0, wsum, f, A);
     return true;
 }
 
@@ -835,9 +833,9 @@ __device__ inline void rows_from_sums(T wsum, T S0, T T0, T S01, T T01, T Tt, T 
 // wave lies below 28 degrees, the wave runs a branch-free FP32 loop over the
 // points in groups of four (grid-structured rules reuse the blend of each
 // rule row).  Occupancy: 6 waves per SIMD for FP32 (80 VGPRs), 5 for FP64.
-template <typename T, bool SUR = false>
+template <typename T, bool SUR = false, bool PERSIST = false>
 __global__ void __launch_bounds__(kThreads)
-    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 5)))
+    __attribute__((amdgpu_waves_per_eu(PERSIST ? 4 : (sizeof(T) == 4 ? 6 : 5))))
 rowweight_kernel(CubArgs a) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -848,6 +846,7 @@ rowweight_kernel(CubArgs a) {
     LeanRule lr;
 #pragma unroll
     for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = (cfloat*)(a.lean + j * N4);
+    __shared__ int sur_cnt_s[kThreads / 64];
 
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_w * a.tiles_h;
@@ -857,7 +856,7 @@ rowweight_kernel(CubArgs a) {
     // block's contiguous share of all S x ntiles items, taken as segments
     // split at source boundaries.
     long long i0, i1;
-    if (a.rw_persist) {
+    if (PERSIST) {
         const long long n = (long long)a.S * ntiles;
         i0 = n * blockIdx.x / gridDim.x;
         i1 = n * (blockIdx.x + 1) / gridDim.x;
@@ -873,11 +872,14 @@ rowweight_kernel(CubArgs a) {
     for (int k = tid; k < kND; k += kThreads) hist_s[k] = 0.0;
     __syncthreads();
 
-  for (long long seg = i0; seg < i1;) {
-    const int s = (int)(seg / ntiles);
-    const int t_begin = (int)(seg % ntiles);
-    const int t_end = (int)min((long long)ntiles, t_begin + (i1 - seg));
-    seg += t_end - t_begin;
+  // one segment on the grid of tile runs (the loop runs once, and the
+  // instance keeps no segment state across it)
+  for (long long seg = i0; PERSIST ? seg < i1 : seg == i0;) {
+    const int s = PERSIST ? (int)(seg / ntiles) : (int)blockIdx.y;
+    const int t_begin = PERSIST ? (int)(seg % ntiles) : (int)(i0 - (long long)blockIdx.y * ntiles);
+    const int t_end = PERSIST ? (int)min((long long)ntiles, t_begin + (i1 - seg))
+                              : min(ntiles, t_begin + a.tiles_per_block);
+    seg = PERSIST ? seg + (t_end - t_begin) : i0 + 1;
     const SrcGeom g = a.geo[s];
     const bool do_hist = a.hist && g.in_fov;
     const int lx = tid % kTileW, ly = tid / kTileW;
@@ -990,15 +992,35 @@ rowweight_kernel(CubArgs a) {
             }
             rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
         }
-        if (valid && listed) {
-            // the surrogate pass writes its row weights and histogram rows
-            const int pix = h * a.W + w;
-            rw_row[pix] = r0;
-            const int k = atomicAdd(&a.sur_count[s], 1);
-            float4* e = a.sur_list + 2 * ((size_t)s * plane + k);
-            e[0] = float4{sf[0], sf[1], sf[2], sf[3]};
-            e[1] = float4{sA, __int_as_float(pix), 0.0f, 0.0f};
-        } else if (valid) {
+        if constexpr (SUR) {
+            // the tile's listed pixels, compacted in the tile's own 256 list
+            // slots (no atomics): wave counts in LDS, then each lane's slot
+            const bool app = valid && listed;
+            const unsigned long long m = __ballot(app);
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+            if (lane == 0) sur_cnt_s[wv] = __popcll(m);
+            __syncthreads();
+            int base = 0;
+            for (int q = 0; q < wv; ++q) base += sur_cnt_s[q];
+            if (app) {
+                // the surrogate pass writes its row weights and histogram rows
+                const int pix = h * a.W + w;
+                rw_row[pix] = r0;
+                const int k = base + __popcll(m & ((1ull << lane) - 1));
+                const bool inr = do_hist && in_radius(g, a.centres, h * Wp1 + w);
+                const float om = a.omega ? (float)a.omega[pix] : 1.0f;
+                float4* e = a.sur_list + 2 * (((size_t)s * ntiles + t) * kThreads + k);
+                e[0] = float4{sf[0], sf[1], sf[2], sf[3]};
+                e[1] = float4{sA, __int_as_float(pix), om, __int_as_float(2 * r0 + (inr ? 1 : 0))};
+            }
+            if (threadIdx.x == 0) {
+                int n = 0;
+                for (int q = 0; q < kThreads / 64; ++q) n += sur_cnt_s[q];
+                a.sur_count[(size_t)s * ntiles + t] = n;
+            }
+            __syncthreads();   // sur_cnt_s is rewritten by the next tile
+        }
+        if (valid && !listed) {
             const int pix = h * a.W + w;
             if (fast) {
                 // row weights x solid angle
@@ -1019,14 +1041,15 @@ rowweight_kernel(CubArgs a) {
         }
     }
     if (do_hist) {
-        // the segment's rows into the source's histogram, cleared for the next
+        // the segment's rows into the source's histogram (cleared for the
+        // next segment of a persistent block)
         __syncthreads();
         for (int k = tid; k < kND; k += kThreads)
             if (hist_s[k] != 0.0) {
                 atomicAdd(&a.hist[(size_t)s * kND + k], hist_s[k]);
-                hist_s[k] = 0.0;
+                if (PERSIST) hist_s[k] = 0.0;
             }
-        __syncthreads();
+        if (PERSIST) __syncthreads();
     }
   }
 }
@@ -1043,9 +1066,9 @@ sur_rows_kernel(CubArgs a) {
     __shared__ MlpPacked net;
     __shared__ double hblk[kNDPad];
     const int s = blockIdx.y;
-    const int count = a.sur_count[s];
+    const int ntiles = a.tiles_w * a.tiles_h;
     const int waves = kSurThreads / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-    if ((int)blockIdx.x * waves * 16 >= count) return;
+    if ((int)blockIdx.x * waves >= ntiles) return;
     const bool use_net = a.sur == 1;
     if (use_net) {
         const float4* src = reinterpret_cast<const float4*>(a.sur_net);
@@ -1058,37 +1081,35 @@ sur_rows_kernel(CubArgs a) {
     const SrcGeom g = a.geo[s];
     const bool do_hist = a.hist && g.in_fov;
     const size_t plane = (size_t)a.H * a.W;
-    const int Wp1 = a.W + 1;
-    const float4* list = a.sur_list + 2 * (size_t)s * plane;
     float4* rw = reinterpret_cast<float4*>(a.rw) + (size_t)s * plane;
-    const int* rw_row = a.rw_row + (size_t)s * plane;
     const float wsum = (float)a.wsum;
-    // every wave runs the same number of groups (uniform exit), lanes past
-    // the end evaluate the last entry and store nothing
-    const int ngroups = (count + 15) / 16;
-    for (int grp = blockIdx.x * waves + wave; grp < ngroups; grp += gridDim.x * waves) {
-        const int i = grp * 16 + (lane & 15);
-        const int ic = min(i, count - 1);
-        const float4 f = list[2 * ic], e = list[2 * ic + 1];
-        float y = 0.0f;
-        if (use_net) {
-            const float x[4] = {f.x, f.y, f.z, f.w};
-            y = mlp_infer16(&net, x);
-        }
-        if (lane < 16 && i < count) {
-            const float A = e.x;
-            const int pix = __float_as_int(e.y);
-            float W[4];
-            poly_rows<1>(wsum, A, fabsf(A) + fmaxf(y, 0.0f), 0.0f, W);
-            const float om = a.omega ? (float)a.omega[pix] : 1.0f;
-            const float4 wv = float4{om * W[0], om * W[1], om * W[2], om * W[3]};
-            rw[pix] = wv;
-            const int h = pix / a.W, w = pix % a.W;
-            if (do_hist && in_radius(g, a.centres, h * Wp1 + w)) {
-                const int r0 = rw_row[pix];
-                atomicAdd(&hblk[r0], (double)wv.x);
-                atomicAdd(&hblk[r0 + 1], (double)wv.y);
-                atomicAdd(&hblk[r0 + 2], (double)wv.z);
+    // each wave takes whole tiles: their entries in groups of 16 (the last
+    // group's spare lanes evaluate the tile's last entry and store nothing)
+    for (int t = blockIdx.x * waves + wave; t < ntiles; t += gridDim.x * waves) {
+        const int count = a.sur_count[(size_t)s * ntiles + t];
+        const float4* list = a.sur_list + 2 * ((size_t)s * ntiles + t) * kThreads;
+        for (int i0 = 0; i0 < count; i0 += 16) {
+            const int i = i0 + (lane & 15);
+            const int ic = min(i, count - 1);
+            const float4 f = list[2 * ic], e = list[2 * ic + 1];
+            float y = 0.0f;
+            if (use_net) {
+                const float x[4] = {f.x, f.y, f.z, f.w};
+                y = mlp_infer16(&net, x);
+            }
+            if (lane < 16 && i < count) {
+                const float A = e.x, om = e.z;
+                const int pix = __float_as_int(e.y), rf = __float_as_int(e.w);
+                float W[4];
+                poly_rows<1>(wsum, A, fabsf(A) + fmaxf(y, 0.0f), 0.0f, W);
+                const float4 wv = float4{om * W[0], om * W[1], om * W[2], om * W[3]};
+                rw[pix] = wv;
+                if (rf & 1) {
+                    const int r0 = rf >> 1;
+                    atomicAdd(&hblk[r0], (double)wv.x);
+                    atomicAdd(&hblk[r0 + 1], (double)wv.y);
+                    atomicAdd(&hblk[r0 + 2], (double)wv.z);
+                }
             }
         }
     }
@@ -1783,7 +1804,7 @@ static int check_wcs(const xts_wcs* wcs) {
 
 // Geometry + (optionally) per-source radius into the kSlotGeom slot.
 static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph_dev, int S,
-                          ImageGeom* ig, bool with_radius = true) {
+                          ImageGeom* ig, bool with_radius = true, bool keep = false) {
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     const size_t ncorn = (size_t)(H + 1) * (W + 1);
     const size_t bytes = ncorn * 3 * 8 * 2 + (size_t)H * W * 8;
@@ -1800,8 +1821,11 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
     // the pixel geometry depends on the WCS alone: recomputed only when the
     // image changes (the slot keeps the last image's); always inside an
     // arena, whose contents do not outlive the call
+    // keep: the slot holds this image's geometry from the call just before
+    // (the batched engine's emit half)
     const bool cached = ctx->arena.base == nullptr && ctx->geom_cache;
-    if (!cached || !ctx->geom_valid || std::memcmp(&ctx->geom_wcs, wcs, sizeof(xts_wcs)) != 0) {
+    if (!keep &&
+        (!cached || !ctx->geom_valid || std::memcmp(&ctx->geom_wcs, wcs, sizeof(xts_wcs)) != 0)) {
         geometry_kernel<<<std::min<unsigned>(div_up(ncorn, 256), 4096), 256, 0, ctx->stream>>>(
             w, H, W, ig->corners, ig->centres);
         solid_angle_kernel<<<std::min<unsigned>(div_up((int64_t)H * W, 256), 4096), 256, 0,
@@ -1937,12 +1961,14 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
 // Phases of one model map: kRows needs only the image, the sources and the
 // rule (geometry, radii, row weights, near-pixel rows); kFinish the PSF
 // table, exposure and output (table conversion, map integral and factor,
-// emit).  kAll runs both in one call.
-enum { kPhaseRows = 1, kPhaseFinish = 2, kPhaseAll = 3 };
+// emit).  kAll runs both in one call.  The batched engine splits the second
+// half once more: kPhaseRows | kPhaseTable, then kPhaseEmit alone (the
+// geometry and table of the first call kept).
+enum { kPhaseRows = 1, kPhaseTable = 2, kPhaseEmit = 4, kPhaseFinish = 6, kPhaseAll = 7 };
 
-// Per-source int counters (near, wide and surrogate pixels), in doubles of
-// the accumulator slot.
-static size_t model_map_counter_doubles(int64_t S) { return (size_t)(3 * S + 1) / 2; }
+// Per-source int counters (near and wide pixels), in doubles of the
+// accumulator slot.
+static size_t model_map_counter_doubles(int64_t S) { return (size_t)S; }
 
 static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                             const double* expo, const double* upsf, const double* parint,
@@ -1957,8 +1983,9 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096 && S * E < (1LL << 31),
               "bad S=%lld or E=%lld", (long long)S, (long long)E);
     XTS_CHECK(src_sph, "null device pointer");
-    XTS_CHECK(!(phase & kPhaseFinish) || (upsf && out), "null device pointer");
-    XTS_CHECK(!(phase & kPhaseFinish) || !full || (expo && parint), "null exposure/parint");
+    XTS_CHECK(!(phase & kPhaseTable) || upsf, "null device pointer");
+    XTS_CHECK(!(phase & kPhaseEmit) || out, "null device pointer");
+    XTS_CHECK(!(phase & kPhaseTable) || !full || (expo && parint), "null exposure/parint");
     XTS_CHECK((flags & ~0x3F) == 0, "unknown flags 0x%x", flags);
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     XTS_CHECK((int64_t)H * W <= (int64_t)1 << 30, "image of %lld pixels exceeds 2^30",
@@ -1974,7 +2001,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
 
     // the radii are launched with the table conversion below
     ImageGeom ig;
-    XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig, false));
+    XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig, false, phase == kPhaseEmit));
     const xts_const_tables* ct;
     XTS_TRY(const_tables(ctx, &ct));
     RuleUVW* rp;
@@ -1998,10 +2025,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_TRY(slot_reserve(ctx, kSlotAcc, nacc * 8, &accp));
     double* acc = (double*)accp;
     const size_t nzero = full ? cnt_d + (size_t)S * kND : cnt_d;
-    {
+    if (phase != kPhaseEmit) {
         // radii and cleared accumulators before the row weights; the table
         // once the PSF is known (one launch when both are)
-        const bool rad = phase & kPhaseRows, conv = phase & kPhaseFinish;
+        const bool rad = phase & kPhaseRows, conv = phase & kPhaseTable;
         const unsigned nrad = rad ? (unsigned)(S + 1) : 0u;
         const unsigned nconv = std::min<unsigned>(div_up(conv ? lut_elems : nzero, 256), 8192);
         radius_lut_kernel<<<nrad + std::max(nconv, 1u), 256, 0, ctx->stream>>>(
@@ -2042,14 +2069,17 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.wide_list = ca.near_rr + (size_t)S * plane;
     ca.near_count = (int*)acc;
     ca.wide_count = ca.near_count + S;
-    ca.sur_count = ca.wide_count + S;
     ca.sur = sur;
     ca.sur_net = sur == 1 ? (const MlpPacked*)ctx->surrogate : nullptr;
     ca.sur_list = nullptr;
+    ca.sur_count = nullptr;
     if (sur) {
+        // per tile: 256 list slots of 32 bytes and the tile's count
+        const size_t nt = (size_t)S * ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
         void* sl;
-        XTS_TRY(slot_reserve(ctx, kSlotSurList, (size_t)S * plane * 32, &sl));
+        XTS_TRY(slot_reserve(ctx, kSlotSurList, nt * (kThreads * 32 + 4), &sl));
         ca.sur_list = (float4*)sl;
+        ca.sur_count = (int*)((char*)sl + nt * kThreads * 32);
     }
     ca.near_pool = pool;
     ca.near_pool_cap = (int)pool_cap;
@@ -2091,14 +2121,29 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                                                                  (int64_t)S * ntiles))
                               : dim3((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
-        if (f64) rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
-        else if (sur) rowweight_kernel<float, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
-        else rowweight_kernel<float><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        if (f64) {
+            if (ca.rw_persist)
+                rowweight_kernel<double, false, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
+            else
+                rowweight_kernel<double><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        } else if (sur) {
+            if (ca.rw_persist)
+                rowweight_kernel<float, true, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
+            else
+                rowweight_kernel<float, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        } else if (ca.rw_persist) {
+            rowweight_kernel<float, false, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        } else {
+            rowweight_kernel<float><<<grid, kThreads, sm, ctx->stream>>>(ca);
+        }
         XTS_HIP(hipGetLastError());
     }
     // pass 1b: the surrogate's pixels, blocks stride over each source's list
     if (sur && (phase & kPhaseRows)) {
-        const dim3 grid((unsigned)std::max<int64_t>(1, 4 * want_blocks / S), (unsigned)S);
+        const int64_t tiles = (int64_t)ca.tiles_w * ca.tiles_h;
+        const dim3 grid((unsigned)std::min<int64_t>(std::max<int64_t>(1, 4 * want_blocks / S),
+                                                   div_up(tiles, kSurThreads / 64)),
+                        (unsigned)S);
         TimedRegion tr(ctx, "surrogate");
         sur_rows_kernel<<<grid, kSurThreads, 0, ctx->stream>>>(ca);
         XTS_HIP(hipGetLastError());
@@ -2118,14 +2163,14 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         }
         XTS_HIP(hipGetLastError());
     }
-    if (!(phase & kPhaseFinish)) return XTS_OK;
-    if (full) {
+    if (full && (phase & kPhaseTable)) {
         TimedRegion tr(ctx, "factor");
         hist_integral_kernel<<<(unsigned)S, kHistThreads, 0, ctx->stream>>>(
             (int)E, ca.hist, upsf, ca.mapint, ig.geo, parint, ct->sep, fac,
             (flags & XTS_CF_EXACT_RADIUS) ? 1 : 0);
         XTS_HIP(hipGetLastError());
     }
+    if (!(phase & kPhaseEmit)) return XTS_OK;
     // pass 3: blocks of kEmitBlockPix pixels of one source, then kWideBlocks
     // blocks per source for the wide pixels (they depend only on the factors
     // and the near lists and write values no other pass writes, so they run
@@ -2219,9 +2264,10 @@ static size_t model_map_arena_bytes(int64_t S, const xts_wcs* wcs, int64_t E, in
 // The one-call device model map for the batched engine (xts_batch.hip).
 int model_map_dev_call(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                        const double* expo, const double* upsf, const double* parint, int64_t E,
-                       const double* rule, int64_t N, int32_t flags, void* out) {
+                       const double* rule, int64_t N, int32_t flags, void* out, int phase) {
+    XTS_CHECK(phase >= 1 && phase <= kPhaseAll, "bad phase %d", phase);
     return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out,
-                            true);
+                            true, phase);
 }
 
 static size_t out_bytes(int64_t S, const xts_wcs* wcs, int64_t E, int32_t flags) {
