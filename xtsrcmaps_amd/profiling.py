@@ -4,6 +4,7 @@ Workloads to profile (run under ``rocprofv3 ... -- python3 -m
 xtsrcmaps_amd.profiling <workload>``):
 
     modelmap --config c1 --calls 3   the model-map passes on resident inputs
+    stages --config c3 --precision 1 the PSF stages (0 FP64, 1 BF16x3 contractions)
     copy --mib 592 --calls 3         an FP32 copy kernel of known size: the
                                      calibration of FETCH_SIZE / WRITE_SIZE
 
@@ -13,6 +14,7 @@ Summaries of rocprofv3 CSV output (plain Python, no GPU):
     traffic --mm-fetch D --mm-write D --copy-fetch D --copy-write D
             --copy-mib M --cmd CMD               JSON: HBM bytes per model map
     valu <pmc run dir> --kernel K --cmd CMD      JSON: VALU issue of one kernel
+    mfma <pmc run dir> --kernel RX --cmd CMD     JSON: matrix-core use of kernels
 """
 from __future__ import annotations
 
@@ -46,6 +48,23 @@ def _modelmap(config: str, calls: int, surrogate: bool = False) -> None:
     torch.cuda.synchronize()
     print(f"modelmap: {calls} calls of {config}" + (" with the surrogate" if surrogate else ""),
           flush=True)
+
+
+def _stages(config: str, calls: int, precision: int) -> None:
+    """The PSF stages (effective area, cosine bins, exposure, mean PSF and
+    its integrals) of a workload, `calls` times at the given contraction
+    precision (0 FP64, 1 BF16x3 on the matrix cores)."""
+    import torch
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    cfg, inp, ctx, d = bench.setup(config, surrogate=False)
+    bench.psf_stages(ctx, inp, d, precision)
+    torch.cuda.synchronize()
+    for _ in range(calls):
+        bench.psf_stages(ctx, inp, d, precision)
+    torch.cuda.synchronize()
+    print(f"stages: {calls} calls of {config} at precision {precision}", flush=True)
 
 
 def _copy(mib: int, calls: int) -> None:
@@ -133,6 +152,44 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
 VALU_LANE_OPS_PEAK = 157.3e12 / 2
 
 
+# Matrix-core counters of the kernels matching kernel_rx (one --pmc pass
+# with SQ_INSTS_VALU_MFMA_MOPS_BF16, SQ_INSTS_MFMA, SQ_VALU_MFMA_BUSY_CYCLES,
+# SQ_BUSY_CYCLES and GRBM_GUI_ACTIVE).  One MOPS count is 512 floating-point
+# operations of the matrix cores.
+MFMA_PEAK_BF16 = 2.5e15
+
+
+def mfma_json(run_dir: str, kernel_rx: str, cmd: str) -> dict:
+    path = _one(os.path.join(run_dir, "**", "*counter_collection.csv"))
+    rx = re.compile(kernel_rx)
+    disp = {}
+    for r in csv.DictReader(open(path)):
+        if not rx.search(r["Kernel_Name"]):
+            continue
+        d = disp.setdefault(r["Dispatch_Id"], {"ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+                                                "name": r["Kernel_Name"].split("(")[0]})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    if not disp:
+        raise ValueError(f"no dispatch of {kernel_rx} in {path}")
+    kernels = {}
+    for d in disp.values():
+        kernels.setdefault(d["name"], []).append(d)
+    out = {"command": cmd, "kernels": {}}
+    for name, rows in kernels.items():
+        mean = lambda k: sum(x.get(k, 0.0) for x in rows) / len(rows)
+        ns = mean("ns")
+        flops = mean("SQ_INSTS_VALU_MFMA_MOPS_BF16") * 512
+        out["kernels"][name] = {
+            "dispatches": len(rows), "duration_us": ns / 1e3,
+            "mfma_insts": mean("SQ_INSTS_MFMA"),
+            "mfma_bf16_flops": flops,
+            "mfma_bf16_tflops": flops / (ns * 1e-9) / 1e12 if ns else 0.0,
+            "frac_of_bf16_dense_peak": flops / (ns * 1e-9) / MFMA_PEAK_BF16 if ns else 0.0,
+            "mfma_busy_over_busy_cycles": (mean("SQ_VALU_MFMA_BUSY_CYCLES") /
+                                           max(mean("SQ_BUSY_CYCLES"), 1.0))}
+    return out
+
+
 def valu_json(run_dir: str, kernel_rx: str, cmd: str) -> dict:
     path = _one(os.path.join(run_dir, "**", "*counter_collection.csv"))
     rx = re.compile(kernel_rx)
@@ -166,6 +223,10 @@ def main(argv=None) -> None:
     p.add_argument("--config", default="c1")
     p.add_argument("--calls", type=int, default=3)
     p.add_argument("--surrogate", action="store_true")
+    p = sub.add_parser("stages")
+    p.add_argument("--config", default="c3")
+    p.add_argument("--calls", type=int, default=5)
+    p.add_argument("--precision", type=int, default=0)
     p = sub.add_parser("copy")
     p.add_argument("--mib", type=int, default=592)
     p.add_argument("--calls", type=int, default=3)
@@ -179,6 +240,10 @@ def main(argv=None) -> None:
     p.add_argument("--cmd", required=True)
     p.add_argument("--kernel-only", default="",
                    help="report this kernel alone (e.g. emit) as traffic_bytes_per_call")
+    p = sub.add_parser("mfma")
+    p.add_argument("run_dir")
+    p.add_argument("--kernel", default="gemm|exposure|mean_psf|contract")
+    p.add_argument("--cmd", required=True)
     p = sub.add_parser("valu")
     p.add_argument("run_dir")
     p.add_argument("--kernel", default="rowweight_kernel<float>")
@@ -186,8 +251,12 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     if a.what == "modelmap":
         _modelmap(a.config, a.calls, a.surrogate)
+    elif a.what == "stages":
+        _stages(a.config, a.calls, a.precision)
     elif a.what == "copy":
         _copy(a.mib, a.calls)
+    elif a.what == "mfma":
+        print(json.dumps(mfma_json(a.run_dir, a.kernel, a.cmd), indent=1))
     elif a.what == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.cmd))
     elif a.what == "valu":
