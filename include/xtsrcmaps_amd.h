@@ -322,9 +322,11 @@ int xts_model_map_footprints(xts_ctx* ctx, int64_t n_images, const xts_wcs* wcs,
  * XTS_LAYOUT_SEHW | XTS_OUT_F32 and FP32 cubature (XTS_SURROGATE and
  * XTS_CF_EXACT_RADIUS apply).  The work runs as a cost-ordered list of
  * source batches alternating between the context's stream and a second
- * internal stream (row weights of one batch beside the emit of the other);
- * it is ordered after the work already on the context stream and the
- * context stream waits for all of it.  Tuning keys: "batch_sources",
+ * internal stream, pipelined: each batch's row weights and factors wait for
+ * the previous batch's, its emit for the previous emit, so the row weights
+ * of one batch run beside the emit of the previous one; it is ordered after
+ * the work already on the context stream and the context stream waits for
+ * all of it.  Tuning keys: "batch_sources",
  * "rowweight_blocks", "autotune" (time candidate shapes on the first call
  * of a workload shape, keep the fastest), "graph" (capture the launches in a
  * HIP graph on the first call with given arguments, replay afterwards). */

@@ -10,14 +10,17 @@
 //                     pixel box of that disk on the source's image
 //   batches           a work list of (image, run of sources) batches ordered
 //                     by cost (pixels x sources, longest first); each batch is
-//                     one device model map (geometry, row weights, near
-//                     pixels, correction factor, emit).  Consecutive batches
-//                     alternate between the context and a second context with
-//                     its own stream and buffers, so the row-weight pass
-//                     (VALU-bound) of one batch runs beside the emit
-//                     (HBM-store-bound) of the previous one; the row-weight
-//                     pass can run on a limited grid of resident blocks
-//                     ("rowweight_blocks") so that the emit's blocks find room
+//                     one device model map in two calls (geometry, row
+//                     weights, near pixels, correction factor; then the
+//                     emit).  Consecutive batches alternate between the
+//                     context and a second context with its own stream and
+//                     buffers; a batch's first call waits for the previous
+//                     batch's first call and its emit for the previous emit,
+//                     so the row-weight pass (VALU-bound) of one batch runs
+//                     beside the emit (HBM-store-bound) of the previous one;
+//                     the row-weight pass can run on a limited grid of
+//                     resident blocks ("rowweight_blocks") so that the emit's
+//                     blocks find room
 //                     on every compute unit
 //   autotune          the batch size and row-weight grid, timed on the call's
 //                     own workload the first time a shape is seen
