@@ -54,6 +54,11 @@ extern "C" {
  * first moment taken as |A| (the value at the points' mean separation): the
  * method without the network, for comparison. */
 #define XTS_SURROGATE_NO_NET 0x20
+/* With XTS_SURROGATE: the network's hidden layers on the FP8 (OCP e4m3)
+ * matrix cores, one weight scale per layer and activations scaled per wave
+ * and layer, FP32 accumulation (the first layer stays BF16 high and low
+ * parts).  No counterpart in the reference. */
+#define XTS_SURROGATE_FP8 0x40
 
 /* contraction precision for the exposure and mean-PSF builds */
 #define XTS_PREC_FP64 0       /* FP64 FMA                                      */
@@ -315,6 +320,24 @@ int xts_model_map_footprints(xts_ctx* ctx, int64_t n_images, const xts_wcs* wcs,
                              const double* parint, int64_t E, double containment,
                              xts_footprint* boxes, double* radius, int64_t* total);
 
+/* The flux-weighted summed model of S point sources on one image: out
+ * device float [E, H, W] = sum_s flux[s] x (source s's map of
+ * xts_point_src_model_map_wcs_dev).  No counterpart in the reference, whose
+ * callers receive one map per source (model_map.point_src_model_map_wcs.cxx
+ * via ModelMap::compute_srcmaps) and sum them themselves; it is for sky
+ * models whose per-source maps do not fit in memory (10,000 sources x 100
+ * energies x 1000 x 1000 pixels is 4 TB).  flux device double [S]; the other
+ * inputs as for xts_point_src_model_map_wcs_dev; FP32 cubature; flags may
+ * hold XTS_SURROGATE (with XTS_SURROGATE_NO_NET) and XTS_CF_EXACT_RADIUS.
+ * The sources run in batches (tuning key "sum_batch_sources"; default: the
+ * batch's working buffers within about 12 GB): per batch the row weights,
+ * near pixels and factors, then every pixel's sum over the batch's sources
+ * accumulated in registers and added to out. */
+int xts_point_src_summed_model_dev(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph,
+                                   int64_t S, const double* flux, const double* expo,
+                                   const double* upsf, const double* parint, int64_t E,
+                                   const double* rule, int64_t N, int32_t flags, float* out);
+
 /* The batched model map: for each source, xts_point_src_model_map_wcs_dev's
  * values (its own image, exposure, PSF table, correction factor) inside its
  * box.  Device pointers as for that entry point; boxes host [S] (from
@@ -413,6 +436,11 @@ int xts_mlp_forward_dev(xts_ctx* ctx, const float* params, const float* norm, co
  * FP32 accumulation) -> y [B]. */
 int xts_mlp_infer_bf16_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,
                            int64_t ldx, int64_t B, float* y);
+/* FP8 inference (the record of XTS_SURROGATE_FP8: e4m3 hidden layers with
+ * per-layer weight scales and per-wave activation scales, FP32
+ * accumulation) -> y [B]. */
+int xts_mlp_infer_fp8_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,
+                          int64_t ldx, int64_t B, float* y);
 /* Gradient of the mean squared error of the normalised outputs over B
  * samples (targets y_target, stride ldy) -> grad [XTS_MLP_PARAMS], loss [1]. */
 int xts_mlp_grad_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,

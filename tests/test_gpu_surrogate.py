@@ -104,6 +104,20 @@ print(f"held-out: |B1 excess| max {float(tb.max()):.3f}; network error max {floa
       f"p99 {float(eb.quantile(0.99)):.2e} mean {float(eb.mean()):.2e}; BF16 vs FP32 "
       f"max {float(db.max()):.2e}", flush=True)
 check("bf16", float(db.max()) < 2e-3, "BF16 inference within 2e-3 of FP32 on the moment")
+# FP8 (e4m3) hidden layers: 3 mantissa bits, a rounding of up to 2^-4 of
+# each weight and activation, partly averaging out over the 64 inputs
+y8 = torch.empty_like(yb)
+ctx.mlp_forward_dev(net.params, net.norm, hold, y8, fp8=True)
+ctx.synchronize()
+d8 = (y8 - y32).abs()
+scale = float(y32.abs().max())
+print(f"FP8 vs FP32: max {float(d8.max()):.2e} p99 {float(d8.quantile(0.99)):.2e} "
+      f"mean {float(d8.mean()):.2e} (outputs up to {scale:.3f})", flush=True)
+# (one output's error can reach a rounding step of its largest terms, the
+# bulk stays near the averaged rounding)
+check("fp8", bool(torch.isfinite(y8).all()) and float(d8.max()) < 0.10 * scale and
+      float(d8.quantile(0.99)) < 0.02 * scale,
+      "FP8 inference within 10% (99th percentile 2%) of the largest output of FP32")
 
 # -- model map with the surrogate against the oracle's full cubature
 from oracle import oracle as orc
@@ -116,6 +130,7 @@ ref = orc.point_src_model_map_wcs(cfg.npix, cfg.npix, oracle_wcs(orc, inp.wcs), 
 base = xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F32
 res = {}
 for name, fl in (("cubature", base), ("surrogate", base | xa.SURROGATE),
+                 ("fp8 surrogate", base | xa.SURROGATE | xa.SURROGATE_FP8),
                  ("no network", base | xa.SURROGATE | xa.SURROGATE_NO_NET)):
     got = ctx.point_src_model_map_wcs(inp.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
                                       inp.rule, flags=fl)
@@ -127,6 +142,8 @@ check("cubature", res["cubature"].max() < 1e-5, "plain FP32 cubature within 1e-5
 check("surrogate", res["surrogate"].max() <= 1e-4, "surrogate within 1e-4")
 check("ablation", res["no network"].max() > 10 * res["surrogate"].max(),
       "without the network at least ten times worse")
+check("fp8 map", res["fp8 surrogate"].max() <= 1e-4,
+      "FP8 surrogate within 1e-4")
 sys.exit(0 if ok else 1)
 """
 

@@ -16,7 +16,7 @@ import numpy as np
 from . import capi
 from .capi import (XtsError, Wcs, PROJ_CAR, PROJ_AIT, LAYOUT_SHWE, LAYOUT_SEHW,
                    OUT_F64, OUT_F32, COMPUTE_F32, COMPUTE_F64, CF_EXACT_RADIUS, SURROGATE,
-                   SURROGATE_NO_NET, PREC_FP64, PREC_BF16X3)
+                   SURROGATE_NO_NET, SURROGATE_FP8, PREC_FP64, PREC_BF16X3)
 
 ND = 401
 
@@ -24,7 +24,7 @@ __all__ = [
     "Context", "Wcs", "XtsError", "separations", "costhetas",
     "livetime_efficiency_factors", "PROJ_CAR", "PROJ_AIT", "LAYOUT_SHWE",
     "LAYOUT_SEHW", "OUT_F64", "OUT_F32", "COMPUTE_F32", "COMPUTE_F64", "CF_EXACT_RADIUS",
-    "SURROGATE", "SURROGATE_NO_NET",
+    "SURROGATE", "SURROGATE_NO_NET", "SURROGATE_FP8",
     "PREC_FP64", "PREC_BF16X3", "ND", "write_src_model", "read_irf_pars", "read_expcube",
 ]
 
@@ -452,13 +452,14 @@ class Context:
                 ("norm", norm, "float32", (capi.MLP_NORM,)),
                 ("x", x, "float32", (x.shape[0], x.shape[1]))]
 
-    def mlp_forward_dev(self, params, norm, x, y, bf16=False):
-        """FP32 forward (or BF16 matrix-core inference) of rows x [B, >=4] -> y [B]."""
+    def mlp_forward_dev(self, params, norm, x, y, bf16=False, fp8=False):
+        """FP32 forward (or BF16 / FP8 matrix-core inference) of rows x [B, >=4] -> y [B]."""
         if x.dim() != 2 or x.shape[1] < 4:
             raise capi.XtsError(1, "x: expected [B, >= 4]")
         B = x.shape[0]
         p = capi.devs(self._mlp_specs(params, norm, x) + [("y", y, "float32", (B,))])
-        fn = capi.lib().xts_mlp_infer_bf16_dev if bf16 else capi.lib().xts_mlp_forward_dev
+        fn = (capi.lib().xts_mlp_infer_fp8_dev if fp8 else
+              capi.lib().xts_mlp_infer_bf16_dev if bf16 else capi.lib().xts_mlp_forward_dev)
         capi.check(fn(self._h, p[0], p[1], p[2], x.shape[1], B, p[3]))
 
     def mlp_grad_dev(self, params, norm, x, y_target, grad, loss):
@@ -576,6 +577,23 @@ class Context:
         rp, N = capi.rule_ptr(rule)
         capi.check(capi.lib().xts_point_src_model_map_wcs_dev(
             self._h, C.byref(wcs), p[0], S, p[1], p[2], p[3], E, rp, N, int(flags), p[4]))
+
+    def point_src_summed_model_dev(self, wcs: Wcs, src_sph, flux, exposure, upsf, parint, out,
+                                   rule=None, flags=0):
+        """Flux-weighted summed model out [E, H, W] float32 = sum_s flux[s] x
+        source s's map, on torch CUDA tensors (async on the context stream).
+        flags: SURROGATE (with SURROGATE_NO_NET or SURROGATE_FP8), CF_EXACT_RADIUS."""
+        S, E = exposure.shape
+        H, W = wcs.naxis1, wcs.naxis2
+        p = capi.devs([("src_sph", src_sph, "float64", (S, 2)),
+                       ("flux", flux, "float64", (S,)),
+                       ("exposure", exposure, "float64", (S, E)),
+                       ("upsf", upsf, "float64", (S, 401, E)),
+                       ("parint", parint, "float64", (S, 401, E)),
+                       ("out", out, "float32", (E, H, W))])
+        rp, N = capi.rule_ptr(rule)
+        capi.check(capi.lib().xts_point_src_summed_model_dev(
+            self._h, C.byref(wcs), p[0], S, p[1], p[2], p[3], p[4], E, rp, N, int(flags), p[5]))
 
     def point_src_model_map_rows_dev(self, wcs: Wcs, src_sph, E, rule=None,
                                      flags=LAYOUT_SEHW | OUT_F32 | COMPUTE_F32):

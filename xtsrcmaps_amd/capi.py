@@ -15,7 +15,7 @@ LAYOUT_SHWE, LAYOUT_SEHW = 0x0, 0x1
 OUT_F64, OUT_F32 = 0x0, 0x2
 COMPUTE_F32, COMPUTE_F64 = 0x0, 0x4
 CF_EXACT_RADIUS = 0x8
-SURROGATE, SURROGATE_NO_NET = 0x10, 0x20
+SURROGATE, SURROGATE_NO_NET, SURROGATE_FP8 = 0x10, 0x20, 0x40
 MLP_PARAMS, MLP_NORM = 8705, 10
 PREC_FP64, PREC_BF16X3 = 0, 1
 SCALE_PIXEL, SCALE_SRC_ENERGY = 0, 1
@@ -107,6 +107,8 @@ _SIGS = {
     "xts_point_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _i64,
                                         _d, _i64, C.c_int32, _vp],
     "xts_ctx_get_tuning": [_vp, C.c_char_p, C.POINTER(_i64)],
+    "xts_point_src_summed_model_dev": [_vp, C.POINTER(Wcs), _vp, _i64, _vp, _vp, _vp, _vp, _i64,
+                                       _d, _i64, C.c_int32, _vp],
     "xts_model_map_footprints": [_vp, _i64, C.POINTER(Wcs), C.POINTER(_i64), _vp, _vp, _i64,
                                  C.c_double, C.POINTER(Footprint), _d, C.POINTER(_i64)],
     "xts_point_src_model_maps_batched_dev": [_vp, _i64, C.POINTER(Wcs), C.POINTER(_i64), _vp,
@@ -117,6 +119,7 @@ _SIGS = {
                                   _vp],
     "xts_mlp_forward_dev": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "xts_mlp_infer_bf16_dev": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
+    "xts_mlp_infer_fp8_dev": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "xts_mlp_grad_dev": [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp],
     "xts_mlp_adam_dev": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float,
                          C.c_float],

@@ -187,6 +187,7 @@ static int ensure_aux(xts_ctx* ctx) {
     a->geom_cache = false;
     a->tune_tiles_per_block = ctx->tune_tiles_per_block;
     a->surrogate = ctx->surrogate;
+    a->surrogate8 = ctx->surrogate8;
     a->surrogate_ready = ctx->surrogate_ready;
     return XTS_OK;
 }

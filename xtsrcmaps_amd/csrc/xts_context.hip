@@ -256,6 +256,16 @@ int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
         ctx->tune_batch_sources = value;
         return XTS_OK;
     }
+    if (strcmp(key, "sum_batch_sources") == 0) {
+        XTS_CHECK(value >= 0, "sum_batch_sources >= 0");
+        ctx->tune_sum_batch = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "sum_group") == 0) {
+        XTS_CHECK(value == 4 || value == 8, "sum_group is 4 or 8");
+        ctx->tune_sum_group = value;
+        return XTS_OK;
+    }
     if (strcmp(key, "autotune") == 0) {
         XTS_CHECK(value == 0 || value == 1, "autotune is 0 or 1");
         ctx->autotune = (int)value;

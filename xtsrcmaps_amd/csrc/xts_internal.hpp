@@ -64,7 +64,8 @@ enum SlotId {
     kSlotIn6, kSlotIn7, kSlotOut0, kSlotOut1, kSlotOut2, kSlotOut3,
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
     kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotBilerp,
-    kSlotMlpState, kSlotMlpPacked, kSlotSurrogate, kSlotSurList, kSlotCount
+    kSlotMlpState, kSlotMlpPacked, kSlotSurrogate, kSlotSurList, kSlotSumFac,
+    kSlotSurrogate8, kSlotCount
 };
 
 }  // namespace xts
@@ -118,6 +119,7 @@ struct xts_ctx {
     xts_const_tables arena_consts;         // the constant tables inside the arena
     // the surrogate network in its matrix-core record (xts_ctx_set_surrogate)
     void* surrogate = nullptr;
+    void* surrogate8 = nullptr;   // the same network's FP8 record
     bool surrogate_ready = false;
     // batched engine: the second context its odd batches run on (own slots,
     // own stream), fork/join events, the tuned launch shapes per workload
@@ -135,6 +137,8 @@ struct xts_ctx {
     std::vector<Tuned> tuned;
     int autotune = 0, use_graph = 0;
     int64_t tune_batch_sources = 0;        // >0: sources per batch of the batched engine
+    int64_t tune_sum_batch = 0;            // >0: sources per batch of a summed model
+    int64_t tune_sum_group = 4;            // sources per staging round of the summed model (4, 8)
     int64_t last_batch_sources = 0, last_rw_blocks = 0, last_batches = 0;  // last batched call
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;

@@ -213,4 +213,164 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
     return fmaf(y + m->b4, m->norm[8], m->norm[9]);
 }
 
+// ---------------------------------------------------------------- FP8
+// The FP8 (OCP e4m3) inference record: layer 1 as in MlpPacked (BF16 high
+// and low parts: its four inputs carry the features' full precision), the
+// two hidden layers' weights as e4m3 with one scale per layer (w = q sw,
+// the layer's largest |w| at kFp8Max), in v_mfma_f32_16x16x32_fp8_fp8
+// operand order: tile To * 2 + Ti, lane l's eight bytes are the weights of
+// output 16 To + l % 16 for inputs mlp_kslot(Ti, l / 16, j), j = 0..7 (the
+// contraction pairs A's and B's bytes by lane group and byte, so the
+// previous layer's accumulator lanes feed B as in the BF16 record).
+// Activations are quantized per wave and layer at run time: the wave's
+// largest |h| at kFp8Max.  e4m3 has 3 mantissa bits (relative rounding
+// 2^-4) and no saturation (beyond 464 it converts to NaN): kFp8Max = 440
+// keeps every scaled value finite.
+constexpr float kFp8Max = 440.0f;
+typedef long mlp_fp8x8;
+struct MlpPacked8 {
+    mlp_bf16x4 w1[4][64];
+    mlp_fp8x8 w2[8][64];
+    mlp_fp8x8 w3[8][64];
+    float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
+    float b4, norm[kMlpNorm];
+    float sw2, sw3;
+    float pad[3];
+};
+static_assert(sizeof(MlpPacked8) % 16 == 0, "MlpPacked8 size");
+
+// Eight floats to e4m3 bytes (v_cvt_pk_fp8_f32, round to nearest even).
+__device__ inline mlp_fp8x8 mlp_fp8_pack8(const float (&v)[8]) {
+    int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+    int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+    return (mlp_fp8x8)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Pack FP32 parameters into the FP8 record (one block); sw2, sw3: the
+// hidden layers' weight scales (largest |w| / kFp8Max, computed by the
+// caller over each layer).
+__device__ inline void mlp_pack8(const float* __restrict__ p, const float* __restrict__ norm,
+                                 float sw2, float sw3, MlpPacked8* __restrict__ out, int tid,
+                                 int nthr) {
+    for (int k = tid; k < 4 * 64; k += nthr) {
+        const int To = k / 64, l = k % 64, g = l / 16, o = 16 * To + l % 16;
+        mlp_bf16x4 v;
+        for (int j = 0; j < 4; ++j) {
+            const float w = p[kOffW1 + o * kMlpIn + j];
+            const uint16_t hi = mlp_bf16(w), lo = mlp_bf16(w - mlp_from_bf16(hi));
+            v[j] = (short)(g <= 1 ? hi : (g == 2 ? lo : 0));
+        }
+        out->w1[To][l] = v;
+    }
+    for (int k = tid; k < 2 * 8 * 64; k += nthr) {
+        const int layer = k / (8 * 64), r = k % (8 * 64);
+        const int t = r / 64, l = r % 64, To = t / 2, Ti = t % 2;
+        const float* W = p + (layer == 0 ? kOffW2 : kOffW3);
+        const float isw = 1.0f / (layer == 0 ? sw2 : sw3);
+        float v[8];
+        for (int j = 0; j < 8; ++j)
+            v[j] = W[(16 * To + l % 16) * kMlpH + mlp_kslot(Ti, l / 16, j)] * isw;
+        (layer == 0 ? out->w2 : out->w3)[t][l] = mlp_fp8_pack8(v);
+    }
+    for (int k = tid; k < kMlpH; k += nthr) {
+        out->b1[k] = p[kOffB1 + k];
+        out->b2[k] = p[kOffB2 + k];
+        out->b3[k] = p[kOffB3 + k];
+        out->w4[k] = p[kOffW4 + k];
+    }
+    if (tid == 0) {
+        out->b4 = p[kOffB4];
+        for (int k = 0; k < kMlpNorm; ++k) out->norm[k] = norm[k];
+        out->sw2 = sw2;
+        out->sw3 = sw3;
+        for (int k = 0; k < 3; ++k) out->pad[k] = 0.0f;
+    }
+}
+
+// Largest |x| over the wave (every lane gets it).
+__device__ inline float mlp_wave_absmax(float x) {
+    x = fabsf(x);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+    return x;
+}
+
+// One hidden layer on the FP8 matrix cores: h (this lane's 16 inputs: tile
+// To's outputs 4 (l / 16) + i of the previous layer) quantized at the
+// wave's scale, times the layer's e4m3 weights, rescaled, plus the bias.
+__device__ inline void mlp_fp8_layer(const mlp_fp8x8 (*w)[64], float sw, const float* bias,
+                                     const float (&h)[4][4], float (&out)[4][4]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    float m = 0.0f;
+#pragma unroll
+    for (int To = 0; To < 4; ++To)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m = fmaxf(m, fabsf(h[To][i]));
+    m = mlp_wave_absmax(m);
+    const float sh = m > 0.0f ? m / kFp8Max : 1.0f;
+    const float ish = 1.0f / sh;
+    // a 32-wide input chunk Ti is the previous layer's tiles 2 Ti, 2 Ti + 1
+    mlp_fp8x8 X[2];
+#pragma unroll
+    for (int Ti = 0; Ti < 2; ++Ti) {
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            v[i] = h[2 * Ti][i] * ish;
+            v[4 + i] = h[2 * Ti + 1][i] * ish;
+        }
+        X[Ti] = mlp_fp8_pack8(v);
+    }
+    const float sc = sw * sh;
+#pragma unroll
+    for (int To = 0; To < 4; ++To) {
+        mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int Ti = 0; Ti < 2; ++Ti)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(w[To * 2 + Ti][lane], X[Ti], acc, 0,
+                                                             0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[To][i] = fmaf(acc[i], sc, bias[16 * To + 4 * g + i]);
+    }
+}
+
+// FP8 inference of 16 samples per wave (as mlp_infer16).
+__device__ inline float mlp_infer16_fp8(const MlpPacked8* __restrict__ m, const float (&x)[4]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    mlp_bf16x4 in;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float xn = (x[j] - m->norm[j]) * m->norm[4 + j];
+        const uint16_t hi = mlp_bf16(xn);
+        const uint16_t lo = mlp_bf16(xn - mlp_from_bf16(hi));
+        in[j] = (short)(g == 1 ? lo : (g == 3 ? 0 : hi));
+    }
+    float h[4][4], z[4][4];
+#pragma unroll
+    for (int To = 0; To < 4; ++To) {
+        mlp_f32x4 acc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = m->b1[16 * To + 4 * g + i];
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w1[To][lane], in, acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[To][i] = silu(acc[i]);
+    }
+    mlp_fp8_layer(m->w2, m->sw2, m->b2, h, z);
+#pragma unroll
+    for (int To = 0; To < 4; ++To)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[To][i] = silu(z[To][i]);
+    mlp_fp8_layer(m->w3, m->sw3, m->b3, h, z);
+    float y = 0.0f;
+#pragma unroll
+    for (int To = 0; To < 4; ++To)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], silu(z[To][i]), y);
+    y += __shfl_xor(y, 16);
+    y += __shfl_xor(y, 32);
+    return fmaf(y + m->b4, m->norm[8], m->norm[9]);
+}
+
 }  // namespace xts

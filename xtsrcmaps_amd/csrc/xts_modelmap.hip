@@ -210,6 +210,12 @@ struct CubArgs {
     float4* sur_list;          // [S][tiles][256][2]: features; A, pixel, solid angle, 2 r0 + in-radius
     int* sur_count;            // [S][tiles] entries of each tile's 256 list slots
     const MlpPacked* sur_net;
+    const MlpPacked8* sur_net8;
+    // summed model (xts_point_src_summed_model_dev): per-source flux and the
+    // FP32 [E][H W] sum the batch's maps x flux are added to; else null
+    const double* flux;
+    float* sum;
+    float* sum_fac;            // [S][Ep] flux x exposure x correction in FP32
 };
 
 __device__ inline float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994530942f; }
@@ -411,7 +417,8 @@ struct ChunkScale {
             if (tid >= ne) x = c = 1.0;
             ex[tid] = x;
             cf[tid] = c;
-            f32[tid] = (float)(x * c);
+            // a summed model's values also carry the source's flux
+            f32[tid] = a.flux ? (float)(x * c * a.flux[se0 / a.E]) : (float)(x * c);
         }
     }
     template <typename OT, typename T>
@@ -1066,21 +1073,23 @@ rowweight_kernel(CubArgs a) {
 // B1 = |A| + max(y, 0) (y = 0 without the network), the three row weights
 // (poly_rows), stores them x solid angle and adds the in-radius ones to the
 // block's LDS histogram, added to the source's once.  Grid: (blocks
-// striding over the list, S).
+// striding over the list, S).  Rec: MlpPacked (BF16) or MlpPacked8 (FP8).
 constexpr int kSurThreads = 256;
+template <typename Rec>
 __global__ void __launch_bounds__(kSurThreads)
 sur_rows_kernel(CubArgs a) {
-    __shared__ MlpPacked net;
+    __shared__ Rec net;
     __shared__ double hblk[kNDPad];
     const int s = blockIdx.y;
     const int ntiles = a.tiles_w * a.tiles_h;
     const int waves = kSurThreads / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
     if ((int)blockIdx.x * waves >= ntiles) return;
-    const bool use_net = a.sur == 1;
+    constexpr bool F8 = sizeof(Rec) == sizeof(MlpPacked8);
+    const bool use_net = a.sur == (F8 ? 3 : 1);
     if (use_net) {
-        const float4* src = reinterpret_cast<const float4*>(a.sur_net);
+        const float4* src = reinterpret_cast<const float4*>(F8 ? a.sur_net8 : (const void*)a.sur_net);
         float4* dst = reinterpret_cast<float4*>(&net);
-        for (int k = threadIdx.x; k < (int)(sizeof(MlpPacked) / 16); k += kSurThreads)
+        for (int k = threadIdx.x; k < (int)(sizeof(Rec) / 16); k += kSurThreads)
             dst[k] = src[k];
     }
     for (int k = threadIdx.x; k < kND; k += kSurThreads) hblk[k] = 0.0;
@@ -1102,7 +1111,8 @@ sur_rows_kernel(CubArgs a) {
             float y = 0.0f;
             if (use_net) {
                 const float x[4] = {f.x, f.y, f.z, f.w};
-                y = mlp_infer16(&net, x);
+                if constexpr (F8) y = mlp_infer16_fp8(&net, x);
+                else y = mlp_infer16(&net, x);
             }
             if (lane < 16 && i < count) {
                 const float A = e.x, om = e.z;
@@ -1286,7 +1296,7 @@ near_rows_kernel(CubArgs a) {
 // vectors and accumulates W_r L[r], and the wave sums the 64 partial
 // vectors.  x solid angle, x exposure x correction when SCALED.
 constexpr int kWideBlocks = 2;   // extra emit blocks per source and energy chunk
-template <typename T, int EC, typename OT, bool SCALED>
+template <typename T, int EC, typename OT, bool SCALED, bool SUM = false>
 __device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
     using V = typename Vec<T>::type;
     constexpr int VN = Vec<T>::n;
@@ -1343,6 +1353,13 @@ __device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
                 const int e = v * VN + j;
                 if (lane == 0 && e < ne) {
                     x *= om;
+                    if constexpr (SUM) {
+                        // summed model: x flux into the [E][H W] sum
+                        const int se = s * a.E + e0 + e;
+                        atomicAdd(a.sum + (size_t)(e0 + e) * plane + pix,
+                                  (float)x * (float)(a.expo[se] * a.cf[se] * a.flux[s]));
+                        continue;
+                    }
                     OT* po = reinterpret_cast<OT*>(a.out) +
                              out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
                     if (SCALED) {
@@ -1365,8 +1382,8 @@ __device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
 // Pass 2b (after the correction factor): the pooled listed pixels' map
 // values, written once: sum_r W_r L[rbase + r] (W already x solid angle),
 // x exposure x correction when SCALED.  Grid: (blocks striding over the
-// list, S, energy chunks).
-template <typename T, int EC, typename OT, bool SCALED>
+// list, S, energy chunks).  SUM: added x flux to a summed model instead.
+template <typename T, int EC, typename OT, bool SCALED, bool SUM = false>
 __global__ void __launch_bounds__(kNearThreads)
 near_emit_kernel(CubArgs a) {
     using V = typename Vec<T>::type;
@@ -1428,9 +1445,14 @@ near_emit_kernel(CubArgs a) {
                 const int e = v * VN + j;
                 if (e < ne) {
                     const T x = reinterpret_cast<const T*>(&acc[v])[j];
-                    OT* po = reinterpret_cast<OT*>(a.out) +
-                             out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
-                    *po = SCALED ? sc.template apply<OT>(x, e) : (OT)x;
+                    if constexpr (SUM) {
+                        atomicAdd(a.sum + (size_t)(e0 + e) * plane + pix,
+                                  sc.template apply<float>(x, e));
+                    } else {
+                        OT* po = reinterpret_cast<OT*>(a.out) +
+                                 out_index(a.layout_sehw, s, pix, e0 + e, a.E, plane);
+                        *po = SCALED ? sc.template apply<OT>(x, e) : (OT)x;
+                    }
                 }
             }
         }
@@ -1599,6 +1621,246 @@ emit_shwe_kernel(CubArgs a) {
 }
 
 
+// The summed model's per-(source, energy) factor flux x exposure x
+// correction, rounded to FP32 once ([S][Ep], zero past E).
+__global__ void sum_factor_kernel(CubArgs a) {
+    const int n = a.S * a.Ep;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int s = k / a.Ep, e = k % a.Ep;
+        a.sum_fac[k] = e < a.E ? (float)(a.expo[s * a.E + e] * a.cf[s * a.E + e] * a.flux[s])
+                               : 0.0f;
+    }
+}
+
+// Zeroes n floats (a kernel rather than a memset, so that captured graphs
+// replay it correctly).
+__global__ void zero_kernel(float* __restrict__ p, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0.0f;
+}
+
+// Pass 3 of a summed model (xts_point_src_summed_model_dev), FP32 on the
+// matrix cores: per 16x16 pixel tile and chunk of 16 NT energies, the sum
+// over the batch's sources of flux x exposure x correction x sum_k W_k
+// L[r0 + k] for the far pixels.  The block takes the sources in groups of
+// G: it loads the group's row indices and weights together, stages each
+// source's scaled table rows M[r][e] = L[r][e] x factor[e] over its far
+// pixels' window [rmin, rmax + 3] (at most kSumWin rows) in LDS, and then
+// each wave forms D[pixel][e] += A[pixel][r] M[r][e] on
+// v_mfma_f32_16x16x4_f32 for every source of the group, four window rows a
+// step: A holds each pixel's four row weights in its rows' columns (zero
+// elsewhere).  One load latency and two barriers serve G sources.  Wave w
+// owns tile rows 4w .. 4w+3 (one 16-pixel m-tile each); lane l owns the
+// pixel of m-tile l / 16, column l % 16, and supplies A[l % 16][l / 16] for
+// each m-tile from that pixel's lane by a shuffle; B[k][n] = M[4 kk + l /
+// 16][16 n + l % 16]; D[4 (l / 16) + i][l % 16] is four consecutive pixels
+// of one energy.  Wider windows (tiles next to the source) run afterwards,
+// one source at a time, in passes of G kSumWin rows.  Listed and wide
+// pixels are added by near_emit_kernel and wide_sum_kernel; the launches of
+// one call are ordered on one stream, so the read-add-write of the sum
+// needs no atomics.
+constexpr int kSumWin = 16;
+typedef float sum_f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__device__ inline void sum_mfma_window(const float* __restrict__ Mw, int c0, int nk, int lane,
+                                       const int (&ra)[4], const float (&wa)[4][4],
+                                       sum_f32x4 (&acc)[4][NT]) {
+    constexpr int EC = 16 * NT;
+    for (int kk = 0; kk < nk; ++kk) {
+        // A[l % 16][l / 16]: the weight of window row c0 + 4 kk + l / 16
+        const int row = c0 + 4 * kk + (lane >> 4);
+        float A[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int j = row - ra[m];
+            A[m] = j == 0 ? wa[m][0] : j == 1 ? wa[m][1] : j == 2 ? wa[m][2]
+                                                          : j == 3 ? wa[m][3] : 0.0f;
+        }
+        const float* Mk = Mw + (4 * kk + (lane >> 4)) * EC + (lane & 15);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const float B = Mk[16 * n];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[m], B, acc[m][n], 0, 0, 0);
+        }
+    }
+}
+
+// Rows [c0, c0 + nr) of source s's scaled table into Mw (4 nk rows, the
+// rest zero), by all threads of the block.
+template <int EC>
+__device__ inline void sum_stage(const CubArgs& a, int s, int e0, int ne, int c0, int nr, int nk,
+                                 float* __restrict__ Mw, int tid) {
+    const float* L = reinterpret_cast<const float*>(a.lut) + (size_t)s * kND * a.Ep + e0;
+    const float* f = a.sum_fac + (size_t)s * a.Ep + e0;
+    for (int k = tid; k < 4 * nk * EC; k += kThreads) {
+        const int r = k / EC, e = k % EC;
+        float x = 0.0f;
+        if (r < nr && e < ne) x = L[(size_t)(c0 + r) * a.Ep + e] * f[e];
+        Mw[k] = x;
+    }
+}
+
+template <int NT, int G>
+__global__ void __launch_bounds__(kThreads)
+sum_far_kernel(CubArgs a) {
+    constexpr int EC = 16 * NT;
+    constexpr int kWaves = kThreads / 64;
+    __shared__ __align__(16) float M[G * kSumWin * EC];
+    __shared__ int red[G][2 * kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int th = blockIdx.x / a.tiles_w, tw = blockIdx.x % a.tiles_w;
+    // this lane's pixel: m-tile lane / 16 of the wave, column lane % 16
+    const int h = th * kTileH + 4 * wv + (lane >> 4), w = tw * kTileW + (lane & 15);
+    const bool valid = h < a.H && w < a.W;
+    const int pix = valid ? h * a.W + w : 0;
+    const int e0 = blockIdx.y * EC;
+    const int ne = min(EC, a.E - e0);
+    const size_t plane = (size_t)a.H * a.W;
+    const float4* rw4 = reinterpret_cast<const float4*>(a.rw);
+    sum_f32x4 acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = sum_f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s0 = 0; s0 < a.S; s0 += G) {
+        const int ng = min(G, a.S - s0);
+        int r0g[G];
+        float4 Wg[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            r0g[g] = kRowWide;
+            Wg[g] = float4{0.f, 0.f, 0.f, 0.f};
+            if (g < ng && valid) {
+                r0g[g] = a.rw_row[(size_t)(s0 + g) * plane + pix];
+                Wg[g] = rw4[(size_t)(s0 + g) * plane + pix];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const bool far = r0g[g] >= 0;
+            int lo = far ? r0g[g] : kND, hi = far ? r0g[g] : -1;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                lo = min(lo, __shfl_xor(lo, o));
+                hi = max(hi, __shfl_xor(hi, o));
+            }
+            if (lane == 0) {
+                red[g][wv] = lo;
+                red[g][kWaves + wv] = hi;
+            }
+        }
+        __syncthreads();
+        int rmin[G], rmax[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            rmin[g] = kND;
+            rmax[g] = -1;
+#pragma unroll
+            for (int q = 0; q < kWaves; ++q) {
+                rmin[g] = min(rmin[g], red[g][q]);
+                rmax[g] = max(rmax[g], red[g][kWaves + q]);
+            }
+            if (g >= ng) rmax[g] = -1;
+        }
+        // the group's narrow windows, staged together
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nr = rmax[g] + 4 - rmin[g];
+            if (rmax[g] >= 0 && nr <= kSumWin)
+                sum_stage<EC>(a, s0 + g, e0, ne, rmin[g], nr, (nr + 3) / 4,
+                              M + g * kSumWin * EC, tid);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nr = rmax[g] + 4 - rmin[g];
+            if (rmax[g] < 0 || nr > kSumWin) continue;
+            const int rr = r0g[g] >= 0 ? r0g[g] : -1000;   // not far: out of every window
+            int ra[4];
+            float wa[4][4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int src = 16 * m + (lane & 15);
+                ra[m] = __shfl(rr, src);
+                wa[m][0] = __shfl(Wg[g].x, src);
+                wa[m][1] = __shfl(Wg[g].y, src);
+                wa[m][2] = __shfl(Wg[g].z, src);
+                wa[m][3] = __shfl(Wg[g].w, src);
+            }
+            sum_mfma_window<NT>(M + g * kSumWin * EC, rmin[g], (nr + 3) / 4, lane, ra, wa, acc);
+        }
+        // wide windows: one source at a time, in passes of G kSumWin rows
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (rmax[g] < 0 || rmax[g] + 4 - rmin[g] <= kSumWin) continue;
+            const int rr = r0g[g] >= 0 ? r0g[g] : -1000;
+            int ra[4];
+            float wa[4][4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int src = 16 * m + (lane & 15);
+                ra[m] = __shfl(rr, src);
+                wa[m][0] = __shfl(Wg[g].x, src);
+                wa[m][1] = __shfl(Wg[g].y, src);
+                wa[m][2] = __shfl(Wg[g].z, src);
+                wa[m][3] = __shfl(Wg[g].w, src);
+            }
+            for (int c0 = rmin[g]; c0 <= rmax[g] + 3; c0 += G * kSumWin) {
+                const int nr = min(G * kSumWin, rmax[g] + 4 - c0);
+                __syncthreads();   // M's previous readers are done
+                sum_stage<EC>(a, s0 + g, e0, ne, c0, nr, (nr + 3) / 4, M, tid);
+                __syncthreads();
+                sum_mfma_window<NT>(M, c0, (nr + 3) / 4, lane, ra, wa, acc);
+            }
+        }
+        __syncthreads();   // M and red are rewritten by the next group
+    }
+    // D[4 (l / 16) + i][l % 16] of m-tile m: pixels 4 (l / 16) .. +3 of tile
+    // row 4 wv + m, energy 16 n + l % 16
+    const int hh = th * kTileH + 4 * wv;
+    const int wq = tw * kTileW + 4 * (lane >> 4);
+    const bool vec = (a.W & 3) == 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (hh + m >= a.H) break;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int e = 16 * n + (lane & 15);
+            if (e >= ne) continue;
+            float* po = a.sum + (size_t)(e0 + e) * plane + (size_t)(hh + m) * a.W + wq;
+            if (vec && wq + 3 < a.W) {
+                float4 y = *reinterpret_cast<float4*>(po);
+                y.x += acc[m][n][0];
+                y.y += acc[m][n][1];
+                y.z += acc[m][n][2];
+                y.w += acc[m][n][3];
+                *reinterpret_cast<float4*>(po) = y;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (wq + i < a.W) po[i] += acc[m][n][i];
+            }
+        }
+    }
+}
+
+// The wide pixels of a summed model's batch: kWideBlocks blocks per source
+// and energy chunk, as in the emit.
+template <int EC>
+__global__ void __launch_bounds__(kEmitThreads)
+wide_sum_kernel(CubArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float* H = reinterpret_cast<float*>(smem_raw);
+    constexpr int kWaves = kEmitThreads / 64;
+    wide_pixels<float, EC, float, true, true>(a, blockIdx.x * kWaves + threadIdx.x / 64,
+                                              kWideBlocks * kWaves,
+                                              H + (threadIdx.x / 64) * kNDPad);
+}
+
 template <typename T, int EC, typename OT, bool SCALED>
 static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) {
     const size_t sm = (size_t)kND * (sehw ? EC + 1 : EC) * sizeof(T);
@@ -1613,11 +1875,11 @@ static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) 
 
 // Pooled near pixels (wide = false) or wide pixels (wide = true: a few
 // waves per source scan the list).
-template <typename T, int EC, typename OT, bool SCALED>
+template <typename T, int EC, typename OT, bool SCALED, bool SUM = false>
 static int launch_near_emit(hipStream_t st, const CubArgs& ca, dim3 grid) {
     const size_t sm = (size_t)kND * EC * sizeof(T);
-    XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED>, sm));
-    near_emit_kernel<T, EC, OT, SCALED><<<grid, kNearThreads, sm, st>>>(ca);
+    XTS_TRY(allow_lds((const void*)near_emit_kernel<T, EC, OT, SCALED, SUM>, sm));
+    near_emit_kernel<T, EC, OT, SCALED, SUM><<<grid, kNearThreads, sm, st>>>(ca);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -1971,7 +2233,8 @@ static int upload_rule(xts_ctx* ctx, const double* rule, int64_t N, RuleUVW** de
 // emit).  kAll runs both in one call.  The batched engine splits the second
 // half once more: kPhaseRows | kPhaseTable, then kPhaseEmit alone (the
 // geometry and table of the first call kept).
-enum { kPhaseRows = 1, kPhaseTable = 2, kPhaseEmit = 4, kPhaseFinish = 6, kPhaseAll = 7 };
+enum { kPhaseRows = 1, kPhaseTable = 2, kPhaseEmit = 4, kPhaseFinish = 6, kPhaseAll = 7,
+       kPhaseSum = 8 };   // in place of kPhaseEmit: the batch added x flux to a summed model
 
 // Per-source int counters (near and wide pixels), in doubles of the
 // accumulator slot.
@@ -1980,7 +2243,7 @@ static size_t model_map_counter_doubles(int64_t S) { return (size_t)S; }
 static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                             const double* expo, const double* upsf, const double* parint,
                             int64_t E, const double* rule, int64_t N, int32_t flags, void* out,
-                            bool full, int phase = kPhaseAll) {
+                            bool full, int phase = kPhaseAll, const double* flux = nullptr) {
     // any model map other than the second half of a split one overwrites the
     // near lists, counters and radii that a pending first half left: its
     // second half is refused from now on
@@ -1991,17 +2254,25 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
               "bad S=%lld or E=%lld", (long long)S, (long long)E);
     XTS_CHECK(src_sph, "null device pointer");
     XTS_CHECK(!(phase & kPhaseTable) || upsf, "null device pointer");
-    XTS_CHECK(!(phase & kPhaseEmit) || out, "null device pointer");
+    XTS_CHECK(!(phase & (kPhaseEmit | kPhaseSum)) || out, "null device pointer");
+    XTS_CHECK(!(phase & kPhaseSum) || (flux && !(phase & kPhaseEmit) && full &&
+                                       !(flags & XTS_COMPUTE_F64) && (flags & XTS_OUT_F32)),
+              "a summed model needs the flux, the FP32 cubature and no per-source output");
     XTS_CHECK(!(phase & kPhaseTable) || !full || (expo && parint), "null exposure/parint");
-    XTS_CHECK((flags & ~0x3F) == 0, "unknown flags 0x%x", flags);
+    XTS_CHECK((flags & ~0x7F) == 0, "unknown flags 0x%x", flags);
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     XTS_CHECK((int64_t)H * W <= (int64_t)1 << 30, "image of %lld pixels exceeds 2^30",
               (long long)H * W);
     const bool f64 = (flags & XTS_COMPUTE_F64) != 0;
-    const int sur = (flags & XTS_SURROGATE) ? ((flags & XTS_SURROGATE_NO_NET) ? 2 : 1) : 0;
+    // 1 the BF16 network, 2 no network, 3 the FP8 network
+    const int sur = (flags & XTS_SURROGATE)
+                        ? ((flags & XTS_SURROGATE_NO_NET) ? 2 : (flags & XTS_SURROGATE_FP8) ? 3 : 1)
+                        : 0;
+    XTS_CHECK(!(flags & XTS_SURROGATE_FP8) || (sur && !(flags & XTS_SURROGATE_NO_NET)),
+              "XTS_SURROGATE_FP8 needs XTS_SURROGATE (and the network)");
     XTS_CHECK(!(flags & XTS_SURROGATE_NO_NET) || sur, "XTS_SURROGATE_NO_NET needs XTS_SURROGATE");
     XTS_CHECK(!sur || (!f64 && full), "the surrogate serves the FP32 model map only");
-    XTS_CHECK(sur != 1 || ctx->surrogate_ready,
+    XTS_CHECK((sur != 1 && sur != 3) || ctx->surrogate_ready,
               "XTS_SURROGATE: no network set (xts_ctx_set_surrogate)");
     const int sehw = (flags & XTS_LAYOUT_SEHW) ? 1 : 0;
     const int of32 = (flags & XTS_OUT_F32) ? 1 : 0;
@@ -2078,6 +2349,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.wide_count = ca.near_count + S;
     ca.sur = sur;
     ca.sur_net = sur == 1 ? (const MlpPacked*)ctx->surrogate : nullptr;
+    ca.sur_net8 = sur == 3 ? (const MlpPacked8*)ctx->surrogate8 : nullptr;
     ca.sur_list = nullptr;
     ca.sur_count = nullptr;
     if (sur) {
@@ -2152,7 +2424,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                                                    div_up(tiles, kSurThreads / 64)),
                         (unsigned)S);
         TimedRegion tr(ctx, "surrogate");
-        sur_rows_kernel<<<grid, kSurThreads, 0, ctx->stream>>>(ca);
+        if (sur == 3) sur_rows_kernel<MlpPacked8><<<grid, kSurThreads, 0, ctx->stream>>>(ca);
+        else sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, ctx->stream>>>(ca);
         XTS_HIP(hipGetLastError());
     }
     // pass 2a: blocks stride over each source's list
@@ -2176,6 +2449,36 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
             (int)E, ca.hist, upsf, ca.mapint, ig.geo, parint, ct->sep, fac,
             (flags & XTS_CF_EXACT_RADIUS) ? 1 : 0);
         XTS_HIP(hipGetLastError());
+    }
+    if (phase & kPhaseSum) {
+        // the batch's maps x flux added to the summed model out [E][H W]
+        ca.flux = flux;
+        ca.sum = (float*)out;
+        void* sf;
+        XTS_TRY(slot_reserve(ctx, kSlotSumFac, (size_t)S * Ep * sizeof(float), &sf));
+        ca.sum_fac = (float*)sf;
+        TimedRegion tr(ctx, "sum");
+        sum_factor_kernel<<<(unsigned)std::min<int64_t>(div_up((int64_t)S * Ep, 256), 1024), 256, 0,
+                            ctx->stream>>>(ca);
+        const unsigned nt = (unsigned)(ca.tiles_w * ca.tiles_h);
+        const dim3 g48(nt, (unsigned)div_up(E, 48)), g112(nt, (unsigned)div_up(E, 112));
+        const bool g4 = ctx->tune_sum_group != 8;
+        if (E <= 48) {
+            if (g4) sum_far_kernel<3, 4><<<g48, kThreads, 0, ctx->stream>>>(ca);
+            else sum_far_kernel<3, 8><<<g48, kThreads, 0, ctx->stream>>>(ca);
+        } else {
+            if (g4) sum_far_kernel<7, 4><<<g112, kThreads, 0, ctx->stream>>>(ca);
+            else sum_far_kernel<7, 8><<<g112, kThreads, 0, ctx->stream>>>(ca);
+        }
+        XTS_HIP(hipGetLastError());
+        const size_t smw = (size_t)(kEmitThreads / 64) * kNDPad * sizeof(float);
+        XTS_TRY(allow_lds((const void*)wide_sum_kernel<40>, smw));
+        wide_sum_kernel<40><<<dim3(kWideBlocks, (unsigned)S, (unsigned)div_up(E, 40)), kEmitThreads,
+                              smw, ctx->stream>>>(ca);
+        XTS_HIP(hipGetLastError());
+        const dim3 gn((unsigned)near_gx, (unsigned)S, (unsigned)div_up(E, 40));
+        XTS_TRY((launch_near_emit<float, 40, float, true, true>(ctx->stream, ca, gn)));
+        return XTS_OK;
     }
     if (!(phase & kPhaseEmit)) return XTS_OK;
     // pass 3: blocks of kEmitBlockPix pixels of one source, then kWideBlocks
@@ -2295,6 +2598,44 @@ int xts_point_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* wcs, const doub
     return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out, true);
 }
 
+int xts_point_src_summed_model_dev(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph,
+                                   int64_t S, const double* flux, const double* expo,
+                                   const double* upsf, const double* parint, int64_t E,
+                                   const double* rule, int64_t N, int32_t flags, float* out) {
+    XTS_TRY(activate(ctx));
+    XTS_TRY(check_wcs(wcs));
+    XTS_CHECK(S > 0 && S <= (1 << 24) && E > 0 && E <= 4096, "bad S=%lld or E=%lld",
+              (long long)S, (long long)E);
+    XTS_CHECK(src_sph && flux && expo && upsf && parint && out, "null device pointer");
+    XTS_CHECK((flags & ~(XTS_SURROGATE | XTS_SURROGATE_NO_NET | XTS_SURROGATE_FP8 |
+                         XTS_CF_EXACT_RADIUS)) == 0,
+              "xts_point_src_summed_model_dev: flags 0x%x (only XTS_SURROGATE, "
+              "XTS_SURROGATE_NO_NET, XTS_SURROGATE_FP8 and XTS_CF_EXACT_RADIUS apply)", flags);
+    const size_t plane = (size_t)wcs->naxis1 * wcs->naxis2;
+    XTS_CHECK(plane <= (size_t)1 << 30, "image too large");
+    const bool sur = (flags & XTS_SURROGATE) != 0;
+    // sources per batch: the per-source working buffers (row weights, row
+    // indices, near lists: 36 B per pixel; the surrogate's tile lists 32 B
+    // more) within about 12 GB, unless tuned
+    const double per_src = (double)plane * (36.0 + (sur ? 33.0 : 0.0)) + 401.0 * 4 * (E + 3);
+    int64_t B = ctx->tune_sum_batch > 0 ? ctx->tune_sum_batch
+                                        : (int64_t)std::max(1.0, std::floor(12e9 / per_src));
+    B = std::max<int64_t>(1, std::min(B, S));
+    XTS_CHECK(B * E < (1LL << 31), "batch of %lld sources x %lld energies too large",
+              (long long)B, (long long)E);
+    zero_kernel<<<(unsigned)std::min<size_t>(div_up(E * plane, 256), 65536), 256, 0, ctx->stream>>>(
+        out, (size_t)E * plane);
+    XTS_HIP(hipGetLastError());
+    for (int64_t b0 = 0; b0 < S; b0 += B) {
+        const int64_t n = std::min(B, S - b0);
+        XTS_TRY(model_map_device(ctx, wcs, src_sph + 2 * b0, n, expo + b0 * E,
+                                 upsf + (size_t)b0 * kND * E, parint + (size_t)b0 * kND * E, E,
+                                 rule, N, flags | XTS_LAYOUT_SEHW | XTS_OUT_F32, out, true,
+                                 kPhaseRows | kPhaseTable | kPhaseSum, flux + b0));
+    }
+    return XTS_OK;
+}
+
 int xts_point_src_model_map_wcs_dev_noalloc(xts_ctx* ctx, const xts_wcs* wcs,
                                             const double* src_sph, int64_t S,
                                             const double* expo, const double* upsf,
@@ -2307,7 +2648,7 @@ int xts_point_src_model_map_wcs_dev_noalloc(xts_ctx* ctx, const xts_wcs* wcs,
     XTS_CHECK(rule == nullptr || (N > 0 && N <= 1024), "rule size %lld out of range (1..1024)",
               (long long)N);
     XTS_CHECK((int64_t)wcs->naxis1 * wcs->naxis2 <= (int64_t)1 << 30, "image too large");
-    XTS_CHECK(!(flags & (XTS_SURROGATE | XTS_SURROGATE_NO_NET)),
+    XTS_CHECK(!(flags & (XTS_SURROGATE | XTS_SURROGATE_NO_NET | XTS_SURROGATE_FP8)),
               "xts_point_src_model_map_wcs_dev_noalloc: the surrogate is not offered here");
     void* base;
     size_t cap;

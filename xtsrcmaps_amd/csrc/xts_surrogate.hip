@@ -396,14 +396,44 @@ __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ P,
     mlp_pack(P, norm, out, threadIdx.x, blockDim.x);
 }
 
+// The FP8 record: the hidden layers' weight scales from their largest
+// |w|, then the packing (one block of 256 threads).
+__global__ void __launch_bounds__(256) pack8_kernel(const float* __restrict__ P,
+                                                    const float* __restrict__ norm,
+                                                    MlpPacked8* __restrict__ out) {
+    __shared__ float red[2][256];
+    const int tid = threadIdx.x;
+    float m2 = 0.0f, m3 = 0.0f;
+    for (int k = tid; k < kMlpH * kMlpH; k += 256) {
+        m2 = fmaxf(m2, fabsf(P[kOffW2 + k]));
+        m3 = fmaxf(m3, fabsf(P[kOffW3 + k]));
+    }
+    red[0][tid] = m2;
+    red[1][tid] = m3;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            red[0][tid] = fmaxf(red[0][tid], red[0][tid + o]);
+            red[1][tid] = fmaxf(red[1][tid], red[1][tid + o]);
+        }
+        __syncthreads();
+    }
+    const float sw2 = red[0][0] > 0.0f ? red[0][0] / kFp8Max : 1.0f;
+    const float sw3 = red[1][0] > 0.0f ? red[1][0] / kFp8Max : 1.0f;
+    mlp_pack8(P, norm, sw2, sw3, out, tid, 256);
+}
+
+// Inference of B samples with the BF16 (MlpPacked) or FP8 (MlpPacked8)
+// record, staged in LDS.
+template <typename Rec>
 __global__ void __launch_bounds__(256)
-infer_kernel(const MlpPacked* __restrict__ mp, const float* __restrict__ X, int ldx,
+infer_kernel(const Rec* __restrict__ mp, const float* __restrict__ X, int ldx,
              long long B, float* __restrict__ y) {
-    __shared__ MlpPacked m;
+    __shared__ Rec m;
     {
         const float4* src = reinterpret_cast<const float4*>(mp);
         float4* dst = reinterpret_cast<float4*>(&m);
-        for (int k = threadIdx.x; k < (int)(sizeof(MlpPacked) / 16); k += blockDim.x)
+        for (int k = threadIdx.x; k < (int)(sizeof(Rec) / 16); k += blockDim.x)
             dst[k] = src[k];
     }
     __syncthreads();
@@ -416,7 +446,9 @@ infer_kernel(const MlpPacked* __restrict__ mp, const float* __restrict__ X, int 
         float x[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) x[i] = X[rr * ldx + i];
-        const float out = mlp_infer16(&m, x);
+        float out;
+        if constexpr (sizeof(Rec) == sizeof(MlpPacked8)) out = mlp_infer16_fp8(&m, x);
+        else out = mlp_infer16(&m, x);
         if (lane < 16 && r < B) y[r] = out;
     }
 }
@@ -551,7 +583,23 @@ int xts_mlp_infer_bf16_dev(xts_ctx* ctx, const float* params, const float* norm,
     XTS_TRY(slot_reserve(ctx, kSlotMlpPacked, sizeof(MlpPacked), &mp));
     pack_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked*)mp);
     const unsigned grid = std::min<unsigned>(div_up(B, 64), 4096);
-    infer_kernel<<<grid, 256, 0, ctx->stream>>>((const MlpPacked*)mp, x, (int)ldx, B, y);
+    infer_kernel<MlpPacked><<<grid, 256, 0, ctx->stream>>>((const MlpPacked*)mp, x, (int)ldx, B,
+                                                           y);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
+int xts_mlp_infer_fp8_dev(xts_ctx* ctx, const float* params, const float* norm, const float* x,
+                          int64_t ldx, int64_t B, float* y) {
+    XTS_TRY(activate(ctx));
+    XTS_TRY(check_mlp_args(params, norm, x, B));
+    XTS_CHECK(y && ldx >= kMlpIn, "null output or row stride below 4");
+    void* mp;
+    XTS_TRY(slot_reserve(ctx, kSlotMlpPacked, sizeof(MlpPacked8), &mp));
+    pack8_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked8*)mp);
+    const unsigned grid = std::min<unsigned>(div_up(B, 64), 4096);
+    infer_kernel<MlpPacked8><<<grid, 256, 0, ctx->stream>>>((const MlpPacked8*)mp, x, (int)ldx,
+                                                            B, y);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -563,13 +611,16 @@ int xts_ctx_set_surrogate(xts_ctx* ctx, const float* params, const float* norm) 
         return XTS_OK;
     }
     XTS_CHECK(norm, "null normalisation record");
-    void* mp;
+    void *mp, *mp8;
     XTS_TRY(slot_reserve(ctx, kSlotSurrogate, sizeof(MlpPacked), &mp));
+    XTS_TRY(slot_reserve(ctx, kSlotSurrogate8, sizeof(MlpPacked8), &mp8));
     pack_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked*)mp);
+    pack8_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked8*)mp8);
     XTS_HIP(hipGetLastError());
     // packed before any later model map reads it, whatever stream that uses
     XTS_HIP(hipStreamSynchronize(ctx->stream));
     ctx->surrogate = mp;
+    ctx->surrogate8 = mp8;
     ctx->surrogate_ready = true;
     ++ctx->generation;
     return XTS_OK;

@@ -47,6 +47,16 @@ CONFIGS = {
     # the surrogate workload: the same 2,000 sources on 400x400 at 0.05 deg,
     # 37 energies, the 8x8 Gauss-Legendre rule, FP32 map
     "c3": Config("c3", 2000, 400, 0.05, rule="gl8", seed=3, out_f32=True, src_mode="uniform"),
+    # the large sky: 10,000 sources with fluxes log-uniform over 4 decades,
+    # uniform over a 1000x1000 image at 0.02 deg (20 x 20 deg), 100 energies,
+    # 8x8 Gauss-Legendre rule; output the flux-weighted summed model
+    # [100, 1000, 1000] FP32 (the per-source maps would be 4 TB)
+    "c4": Config("c4", 10000, 1000, 0.02, n_energy=100, rule="gl8", seed=4, out_f32=True,
+                 src_mode="uniform", extra={"flux_decades": 4}),
+    # the large sky's sources and physics on a 160x160 window at 0.02 deg
+    # (12 sources), for oracle comparisons of the summed model
+    "c4w": Config("c4w", 12, 160, 0.02, n_energy=100, rule="gl8", seed=4, out_f32=True,
+                  src_mode="uniform", extra={"flux_decades": 4}),
 }
 
 
@@ -89,6 +99,16 @@ def sources(cfg: Config, wcs: Wcs) -> np.ndarray:
         p = rng.uniform(0.5, cfg.npix + 0.5, (cfg.n_src, 2))
     lon, lat = pix2sph_car(wcs, p[:, 0], p[:, 1])
     return np.stack([lon, lat], axis=1)
+
+
+def fluxes(cfg: Config) -> np.ndarray:
+    """Per-source flux, log-uniform over cfg.extra["flux_decades"] decades
+    (1 for every source without it)."""
+    dec = cfg.extra.get("flux_decades", 0)
+    if not dec:
+        return np.ones(cfg.n_src)
+    rng = np.random.default_rng(cfg.seed + 2000)
+    return 10.0 ** rng.uniform(-dec, 0.0, cfg.n_src)
 
 
 def _padded_grid(e_edges, c_edges, params_raw):
