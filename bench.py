@@ -494,6 +494,7 @@ def main():
         paths = {"cubature": (step, base), "batched cubature": (step_batched, base)}
         if sur:
             paths["surrogate"] = (step, base | xa.SURROGATE)
+            paths["fp8 surrogate"] = (step, base | xa.SURROGATE | xa.SURROGATE_FP8)
             paths["batched surrogate"] = (step_batched, base | xa.SURROGATE)
         evals = cfg.evals
         workload = (f"{cfg.name}: {cfg.n_src} point sources, {cfg.npix}x{cfg.npix} CAR "

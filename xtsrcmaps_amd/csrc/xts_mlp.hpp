@@ -133,11 +133,21 @@ __device__ inline void mlp_pack(const float* __restrict__ p, const float* __rest
     }
 }
 
+typedef float mlp_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned mlp_u32x2 __attribute__((ext_vector_type(2)));
+
+// Four floats to BF16 by the hardware's packed conversion (v_cvt_pk_bf16_f32,
+// round to nearest even, as mlp_bf16).
+__device__ inline mlp_bf16x4 mlp_to_bf16x4(float a, float b, float c, float d) {
+    const mlp_bf16x2 lo = __builtin_convertvector((mlp_f32x2{a, b}), mlp_bf16x2);
+    const mlp_bf16x2 hi = __builtin_convertvector((mlp_f32x2{c, d}), mlp_bf16x2);
+    const mlp_u32x2 u = {__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)};
+    return __builtin_bit_cast(mlp_bf16x4, u);
+}
+
 __device__ inline mlp_bf16x4 mlp_silu_bf16(mlp_f32x4& acc) {
-    mlp_bf16x4 h;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) h[i] = (short)mlp_bf16(silu(acc[i]));
-    return h;
+    return mlp_to_bf16x4(silu(acc[0]), silu(acc[1]), silu(acc[2]), silu(acc[3]));
 }
 
 // Two of the previous layer's 16-output tiles as one 32-wide input chunk.
@@ -152,20 +162,27 @@ __device__ inline mlp_f32x4 mlp_mfma32(const mlp_s16x8& w, const mlp_bf16x8& h,
                                                    0, 0, 0);
 }
 
+// The normalised inputs of layer 1, by lane group g: the high BF16 parts
+// (g = 0, 2), the low parts (g = 1), zero (g = 3).
+__device__ inline mlp_bf16x4 mlp_input_parts(const float* norm, const float (&x)[4], int g) {
+    float xn[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xn[j] = (x[j] - norm[j]) * norm[4 + j];
+    const mlp_bf16x4 hi = mlp_to_bf16x4(xn[0], xn[1], xn[2], xn[3]);
+    if (g != 1) return g == 3 ? mlp_bf16x4{0, 0, 0, 0} : hi;
+    float r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = xn[j] - mlp_from_bf16((uint16_t)hi[j]);
+    return mlp_to_bf16x4(r[0], r[1], r[2], r[3]);
+}
+
 // One wave evaluates 16 samples: lane l carries sample l % 16's raw
 // features x[4] (every lane of the sample the same values); the result, for
 // sample l % 16, is returned in every lane.  m: the record (LDS or global).
 __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float (&x)[4]) {
     const int lane = threadIdx.x & 63, g = lane >> 4;
     // normalised inputs, split into BF16 high and low parts by lane group
-    mlp_bf16x4 in;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float xn = (x[j] - m->norm[j]) * m->norm[4 + j];
-        const uint16_t hi = mlp_bf16(xn);
-        const uint16_t lo = mlp_bf16(xn - mlp_from_bf16(hi));
-        in[j] = (short)(g == 1 ? lo : (g == 3 ? 0 : hi));
-    }
+    mlp_bf16x4 in = mlp_input_parts(m->norm, x, g);
     mlp_bf16x4 H[4];
 #pragma unroll
     for (int To = 0; To < 4; ++To) {
@@ -339,14 +356,7 @@ __device__ inline void mlp_fp8_layer(const mlp_fp8x8 (*w)[64], float sw, const f
 // FP8 inference of 16 samples per wave (as mlp_infer16).
 __device__ inline float mlp_infer16_fp8(const MlpPacked8* __restrict__ m, const float (&x)[4]) {
     const int lane = threadIdx.x & 63, g = lane >> 4;
-    mlp_bf16x4 in;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float xn = (x[j] - m->norm[j]) * m->norm[4 + j];
-        const uint16_t hi = mlp_bf16(xn);
-        const uint16_t lo = mlp_bf16(xn - mlp_from_bf16(hi));
-        in[j] = (short)(g == 1 ? lo : (g == 3 ? 0 : hi));
-    }
+    mlp_bf16x4 in = mlp_input_parts(m->norm, x, g);
     float h[4][4], z[4][4];
 #pragma unroll
     for (int To = 0; To < 4; ++To) {

@@ -32,7 +32,7 @@ COPY_KERNELS = r"MulFunctor"          # torch.mul(src, 1.0, out=dst): reads and 
 
 
 # ---------------------------------------------------------------- workloads
-def _modelmap(config: str, calls: int, surrogate: bool = False) -> None:
+def _modelmap(config: str, calls: int, surrogate: bool = False, fp8: bool = False) -> None:
     import torch
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     sys.path.insert(0, root)
@@ -40,6 +40,8 @@ def _modelmap(config: str, calls: int, surrogate: bool = False) -> None:
     import xtsrcmaps_amd as xa
     cfg, inp, ctx, d = bench.setup(config, surrogate=surrogate)
     flags = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32 | (xa.SURROGATE if surrogate else 0)
+    if surrogate and fp8:
+        flags |= xa.SURROGATE_FP8
     bench.step(ctx, inp, d, flags)
     torch.cuda.synchronize()
     for _ in range(calls):
@@ -223,6 +225,7 @@ def main(argv=None) -> None:
     p.add_argument("--config", default="c1")
     p.add_argument("--calls", type=int, default=3)
     p.add_argument("--surrogate", action="store_true")
+    p.add_argument("--fp8", action="store_true", help="with --surrogate: the FP8 network")
     p = sub.add_parser("stages")
     p.add_argument("--config", default="c3")
     p.add_argument("--calls", type=int, default=5)
@@ -250,7 +253,7 @@ def main(argv=None) -> None:
     p.add_argument("--cmd", required=True)
     a = ap.parse_args(argv)
     if a.what == "modelmap":
-        _modelmap(a.config, a.calls, a.surrogate)
+        _modelmap(a.config, a.calls, a.surrogate, a.fp8)
     elif a.what == "stages":
         _stages(a.config, a.calls, a.precision)
     elif a.what == "copy":
