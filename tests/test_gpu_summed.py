@@ -7,6 +7,8 @@ energies):
   within 1e-5), and with the FP8 surrogate (trained on the device for the
   0.02-degree pixels) within 1e-4;
 - two sources on the full 1000x1000 image against the oracle, per plane;
+- misuse (flags, shapes, dtypes, a missing network, tuning values)
+  refused before any launch; batches of 5 sources equal to one batch;
 - the full 10,000 sources: every value finite and positive; the sums over
   two halves of the source list add up to the whole (FP32 accumulation
   order only, 1e-5); doubling every flux doubles the model (1e-6: the near
@@ -68,6 +70,38 @@ ctx, d = bench.setup_inputs(inp, with_out=False)
 bench.psf_stages(ctx, inp, d)
 torch.cuda.synchronize()
 ft = torch.as_tensor(flux, dtype=torch.float64, device="cuda")
+
+# -- misuse refused before any launch
+def refused(fn):
+    try:
+        fn()
+    except xa.XtsError:
+        return True
+    return False
+E, H, W = len(inp.logE), inp.wcs.naxis1, inp.wcs.naxis2
+out = torch.empty((E, H, W), dtype=torch.float32, device="cuda")
+call = lambda o=out, f=ft, fl=0: ctx.point_src_summed_model_dev(
+    inp.wcs, d["src"], f, d["expo"], d["upsf"], d["parint"], o, inp.rule, flags=fl)
+check("refusals", all([
+    refused(lambda: call(fl=xa.COMPUTE_F64)),                          # FP64 cubature
+    refused(lambda: call(fl=xa.LAYOUT_SEHW)),                          # a per-source layout
+    refused(lambda: call(fl=xa.SURROGATE)),                            # no network set yet
+    refused(lambda: call(fl=xa.SURROGATE_FP8)),                        # FP8 without SURROGATE
+    refused(lambda: call(o=out.double())),                             # output dtype
+    refused(lambda: call(o=out[:, :, :W - 1])),                        # output shape
+    refused(lambda: call(f=ft[:-1].contiguous())),                     # flux length
+    refused(lambda: ctx.set_tuning("sum_group", 3)),
+    refused(lambda: ctx.set_tuning("sum_batch_sources", -1)),
+]), "bad flags, shapes, dtypes, a missing network and bad tuning values raise")
+# batches of any size give the same model (up to the atomics' order)
+ctx.set_tuning("sum_batch_sources", 5)
+small = torch.empty_like(out)
+call(o=small)
+ctx.set_tuning("sum_batch_sources", 0)
+call()
+torch.cuda.synchronize()
+e = float(((small - out).abs() / out).max())
+check("batches", e <= 1e-6, f"batches of 5 sources: max rel {e:.1e} from one batch")
 t0 = time.perf_counter()
 ref = oracle_sum(inp, np.arange(cfg.n_src), flux)
 print(f"window: oracle {time.perf_counter() - t0:.1f} s", flush=True)

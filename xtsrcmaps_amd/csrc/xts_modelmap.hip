@@ -2611,6 +2611,13 @@ int xts_point_src_summed_model_dev(xts_ctx* ctx, const xts_wcs* wcs, const doubl
                          XTS_CF_EXACT_RADIUS)) == 0,
               "xts_point_src_summed_model_dev: flags 0x%x (only XTS_SURROGATE, "
               "XTS_SURROGATE_NO_NET, XTS_SURROGATE_FP8 and XTS_CF_EXACT_RADIUS apply)", flags);
+    // the checks of the per-batch model maps, before anything is launched
+    XTS_CHECK(!(flags & (XTS_SURROGATE_NO_NET | XTS_SURROGATE_FP8)) || (flags & XTS_SURROGATE),
+              "XTS_SURROGATE_NO_NET and XTS_SURROGATE_FP8 need XTS_SURROGATE");
+    XTS_CHECK(!(flags & XTS_SURROGATE_FP8) || !(flags & XTS_SURROGATE_NO_NET),
+              "XTS_SURROGATE_FP8 needs the network");
+    XTS_CHECK(!(flags & XTS_SURROGATE) || (flags & XTS_SURROGATE_NO_NET) || ctx->surrogate_ready,
+              "XTS_SURROGATE: no network set (xts_ctx_set_surrogate)");
     const size_t plane = (size_t)wcs->naxis1 * wcs->naxis2;
     XTS_CHECK(plane <= (size_t)1 << 30, "image too large");
     const bool sur = (flags & XTS_SURROGATE) != 0;
