@@ -491,7 +491,13 @@ def main():
         cfg, inp, ctx, d = setup(args.config, surrogate=sur)
         d["serial"] = args.serial
         setup_batched(ctx, inp, d, [inp.wcs], np.array([0, cfg.n_src], np.int64))
-        paths = {"cubature": (step, base), "batched cubature": (step_batched, base)}
+        # "cubature": the library's default for the map (at this size the
+        # pipelined source batches); "cubature, one pass": every pass over all
+        # sources at once, one after the other (its emit runs alone: the
+        # roofline's measurement)
+        paths = {"cubature": (step, base),
+                 "cubature, one pass": (step, base, {"pipeline_batches": 1}),
+                 "batched cubature": (step_batched, base)}
         if sur:
             paths["surrogate"] = (step, base | xa.SURROGATE)
             paths["fp8 surrogate"] = (step, base | xa.SURROGATE | xa.SURROGATE_FP8)
@@ -502,7 +508,11 @@ def main():
                     f"cubature, FP32 [S,E,H,W] map")
     res, maps = {}, {}
     S = inp.src_sph.shape[0]
-    for name, (fn, flags) in paths.items():
+    for name, spec in paths.items():
+        fn, flags = spec[0], spec[1]
+        tuning = spec[2] if len(spec) > 2 else {}
+        for k, v in tuning.items():
+            ctx.set_tuning(k, v)
         _log(f"bench: {name}: warmup {args.warmup}, timing {args.steps} steps")
         ms = timed(ctx, inp, d, flags, args.steps, args.warmup, args.precision, fn)
         entry = {"ms_per_step": ms, "value": evals / (ms / 1e3)}
@@ -519,8 +529,10 @@ def main():
             step(ctx, inp, d, flags, args.precision)
             torch.cuda.synchronize()
             maps[name] = d["out"][:min(args.cpu_sources, S)].cpu().numpy()
-    sur_paths = [k for k in res if k.endswith("surrogate")]
-    cub_paths = [k for k in res if k.endswith("cubature")]
+        for k in tuning:
+            ctx.set_tuning(k, 0)
+    sur_paths = [k for k in res if "surrogate" in k]
+    cub_paths = [k for k in res if "cubature" in k]
     best = lambda ks: min(ks, key=lambda k: res[k]["ms_per_step"])
     head = best(sur_paths) if sur_paths else best(cub_paths)
     ms = res[head]["ms_per_step"]
@@ -553,7 +565,10 @@ def main():
     if "footprint" in d:
         result["footprint"] = d["footprint"]
     if timed_paths:
-        hp = head if "stages" in res[head] else timed_paths[-1]
+        # the emit's launches measured where they run alone (in the pipelined
+        # map they share the CUs with the next batch's row weights)
+        hp = "cubature, one pass" if "cubature, one pass" in timed_paths else (
+            head if "stages" in res[head] else timed_paths[-1])
         emit = res[hp]["stages"]["emit"]
         achieved = 4.0 * evals / (emit["ms"] / 1e3) / 1e9
         traffic = _profile(f"traffic_{cfg.name}_emit.json") if cfg else None
@@ -564,7 +579,10 @@ def main():
             "kernel": f"emit_sehw_kernel (the model map's store pass) of the {hp} path; "
                       "algorithmic bytes = 4 B per FP32 map value, time = its average launch "
                       "duration from HIP events on the library's stream",
-            "launch_ms": emit["ms"]}
+            "launch_ms": emit["ms"],
+            "step_frac": 4.0 * evals / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "best_cubature_step_frac": 4.0 * evals / (res[best(cub_paths)]["ms_per_step"] / 1e3)
+                                       / 1e9 / HBM_PEAK_GBS}
     if sur:
         n = d["net"]
         result["surrogate"] = {

@@ -111,7 +111,8 @@ int xts_abi_version(void);
 /* Instrumentation (no reference counterpart): when enabled, kernel launches
  * of the named model-map stages ("cubature": the row-weight pass, "near":
  * the near-pixel rows, "factor", "emit", "near_emit": the near and wide
- * pixels' map values) are bracketed by events on the
+ * pixels' map values, "fused": the pipelined map's launches of one batch's
+ * row weights beside the previous batch's map) are bracketed by events on the
  * context stream; xts_ctx_kernel_times synchronises and returns (and
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
@@ -119,8 +120,12 @@ int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
  * "tiles_per_block" (16x16-pixel tiles per block of the row-weight pass;
  * 0 restores the default choice); "geometry_cache" (1, the default: the
  * pixel geometry and solid angles of the last image are kept between model
- * maps of that image; 0: recomputed by every call).  Results do not depend
- * on either. */
+ * maps of that image; 0: recomputed by every call); "pipeline_batches" (the
+ * FP32 [S, E, H, W] model map in this many source batches, each batch's row
+ * weights computed by the same launch as the previous batch's map values;
+ * 1: the passes over all sources at once; 0: the default).  Results do not
+ * depend on any of them.  Setting a key cancels a pending first half of a
+ * split model map. */
 int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value);
 int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms,
                          int64_t* count);

@@ -214,6 +214,11 @@ int xts_ctx_destroy(xts_ctx* ctx) {
     if (ctx->join_ev) hipEventDestroy(ctx->join_ev);
     if (ctx->rows_ev) hipEventDestroy(ctx->rows_ev);
     if (ctx->fin_ev) hipEventDestroy(ctx->fin_ev);
+    if (ctx->pipe_stream) {
+        hipStreamSynchronize(ctx->pipe_stream);
+        hipStreamDestroy(ctx->pipe_stream);
+    }
+    for (auto e : ctx->pipe_ev) hipEventDestroy(e);
     if (ctx->aux) xts_ctx_destroy(ctx->aux);
     hipSetDevice(ctx->device);
     for (auto& s : ctx->slots)
@@ -242,6 +247,37 @@ int xts_ctx_set_stream(xts_ctx* ctx, void* stream) {
 int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
     XTS_CHECK(ctx != nullptr && key != nullptr, "null argument");
     XTS_CHECK(value >= 0 && value <= 1 << 20, "tuning value %lld out of range", (long long)value);
+    // a pending first half of a split model map was shaped by the old
+    // settings: its second half is refused from now on
+    ctx->pending_rows.valid = false;
+    if (strcmp(key, "rowweight_frames") == 0) {
+        ctx->tune_rw_frames = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "rowweight_legacy") == 0) {
+        XTS_CHECK(value == 0 || value == 1, "rowweight_legacy is 0 or 1");
+        ctx->tune_rw_legacy = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "emit_energies") == 0) {
+        XTS_CHECK(value == 0 || value == 20 || value == 40, "emit_energies is 20 or 40");
+        ctx->tune_emit_ec = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "emit_priority") == 0) {
+        XTS_CHECK(value <= 3, "emit_priority is 0 .. 3");
+        ctx->tune_emit_prio = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "rowweight_frames_cu") == 0) {
+        XTS_CHECK(value <= 16, "rowweight_frames_cu at most 16");
+        ctx->tune_rw_frames_cu = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "pipeline_batches") == 0) {
+        ctx->tune_pipe_batches = value;
+        return XTS_OK;
+    }
     if (strcmp(key, "geometry_cache") == 0) {
         XTS_CHECK(value == 0 || value == 1, "geometry_cache is 0 or 1");
         ctx->geom_cache = value != 0;

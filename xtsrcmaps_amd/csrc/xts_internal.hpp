@@ -139,6 +139,16 @@ struct xts_ctx {
     int64_t tune_batch_sources = 0;        // >0: sources per batch of the batched engine
     int64_t tune_sum_batch = 0;            // >0: sources per batch of a summed model
     int64_t tune_sum_group = 4;            // sources per staging round of the summed model (4, 8)
+    int64_t tune_pipe_batches = 0;         // source batches of the pipelined map (0 default, 1 off)
+    int64_t tune_rw_frames = 0;            // >0: row weights by tile frames, this many sources a block
+    int64_t tune_rw_frames_cu = 0;         // >0: resident tile-frame blocks per CU
+    int64_t tune_emit_prio = 0;            // wave priority of the pipelined map's emit (0-3)
+    int64_t tune_emit_ec = 0;              // energies per block of the pipelined map's emit (20, 40)
+    int64_t tune_rw_legacy = 0;            // 1: FP32 row weights by per-source tile blocks
+    // the pipelined map's second stream (row weights of the next batch) and
+    // its events: [0, B) row weights done, [B, 2B) batch emitted, 2B fork
+    hipStream_t pipe_stream = nullptr;
+    std::vector<hipEvent_t> pipe_ev;
     int64_t last_batch_sources = 0, last_rw_blocks = 0, last_batches = 0;  // last batched call
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;

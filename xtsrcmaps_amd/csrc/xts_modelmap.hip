@@ -198,7 +198,8 @@ struct CubArgs {
     int S, H, W, E, Ep, N;
     int tiles_w, tiles_h, tiles_per_block;
     int rw_persist;            // row-weight pass on a 1-D grid of resident blocks
-    int emit_gx;               // emit blocks per source (x); the next kWideBlocks: wide pixels
+    int emit_gx;               // emit blocks per source (x)
+    int emit_prio;             // wave priority of the emit's waves (s_setprio), 0: default
     int layout_sehw, out_f32;
     int tensor_nu, tensor_nv;  // rule is an nu x nv grid (p = i nv + j), else 0
     double rho_scale;          // max over points of sum_k |b_k|
@@ -570,6 +571,7 @@ __device__ inline LeanPixel lean_pixel(const PixelGeom<float>& pg, int r0) {
 using cfloat = const __attribute__((address_space(4))) float;
 struct LeanRule {
     cfloat* p[3];   // u, v, wq
+    const double* isep64;   // [401] 2 R2D / sep (global, or a copy in LDS)
 };
 constexpr int kLeanArrays = 3;
 
@@ -624,7 +626,7 @@ constexpr double kAsinSeries[9] = {1.0, 1.0 / 6, 3.0 / 40, 5.0 / 112, 35.0 / 115
                                    231.0 / 13312, 143.0 / 10240, 6435.0 / 557056};
 
 template <int D>
-__device__ inline PolyPixel poly_pixel(const CubArgs& a, const PixelGeom<float>& pg, int r0) {
+__device__ inline PolyPixel poly_pixel(const double* isep64, const PixelGeom<float>& pg, int r0) {
     const float* A = pg.A; const float* B = pg.B; const float* C = pg.C; const float* E = pg.E;
     auto dot = [](const float* x, const float* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
     const float yc = dot(A, A);
@@ -651,7 +653,7 @@ __device__ inline PolyPixel poly_pixel(const CubArgs& a, const PixelGeom<float>&
 #pragma unroll
     for (int k = 7; k >= 0; --k) G = fma(G, ye, kAsinSeries[k]);
     // inv_sep64[k] = 2 R2D / sep[k]
-    const double em = fma(sq * G, a.inv_sep64[r0 + 1], -1.0);
+    const double em = fma(sq * G, isep64[r0 + 1], -1.0);
     pp.c[0] = (float)(em * kInvD0);
     // exp(g(y_e (1 + x)) - g(y_e)) = 1 + e1 x + e2 x^2 + e3 x^3 + ..., times
     // K = (s_e / s1) / (1 - e^-step)
@@ -713,7 +715,7 @@ template <int D, int NM>
 __device__ inline void poly_tensor_loop(const CubArgs& a, const LeanRule& lr,
                                         const PixelGeom<float>& pg, int r0, float wtot,
                                         float (&W)[4]) {
-    const PolyPixel pp = poly_pixel<D>(a, pg, r0);
+    const PolyPixel pp = poly_pixel<D>(lr.isep64, pg, r0);
     const int nv = a.tensor_nv;
     float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
     for (int i = 0; i < a.tensor_nu; ++i) {
@@ -738,7 +740,7 @@ template <int D, int NM>
 __device__ inline void poly_list_loop(const CubArgs& a, int N4, const LeanRule& lr,
                                       const PixelGeom<float>& pg, int r0, float wtot,
                                       float (&W)[4]) {
-    const PolyPixel pp = poly_pixel<D>(a, pg, r0);
+    const PolyPixel pp = poly_pixel<D>(lr.isep64, pg, r0);
     float A = 0.0f, B1 = 0.0f, B2 = 0.0f;
     for (int p0 = 0; p0 < N4; p0 += 4) {
         const float4 u = lean_load4(lr.p[0] + p0), v = lean_load4(lr.p[1] + p0),
@@ -782,9 +784,9 @@ __device__ inline bool poly_dispatch(const CubArgs& a, const LeanRule& lr,
 // pixel's (u, v) square, and A from the 2x2 Gauss-Legendre rule of sigma
 // (times the rule's weight total) in place of the point loop.
 template <int D>
-__device__ inline void sur_pixel(const CubArgs& a, const PixelGeom<float>& pg, int r0,
+__device__ inline void sur_pixel(const double* isep64, const PixelGeom<float>& pg, int r0,
                                  float wsum, float (&f)[4], float& A) {
-    const PolyPixel pp = poly_pixel<D>(a, pg, r0);
+    const PolyPixel pp = poly_pixel<D>(isep64, pg, r0);
     auto sig = [&](float u, float v) {
         const float xu = fmaf(fmaf(pp.x2, u, pp.x1), u, pp.x0);
         const float bu = fmaf(fmaf(pp.b2, u, pp.b1), u, pp.b0);
@@ -807,10 +809,10 @@ __device__ inline void sur_pixel(const CubArgs& a, const PixelGeom<float>& pg, i
 
 // Nine series evaluations per pixel: the highest degree for every wave (it
 // reaches every one-node pixel) keeps a single instance of the set-up.
-__device__ inline bool sur_dispatch(const CubArgs& a, const PixelGeom<float>& pg, int r0,
+__device__ inline bool sur_dispatch(const double* isep64, const PixelGeom<float>& pg, int r0,
                                     float wsum, float rx, float (&f)[4], float& A) {
     if (!__all(rx <= kPolyReach[kPolyMaxD - 2])) return false;
-    sur_pixel<kPolyMaxD>(a, pg, r0, wsum, f, A);
+    sur_pixel<kPolyMaxD>(isep64, pg, r0, wsum, f, A);
     return true;
 }
 
@@ -840,19 +842,22 @@ __device__ inline void rows_from_sums(T wsum, T S0, T T0, T S01, T T01, T Tt, T 
 // wave lies below 28 degrees, the wave runs a branch-free FP32 loop over the
 // points in groups of four (grid-structured rules reuse the blend of each
 // rule row).  Occupancy: 6 waves per SIMD for FP32 (80 VGPRs), 5 for FP64.
+//
+// rowweight_block is one block's share: lin is the block's index on the
+// (tile runs, sources) grid, or with PERSIST its index among nblk resident
+// blocks; smem holds the dynamic LDS described above.
 template <typename T, bool SUR = false, bool PERSIST = false>
-__global__ void __launch_bounds__(kThreads)
-    __attribute__((amdgpu_waves_per_eu(PERSIST ? 4 : (sizeof(T) == 4 ? 6 : 5))))
-rowweight_kernel(CubArgs a) {
+__device__ inline void rowweight_block(const CubArgs& a, long long lin, long long nblk,
+                                       unsigned char* smem) {
     using V4 = typename Vec4<T>::type;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* hist_s = reinterpret_cast<double*>(smem_raw);
+    double* hist_s = reinterpret_cast<double*>(smem);
     T* isep_s = reinterpret_cast<T*>(hist_s + kNDPad);
     V4* rule_s = reinterpret_cast<V4*>(isep_s + kNDPad);
     const int N4 = (a.N + 3) & ~3;
     LeanRule lr;
 #pragma unroll
     for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = (cfloat*)(a.lean + j * N4);
+    lr.isep64 = a.inv_sep64;
     __shared__ int sur_cnt_s[kThreads / 64];
 
     const int tid = threadIdx.x;
@@ -866,14 +871,13 @@ rowweight_kernel(CubArgs a) {
     int sx = 0;   // the source of a non-persistent block
     if (PERSIST) {
         const long long n = (long long)a.S * ntiles;
-        i0 = n * blockIdx.x / gridDim.x;
-        i1 = n * (blockIdx.x + 1) / gridDim.x;
+        i0 = n * lin / nblk;
+        i1 = n * (lin + 1) / nblk;
     } else {
         // tile-major order of the (tile runs, sources) grid: consecutive
         // blocks take one tile run for consecutive sources, so the blocks in
         // flight share that run's pixel geometry in L2 (the whole image's,
         // 9 MB in FP64 at 400 x 400, would stream through it per source)
-        const long long lin = blockIdx.x + (long long)gridDim.x * blockIdx.y;
         sx = (int)(lin % a.S);
         const int tb = (int)(lin / a.S) * a.tiles_per_block;
         if (tb >= ntiles) return;
@@ -942,7 +946,7 @@ rowweight_kernel(CubArgs a) {
                     // surrogate: A from four points; pixels whose bounds hold
                     // no node have B1 = |A| exactly, the others go to the
                     // network
-                    series = sur_dispatch(a, pg, r0, wsum, rx, sf, sA);
+                    series = sur_dispatch(lr.isep64, pg, r0, wsum, rx, sf, sA);
                     if (series) {
                         listed = fast && ihi == ilo + 1;
                         poly_rows<1>(wsum, sA, fabsf(sA), 0.0f, W);
@@ -1064,6 +1068,228 @@ rowweight_kernel(CubArgs a) {
                 if (PERSIST) hist_s[k] = 0.0;
             }
         if (PERSIST) __syncthreads();
+    }
+  }
+}
+
+template <typename T, bool SUR = false, bool PERSIST = false>
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(PERSIST ? 4 : (sizeof(T) == 4 ? 6 : 5))))
+rowweight_kernel(CubArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    if (PERSIST)
+        rowweight_block<T, SUR, true>(a, blockIdx.x, gridDim.x, smem_raw);
+    else
+        rowweight_block<T, SUR, false>(a, blockIdx.x + (long long)gridDim.x * blockIdx.y, 0,
+                                       smem_raw);
+}
+
+
+// Pass 1, a tile across many sources (FP32 cubature): a block takes one
+// 16 x 16 tile and a range of sources.  The tile's source-independent pixel
+// frame is formed once into LDS (half the corners' mean m, so that the
+// blend's constant term is A = m - src / 2; the blend vectors B, C, E and the
+// corners' spread hr0 of pixel_geom; the centre for the radius test; the
+// solid angle), and so are the tables of 2 R2D / sep: the loop over the
+// sources then reads no global memory but each source's SrcGeom (scalar
+// loads, the next one's issued before the current one's work).  Beside a
+// store-bound pass a load waits many microseconds; this pass issues none per
+// source and keeps its compute busy.  The results equal rowweight_kernel's
+// up to FP32 rounding (A, B, C, E are formed in FP64 from the corners and
+// rounded once).  Items: (tile, run of sources), tile-major; block b takes
+// items b, b + grid, ... (one per block, or a grid of a few resident blocks
+// per CU that leaves room beside them for the store-bound emit).
+struct PixFrame {
+    double m[3];     // half the mean of the corners
+    double c[3];     // centre
+    float B[3], C[3], E[3];
+    float hr0;       // max over corners |c_k / 2 - m|
+    float om;        // solid angle (1 without)
+};
+
+__device__ inline bool in_radius_dir(const SrcGeom& g, const double* c) {
+    const double ex = g.dir[0] - c[0], ey = g.dir[1] - c[1], ez = g.dir[2] - c[2];
+    const double c2 = ex * ex + ey * ey + ez * ez;
+    if (c2 < g.chord2_lo) return true;
+    if (c2 > g.chord2_hi) return false;
+    return in_radius_exact(c2, g.radius_deg);
+}
+
+// A source's geometry by scalar loads (the constant address space; written
+// by an earlier launch, so the scalar cache holds no stale copy).
+__device__ inline SrcGeom load_geo_scalar(const SrcGeom* p) {
+    using cdouble = const __attribute__((address_space(4))) double;
+    using cint = const __attribute__((address_space(4))) int;
+    cdouble* d = (cdouble*)p;
+    SrcGeom g;
+    g.dir[0] = d[0];
+    g.dir[1] = d[1];
+    g.dir[2] = d[2];
+    g.radius_deg = d[3];
+    g.chord2_lo = d[4];
+    g.chord2_hi = d[5];
+    g.in_fov = ((cint*)p)[12];
+    g.pad = 0;
+    return g;
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+rowweight_frames_kernel(CubArgs a, int nsplit) {
+    __shared__ PixFrame fr_s[kThreads];
+    __shared__ double isep_s[kNDPad];
+    __shared__ double hist_s[kNDPad];
+    const int tid = threadIdx.x;
+    const int ntiles = a.tiles_w * a.tiles_h;
+    const long long nitem = (long long)ntiles * nsplit;
+    if ((long long)blockIdx.x >= nitem) return;
+    for (int r = tid; r < kNDPad; r += kThreads) {
+        isep_s[r] = r < kND ? a.inv_sep64[r] : 0.0;
+        hist_s[r] = 0.0;
+    }
+    const int N4 = (a.N + 3) & ~3;
+    LeanRule lr;
+#pragma unroll
+    for (int j = 0; j < kLeanArrays; ++j) lr.p[j] = (cfloat*)(a.lean + j * N4);
+    lr.isep64 = isep_s;
+    const float wsum = (float)a.wsum;
+    const float rho = (float)a.rho_scale;
+    const size_t plane = (size_t)a.H * a.W;
+    const int Wp1 = a.W + 1;
+    const int lx = tid % kTileW, ly = tid / kTileW;
+    __syncthreads();
+
+  // items (tile item / nsplit, sources split nsplit ways) strided over the
+  // grid: one item per block, or a few resident blocks per CU
+  for (long long item = blockIdx.x; item < nitem; item += gridDim.x) {
+    const int t = (int)(item / nsplit), ks = (int)(item % nsplit);
+    const int s0 = (int)((long long)a.S * ks / nsplit);
+    const int s1 = (int)((long long)a.S * (ks + 1) / nsplit);
+    if (s0 >= s1) continue;
+    const int th = t / a.tiles_w, tw = t % a.tiles_w;
+    const int h = th * kTileH + ly, w = tw * kTileW + lx;
+    const bool valid = h < a.H && w < a.W;
+    // out-of-image lanes take an in-image pixel and store nothing
+    const int hc = min(h, a.H - 1), wc = min(w, a.W - 1);
+    const int pix = h * a.W + w;
+    {
+        // this thread's frame (only this thread reads its slot)
+        const int c00 = hc * Wp1 + wc;
+        const int cidx[4] = {c00, c00 + 1, c00 + Wp1 + 1, c00 + Wp1};  // LL LR UR UL
+        double cc[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cc[k][c] = a.corners[3 * cidx[k] + c];
+        PixFrame f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            f.m[c] = 0.125 * ((cc[0][c] + cc[1][c]) + (cc[2][c] + cc[3][c]));
+            f.B[c] = (float)(0.125 * ((cc[1][c] - cc[0][c]) + (cc[2][c] - cc[3][c])));
+            f.C[c] = (float)(0.125 * ((cc[3][c] - cc[0][c]) + (cc[2][c] - cc[1][c])));
+            f.E[c] = (float)(0.125 * ((cc[0][c] - cc[1][c]) + (cc[2][c] - cc[3][c])));
+        }
+        double hr = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double ex = 0.5 * cc[k][0] - f.m[0], ey = 0.5 * cc[k][1] - f.m[1],
+                         ez = 0.5 * cc[k][2] - f.m[2];
+            hr = fmax(hr, sqrt(ex * ex + ey * ey + ez * ez));
+        }
+        f.hr0 = (float)hr;
+        const double* ce = a.centres + 3 * (size_t)c00;
+        f.c[0] = ce[0];
+        f.c[1] = ce[1];
+        f.c[2] = ce[2];
+        f.om = a.omega ? (float)a.omega[hc * a.W + wc] : 1.0f;
+        fr_s[tid] = f;
+    }
+
+    // the sources' geometry by scalar loads (constant address space): vector
+    // loads would wait behind the previous source's stores
+    SrcGeom gn = load_geo_scalar(a.geo + s0);
+    for (int s = s0; s < s1; ++s) {
+        const SrcGeom g = gn;
+        if (s + 1 < s1) gn = load_geo_scalar(a.geo + s + 1);
+        const bool do_hist = a.hist && g.in_fov;
+        const PixFrame f = fr_s[tid];
+        PixelGeom<float> pg;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pg.A[c] = (float)(f.m[c] - 0.5 * g.dir[c]);
+            pg.B[c] = f.B[c];
+            pg.C[c] = f.C[c];
+            pg.E[c] = f.E[c];
+        }
+        const float x0 = fast_sqrt(pg.A[0] * pg.A[0] + pg.A[1] * pg.A[1] + pg.A[2] * pg.A[2]);
+        const float hr = f.hr0 * rho * 1.0001f + 1e-5f * x0;
+        pg.x0 = x0;
+        pg.hr = hr;
+        pg.slo = float(2.0 * kR2D) * asin_pos(fmaxf(x0 - hr, 0.0f));
+        pg.shi = float(2.0 * kR2D) * asin_pos(fminf(x0 + hr, 1.0f));
+        const int ilo = sep_index_fast(pg.slo), ihi = sep_index_fast(pg.shi);
+        const bool fast = (ihi - ilo) <= 2 && pg.slo >= 1e-4f && pg.shi < 69.9f;
+        const int r0 = min(max(ilo - 1, 0), kND - 4);
+        float W[4] = {0.f, 0.f, 0.f, 0.f};
+        // lanes that are not fast are listed and their sums discarded (see
+        // rowweight_block)
+        const bool lean6 = __any(fast) && __all(!fast || pg.shi < 28.0f);
+        bool series = false;
+        if (lean6) {
+            const bool one = __all(!fast || ihi - ilo <= 1);
+            const float rx = fast ? poly_reach(pg) : 0.0f;
+            series = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
+                         : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
+        }
+        if (!series && __any(fast)) {
+            float S0 = 0.f, T0 = 0.f, S01 = 0.f, T01 = 0.f, Tt = 0.f;
+            for (int p = 0; p < a.N; ++p) {
+                const float u = lr.p[0][p], v = lr.p[1][p], wq = lr.p[2][p];
+                const float dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+                const float dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+                const float dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+                const float x = fast_sqrt(dx * dx + dy * dy + dz * dz);
+                const float sd = float(2.0 * kR2D) * asin_pos(fminf(x, 1.0f));
+                const int i = sep_index_fast(sd);
+                // (float)isep_s[k] is the FP32 table's entry
+                const float tt =
+                    float(kRedm1) * (sd * ((float)isep_s[i - 1] * float(0.5 / kR2D)) - 1.0f);
+                const int rel = min(max(i - 1 - r0, 0), 2);
+                const float wt = wq * tt;
+                S0 += rel == 0 ? wq : 0.f;
+                T0 += rel == 0 ? wt : 0.f;
+                S01 += rel <= 1 ? wq : 0.f;
+                T01 += rel <= 1 ? wt : 0.f;
+                Tt += wt;
+            }
+            rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
+        }
+        if (valid) {
+            if (fast) {
+                const float4 wv = float4{f.om * W[0], f.om * W[1], f.om * W[2], f.om * W[3]};
+                reinterpret_cast<float4*>(a.rw)[(size_t)s * plane + pix] = wv;
+                a.rw_row[(size_t)s * plane + pix] = r0;
+                if (do_hist && in_radius_dir(g, f.c)) {
+                    atomicAdd(&hist_s[r0], (double)wv.x);
+                    atomicAdd(&hist_s[r0 + 1], (double)wv.y);
+                    atomicAdd(&hist_s[r0 + 2], (double)wv.z);
+                    atomicAdd(&hist_s[r0 + 3], (double)wv.w);
+                }
+            } else {
+                a.rw_row[(size_t)s * plane + pix] = kRowListed;
+                a.near_list[(size_t)s * plane + atomicAdd(&a.near_count[s], 1)] = pix;
+            }
+        }
+        if (do_hist) {
+            __syncthreads();
+            for (int r = tid; r < kND; r += kThreads) {
+                const double v = hist_s[r];
+                if (v != 0.0) {
+                    atomicAdd(&a.hist[(size_t)s * kND + r], v);
+                    hist_s[r] = 0.0;
+                }
+            }
+            __syncthreads();
+        }
     }
   }
 }
@@ -1236,6 +1462,28 @@ near_rows_kernel(CubArgs a) {
                     myh[(l + 1) * kNearThreads] += (T)wt;
                 }
             }
+        } else if (!narrow) {
+            // a wide pixel's in-radius weights, point by point in FP64 (its
+            // points' offsets from the source cancel; see wide_pixels)
+            if (inrad) {
+                const double omd = a.omega ? a.omega[pix] : 1.0;
+                const PixelGeom<double> pd =
+                    pixel_geom<double>(a.corners, h, w, Wp1, g.dir, a.rho_scale);
+                for (int p = part; p < a.N; p += kNearSplit) {
+                    const RuleUVW q = a.uvw[p];
+                    const double u = q.u, v = q.v;
+                    const double dx = pd.A[0] + u * pd.B[0] + v * (pd.C[0] + u * pd.E[0]);
+                    const double dy = pd.A[1] + u * pd.B[1] + v * (pd.C[1] + u * pd.E[1]);
+                    const double dz = pd.A[2] + u * pd.B[2] + v * (pd.C[2] + u * pd.E[2]);
+                    const double sd = 2.0 * kR2D * asin(fmin(sqrt(dx * dx + dy * dy + dz * dz),
+                                                             1.0));
+                    const int i = sep_index_fast(sd);
+                    const double tt =
+                        kRedm1 * (sd * (a.inv_sep64[i - 1] * (0.5 / kR2D)) - 1.0);
+                    atomicAdd(&hblk[i - 1], omd * (q.wq * (1.0 - tt)));
+                    atomicAdd(&hblk[i], omd * (q.wq * tt));
+                }
+            }
         } else {
             for (int p = part; p < a.N; p += kNearSplit) {
                 const V4 rr = rule_s[p];
@@ -1248,14 +1496,9 @@ near_rows_kernel(CubArgs a) {
                 const int i = sep_index_fast(sd);
                 const T tt = T(kRedm1) * (sd * (isep_s[i - 1] * T(0.5 / kR2D)) - T(1));
                 const T wa = wq * (T(1) - tt), wb = wq * tt;
-                if (narrow) {
-                    const int l = min(max(i - 1 - rbase, 0), R - 2);
-                    myh[l * kNearThreads] += wa;
-                    myh[(l + 1) * kNearThreads] += wb;
-                } else if (inrad) {
-                    atomicAdd(&hblk[i - 1], (double)(om * wa));
-                    atomicAdd(&hblk[i], (double)(om * wb));
-                }
+                const int l = min(max(i - 1 - rbase, 0), R - 2);
+                myh[l * kNearThreads] += wa;
+                myh[(l + 1) * kNearThreads] += wb;
             }
         }
         if (narrow) {
@@ -1297,13 +1540,11 @@ near_rows_kernel(CubArgs a) {
 // vectors.  x solid angle, x exposure x correction when SCALED.
 constexpr int kWideBlocks = 2;   // extra emit blocks per source and energy chunk
 template <typename T, int EC, typename OT, bool SCALED, bool SUM = false>
-__device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
+__device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H, int s, int e0) {
     using V = typename Vec<T>::type;
     constexpr int VN = Vec<T>::n;
     constexpr int NV = EC / VN;
-    const int s = blockIdx.y;
     const int lane = threadIdx.x % 64;
-    const int e0 = blockIdx.z * EC;
     const int ne = min(EC, a.E - e0);
     const SrcGeom g = a.geo[s];
     const size_t plane = (size_t)a.H * a.W;
@@ -1314,22 +1555,27 @@ __device__ inline void wide_pixels(const CubArgs& a, int gw, int nwaves, T* H) {
     for (int k = gw; k < nw; k += nwaves) {
         const int pix = list[a.wide_list[(size_t)s * plane + k]];
         const int h = pix / a.W, w = pix % a.W;
-        const PixelGeom<T> pg = pixel_geom<T>(a.corners, h, w, a.W + 1, g.dir, a.rho_scale);
+        // The points' offsets from the source in FP64 whatever T: a wide
+        // pixel holds (or touches) the source, and its points' offsets cancel
+        // the pixel-sized blend terms to a small fraction of them.  FP32
+        // would leave errors of 1e-4 in t that a table varying fast from row
+        // to row turns into errors of 1e-4 in the value.
+        const PixelGeom<double> pg =
+            pixel_geom<double>(a.corners, h, w, a.W + 1, g.dir, a.rho_scale);
         const T om = a.omega ? (T)a.omega[pix] : T(1);
         for (int r = lane; r < kND; r += 64) H[r] = T(0);
         for (int p = lane; p < a.N; p += 64) {
             const RuleUVW q = a.uvw[p];
-            const T u = (T)q.u, v = (T)q.v, wq = (T)q.wq;
-            const T dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
-            const T dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
-            const T dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
-            const T x = fast_sqrt(dx * dx + dy * dy + dz * dz);
-            const T sd = T(2.0 * kR2D) * asin_pos(fmin(x, T(1)));
+            const double u = q.u, v = q.v;
+            const double dx = pg.A[0] + u * pg.B[0] + v * (pg.C[0] + u * pg.E[0]);
+            const double dy = pg.A[1] + u * pg.B[1] + v * (pg.C[1] + u * pg.E[1]);
+            const double dz = pg.A[2] + u * pg.B[2] + v * (pg.C[2] + u * pg.E[2]);
+            const double x = sqrt(dx * dx + dy * dy + dz * dz);
+            const double sd = 2.0 * kR2D * asin(fmin(x, 1.0));
             const int i = sep_index_fast(sd);
-            const T isep = sizeof(T) == 4 ? (T)a.inv_sep[i - 1] : (T)a.inv_sep64[i - 1];
-            const T tt = T(kRedm1) * (sd * (isep * T(0.5 / kR2D)) - T(1));
-            atomicAdd(&H[i - 1], wq * (T(1) - tt));
-            atomicAdd(&H[i], wq * tt);
+            const double tt = kRedm1 * (sd * (a.inv_sep64[i - 1] * (0.5 / kR2D)) - 1.0);
+            atomicAdd(&H[i - 1], (T)(q.wq * (1.0 - tt)));
+            atomicAdd(&H[i], (T)(q.wq * tt));
         }
         V acc[NV];
 #pragma unroll
@@ -1461,7 +1707,8 @@ near_emit_kernel(CubArgs a) {
 
 // Pass 3: per (source, pixel range, energy chunk): map = sum_k W_k L[r0+k]
 // for pass-1 pixels, x exposure x correction when SCALED.  Listed pixels are
-// left to near_emit_kernel.  OT: output type.  Each thread first loads the
+// left to near_emit_kernel, wide ones to wide_emit_kernel (a launch of its
+// own: its registers would limit the emit's residency beside other work).  OT: output type.  Each thread first loads the
 // row weights of its kEmitPix pixels and then only stores: a load issued
 // after stores would wait for them to drain (the vector memory counter
 // counts both).
@@ -1481,13 +1728,11 @@ emit_sehw_kernel(CubArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     __shared__ ChunkScale<EC> sc;
-    if ((int)blockIdx.x >= a.emit_gx) {
-        constexpr int kWaves = kEmitThreads / 64;
-        wide_pixels<T, EC, OT, SCALED>(a, (blockIdx.x - a.emit_gx) * kWaves + threadIdx.x / 64,
-                                       kWideBlocks * kWaves, lut_s + (threadIdx.x / 64) * kNDPad);
-        return;
-    }
 
+    // beside a compute-bound pass, the stores' waves may take issue first
+    if (a.emit_prio == 1) __builtin_amdgcn_s_setprio(1);
+    else if (a.emit_prio == 2) __builtin_amdgcn_s_setprio(2);
+    else if (a.emit_prio >= 3) __builtin_amdgcn_s_setprio(3);
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
     const int ne = min(EC, a.E - e0);
@@ -1561,12 +1806,6 @@ emit_shwe_kernel(CubArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     __shared__ ChunkScale<EC> sc;
-    if ((int)blockIdx.x >= a.emit_gx) {
-        constexpr int kWaves = kEmitThreads / 64;
-        wide_pixels<T, EC, OT, SCALED>(a, (blockIdx.x - a.emit_gx) * kWaves + threadIdx.x / 64,
-                                       kWideBlocks * kWaves, lut_s + (threadIdx.x / 64) * kNDPad);
-        return;
-    }
 
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
@@ -1858,9 +2097,24 @@ wide_sum_kernel(CubArgs a) {
     constexpr int kWaves = kEmitThreads / 64;
     wide_pixels<float, EC, float, true, true>(a, blockIdx.x * kWaves + threadIdx.x / 64,
                                               kWideBlocks * kWaves,
-                                              H + (threadIdx.x / 64) * kNDPad);
+                                              H + (threadIdx.x / 64) * kNDPad, blockIdx.y,
+                                              blockIdx.z * EC);
 }
 
+// The wide pixels of every source: kWideBlocks blocks per source and energy
+// chunk (grid (kWideBlocks, S, chunks)), one wave per pixel (wide_pixels).
+template <typename T, int EC, typename OT, bool SCALED>
+__global__ void __launch_bounds__(kEmitThreads) wide_emit_kernel(CubArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* H = reinterpret_cast<T*>(smem_raw);
+    constexpr int kWaves = kEmitThreads / 64;
+    wide_pixels<T, EC, OT, SCALED>(a, blockIdx.x * kWaves + threadIdx.x / 64,
+                                   kWideBlocks * kWaves, H + (threadIdx.x / 64) * kNDPad,
+                                   blockIdx.y, blockIdx.z * EC);
+}
+
+// grid: (emit blocks per source, S, energy chunks); the wide pixels' launch
+// follows on the same stream
 template <typename T, int EC, typename OT, bool SCALED>
 static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) {
     const size_t sm = (size_t)kND * (sehw ? EC + 1 : EC) * sizeof(T);
@@ -1869,6 +2123,11 @@ static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) 
     XTS_TRY(allow_lds(fn, sm));
     if (sehw) emit_sehw_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, st>>>(ca);
     else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, st>>>(ca);
+    XTS_HIP(hipGetLastError());
+    const size_t smw = (size_t)(kEmitThreads / 64) * kNDPad * sizeof(T);
+    XTS_TRY(allow_lds((const void*)wide_emit_kernel<T, EC, OT, SCALED>, smw));
+    wide_emit_kernel<T, EC, OT, SCALED><<<dim3(kWideBlocks, grid.y, grid.z), kEmitThreads, smw,
+                                          st>>>(ca);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -2240,10 +2499,188 @@ enum { kPhaseRows = 1, kPhaseTable = 2, kPhaseEmit = 4, kPhaseFinish = 6, kPhase
 // accumulator slot.
 static size_t model_map_counter_doubles(int64_t S) { return (size_t)S; }
 
+// Source batches of the pipelined map for this call, or 1 (the passes over
+// all sources at once): the FP32 cubature into an FP32 [S, E, H, W] map,
+// full model map, no surrogate, not from a caller's arena; tuning key
+// "pipeline_batches" (0: the default, 1: off, n: n batches).  By default a
+// map of at least kPipeMinValues values runs in kPipeDefaultBatches
+// batches (2,000 sources x 37 x 400 x 400: 15.1 ms against 16.4 for the
+// same kernels one after the other, on one MI355X).
+constexpr int64_t kPipeDefaultBatches = 8;
+constexpr int64_t kPipeMinSources = 16;   // sources per batch at least
+constexpr double kPipeMinValues = 4e9;
+static int64_t pipeline_batches(const xts_ctx* ctx, int64_t S, int64_t plane, int64_t E,
+                                int32_t flags, bool full, int phase, int sur) {
+    const bool ok = full && !sur && !(flags & XTS_COMPUTE_F64) && (flags & XTS_OUT_F32) &&
+                    (flags & XTS_LAYOUT_SEHW) && !(phase & kPhaseSum) &&
+                    ctx->arena.base == nullptr && phase != kPhaseEmit;
+    if (!ok) return 1;
+    int64_t B = ctx->tune_pipe_batches;
+    if (B == 0) B = (double)S * plane * E >= kPipeMinValues ? kPipeDefaultBatches : 1;
+    B = std::min<int64_t>(B, S / kPipeMinSources);
+    return B > 1 ? B : 1;
+}
+
+// Sources per block of the tile-frame row-weight pass: at most 64, fewer
+// when that leaves under about eight blocks per CU.
+static int64_t rw_frames_per_block(const xts_ctx* ctx, int64_t S, int64_t ntiles) {
+    if (ctx->tune_rw_frames > 0) return ctx->tune_rw_frames;
+    const int64_t fill = std::max<int64_t>(1, S * ntiles / ((int64_t)ctx->cu_count * 8));
+    return std::max<int64_t>(4, std::min<int64_t>(64, fill));
+}
+
+// Launches the tile-frame row-weight pass over the sources of c on stream
+// st: items of one tile and a run of about `per_block` sources, one block
+// per item, or per_cu resident blocks per CU striding over the items.
+static int launch_rw_frames(xts_ctx* ctx, const CubArgs& c, int64_t per_block, int64_t per_cu,
+                            hipStream_t st) {
+    const int64_t ntiles = (int64_t)c.tiles_w * c.tiles_h;
+    const int64_t nsplit = std::max<int64_t>(1, (c.S + per_block - 1) / per_block);
+    const int64_t nitem = ntiles * nsplit;
+    const int64_t grid = per_cu > 0 ? std::min<int64_t>(nitem, per_cu * ctx->cu_count) : nitem;
+    rowweight_frames_kernel<<<(unsigned)grid, kThreads, 0, st>>>(c, (int)nsplit);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
+// The pipelined map (FP32 [S, E, H, W], B source batches of Sb): a second
+// stream computes the row weights and near rows of batch b + 1 while the
+// context stream takes batch b's correction factors, map values and near
+// pixels.  The row-weight pass (rowweight_frames_kernel) reads no global
+// memory per source, so it keeps its pace beside the store-bound emit.  Row
+// weights and lists of two batches alternate between two buffers (parity
+// b & 1): batch b + 1's row weights wait for batch b - 1's map.  kPhaseRows
+// enqueues batch 0's row weights (the radii and cleared accumulators are
+// already enqueued on the context stream); the emit phase the rest.  ca
+// describes all S sources, with the two batches' row buffers at its list
+// pointers.
+static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, int phase,
+                        const double* upsf, const double* parint, const double* sep, int exact,
+                        size_t pool_cap) {
+    const int64_t S = ca.S, E = ca.E;
+    const size_t plane = (size_t)ca.H * ca.W;
+    const int64_t want_blocks = (int64_t)ctx->cu_count * 4;
+    if (!ctx->pipe_stream)
+        XTS_HIP(hipStreamCreateWithFlags(&ctx->pipe_stream, hipStreamNonBlocking));
+    while ((int64_t)ctx->pipe_ev.size() < 2 * B + 1) {
+        hipEvent_t e;
+        XTS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->pipe_ev.push_back(e);
+    }
+    hipEvent_t* evR = ctx->pipe_ev.data();
+    hipEvent_t* evM = evR + B;
+    hipEvent_t fork = evR[2 * B];
+    const hipStream_t M = ctx->stream, R = ctx->pipe_stream;
+    const int64_t per_block =
+        rw_frames_per_block(ctx, std::min(Sb, S), (int64_t)ca.tiles_w * ca.tiles_h);
+    // resident row-weight blocks per CU beside the emit: two leave room for
+    // one emit block (LDS 2 x 31 + 66 KB); more crowd the emit out
+    const int64_t per_cu = ctx->tune_rw_frames_cu > 0 ? ctx->tune_rw_frames_cu : 2;
+    auto batch = [&](int64_t b) {
+        CubArgs c = ca;
+        const int64_t b0 = b * Sb;
+        const size_t par = (size_t)(b & 1) * (size_t)Sb * plane;
+        c.S = (int)std::min(Sb, S - b0);
+        c.tiles_per_block = 1;
+        c.rw_persist = 0;
+        c.geo = ca.geo + b0;
+        c.lut = (const float*)ca.lut + (size_t)b0 * kND * ca.Ep;
+        c.near_count = ca.near_count + b0;
+        c.wide_count = ca.wide_count + b0;
+        c.hist = ca.hist + (size_t)b0 * kND;
+        c.mapint = ca.mapint + (size_t)b0 * E;
+        c.cf = ca.cf + (size_t)b0 * E;
+        c.expo = ca.expo ? ca.expo + (size_t)b0 * E : nullptr;
+        c.out = ca.out ? (void*)((float*)ca.out + (size_t)b0 * E * plane) : nullptr;
+        c.rw = (float4*)ca.rw + par;
+        c.rw_row = ca.rw_row + par;
+        c.near_list = ca.near_list + par;
+        c.near_off = ca.near_off + par;
+        c.near_rr = ca.near_rr + par;
+        c.wide_list = ca.wide_list + par;
+        c.near_pool = (float*)ca.near_pool + (size_t)(b & 1) * (size_t)Sb * pool_cap;
+        c.emit_gx = (int)div_up(plane, kEmitBlockPix);
+        c.emit_prio = (int)ctx->tune_emit_prio;
+        return c;
+    };
+    const size_t tsz = 4;
+    const int N4 = (ca.N + 3) & ~3;
+    const size_t sm_near = (size_t)kNDPad * tsz + (size_t)N4 * 4 * tsz + (size_t)N4 * 16 +
+                           (size_t)kNearRows * kNearThreads * tsz;
+    XTS_TRY(allow_lds((const void*)near_rows_kernel<float>, sm_near));
+    auto near_gx = [&](const CubArgs& c) {
+        return std::min<int64_t>(std::max<int64_t>(1, 2 * want_blocks / c.S),
+                                 div_up(plane, kNearPix));
+    };
+    // batch b's row weights and near rows on R
+    auto rows = [&](int64_t b) -> int {
+        const CubArgs c = batch(b);
+        // the first batch's row weights run alone: one block per item
+        XTS_TRY(launch_rw_frames(ctx, c, per_block, b == 0 ? 0 : per_cu, R));
+        near_rows_kernel<float><<<dim3((unsigned)near_gx(c), (unsigned)c.S), kNearThreads,
+                                  sm_near, R>>>(c);
+        XTS_HIP(hipGetLastError());
+        XTS_HIP(hipEventRecord(evR[b], R));
+        return XTS_OK;
+    };
+    // batch b's factors, map values and near pixels on M
+    auto emit_batch = [&](int64_t b) -> int {
+        const CubArgs c = batch(b);
+        const size_t b0 = (size_t)(b * Sb);
+        XTS_HIP(hipStreamWaitEvent(M, evR[b], 0));
+        {
+            TimedRegion tr(ctx, "factor");
+            hist_integral_kernel<<<(unsigned)c.S, kHistThreads, 0, M>>>(
+                (int)E, c.hist, upsf + b0 * kND * E, c.mapint, c.geo, parint + b0 * kND * E,
+                sep, const_cast<double*>(c.cf), exact);
+            XTS_HIP(hipGetLastError());
+        }
+        {
+            TimedRegion tr(ctx, "emit");
+            // energies per emit block: 40 (66 KB of LDS for the table) or
+            // 20 (34 KB: two emit blocks fit beside the row-weight blocks)
+            if (ctx->tune_emit_ec == 20) {
+                const dim3 grid((unsigned)c.emit_gx, (unsigned)c.S, (unsigned)div_up(E, 20));
+                XTS_TRY((emit<float, 20>(M, c, grid, true)));
+            } else {
+                const dim3 grid((unsigned)c.emit_gx, (unsigned)c.S,
+                                (unsigned)div_up(E, kEmitEC32));
+                XTS_TRY((emit<float, kEmitEC32>(M, c, grid, true)));
+            }
+        }
+        {
+            TimedRegion tr(ctx, "near_emit");
+            XTS_TRY((near_emit<float, 40>(M, c, near_gx(c), true)));
+        }
+        XTS_HIP(hipEventRecord(evM[b], M));
+        return XTS_OK;
+    };
+    if (phase & kPhaseRows) {
+        // R starts after the radii and cleared accumulators on M
+        XTS_HIP(hipEventRecord(fork, M));
+        XTS_HIP(hipStreamWaitEvent(R, fork, 0));
+        XTS_TRY(rows(0));
+    }
+    if (!(phase & kPhaseEmit)) return XTS_OK;
+    for (int64_t b = 0; b < B; ++b) {
+        XTS_TRY(emit_batch(b));
+        if (b + 1 < B) {
+            // batch b + 1 reuses the buffers of batch b - 1
+            if (b >= 1) XTS_HIP(hipStreamWaitEvent(R, evM[b - 1], 0));
+            XTS_TRY(rows(b + 1));
+        }
+    }
+    return XTS_OK;
+}
+
+// pipe: the map may run as the pipelined source batches (run_pipeline);
+// false for callers that split a map into phases of their own (the batched
+// engine).
 static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, int64_t S,
                             const double* expo, const double* upsf, const double* parint,
                             int64_t E, const double* rule, int64_t N, int32_t flags, void* out,
-                            bool full, int phase = kPhaseAll, const double* flux = nullptr) {
+                            bool full, int phase = kPhaseAll, const double* flux = nullptr,
+                            bool pipe = true) {
     // any model map other than the second half of a split one overwrites the
     // near lists, counters and radii that a pending first half left: its
     // second half is refused from now on
@@ -2277,9 +2714,18 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     const int sehw = (flags & XTS_LAYOUT_SEHW) ? 1 : 0;
     const int of32 = (flags & XTS_OUT_F32) ? 1 : 0;
 
-    // the radii are launched with the table conversion below
+    // the pipelined map (run_pipeline): the FP32 [S, E, H, W] model map in
+    // pipe_b source batches of pipe_sb, row weights of two batches resident
+    const int64_t pipe_b = pipe ? pipeline_batches(ctx, S, (int64_t)wcs->naxis1 * wcs->naxis2, E,
+                                                   flags, full, phase, sur)
+                                : 1;
+    const int64_t pipe_sb = pipe_b > 1 ? (S + pipe_b - 1) / pipe_b : S;
+    // the radii are launched with the table conversion below; a pipelined
+    // second half keeps the first half's geometry (its row weights may still
+    // be reading it on the second stream)
     ImageGeom ig;
-    XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig, false, phase == kPhaseEmit));
+    XTS_TRY(build_geometry(ctx, wcs, src_sph, (int)S, &ig, false,
+                           phase == kPhaseEmit || (pipe_b > 1 && !(phase & kPhaseRows))));
     const xts_const_tables* ct;
     XTS_TRY(const_tables(ctx, &ct));
     RuleUVW* rp;
@@ -2316,18 +2762,20 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         XTS_HIP(hipGetLastError());
     }
 
+    // sources whose row weights and lists are resident at once
+    const int64_t nbuf = pipe_b > 1 ? 2 * pipe_sb : S;
     // per-pixel row weights and first rows; listed (near) pixels, their pool
     // offsets and row ranges
     const size_t plane = (size_t)H * W;
-    const size_t rw_bytes = (size_t)S * plane * (f64 ? 32 : 16);
+    const size_t rw_bytes = (size_t)nbuf * plane * (f64 ? 32 : 16);
     void* rows;
-    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)S * plane * 20 + 512, &rows));
+    XTS_TRY(slot_reserve(ctx, kSlotRows, rw_bytes + (size_t)nbuf * plane * 20 + 512, &rows));
     // pool of listed pixels' row weights, one segment per source with a slot
     // of kNearRows entries for each of the first 4096 list positions; a
     // pixel beyond is evaluated directly (as a wide pixel)
-    const int64_t pool_cap = std::min<int64_t>(4096 * kNearRows, ((int64_t)1 << 30) / S);
+    const int64_t pool_cap = std::min<int64_t>(4096 * kNearRows, ((int64_t)1 << 30) / nbuf);
     void* pool;
-    XTS_TRY(slot_reserve(ctx, kSlotNearPool, (size_t)(pool_cap * S) * (f64 ? 8 : 4), &pool));
+    XTS_TRY(slot_reserve(ctx, kSlotNearPool, (size_t)(pool_cap * nbuf) * (f64 ? 8 : 4), &pool));
 
     CubArgs ca{};
     ca.corners = ig.corners;
@@ -2341,10 +2789,10 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.inv_sep64 = ct->inv64;
     ca.rw = rows;
     ca.rw_row = (int*)((char*)rows + rw_bytes);
-    ca.near_list = ca.rw_row + (size_t)S * plane;
-    ca.near_off = ca.near_list + (size_t)S * plane;
-    ca.near_rr = ca.near_off + (size_t)S * plane;
-    ca.wide_list = ca.near_rr + (size_t)S * plane;
+    ca.near_list = ca.rw_row + (size_t)nbuf * plane;
+    ca.near_off = ca.near_list + (size_t)nbuf * plane;
+    ca.near_rr = ca.near_off + (size_t)nbuf * plane;
+    ca.wide_list = ca.near_rr + (size_t)nbuf * plane;
     ca.near_count = (int*)acc;
     ca.wide_count = ca.near_count + S;
     ca.sur = sur;
@@ -2381,6 +2829,9 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.j0 = 1.0 + kRecipStep * (kLn1e4 + std::log(2.0 * kR2D));
     ca.jk = kRecipStep * 0.69314718055994530942;
 
+    if (pipe_b > 1)
+        return run_pipeline(ctx, ca, pipe_b, pipe_sb, phase, upsf, parint, ct->sep,
+                            (flags & XTS_CF_EXACT_RADIUS) ? 1 : 0, (size_t)pool_cap);
     // pass 1: one tile per block by default (many short blocks keep the
     // last round of the grid short); a run of tiles per block on request
     const int ntiles = ca.tiles_w * ca.tiles_h;
@@ -2400,7 +2851,13 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                                                                  (int64_t)S * ntiles))
                               : dim3((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
-        if (f64) {
+        if (!f64 && !sur && !ca.rw_persist && ctx->tune_tiles_per_block == 0 &&
+            !ctx->tune_rw_legacy) {
+            // the tile-frame pass; the per-source tile blocks below on request
+            // (tuning "rowweight_legacy", or their launch-shape keys)
+            XTS_TRY(launch_rw_frames(ctx, ca, rw_frames_per_block(ctx, S, ntiles),
+                                     ctx->tune_rw_frames_cu, ctx->stream));
+        } else if (f64) {
             if (ca.rw_persist)
                 rowweight_kernel<double, false, true><<<grid, kThreads, sm, ctx->stream>>>(ca);
             else
@@ -2481,15 +2938,13 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         return XTS_OK;
     }
     if (!(phase & kPhaseEmit)) return XTS_OK;
-    // pass 3: blocks of kEmitBlockPix pixels of one source, then kWideBlocks
-    // blocks per source for the wide pixels (they depend only on the factors
-    // and the near lists and write values no other pass writes, so they run
-    // beside the far pixels' blocks)
+    // pass 3: blocks of kEmitBlockPix pixels of one source; then the wide
+    // pixels' launch (kWideBlocks blocks per source)
     {
         ca.emit_gx = (int)div_up(plane, kEmitBlockPix);
-        const dim3 grid32((unsigned)(ca.emit_gx + kWideBlocks), (unsigned)S,
+        const dim3 grid32((unsigned)ca.emit_gx, (unsigned)S,
                           (unsigned)div_up(E, kEmitEC32));
-        const dim3 grid64((unsigned)(ca.emit_gx + kWideBlocks), (unsigned)S,
+        const dim3 grid64((unsigned)ca.emit_gx, (unsigned)S,
                           (unsigned)div_up(E, kEmitEC64));
         TimedRegion tr(ctx, "emit");
         if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx->stream, ca, grid64, full)));
@@ -2577,7 +3032,7 @@ int model_map_dev_call(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, 
                        const double* rule, int64_t N, int32_t flags, void* out, int phase) {
     XTS_CHECK(phase >= 1 && phase <= kPhaseAll, "bad phase %d", phase);
     return model_map_device(ctx, wcs, src_sph, S, expo, upsf, parint, E, rule, N, flags, out,
-                            true, phase);
+                            true, phase, nullptr, false);
 }
 
 static size_t out_bytes(int64_t S, const xts_wcs* wcs, int64_t E, int32_t flags) {
