@@ -36,7 +36,7 @@ import numpy as np
 
 METRIC = "srcmap pix·E·src evals/s: surrogate vs cubature at ≤1e-4 max rel err, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-STAGES = ("cubature", "surrogate", "near", "factor", "emit", "near_emit")
+STAGES = ("cubature", "surrogate", "near", "factor", "emit", "wide", "near_emit")
 
 
 def _log(msg):

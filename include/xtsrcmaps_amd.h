@@ -110,9 +110,9 @@ const char* xts_last_error(void);
 int xts_abi_version(void);
 /* Instrumentation (no reference counterpart): when enabled, kernel launches
  * of the named model-map stages ("cubature": the row-weight pass, "near":
- * the near-pixel rows, "factor", "emit", "near_emit": the near and wide
- * pixels' map values, "fused": the pipelined map's launches of one batch's
- * row weights beside the previous batch's map) are bracketed by events on the
+ * the near-pixel rows, "factor", "emit", "wide" and "near_emit": the map
+ * values of the far, wide and near pixels; the row weights of the pipelined
+ * map run on its second stream and are not timed) are bracketed by events on the
  * context stream; xts_ctx_kernel_times synchronises and returns (and
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);

@@ -45,13 +45,13 @@ def _rel(got, ref):
     return np.abs(got.astype(np.float64) - ref) / np.abs(ref)
 
 
-def _maps(ctx, batches, *args):
+def _maps(ctx, batches, *args, extra=0):
     """The map with the passes over all sources, then in `batches` batches
-    (the batches' row weights by the tile-frame pass)."""
+    (the batches' row weights by the tile-frame pass); extra: more flags."""
     ctx.set_tuning("pipeline_batches", 1)
-    a = ctx.point_src_model_map_wcs(*args, flags=F32_SEHW)
+    a = ctx.point_src_model_map_wcs(*args, flags=F32_SEHW | extra)
     ctx.set_tuning("pipeline_batches", batches)
-    b = ctx.point_src_model_map_wcs(*args, flags=F32_SEHW)
+    b = ctx.point_src_model_map_wcs(*args, flags=F32_SEHW | extra)
     ctx.set_tuning("pipeline_batches", 1)
     return a, b
 
@@ -162,3 +162,22 @@ def test_gpu_pipeline_split_halves(ctx, orc, c1):
     with pytest.raises(xa.XtsError):
         ctx.point_src_model_map_finish_dev(c1.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
                                            out2, c1.rule, flags=F32_SEHW)
+
+
+def test_gpu_pipeline_surrogate(ctx, orc, c1):
+    """The surrogate map (BF16 and FP8 networks) in 4 batches, the network
+    pass of each batch on the context stream: equal to the one-pass surrogate
+    map with the same network (a short fit on the 200x200 image) within 2e-6,
+    and finite and positive."""
+    import xtsrcmaps_amd as xa
+    from pipelines import oracle_stages
+    from xtsrcmaps_amd import surrogate as sg
+    _log("test_gpu_pipeline_surrogate: 100 x 37 x 200 x 200, 4 batches")
+    o = oracle_stages(orc, c1)
+    sg.fit(ctx, c1.wcs, c1.rule, n_samples=1 << 18, steps=300, seed=3)
+    for extra in (xa.SURROGATE, xa.SURROGATE | xa.SURROGATE_FP8):
+        a, b = _maps(ctx, 4, c1.wcs, o["src"], o["expo"], o["upsf"], o["parint"], c1.rule,
+                     extra=extra)
+        r = np.max(_rel(b, a.astype(np.float64)))
+        _log(f"  flags {extra:#x}: batched vs one pass max rel {r:.3e}")
+        assert np.isfinite(b).all() and (b > 0).all() and r < 2e-6

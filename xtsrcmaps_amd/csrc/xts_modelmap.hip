@@ -1133,11 +1133,13 @@ __device__ inline SrcGeom load_geo_scalar(const SrcGeom* p) {
     return g;
 }
 
+template <bool SUR>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 rowweight_frames_kernel(CubArgs a, int nsplit) {
     __shared__ PixFrame fr_s[kThreads];
     __shared__ double isep_s[kNDPad];
     __shared__ double hist_s[kNDPad];
+    __shared__ int sur_cnt_s[kThreads / 64];
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_w * a.tiles_h;
     const long long nitem = (long long)ntiles * nsplit;
@@ -1234,11 +1236,23 @@ rowweight_frames_kernel(CubArgs a, int nsplit) {
         // rowweight_block)
         const bool lean6 = __any(fast) && __all(!fast || pg.shi < 28.0f);
         bool series = false;
+        bool listed = false;   // handed to the surrogate pass (sur_rows_kernel)
+        float sf[4] = {0.f, 0.f, 0.f, 0.f}, sA = 0.f;
         if (lean6) {
             const bool one = __all(!fast || ihi - ilo <= 1);
             const float rx = fast ? poly_reach(pg) : 0.0f;
-            series = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
-                         : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
+            if (SUR && one) {
+                // as in rowweight_block: A from four points, B1 = |A| for
+                // pixels whose bounds hold no node, the network for the others
+                series = sur_dispatch(lr.isep64, pg, r0, wsum, rx, sf, sA);
+                if (series) {
+                    listed = fast && ihi == ilo + 1;
+                    poly_rows<1>(wsum, sA, fabsf(sA), 0.0f, W);
+                }
+            } else {
+                series = one ? poly_dispatch<1>(a, lr, pg, r0, N4, wsum, rx, W)
+                             : poly_dispatch<2>(a, lr, pg, r0, N4, wsum, rx, W);
+            }
         }
         if (!series && __any(fast)) {
             float S0 = 0.f, T0 = 0.f, S01 = 0.f, T01 = 0.f, Tt = 0.f;
@@ -1263,7 +1277,33 @@ rowweight_frames_kernel(CubArgs a, int nsplit) {
             }
             rows_from_sums(wsum, S0, T0, S01, T01, Tt, W);
         }
-        if (valid) {
+        if constexpr (SUR) {
+            // the tile's listed pixels, compacted in its 256 list slots of
+            // this source (wave counts in LDS, then each lane's slot)
+            const bool app = valid && listed;
+            const unsigned long long m = __ballot(app);
+            const int lane = tid & 63, wv = tid >> 6;
+            if (lane == 0) sur_cnt_s[wv] = __popcll(m);
+            __syncthreads();
+            int base = 0;
+            for (int q = 0; q < wv; ++q) base += sur_cnt_s[q];
+            if (app) {
+                a.rw_row[(size_t)s * plane + pix] = r0;
+                const int k = base + __popcll(m & ((1ull << lane) - 1));
+                const bool inr = do_hist && in_radius_dir(g, f.c);
+                float4* e = a.sur_list + 2 * (((size_t)s * ntiles + t) * kThreads + k);
+                e[0] = float4{sf[0], sf[1], sf[2], sf[3]};
+                e[1] = float4{sA, __int_as_float(pix), f.om,
+                              __int_as_float(2 * r0 + (inr ? 1 : 0))};
+            }
+            if (tid == 0) {
+                int n = 0;
+                for (int q = 0; q < kThreads / 64; ++q) n += sur_cnt_s[q];
+                a.sur_count[(size_t)s * ntiles + t] = n;
+            }
+            __syncthreads();   // sur_cnt_s is rewritten by the next source
+        }
+        if (valid && !listed) {
             if (fast) {
                 const float4 wv = float4{f.om * W[0], f.om * W[1], f.om * W[2], f.om * W[3]};
                 reinterpret_cast<float4*>(a.rw)[(size_t)s * plane + pix] = wv;
@@ -1392,6 +1432,10 @@ near_rows_kernel(CubArgs a) {
     T* ph = reinterpret_cast<T*>(tco_s + N4);   // [kNearRows][kNearThreads]
     // the block's in-radius row sums, added to the source's histogram once
     __shared__ double hblk[kNDPad];
+    // the pass's wide in-radius pixels, whose rule points the block's waves
+    // then take one pixel at a time (a lane per point, FP64)
+    __shared__ int wide_s[kNearPix];
+    __shared__ int wide_n;
 
     const int s = blockIdx.y;
     const int count = a.near_count[s];
@@ -1399,6 +1443,7 @@ near_rows_kernel(CubArgs a) {
     const int tid = threadIdx.x;
     const int slot = tid / kNearSplit, part = tid % kNearSplit;
     for (int k = tid; k < kND; k += kNearThreads) hblk[k] = 0.0;
+    if (tid == 0) wide_n = 0;
     stage_rule<T>(a, isep_s, rule_s, N4, tid, kNearThreads);
     for (int p = tid; p < N4; p += kNearThreads) {
         const float wq = p < a.N ? (float)a.uvw[p].wq : 0.0f;
@@ -1463,27 +1508,8 @@ near_rows_kernel(CubArgs a) {
                 }
             }
         } else if (!narrow) {
-            // a wide pixel's in-radius weights, point by point in FP64 (its
-            // points' offsets from the source cancel; see wide_pixels)
-            if (inrad) {
-                const double omd = a.omega ? a.omega[pix] : 1.0;
-                const PixelGeom<double> pd =
-                    pixel_geom<double>(a.corners, h, w, Wp1, g.dir, a.rho_scale);
-                for (int p = part; p < a.N; p += kNearSplit) {
-                    const RuleUVW q = a.uvw[p];
-                    const double u = q.u, v = q.v;
-                    const double dx = pd.A[0] + u * pd.B[0] + v * (pd.C[0] + u * pd.E[0]);
-                    const double dy = pd.A[1] + u * pd.B[1] + v * (pd.C[1] + u * pd.E[1]);
-                    const double dz = pd.A[2] + u * pd.B[2] + v * (pd.C[2] + u * pd.E[2]);
-                    const double sd = 2.0 * kR2D * asin(fmin(sqrt(dx * dx + dy * dy + dz * dz),
-                                                             1.0));
-                    const int i = sep_index_fast(sd);
-                    const double tt =
-                        kRedm1 * (sd * (a.inv_sep64[i - 1] * (0.5 / kR2D)) - 1.0);
-                    atomicAdd(&hblk[i - 1], omd * (q.wq * (1.0 - tt)));
-                    atomicAdd(&hblk[i], omd * (q.wq * tt));
-                }
-            }
+            // a wide pixel's in-radius weights: after this pass, by a wave
+            if (inrad && part == 0) wide_s[atomicAdd(&wide_n, 1)] = pix;
         } else {
             for (int p = part; p < a.N; p += kNearSplit) {
                 const V4 rr = rule_s[p];
@@ -1522,6 +1548,36 @@ near_rows_kernel(CubArgs a) {
                 a.wide_list[(size_t)s * plane + atomicAdd(&a.wide_count[s], 1)] = idx;
             }
         }
+        // the pass's wide pixels: a lane per rule point, the points' offsets
+        // from the source in FP64 (they cancel the pixel-sized blend terms;
+        // see wide_pixels)
+        __syncthreads();
+        const int nw = wide_n;
+        const int lane = tid & 63;
+        for (int k = tid >> 6; k < nw; k += kNearThreads / 64) {
+            const int wp = wide_s[k];
+            const double omd = a.omega ? a.omega[wp] : 1.0;
+            const PixelGeom<double> pd =
+                pixel_geom<double>(a.corners, wp / a.W, wp % a.W, Wp1, g.dir, a.rho_scale);
+            for (int p = lane; p < a.N; p += 64) {
+                const RuleUVW q = a.uvw[p];
+                const double u = q.u, v = q.v;
+                const double dx = pd.A[0] + u * pd.B[0] + v * (pd.C[0] + u * pd.E[0]);
+                const double dy = pd.A[1] + u * pd.B[1] + v * (pd.C[1] + u * pd.E[1]);
+                const double dz = pd.A[2] + u * pd.B[2] + v * (pd.C[2] + u * pd.E[2]);
+                const double sd =
+                    2.0 * kR2D * asin(fmin(sqrt(dx * dx + dy * dy + dz * dz), 1.0));
+                const int i = sep_index_fast(sd);
+                const double tt = kRedm1 * (sd * (a.inv_sep64[i - 1] * (0.5 / kR2D)) - 1.0);
+                atomicAdd(&hblk[i - 1], omd * (q.wq * (1.0 - tt)));
+                atomicAdd(&hblk[i], omd * (q.wq * tt));
+            }
+        }
+        // every wave is past the list before it is cleared, and the clear is
+        // seen before the next pass adds to it
+        __syncthreads();
+        if (tid == 0) wide_n = 0;
+        __syncthreads();
     }
     if (do_hist) {
         __syncthreads();
@@ -1707,8 +1763,9 @@ near_emit_kernel(CubArgs a) {
 
 // Pass 3: per (source, pixel range, energy chunk): map = sum_k W_k L[r0+k]
 // for pass-1 pixels, x exposure x correction when SCALED.  Listed pixels are
-// left to near_emit_kernel, wide ones to wide_emit_kernel (a launch of its
-// own: its registers would limit the emit's residency beside other work).  OT: output type.  Each thread first loads the
+// left to near_emit_kernel, wide ones to WIDE's extra blocks or to
+// wide_emit_kernel (a launch of its own: its registers would limit the
+// emit's residency beside other work, as in the pipelined map).  OT: output type.  Each thread first loads the
 // row weights of its kEmitPix pixels and then only stores: a load issued
 // after stores would wait for them to drain (the vector memory counter
 // counts both).
@@ -1720,7 +1777,7 @@ near_emit_kernel(CubArgs a) {
 constexpr int kEmitPix = 4;
 constexpr int kEmitBlockPix = kEmitThreads * kEmitPix;
 constexpr int kEmitEC32 = 40, kEmitEC64 = 16;   // energies per block (LDS: 401 rows of EC + 1)
-template <typename T, int EC, typename OT, bool SCALED>
+template <typename T, int EC, typename OT, bool SCALED, bool WIDE>
 __global__ void __launch_bounds__(kEmitThreads)
 emit_sehw_kernel(CubArgs a) {
     using V4 = typename Vec4<T>::type;
@@ -1728,6 +1785,13 @@ emit_sehw_kernel(CubArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     __shared__ ChunkScale<EC> sc;
+    if (WIDE && (int)blockIdx.x >= a.emit_gx) {
+        constexpr int kWaves = kEmitThreads / 64;
+        wide_pixels<T, EC, OT, SCALED>(a, (blockIdx.x - a.emit_gx) * kWaves + threadIdx.x / 64,
+                                       kWideBlocks * kWaves, lut_s + (threadIdx.x / 64) * kNDPad,
+                                       blockIdx.y, blockIdx.z * EC);
+        return;
+    }
 
     // beside a compute-bound pass, the stores' waves may take issue first
     if (a.emit_prio == 1) __builtin_amdgcn_s_setprio(1);
@@ -1796,7 +1860,7 @@ emit_sehw_kernel(CubArgs a) {
     }
 }
 
-template <typename T, int EC, typename OT, bool SCALED>
+template <typename T, int EC, typename OT, bool SCALED, bool WIDE>
 __global__ void __launch_bounds__(kEmitThreads)
 emit_shwe_kernel(CubArgs a) {
     using V = typename Vec<T>::type;
@@ -1806,6 +1870,13 @@ emit_shwe_kernel(CubArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lut_s = reinterpret_cast<T*>(smem_raw);
     __shared__ ChunkScale<EC> sc;
+    if (WIDE && (int)blockIdx.x >= a.emit_gx) {
+        constexpr int kWaves = kEmitThreads / 64;
+        wide_pixels<T, EC, OT, SCALED>(a, (blockIdx.x - a.emit_gx) * kWaves + threadIdx.x / 64,
+                                       kWideBlocks * kWaves, lut_s + (threadIdx.x / 64) * kNDPad,
+                                       blockIdx.y, blockIdx.z * EC);
+        return;
+    }
 
     const int s = blockIdx.y;
     const int e0 = blockIdx.z * EC;
@@ -2113,21 +2184,36 @@ __global__ void __launch_bounds__(kEmitThreads) wide_emit_kernel(CubArgs a) {
                                    blockIdx.y, blockIdx.z * EC);
 }
 
-// grid: (emit blocks per source, S, energy chunks); the wide pixels' launch
-// follows on the same stream
+// grid: (emit blocks per source, S, energy chunks).  mode kEmitWithWide: the
+// wide pixels by kWideBlocks extra blocks per source of the same launch;
+// kEmitFar: the far pixels only (the emit's registers then leave room for
+// other work on a CU); kEmitWide: the wide pixels' launch alone.
+enum { kEmitWithWide = 0, kEmitFar = 1, kEmitWide = 2 };
 template <typename T, int EC, typename OT, bool SCALED>
-static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw) {
+static int launch_emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool sehw, int mode) {
+    if (mode == kEmitWide) {
+        const size_t smw = (size_t)(kEmitThreads / 64) * kNDPad * sizeof(T);
+        XTS_TRY(allow_lds((const void*)wide_emit_kernel<T, EC, OT, SCALED>, smw));
+        wide_emit_kernel<T, EC, OT, SCALED><<<dim3(kWideBlocks, grid.y, grid.z), kEmitThreads,
+                                              smw, st>>>(ca);
+        XTS_HIP(hipGetLastError());
+        return XTS_OK;
+    }
     const size_t sm = (size_t)kND * (sehw ? EC + 1 : EC) * sizeof(T);
-    const void* fn = sehw ? (const void*)emit_sehw_kernel<T, EC, OT, SCALED>
-                          : (const void*)emit_shwe_kernel<T, EC, OT, SCALED>;
-    XTS_TRY(allow_lds(fn, sm));
-    if (sehw) emit_sehw_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, st>>>(ca);
-    else emit_shwe_kernel<T, EC, OT, SCALED><<<grid, kEmitThreads, sm, st>>>(ca);
-    XTS_HIP(hipGetLastError());
-    const size_t smw = (size_t)(kEmitThreads / 64) * kNDPad * sizeof(T);
-    XTS_TRY(allow_lds((const void*)wide_emit_kernel<T, EC, OT, SCALED>, smw));
-    wide_emit_kernel<T, EC, OT, SCALED><<<dim3(kWideBlocks, grid.y, grid.z), kEmitThreads, smw,
-                                          st>>>(ca);
+    if (mode == kEmitFar) {
+        const void* fn = sehw ? (const void*)emit_sehw_kernel<T, EC, OT, SCALED, false>
+                              : (const void*)emit_shwe_kernel<T, EC, OT, SCALED, false>;
+        XTS_TRY(allow_lds(fn, sm));
+        if (sehw) emit_sehw_kernel<T, EC, OT, SCALED, false><<<grid, kEmitThreads, sm, st>>>(ca);
+        else emit_shwe_kernel<T, EC, OT, SCALED, false><<<grid, kEmitThreads, sm, st>>>(ca);
+    } else {
+        const dim3 g2(grid.x + kWideBlocks, grid.y, grid.z);
+        const void* fn = sehw ? (const void*)emit_sehw_kernel<T, EC, OT, SCALED, true>
+                              : (const void*)emit_shwe_kernel<T, EC, OT, SCALED, true>;
+        XTS_TRY(allow_lds(fn, sm));
+        if (sehw) emit_sehw_kernel<T, EC, OT, SCALED, true><<<g2, kEmitThreads, sm, st>>>(ca);
+        else emit_shwe_kernel<T, EC, OT, SCALED, true><<<g2, kEmitThreads, sm, st>>>(ca);
+    }
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -2153,14 +2239,17 @@ static int near_emit(hipStream_t st, const CubArgs& ca, int64_t gx, bool scaled)
                   : launch_near_emit<T, EC, double, false>(st, ca, grid);
 }
 
+// The map values of the pass-1 pixels and the wide pixels (launch_emit's
+// modes).
 template <typename T, int EC>
-static int emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool scaled) {
+static int emit(hipStream_t st, const CubArgs& ca, dim3 grid, bool scaled,
+                int mode = kEmitWithWide) {
     const bool sehw = ca.layout_sehw != 0;
     if (ca.out_f32)
-        return scaled ? launch_emit<T, EC, float, true>(st, ca, grid, sehw)
-                      : launch_emit<T, EC, float, false>(st, ca, grid, sehw);
-    return scaled ? launch_emit<T, EC, double, true>(st, ca, grid, sehw)
-                  : launch_emit<T, EC, double, false>(st, ca, grid, sehw);
+        return scaled ? launch_emit<T, EC, float, true>(st, ca, grid, sehw, mode)
+                      : launch_emit<T, EC, float, false>(st, ca, grid, sehw, mode);
+    return scaled ? launch_emit<T, EC, double, true>(st, ca, grid, sehw, mode)
+                  : launch_emit<T, EC, double, false>(st, ca, grid, sehw, mode);
 }
 
 // ------------------------------------------------------------------ factors
@@ -2501,8 +2590,10 @@ static size_t model_map_counter_doubles(int64_t S) { return (size_t)S; }
 
 // Source batches of the pipelined map for this call, or 1 (the passes over
 // all sources at once): the FP32 cubature into an FP32 [S, E, H, W] map,
-// full model map, no surrogate, not from a caller's arena; tuning key
-// "pipeline_batches" (0: the default, 1: off, n: n batches).  By default a
+// full model map, not from a caller's arena; tuning key
+// "pipeline_batches" (0: the default, 1: off, n: n batches).  The surrogate
+// map runs in batches only when asked (its network pass on M between the
+// emits leaves the pipeline no faster: 21.4 against 20.9 ms).  By default a
 // map of at least kPipeMinValues values runs in kPipeDefaultBatches
 // batches (2,000 sources x 37 x 400 x 400: 15.1 ms against 16.4 for the
 // same kernels one after the other, on one MI355X).
@@ -2511,7 +2602,8 @@ constexpr int64_t kPipeMinSources = 16;   // sources per batch at least
 constexpr double kPipeMinValues = 4e9;
 static int64_t pipeline_batches(const xts_ctx* ctx, int64_t S, int64_t plane, int64_t E,
                                 int32_t flags, bool full, int phase, int sur) {
-    const bool ok = full && !sur && !(flags & XTS_COMPUTE_F64) && (flags & XTS_OUT_F32) &&
+    const bool ok = full && !(flags & XTS_COMPUTE_F64) && (flags & XTS_OUT_F32) &&
+                    (!sur || ctx->tune_pipe_batches > 1) &&
                     (flags & XTS_LAYOUT_SEHW) && !(phase & kPhaseSum) &&
                     ctx->arena.base == nullptr && phase != kPhaseEmit;
     if (!ok) return 1;
@@ -2538,7 +2630,8 @@ static int launch_rw_frames(xts_ctx* ctx, const CubArgs& c, int64_t per_block, i
     const int64_t nsplit = std::max<int64_t>(1, (c.S + per_block - 1) / per_block);
     const int64_t nitem = ntiles * nsplit;
     const int64_t grid = per_cu > 0 ? std::min<int64_t>(nitem, per_cu * ctx->cu_count) : nitem;
-    rowweight_frames_kernel<<<(unsigned)grid, kThreads, 0, st>>>(c, (int)nsplit);
+    if (c.sur) rowweight_frames_kernel<true><<<(unsigned)grid, kThreads, 0, st>>>(c, (int)nsplit);
+    else rowweight_frames_kernel<false><<<(unsigned)grid, kThreads, 0, st>>>(c, (int)nsplit);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -2546,7 +2639,7 @@ static int launch_rw_frames(xts_ctx* ctx, const CubArgs& c, int64_t per_block, i
 // The pipelined map (FP32 [S, E, H, W], B source batches of Sb): a second
 // stream computes the row weights and near rows of batch b + 1 while the
 // context stream takes batch b's correction factors, map values and near
-// pixels.  The row-weight pass (rowweight_frames_kernel) reads no global
+// pixels (with the surrogate, first batch b's network pass).  The row-weight pass (rowweight_frames_kernel) reads no global
 // memory per source, so it keeps its pace beside the store-bound emit.  Row
 // weights and lists of two batches alternate between two buffers (parity
 // b & 1): batch b + 1's row weights wait for batch b - 1's map.  kPhaseRows
@@ -2567,6 +2660,8 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         XTS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         ctx->pipe_ev.push_back(e);
     }
+    // evR[b]: batch b's row weights and near rows done (R); evM[b]: batch b
+    // written (M)
     hipEvent_t* evR = ctx->pipe_ev.data();
     hipEvent_t* evM = evR + B;
     hipEvent_t fork = evR[2 * B];
@@ -2599,6 +2694,11 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         c.near_rr = ca.near_rr + par;
         c.wide_list = ca.wide_list + par;
         c.near_pool = (float*)ca.near_pool + (size_t)(b & 1) * (size_t)Sb * pool_cap;
+        if (ca.sur_list) {
+            const size_t st = (size_t)(b & 1) * (size_t)Sb * (size_t)ca.tiles_w * ca.tiles_h;
+            c.sur_list = ca.sur_list + 2 * st * kThreads;
+            c.sur_count = ca.sur_count + st;
+        }
         c.emit_gx = (int)div_up(plane, kEmitBlockPix);
         c.emit_prio = (int)ctx->tune_emit_prio;
         return c;
@@ -2628,6 +2728,19 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         const CubArgs c = batch(b);
         const size_t b0 = (size_t)(b * Sb);
         XTS_HIP(hipStreamWaitEvent(M, evR[b], 0));
+        if (c.sur) {
+            // the network pass at full occupancy on M (beside the emit its
+            // matrix and VALU work would run at a fraction of the CUs)
+            const int64_t tiles = (int64_t)c.tiles_w * c.tiles_h;
+            const dim3 grid(
+                (unsigned)std::min<int64_t>(std::max<int64_t>(1, 4 * want_blocks / c.S),
+                                            div_up(tiles, kSurThreads / 64)),
+                (unsigned)c.S);
+            TimedRegion tr(ctx, "surrogate");
+            if (c.sur == 3) sur_rows_kernel<MlpPacked8><<<grid, kSurThreads, 0, M>>>(c);
+            else sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, M>>>(c);
+            XTS_HIP(hipGetLastError());
+        }
         {
             TimedRegion tr(ctx, "factor");
             hist_integral_kernel<<<(unsigned)c.S, kHistThreads, 0, M>>>(
@@ -2635,17 +2748,17 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
                 sep, const_cast<double*>(c.cf), exact);
             XTS_HIP(hipGetLastError());
         }
-        {
-            TimedRegion tr(ctx, "emit");
-            // energies per emit block: 40 (66 KB of LDS for the table) or
-            // 20 (34 KB: two emit blocks fit beside the row-weight blocks)
+        // energies per emit block: 40 (66 KB of LDS for the table) or 20
+        // (34 KB: two emit blocks fit beside the row-weight blocks)
+        for (const int mode : {kEmitFar, kEmitWide}) {
+            TimedRegion tr(ctx, mode == kEmitWide ? "wide" : "emit");
             if (ctx->tune_emit_ec == 20) {
                 const dim3 grid((unsigned)c.emit_gx, (unsigned)c.S, (unsigned)div_up(E, 20));
-                XTS_TRY((emit<float, 20>(M, c, grid, true)));
+                XTS_TRY((emit<float, 20>(M, c, grid, true, mode)));
             } else {
                 const dim3 grid((unsigned)c.emit_gx, (unsigned)c.S,
                                 (unsigned)div_up(E, kEmitEC32));
-                XTS_TRY((emit<float, kEmitEC32>(M, c, grid, true)));
+                XTS_TRY((emit<float, kEmitEC32>(M, c, grid, true, mode)));
             }
         }
         {
@@ -2665,11 +2778,14 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
     for (int64_t b = 0; b < B; ++b) {
         XTS_TRY(emit_batch(b));
         if (b + 1 < B) {
-            // batch b + 1 reuses the buffers of batch b - 1
+            // batch b + 1 reuses the buffers of batch b - 1; R runs up to one
+            // batch ahead of M (waiting for batch b itself, as measured,
+            // leaves M idle between batches)
             if (b >= 1) XTS_HIP(hipStreamWaitEvent(R, evM[b - 1], 0));
             XTS_TRY(rows(b + 1));
         }
     }
+    // M's last wait was on evR[B - 1]: R's work is joined into M
     return XTS_OK;
 }
 
@@ -2802,7 +2918,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.sur_count = nullptr;
     if (sur) {
         // per tile: 256 list slots of 32 bytes and the tile's count
-        const size_t nt = (size_t)S * ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
+        const size_t nt =
+            (size_t)nbuf * ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH);
         void* sl;
         XTS_TRY(slot_reserve(ctx, kSlotSurList, nt * (kThreads * 32 + 4), &sl));
         ca.sur_list = (float4*)sl;
@@ -2851,8 +2968,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
                                                                  (int64_t)S * ntiles))
                               : dim3((unsigned)div_up(ntiles, ca.tiles_per_block), (unsigned)S);
         TimedRegion tr(ctx, "cubature");
-        if (!f64 && !sur && !ca.rw_persist && ctx->tune_tiles_per_block == 0 &&
-            !ctx->tune_rw_legacy) {
+        if (!f64 && !ca.rw_persist && ctx->tune_tiles_per_block == 0 && !ctx->tune_rw_legacy) {
             // the tile-frame pass; the per-source tile blocks below on request
             // (tuning "rowweight_legacy", or their launch-shape keys)
             XTS_TRY(launch_rw_frames(ctx, ca, rw_frames_per_block(ctx, S, ntiles),
@@ -2938,14 +3054,17 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
         return XTS_OK;
     }
     if (!(phase & kPhaseEmit)) return XTS_OK;
-    // pass 3: blocks of kEmitBlockPix pixels of one source; then the wide
-    // pixels' launch (kWideBlocks blocks per source)
+    // pass 3: blocks of kEmitBlockPix pixels of one source, then kWideBlocks
+    // blocks per source for the wide pixels (they depend only on the factors
+    // and the near lists and write values no other pass writes, so they run
+    // beside the far pixels' blocks)
     {
         ca.emit_gx = (int)div_up(plane, kEmitBlockPix);
         const dim3 grid32((unsigned)ca.emit_gx, (unsigned)S,
                           (unsigned)div_up(E, kEmitEC32));
         const dim3 grid64((unsigned)ca.emit_gx, (unsigned)S,
                           (unsigned)div_up(E, kEmitEC64));
+        // the wide pixels by extra blocks of the same launch
         TimedRegion tr(ctx, "emit");
         if (f64) XTS_TRY((emit<double, kEmitEC64>(ctx->stream, ca, grid64, full)));
         else XTS_TRY((emit<float, kEmitEC32>(ctx->stream, ca, grid32, full)));

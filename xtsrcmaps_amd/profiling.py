@@ -4,6 +4,7 @@ Workloads to profile (run under ``rocprofv3 ... -- python3 -m
 xtsrcmaps_amd.profiling <workload>``):
 
     modelmap --config c1 --calls 3   the model-map passes on resident inputs
+             [--pipeline-batches N]  (1: every pass over all sources at once)
     stages --config c3 --precision 1 the PSF stages (0 FP64, 1 BF16x3 contractions)
     copy --mib 592 --calls 3         an FP32 copy kernel of known size: the
                                      calibration of FETCH_SIZE / WRITE_SIZE
@@ -26,19 +27,22 @@ import os
 import re
 import sys
 
-MM_KERNELS = (r"rowweight_kernel|sur_rows_kernel|near_rows_kernel|hist_integral_kernel|"
-              r"factor_kernel|emit_s|near_emit_kernel")
+MM_KERNELS = (r"rowweight_kernel|rowweight_frames_kernel|sur_rows_kernel|near_rows_kernel|"
+              r"hist_integral_kernel|factor_kernel|emit_s|wide_emit_kernel|near_emit_kernel")
 COPY_KERNELS = r"MulFunctor"          # torch.mul(src, 1.0, out=dst): reads and writes N bytes
 
 
 # ---------------------------------------------------------------- workloads
-def _modelmap(config: str, calls: int, surrogate: bool = False, fp8: bool = False) -> None:
+def _modelmap(config: str, calls: int, surrogate: bool = False, fp8: bool = False,
+              pipeline_batches: int | None = None) -> None:
     import torch
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     sys.path.insert(0, root)
     import bench
     import xtsrcmaps_amd as xa
     cfg, inp, ctx, d = bench.setup(config, surrogate=surrogate)
+    if pipeline_batches is not None:
+        ctx.set_tuning("pipeline_batches", pipeline_batches)
     flags = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32 | (xa.SURROGATE if surrogate else 0)
     if surrogate and fp8:
         flags |= xa.SURROGATE_FP8
@@ -226,6 +230,8 @@ def main(argv=None) -> None:
     p.add_argument("--calls", type=int, default=3)
     p.add_argument("--surrogate", action="store_true")
     p.add_argument("--fp8", action="store_true", help="with --surrogate: the FP8 network")
+    p.add_argument("--pipeline-batches", type=int, default=None,
+                   help="tuning key pipeline_batches (1: every pass over all sources at once)")
     p = sub.add_parser("stages")
     p.add_argument("--config", default="c3")
     p.add_argument("--calls", type=int, default=5)
@@ -253,7 +259,7 @@ def main(argv=None) -> None:
     p.add_argument("--cmd", required=True)
     a = ap.parse_args(argv)
     if a.what == "modelmap":
-        _modelmap(a.config, a.calls, a.surrogate, a.fp8)
+        _modelmap(a.config, a.calls, a.surrogate, a.fp8, a.pipeline_batches)
     elif a.what == "stages":
         _stages(a.config, a.calls, a.precision)
     elif a.what == "copy":
