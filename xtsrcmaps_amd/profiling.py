@@ -121,7 +121,9 @@ def _counter_sum(run_dir: str, counter: str, kernel_rx: str) -> tuple[float, int
 
 
 def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
-                 copy_mib: int, cmd: str) -> dict:
+                 copy_mib: int, cmd: str, maps: int = 0) -> dict:
+    """HBM bytes per model map; maps: model maps in the profiled run (0: one
+    per emit dispatch, as without source batches)."""
     copy_bytes = copy_mib * (1 << 20)
     cf, nf = _counter_sum(copy_fetch, "FETCH_SIZE", COPY_KERNELS)
     cw, nw = _counter_sum(copy_write, "WRITE_SIZE", COPY_KERNELS)
@@ -130,12 +132,16 @@ def traffic_json(mm_fetch: str, mm_write: str, copy_fetch: str, copy_write: str,
     f_write = cw * 1024 / (nw * copy_bytes)
     mf, mnf = _counter_sum(mm_fetch, "FETCH_SIZE", MM_KERNELS)
     mw, mnw = _counter_sum(mm_write, "WRITE_SIZE", MM_KERNELS)
-    # one emit dispatch per model map
+    # one emit dispatch per model map unless told (the pipelined map has one
+    # per source batch)
     _, calls = _counter_sum(mm_write, "WRITE_SIZE", r"emit_s")
+    if maps > 0:
+        calls = maps
     fetch_b = mf * 1024 / calls / f_fetch
     write_b = mw * 1024 / calls / f_write
     per_kernel = {}
-    for name, rx in (("rowweight", "rowweight_kernel"), ("surrogate", "sur_rows_kernel"),
+    for name, rx in (("rowweight", "rowweight_kernel|rowweight_frames_kernel"),
+                     ("surrogate", "sur_rows_kernel"), ("wide", "wide_emit_kernel"),
                      ("near_rows", "near_rows_kernel"),
                      ("hist_factor", "hist_integral_kernel|factor_kernel"),
                      ("emit", "emit_s"), ("near_emit", "near_emit_kernel")):
@@ -247,6 +253,8 @@ def main(argv=None) -> None:
         p.add_argument("--" + k, required=True)
     p.add_argument("--copy-mib", type=int, required=True)
     p.add_argument("--cmd", required=True)
+    p.add_argument("--maps", type=int, default=0,
+                   help="model maps in the profiled run (0: one per emit dispatch)")
     p.add_argument("--kernel-only", default="",
                    help="report this kernel alone (e.g. emit) as traffic_bytes_per_call")
     p = sub.add_parser("mfma")
@@ -271,7 +279,8 @@ def main(argv=None) -> None:
     elif a.what == "valu":
         print(json.dumps(valu_json(a.run_dir, re.escape(a.kernel), a.cmd), indent=1))
     else:
-        t = traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write, a.copy_mib, a.cmd)
+        t = traffic_json(a.mm_fetch, a.mm_write, a.copy_fetch, a.copy_write, a.copy_mib, a.cmd,
+                         a.maps)
         if a.kernel_only:
             k = t["per_kernel"][a.kernel_only]
             t = {"command": a.cmd, "kernel": a.kernel_only, "fetch_factor": t["fetch_factor"],
