@@ -254,6 +254,15 @@ int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
         ctx->tune_rw_frames = value;
         return XTS_OK;
     }
+    if (strcmp(key, "emit_wide_apart") == 0) {
+        XTS_CHECK(value == 0 || value == 1, "emit_wide_apart is 0 or 1");
+        ctx->tune_emit_wide_apart = value;
+        return XTS_OK;
+    }
+    if (strcmp(key, "near_emit_blocks") == 0) {
+        ctx->tune_near_emit_gx = value;
+        return XTS_OK;
+    }
     if (strcmp(key, "rowweight_legacy") == 0) {
         XTS_CHECK(value == 0 || value == 1, "rowweight_legacy is 0 or 1");
         ctx->tune_rw_legacy = value;

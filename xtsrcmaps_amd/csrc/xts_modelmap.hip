@@ -2712,6 +2712,13 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         return std::min<int64_t>(std::max<int64_t>(1, 2 * want_blocks / c.S),
                                  div_up(plane, kNearPix));
     };
+    // the near pixels' map values: blocks per source, each staging the
+    // source's table once (2: 0.38 ms per map at 2,000 sources on 400x400,
+    // against 0.62 for a grid as wide as the near rows')
+    auto near_emit_gx = [&](const CubArgs& c) {
+        const int64_t want = ctx->tune_near_emit_gx > 0 ? ctx->tune_near_emit_gx : 2;
+        return std::min<int64_t>(want, near_gx(c));
+    };
     // batch b's row weights and near rows on R
     auto rows = [&](int64_t b) -> int {
         const CubArgs c = batch(b);
@@ -2750,7 +2757,13 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         }
         // energies per emit block: 40 (66 KB of LDS for the table) or 20
         // (34 KB: two emit blocks fit beside the row-weight blocks)
-        for (const int mode : {kEmitFar, kEmitWide}) {
+        // the wide pixels as extra emit blocks (124 VGPRs: two row-weight
+        // waves of 96 and two emit waves still fit a SIMD), or (tuning
+        // emit_wide_apart) a launch of their own after a far-pixel emit of 70
+        // VGPRs: the same time per step
+        const bool apart = ctx->tune_emit_wide_apart != 0;
+        for (const int mode : {apart ? kEmitFar : kEmitWithWide, kEmitWide}) {
+            if (mode == kEmitWide && !apart) break;
             TimedRegion tr(ctx, mode == kEmitWide ? "wide" : "emit");
             if (ctx->tune_emit_ec == 20) {
                 const dim3 grid((unsigned)c.emit_gx, (unsigned)c.S, (unsigned)div_up(E, 20));
@@ -2763,7 +2776,7 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         }
         {
             TimedRegion tr(ctx, "near_emit");
-            XTS_TRY((near_emit<float, 40>(M, c, near_gx(c), true)));
+            XTS_TRY((near_emit<float, 40>(M, c, near_emit_gx(c), true)));
         }
         XTS_HIP(hipEventRecord(evM[b], M));
         return XTS_OK;
