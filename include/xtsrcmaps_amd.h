@@ -409,6 +409,74 @@ int xts_pix2sph(xts_ctx* ctx, const xts_wcs* wcs, const double* pix /*[n,2]*/,
 int xts_sph2pix(xts_ctx* ctx, const xts_wcs* wcs, const double* sph, int64_t n,
                 double* pix);
 
+/* ------------------------------------------------------ diffuse sources */
+/* The diffuse-source model map: ModelMap::diffuse_src_model_map_wcs
+ * (model_map/model_map.diffuse_src_model_map_wcs.cxx:8-97; the reference
+ * declares it but does not compile it, model_map/model_map.hxx:30 and
+ * model_map/CMakeLists.txt:8-15).  A sky cube (the galactic diffuse
+ * MapCube, fits::read_allsky_cropped's input) is cropped to the box of the
+ * model map (fits/fits.read_allsky_cropped.cpp:150-216), resampled by
+ * corner-aligned trilinear interpolation onto a grid of target_pix_size
+ * pixels (sky_geom/sky_geom.hxx:350-380, math/trilerp.hpp:9-56) and across
+ * energy planes, convolved per energy with the PSF kernel of the diffuse
+ * source (psf/psf.make_kernel.cpp:8-90) through FFTs
+ * (model_map/mm_fft/fft.convolve_map_psf.cpp:8-39), sampled bilinearly at
+ * the model map's pixel centres and scaled by pixel solid angle and
+ * exposure (diffuse_src_model_map_wcs.cxx:78-84). */
+
+/* The crop, grid, kernel and transform sizes of a diffuse map. */
+typedef struct {
+    int64_t crop_a1, crop_a2;   /* first cube pixel of the crop (1-based; a1
+                                   may lie outside 1..naxis1 when wrapped) */
+    int64_t crop_n1, crop_n2;   /* crop size in cube pixels */
+    int64_t m1, m2;             /* resampled grid size */
+    double step1, step2;        /* cube pixels per resampled pixel */
+    xts_wcs grid;               /* WCS of the resampled grid */
+    int64_t wrap;               /* 1: all-sky CAR cube, longitude wraps */
+    int64_t nk;                 /* kernel size (odd, min(m1, m2)) */
+    double kc1, kc2;            /* kernel centre on the grid (1-based pixels):
+                                   the model map's reference direction */
+    int64_t fft_p, fft_q;       /* transform lengths along axis 1 and 2 */
+} xts_diffuse_geom;
+
+/* Host only: the geometry xts_diffuse_src_model_map_wcs_dev uses.
+ * margin_deg widens the crop on every side (0: the map's own box plus one
+ * pixel). */
+int xts_diffuse_geometry(const xts_wcs* roi, const xts_wcs* cube, double target_pix_size,
+                         double margin_deg, xts_diffuse_geom* out);
+
+/* Device pointers.  cube: FP32 [n_planes, cube.naxis2, cube.naxis1] (FITS
+ * order, axis 1 fastest); upsf: the diffuse source's PSF table [401, E]
+ * (uPsf[0] of the reference's [S, 401, E]); expo: [E]; out: [E, H, W] with
+ * H = roi.naxis1, W = roi.naxis2 (the model maps' layout), double, or float
+ * with XTS_OUT_F32.  A model map in the celestial frame over a galactic cube
+ * (or the reverse) converts coordinates through the J2000 galactic frame;
+ * pixels whose centres fall outside the cube are 0. */
+int xts_diffuse_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* roi, const xts_wcs* cube_wcs,
+                                      const float* cube, int64_t n_planes, const double* upsf,
+                                      const double* expo, int64_t E, double target_pix_size,
+                                      double margin_deg, void* out, int32_t flags);
+/* The same with host pointers; out double. */
+int xts_diffuse_src_model_map_wcs(xts_ctx* ctx, const xts_wcs* roi, const xts_wcs* cube_wcs,
+                                  const float* cube, int64_t n_planes, const double* upsf,
+                                  const double* expo, int64_t E, double target_pix_size,
+                                  double margin_deg, double* out);
+
+/* Psf::make_kernel (psf/psf.make_kernel.cpp:8-90), host pointers: out
+ * [E, nk, nk], nk odd; kernel pixel (h, w) is the grid's pixel (kc1 + h - c,
+ * kc2 + w - c), c = (nk - 1) / 2, valued by the table at its separation from
+ * the grid direction at (kc1, kc2) and normalised to sum 1 per energy.  The
+ * reference's call is the case of a grid whose CRPIX is (c + 1, c + 1) and
+ * (kc1, kc2) = CRPIX. */
+int xts_psf_make_kernel(xts_ctx* ctx, int64_t nk, const xts_wcs* grid, double kc1, double kc2,
+                        const double* upsf, int64_t E, double* out);
+
+/* FFT::convolve_map_psf (model_map/mm_fft/fft.convolve_map_psf.cpp:8-39),
+ * host pointers: image [C, H, W], kernel [C, KH, KW] -> out [C, H + KH - 1,
+ * W + KW - 1], the full linear convolution of each plane with its kernel. */
+int xts_fft_convolve_map_psf(xts_ctx* ctx, const double* image, int64_t C, int64_t H, int64_t W,
+                             const double* kernel, int64_t KH, int64_t KW, double* out);
+
 /* ------------------------------------------------------ learned surrogate */
 /* The surrogate of the far-pixel cubature: a 4 -> 64 -> 64 -> 64 -> 1 SiLU
  * network that predicts, from four features of a pixel's point separations

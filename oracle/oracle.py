@@ -380,3 +380,91 @@ def point_src_model_map_wcs(Nh, Nw, w, src_sph, expo, upsf, parint, rule,
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+# ---------------------------------------------------------------- diffuse sources
+class DiffuseGeom(C.Structure):
+    _fields_ = [("a1", C.c_long), ("a2", C.c_long), ("n1", C.c_long), ("n2", C.c_long),
+                ("m1", C.c_long), ("m2", C.c_long), ("step1", C.c_double),
+                ("step2", C.c_double), ("grid", Wcs), ("wrap", C.c_int), ("frame", C.c_int),
+                ("nk", C.c_long), ("kc1", C.c_double), ("kc2", C.c_double),
+                ("fft_p", C.c_long), ("fft_q", C.c_long), ("period", C.c_double),
+                ("vlo1", C.c_double), ("vhi1", C.c_double), ("vlo2", C.c_double),
+                ("vhi2", C.c_double)]
+
+
+def cel2gal(ra, dec):
+    """J2000 celestial -> galactic (degrees), or_cel2gal."""
+    l, b = C.c_double(), C.c_double()
+    lib().or_cel2gal(C.c_double(ra), C.c_double(dec), C.byref(l), C.byref(b))
+    return l.value, b.value
+
+
+def gal2cel(l, b):
+    ra, dec = C.c_double(), C.c_double()
+    lib().or_gal2cel(C.c_double(l), C.c_double(b), C.byref(ra), C.byref(dec))
+    return ra.value, dec.value
+
+
+def diffuse_geometry(roi, roi_gal, cube, cube_gal, target, margin=0.0) -> DiffuseGeom:
+    """or_diffuse_geometry: crop box, resampled grid, kernel and transform sizes."""
+    g = DiffuseGeom()
+    rc = lib().or_diffuse_geometry(C.byref(roi), C.c_int(int(roi_gal)), C.byref(cube),
+                                   C.c_int(int(cube_gal)), C.c_double(target),
+                                   C.c_double(margin), C.byref(g))
+    if rc:
+        raise ValueError(f"or_diffuse_geometry failed ({rc})")
+    return g
+
+
+def diffuse_resample(cube_data, cube, g, E):
+    """or_diffuse_resample (math/trilerp.hpp:9-56 over the crop): [E, m1, m2]."""
+    d = _c(cube_data, np.float32)
+    out = np.zeros((E, g.m1, g.m2))
+    lib().or_diffuse_resample(_p(d, _f), _l(cube.naxis1), _l(cube.naxis2), _l(d.shape[0]),
+                              C.byref(g), _l(E), _p(out))
+    return out
+
+
+def psf_make_kernel(nk, grid, kc1, kc2, upsf):
+    """or_psf_make_kernel (psf/psf.make_kernel.cpp:8-90): [E, nk, nk]."""
+    u = _c(upsf)
+    E = u.shape[1]
+    out = np.zeros((E, nk, nk))
+    lib().or_psf_make_kernel(_l(nk), C.byref(grid), C.c_double(kc1), C.c_double(kc2), _p(u),
+                             _l(E), _p(out))
+    return out
+
+
+def convolve_same(img, ker, c=None):
+    """or_convolve_same: direct convolution, rows/cols c .. c + m - 1 of the full one."""
+    im, k = _c(img), _c(ker)
+    E, m1, m2 = im.shape
+    nk = k.shape[1]
+    c = (nk - 1) // 2 if c is None else c
+    out = np.zeros_like(im)
+    lib().or_convolve_same(_p(im), _l(E), _l(m1), _l(m2), _p(k), _l(nk), _l(c), _p(out))
+    return out
+
+
+def diffuse_sample(conv, g, roi, omega, expo):
+    """or_diffuse_sample: bilinear samples x solid angle x exposure, [E, H, W]."""
+    cv, om, ex = _c(conv), _c(omega), _c(expo)
+    E = ex.shape[0]
+    out = np.zeros((E, roi.naxis1, roi.naxis2))
+    lib().or_diffuse_sample(_p(cv), C.byref(g), C.byref(roi), _p(om), _p(ex), _l(E), _p(out))
+    return out
+
+
+def diffuse_src_model_map_wcs(roi, roi_gal, cube, cube_gal, cube_data, upsf, expo, target,
+                              margin=0.0):
+    """or_diffuse_src_model_map_wcs -> [E, roi.naxis1, roi.naxis2]."""
+    d, u, ex = _c(cube_data, np.float32), _c(upsf), _c(expo)
+    E = ex.shape[0]
+    out = np.zeros((E, roi.naxis1, roi.naxis2))
+    rc = lib().or_diffuse_src_model_map_wcs(
+        C.byref(roi), C.c_int(int(roi_gal)), C.byref(cube), C.c_int(int(cube_gal)), _p(d, _f),
+        _l(d.shape[0]), _p(u), _p(ex), _l(E), C.c_double(target), C.c_double(margin), _p(out))
+    if rc:
+        raise ValueError(f"or_diffuse_src_model_map_wcs failed ({rc})")
+    return out

@@ -1012,3 +1012,341 @@ void or_point_src_model_map_wcs(long Nh, long Nw, const or_wcs* wc,
     free(pw);
     free(om);
 }
+
+/* ======================================================================
+ * Diffuse-source model map (model_map/model_map.diffuse_src_model_map_wcs.cxx:8-97,
+ * a function the reference declares but does not compile).  The steps it
+ * names, with the choices where its sources are unfinished: the crop box
+ * bounds the cube coordinates of the model map's whole border (the
+ * reference's fits/fits.read_allsky_cropped.cpp:170-200 takes two corners),
+ * unwrapped across the seam of an all-sky CAR cube; the resampled grid is
+ * trilerp's corner-aligned grid (math/trilerp.hpp:22-25); the kernel is
+ * centred on the model map's reference direction and blends table nodes as
+ * mm_cubature/convolve_pixel_psf_logsep.hpp:45-55,72-73 does (the
+ * reference's psf/psf.make_kernel.cpp:38-40 swaps the two weights); the
+ * convolution is per energy plane; the convolved grid is sampled
+ * bilinearly at the model map's pixel centres, then scaled by solid angle
+ * and exposure (diffuse_src_model_map_wcs.cxx:78-84).
+ * ====================================================================== */
+
+/* J2000 galactic frame from the north galactic pole (192.85948, 27.12825)
+ * and the galactic longitude of the north celestial pole, 122.93192, as
+ * explicit axes: z the pole, x and y in the galactic plane placed so that
+ * the celestial pole's projection lies at longitude 122.93192. */
+static void gal_axes(double x[3], double y[3], double z[3]) {
+    const double ra = 192.85948 * D2R, dec = 27.12825 * D2R, l = 122.93192 * D2R;
+    z[0] = cos(dec) * cos(ra); z[1] = cos(dec) * sin(ra); z[2] = sin(dec);
+    const double p[3] = {0.0, 0.0, 1.0};
+    const double pz = p[0] * z[0] + p[1] * z[1] + p[2] * z[2];
+    double u[3] = {p[0] - pz * z[0], p[1] - pz * z[1], p[2] - pz * z[2]};
+    const double nu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int k = 0; k < 3; ++k) u[k] /= nu;
+    const double w[3] = {z[1] * u[2] - z[2] * u[1], z[2] * u[0] - z[0] * u[2],
+                         z[0] * u[1] - z[1] * u[0]};
+    for (int k = 0; k < 3; ++k) {
+        x[k] = cos(l) * u[k] - sin(l) * w[k];
+        y[k] = sin(l) * u[k] + cos(l) * w[k];
+    }
+}
+
+static void vec_sph(const double v[3], double* lon, double* lat) {
+    double a = atan2(v[1], v[0]) * R2D;
+    if (a < 0.0) a += 360.0;
+    if (a >= 360.0) a -= 360.0;
+    double z = v[2] > 1.0 ? 1.0 : (v[2] < -1.0 ? -1.0 : v[2]);
+    *lon = a;
+    *lat = asin(z) * R2D;
+}
+
+void or_cel2gal(double ra, double dec, double* l, double* b) {
+    double x[3], y[3], z[3], v[3];
+    gal_axes(x, y, z);
+    or_sph2dir(ra, dec, v);
+    const double g[3] = {v[0] * x[0] + v[1] * x[1] + v[2] * x[2],
+                         v[0] * y[0] + v[1] * y[1] + v[2] * y[2],
+                         v[0] * z[0] + v[1] * z[1] + v[2] * z[2]};
+    vec_sph(g, l, b);
+}
+
+void or_gal2cel(double l, double b, double* ra, double* dec) {
+    double x[3], y[3], z[3], g[3];
+    gal_axes(x, y, z);
+    or_sph2dir(l, b, g);
+    const double v[3] = {g[0] * x[0] + g[1] * y[0] + g[2] * z[0],
+                         g[0] * x[1] + g[1] * y[1] + g[2] * z[1],
+                         g[0] * x[2] + g[1] * y[2] + g[2] * z[2]};
+    vec_sph(v, ra, dec);
+}
+
+static void frame_conv(int frame, double lon, double lat, double* olon, double* olat) {
+    if (frame == 1) or_cel2gal(lon, lat, olon, olat);
+    else if (frame == 2) or_gal2cel(lon, lat, olon, olat);
+    else { *olon = lon; *olat = lat; }
+}
+
+static int roi2cube(const or_wcs* roi, const or_wcs* cube, int frame, double p1, double p2,
+                    double* q1, double* q2) {
+    double lon, lat, l2, b2;
+    if (or_pix2sph(roi, p1, p2, &lon, &lat) != 0) return -1;
+    frame_conv(frame, lon, lat, &l2, &b2);
+    or_sph2pix(cube, l2, b2, q1, q2);
+    return 0;
+}
+
+static long fft_len(long need) {
+    for (long n = need < 1 ? 1 : need; n <= 4096; ++n) {
+        long m = n;
+        while (m % 2 == 0) m /= 2;
+        while (m % 3 == 0) m /= 3;
+        while (m % 5 == 0) m /= 5;
+        if (m == 1) return n;
+    }
+    return 0;
+}
+
+int or_diffuse_geometry(const or_wcs* roi, int roi_gal, const or_wcs* cube, int cube_gal,
+                        double target, double margin, or_diffuse_geom* g) {
+    memset(g, 0, sizeof(*g));
+    g->frame = roi_gal == cube_gal ? 0 : (cube_gal ? 1 : 2);
+    const int back = g->frame == 0 ? 0 : 3 - g->frame;
+    const long N1 = cube->naxis1, N2 = cube->naxis2, H = roi->naxis1, W = roi->naxis2;
+    const double c1 = fabs(cube->cdelt1), c2 = fabs(cube->cdelt2);
+    const double turn = cube->proj == OR_PROJ_CAR && cube->crota2 == 0.0 ? 360.0 / c1 : 0.0;
+    const int wrap = turn > 0.0 && fabs(N1 * c1 - 360.0) <= 1e-6 * 360.0;
+    double x0, y0;
+    if (roi2cube(roi, cube, g->frame, 0.5 * (H + 1), 0.5 * (W + 1), &x0, &y0)) return -4;
+    double lo1 = x0, hi1 = x0, lo2 = y0, hi2 = y0;
+    for (int side = 0; side < 2; ++side) {
+        const long n = side == 0 ? H : W;
+        for (long k = 0; k <= 2 * (n - 1); ++k) {
+            for (int edge = 0; edge < 2; ++edge) {
+                double p1, p2, q1, q2;
+                if (side == 0) { p1 = 1.0 + 0.5 * k; p2 = edge ? (double)W : 1.0; }
+                else { p2 = 1.0 + 0.5 * k; p1 = edge ? (double)H : 1.0; }
+                if (roi2cube(roi, cube, g->frame, p1, p2, &q1, &q2)) continue;
+                if (turn > 0.0) q1 += turn * nearbyint((x0 - q1) / turn);
+                if (q1 < lo1) lo1 = q1;
+                if (q1 > hi1) hi1 = q1;
+                if (q2 < lo2) lo2 = q2;
+                if (q2 > hi2) hi2 = q2;
+            }
+        }
+    }
+    int pole = 0;
+    for (int k = 0; k < 2 && turn > 0.0; ++k) {
+        double lon, lat, p1, p2, q1, q2;
+        frame_conv(back, 0.0, k == 0 ? 90.0 : -90.0, &lon, &lat);
+        or_sph2pix(roi, lon, lat, &p1, &p2);
+        if (p1 >= 0.5 && p1 <= H + 0.5 && p2 >= 0.5 && p2 <= W + 0.5) {
+            pole = 1;
+            or_sph2pix(cube, 0.0, k == 0 ? 90.0 : -90.0, &q1, &q2);
+            if (q2 < lo2) lo2 = q2;
+            if (q2 > hi2) hi2 = q2;
+        }
+    }
+    const long mg1 = 1 + (long)ceil(margin / c1), mg2 = 1 + (long)ceil(margin / c2);
+    long a1 = (long)floor(lo1) - mg1, b1 = (long)ceil(hi1) + mg1;
+    long a2 = (long)floor(lo2) - mg2, b2 = (long)ceil(hi2) + mg2;
+    if (pole) {
+        a1 = wrap ? (long)floor(x0) - N1 / 2 : 1;
+        b1 = wrap ? a1 + N1 - 1 : N1;
+    }
+    if (wrap) {
+        if (b1 - a1 + 1 > N1) b1 = a1 + N1 - 1;
+    } else {
+        if (a1 < 1) a1 = 1;
+        if (b1 > N1) b1 = N1;
+    }
+    if (a2 < 1) a2 = 1;
+    if (b2 > N2) b2 = N2;
+    if (b1 - a1 + 1 < 2 || b2 - a2 + 1 < 2) return -1;
+    g->a1 = a1; g->a2 = a2; g->n1 = b1 - a1 + 1; g->n2 = b2 - a2 + 1;
+    /* sky_geom/sky_geom.hxx:350-372: one pixel size, the target when finer
+     * than the cube's coarser axis, naxis = round(naxis * scale) */
+    const double pixsz = target < (c1 > c2 ? c1 : c2) ? target : c1;
+    g->m1 = lround(g->n1 * (c1 / pixsz));
+    g->m2 = lround(g->n2 * (c2 / pixsz));
+    if (g->m1 < 2 || g->m2 < 2 || g->m1 > 4096 || g->m2 > 4096) return -2;
+    g->step1 = (double)(g->n1 - 1) / (double)(g->m1 - 1);
+    g->step2 = (double)(g->n2 - 1) / (double)(g->m2 - 1);
+    g->grid = *cube;
+    g->grid.naxis1 = g->m1;
+    g->grid.naxis2 = g->m2;
+    g->grid.crpix1 = (cube->crpix1 - a1) / g->step1 + 1.0;
+    g->grid.crpix2 = (cube->crpix2 - a2) / g->step2 + 1.0;
+    g->grid.cdelt1 = cube->cdelt1 * g->step1;
+    g->grid.cdelt2 = cube->cdelt2 * g->step2;
+    or_wcs_setup(&g->grid);
+    g->wrap = wrap;
+    g->period = wrap ? N1 / g->step1 : 0.0;
+    g->vlo1 = wrap ? -INFINITY : (0.5 - a1) / g->step1 + 1.0;
+    g->vhi1 = wrap ? INFINITY : (N1 + 0.5 - a1) / g->step1 + 1.0;
+    g->vlo2 = (0.5 - a2) / g->step2 + 1.0;
+    g->vhi2 = (N2 + 0.5 - a2) / g->step2 + 1.0;
+    g->nk = g->m1 < g->m2 ? g->m1 : g->m2;
+    if (g->nk % 2 == 0) g->nk -= 1;
+    double lon, lat;
+    frame_conv(g->frame, roi->crval1, roi->crval2, &lon, &lat);
+    or_sph2pix(&g->grid, lon, lat, &g->kc1, &g->kc2);
+    if (wrap) g->kc1 += g->period * nearbyint((0.5 * (g->m1 + 1) - g->kc1) / g->period);
+    const long c = (g->nk - 1) / 2;
+    g->fft_p = fft_len(g->m1 + c);
+    g->fft_q = fft_len(g->m2 + c);
+    return (g->fft_p > 0 && g->fft_q > 0) ? 0 : -3;
+}
+
+/* math/trilerp.hpp:9-56 over the crop of the cube (cube [Ec][N2][N1], FITS
+ * order): out [E][m1][m2], out[k][i][j] at crop position (i s1, j s2) and
+ * cube plane k se, s = (src - 1) / (dst - 1); crop column i is cube pixel
+ * a1 + i along axis 1, taken modulo N1 when the cube wraps. */
+void or_diffuse_resample(const float* cube, long N1, long N2, long Ec, const or_diffuse_geom* g,
+                         long E, double* out) {
+    const double se = E > 1 ? (double)(Ec - 1) / (double)(E - 1) : 0.0;
+    const double sh = (double)(g->n1 - 1) / (double)(g->m1 - 1);
+    const double sw = (double)(g->n2 - 1) / (double)(g->m2 - 1);
+#pragma omp parallel for schedule(static, 1)
+    for (long k = 0; k < E; ++k) {
+        const double e = k * se;
+        const long e0 = (long)e, e1 = e0 + 1 < Ec - 1 ? e0 + 1 : Ec - 1;
+        const double fe = e - e0, ce = 1. - fe;
+        for (long i = 0; i < g->m1; ++i) {
+            const double h = i * sh;
+            const long h0 = (long)h, h1 = h0 + 1 < g->n1 - 1 ? h0 + 1 : g->n1 - 1;
+            const double fh = h - h0, ch = 1. - fh;
+            long x0 = g->a1 + h0 - 1, x1 = g->a1 + h1 - 1;
+            if (g->wrap) { x0 = ((x0 % N1) + N1) % N1; x1 = ((x1 % N1) + N1) % N1; }
+            for (long j = 0; j < g->m2; ++j) {
+                const double w = j * sw;
+                const long w0 = (long)w, w1 = w0 + 1 < g->n2 - 1 ? w0 + 1 : g->n2 - 1;
+                const double fw = w - w0, cw = 1. - fw;
+                const long y0 = g->a2 + w0 - 1, y1 = g->a2 + w1 - 1;
+#define CV(ee, xx, yy) ((double)cube[((ee) * N2 + (yy)) * N1 + (xx)])
+                const double C00 = ch * CV(e0, x0, y0) + fh * CV(e0, x1, y0);
+                const double C01 = ch * CV(e1, x0, y0) + fh * CV(e1, x1, y0);
+                const double C10 = ch * CV(e0, x0, y1) + fh * CV(e0, x1, y1);
+                const double C11 = ch * CV(e1, x0, y1) + fh * CV(e1, x1, y1);
+#undef CV
+                const double C0 = cw * C00 + fw * C10;
+                const double C1 = cw * C01 + fw * C11;
+                out[(k * g->m1 + i) * g->m2 + j] = ce * C0 + fe * C1;
+            }
+        }
+    }
+}
+
+/* psf/psf.make_kernel.cpp:8-90: kernel pixel (h, w) at grid pixel
+ * (kc1 + h - c, kc2 + w - c), its table value at the separation from the
+ * grid direction at (kc1, kc2) (the logsep blend of one unit-weight point,
+ * or_pixel_logsep), normalised to sum 1 per energy: out [E][nk][nk]. */
+void or_psf_make_kernel(long nk, const or_wcs* grid, double kc1, double kc2,
+                        const double* upsf, long E, double* out) {
+    const long c = (nk - 1) / 2;
+    double d0[3];
+    or_pix2dir(grid, kc1, kc2, d0);
+    double* val = (double*)malloc((size_t)E * sizeof(double));
+    double* tot = (double*)calloc((size_t)E, sizeof(double));
+    for (long h = 0; h < nk; ++h) {
+        for (long w = 0; w < nk; ++w) {
+            double pw[4];
+            for (long e = 0; e < E; ++e) val[e] = 0.0;
+            /* a kernel pixel outside the grid's projection is 0 */
+            if (or_pix2dir(grid, kc1 + (double)(h - c), kc2 + (double)(w - c), pw) == 0) {
+                pw[3] = 1.0;
+                or_pixel_logsep(val, E, pw, 1, upsf, d0);
+            }
+            for (long e = 0; e < E; ++e) {
+                out[(e * nk + h) * nk + w] = val[e];
+                tot[e] += val[e];
+            }
+        }
+    }
+    for (long e = 0; e < E; ++e) {
+        const double inv = 1.0 / tot[e];
+        for (long k = 0; k < nk * nk; ++k) out[e * nk * nk + k] *= inv;
+    }
+    free(val);
+    free(tot);
+}
+
+/* Direct linear convolution per plane: out[e][i][j] = sum_{h,w}
+ * ker[e][h][w] img[e][i + c - h][j + c - w] (zero outside), i.e. rows and
+ * columns c .. c + m - 1 of the full convolution that
+ * math/convolve/fft/fft.fftw_convolve_CHW.cpp:7-90 computes by FFT. */
+void or_convolve_same(const double* img, long E, long m1, long m2, const double* ker, long nk,
+                      long c, double* out) {
+#pragma omp parallel for collapse(2) schedule(static, 4)
+    for (long e = 0; e < E; ++e) {
+        for (long i = 0; i < m1; ++i) {
+            const double* K = ker + e * nk * nk;
+            const double* I = img + e * m1 * m2;
+            for (long j = 0; j < m2; ++j) {
+                double s = 0.0;
+                for (long h = 0; h < nk; ++h) {
+                    const long r = i + c - h;
+                    if (r < 0 || r >= m1) continue;
+                    for (long w = 0; w < nk; ++w) {
+                        const long q = j + c - w;
+                        if (q < 0 || q >= m2) continue;
+                        s += K[h * nk + w] * I[r * m2 + q];
+                    }
+                }
+                out[(e * m1 + i) * m2 + j] = s;
+            }
+        }
+    }
+}
+
+/* The convolved grid [E][m1][m2] sampled bilinearly at the model map's
+ * pixel centres (clamped to the grid; 0 outside the cube), times solid
+ * angle and exposure: out [E][H][W]. */
+void or_diffuse_sample(const double* conv, const or_diffuse_geom* g, const or_wcs* roi,
+                       const double* omega, const double* expo, long E, double* out) {
+    const long H = roi->naxis1, W = roi->naxis2, m1 = g->m1, m2 = g->m2;
+    for (long h = 0; h < H; ++h) {
+        for (long w = 0; w < W; ++w) {
+            double lon = 0.0, lat = 0.0, l2, b2, v1, v2;
+            const int valid = or_pix2sph(roi, h + 1.0, w + 1.0, &lon, &lat) == 0;
+            frame_conv(g->frame, lon, lat, &l2, &b2);
+            or_sph2pix(&g->grid, l2, b2, &v1, &v2);
+            if (g->period > 0.0) v1 += g->period * nearbyint((0.5 * (m1 + 1) - v1) / g->period);
+            const int inside = valid && v1 >= g->vlo1 && v1 <= g->vhi1 && v2 >= g->vlo2 && v2 <= g->vhi2;
+            double x = v1 - 1.0, y = v2 - 1.0;
+            x = x < 0.0 ? 0.0 : (x > m1 - 1 ? (double)(m1 - 1) : x);
+            y = y < 0.0 ? 0.0 : (y > m2 - 1 ? (double)(m2 - 1) : y);
+            long i0 = (long)x, j0 = (long)y;
+            if (i0 > m1 - 2) i0 = m1 - 2;
+            if (j0 > m2 - 2) j0 = m2 - 2;
+            const double fx = x - i0, fy = y - j0;
+            for (long e = 0; e < E; ++e) {
+                const double* c = conv + (e * m1 + i0) * m2 + j0;
+                const double v = (1.0 - fx) * (1.0 - fy) * c[0] + (1.0 - fx) * fy * c[1] +
+                                 fx * (1.0 - fy) * c[m2] + fx * fy * c[m2 + 1];
+                out[(e * H + h) * W + w] = inside ? v * omega[h * W + w] * expo[e] : 0.0;
+            }
+        }
+    }
+}
+
+int or_diffuse_src_model_map_wcs(const or_wcs* roi, int roi_gal, const or_wcs* cube,
+                                 int cube_gal, const float* data, long Ec, const double* upsf,
+                                 const double* expo, long E, double target, double margin,
+                                 double* out) {
+    or_diffuse_geom g;
+    const int rc = or_diffuse_geometry(roi, roi_gal, cube, cube_gal, target, margin, &g);
+    if (rc) return rc;
+    const size_t plane = (size_t)g.m1 * g.m2;
+    double* img = (double*)malloc((size_t)E * plane * sizeof(double));
+    double* ker = (double*)malloc((size_t)E * g.nk * g.nk * sizeof(double));
+    double* conv = (double*)malloc((size_t)E * plane * sizeof(double));
+    double* om = (double*)malloc((size_t)roi->naxis1 * roi->naxis2 * sizeof(double));
+    or_diffuse_resample(data, cube->naxis1, cube->naxis2, Ec, &g, E, img);
+    or_psf_make_kernel(g.nk, &g.grid, g.kc1, g.kc2, upsf, E, ker);
+    or_convolve_same(img, E, g.m1, g.m2, ker, g.nk, (g.nk - 1) / 2, conv);
+    or_solid_angle(roi->naxis1, roi->naxis2, roi, om);
+    or_diffuse_sample(conv, &g, roi, om, expo, E, out);
+    free(img);
+    free(ker);
+    free(conv);
+    free(om);
+    return 0;
+}

@@ -173,6 +173,37 @@ void or_point_src_model_map_wcs(long Nh, long Nw, const or_wcs* w,
                                 double* inv_mapinteg, double* cf);
 int  or_num_threads(void);
 
+/* diffuse-source model map (model_map/model_map.diffuse_src_model_map_wcs.cxx:8-97
+ * and the steps it names; see the note above or_cel2gal in xts_oracle.c) */
+typedef struct {
+    long   a1, a2, n1, n2;     /* crop start (1-based cube pixels) and size */
+    long   m1, m2;             /* resampled grid */
+    double step1, step2;       /* cube pixels per grid pixel */
+    or_wcs grid;               /* WCS of the resampled grid */
+    int    wrap, frame;        /* longitude wrap; 0 same frame, 1 cel->gal, 2 gal->cel */
+    long   nk;                 /* kernel size */
+    double kc1, kc2;           /* kernel centre on the grid */
+    long   fft_p, fft_q;       /* transform lengths */
+    double period;             /* grid pixels per turn (wrap) */
+    double vlo1, vhi1, vlo2, vhi2;  /* grid coordinates inside the cube */
+} or_diffuse_geom;
+void or_cel2gal(double ra, double dec, double* l, double* b);
+void or_gal2cel(double l, double b, double* ra, double* dec);
+int  or_diffuse_geometry(const or_wcs* roi, int roi_gal, const or_wcs* cube, int cube_gal,
+                         double target, double margin, or_diffuse_geom* g);
+void or_diffuse_resample(const float* cube, long N1, long N2, long Ec,
+                         const or_diffuse_geom* g, long E, double* out);
+void or_psf_make_kernel(long nk, const or_wcs* grid, double kc1, double kc2,
+                        const double* upsf, long E, double* out);
+void or_convolve_same(const double* img, long E, long m1, long m2, const double* ker,
+                      long nk, long c, double* out);
+void or_diffuse_sample(const double* conv, const or_diffuse_geom* g, const or_wcs* roi,
+                       const double* omega, const double* expo, long E, double* out);
+int  or_diffuse_src_model_map_wcs(const or_wcs* roi, int roi_gal, const or_wcs* cube,
+                                  int cube_gal, const float* data, long Ec,
+                                  const double* upsf, const double* expo, long E,
+                                  double target, double margin, double* out);
+
 #ifdef __cplusplus
 }
 #endif

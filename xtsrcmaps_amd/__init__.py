@@ -56,6 +56,21 @@ def livetime_efficiency_factors(logEs, p0, p1) -> np.ndarray:
     return out
 
 
+def diffuse_geometry(roi: Wcs, cube_wcs: Wcs, target_pix_size: float,
+                     margin_deg: float = 0.0) -> dict:
+    """The crop, resampled grid, kernel and transform sizes of a diffuse map
+    (xts_diffuse_geometry; host only)."""
+    g = capi.DiffuseGeom()
+    capi.check(capi.lib().xts_diffuse_geometry(C.byref(roi), C.byref(cube_wcs),
+                                               float(target_pix_size), float(margin_deg),
+                                               C.byref(g)))
+    d = {n: getattr(g, n) for n, _ in capi.DiffuseGeom._fields_ if n != "grid"}
+    gw = Wcs()
+    C.memmove(C.byref(gw), C.byref(g.grid), C.sizeof(Wcs))
+    d["grid"] = gw
+    return d
+
+
 def write_src_model(path, model_map, src_names, layout: int = LAYOUT_SHWE,
                     float32: bool = False, wcs: Wcs | None = None) -> None:
     """fits::write_src_model (fits/fits.write_src_model.cxx:10-66): one IMAGE
@@ -325,6 +340,61 @@ class Context:
         """Math::solid_angle (math/solid_angle.hpp:42-79) -> [H, W] sr."""
         out = np.zeros((wcs.naxis1, wcs.naxis2))
         capi.check(capi.lib().xts_solid_angle(self._h, C.byref(wcs), capi.dptr(out)))
+        return out
+
+    # ------------------------------------------------------ diffuse sources
+    def diffuse_src_model_map_wcs(self, roi: Wcs, cube_wcs: Wcs, cube, upsf, exposure,
+                                  target_pix_size, margin_deg=0.0):
+        """ModelMap::diffuse_src_model_map_wcs
+        (model_map/model_map.diffuse_src_model_map_wcs.cxx:8-97) on host
+        arrays: cube [planes, cube.naxis2, cube.naxis1] float32 (FITS order),
+        upsf [401, E], exposure [E] -> [E, roi.naxis1, roi.naxis2] float64."""
+        cb = np.ascontiguousarray(cube, dtype=np.float32)
+        u, ex = capi.f64(upsf), capi.f64(exposure)
+        if cb.ndim != 3 or cb.shape[1:] != (cube_wcs.naxis2, cube_wcs.naxis1):
+            raise capi.XtsError(1, f"cube shape {cb.shape} does not match its WCS")
+        E = ex.shape[0]
+        if u.shape != (401, E):
+            raise capi.XtsError(1, f"upsf shape {u.shape}, expected (401, {E})")
+        out = np.zeros((E, roi.naxis1, roi.naxis2))
+        capi.check(capi.lib().xts_diffuse_src_model_map_wcs(
+            self._h, C.byref(roi), C.byref(cube_wcs), capi.fptr(cb), cb.shape[0], capi.dptr(u),
+            capi.dptr(ex), E, float(target_pix_size), float(margin_deg), capi.dptr(out)))
+        return out
+
+    def diffuse_src_model_map_wcs_dev(self, roi: Wcs, cube_wcs: Wcs, cube, upsf, exposure,
+                                      target_pix_size, out, margin_deg=0.0, flags=0):
+        """The diffuse model map on torch CUDA tensors (async on the context
+        stream): out [E, roi.naxis1, roi.naxis2] float64, or float32 with
+        OUT_F32."""
+        E = exposure.shape[0]
+        dt = "float32" if flags & OUT_F32 else "float64"
+        p = capi.devs([("cube", cube, "float32", (cube.shape[0], cube_wcs.naxis2, cube_wcs.naxis1)),
+                       ("upsf", upsf, "float64", (401, E)), ("exposure", exposure, "float64", (E,)),
+                       ("out", out, dt, (E, roi.naxis1, roi.naxis2))])
+        capi.check(capi.lib().xts_diffuse_src_model_map_wcs_dev(
+            self._h, C.byref(roi), C.byref(cube_wcs), p[0], cube.shape[0], p[1], p[2], E,
+            float(target_pix_size), float(margin_deg), p[3], int(flags)))
+
+    def psf_make_kernel(self, nk, grid: Wcs, kc1, kc2, upsf):
+        """Psf::make_kernel (psf/psf.make_kernel.cpp:8-90): upsf [401, E] ->
+        [E, nk, nk], centred at grid pixel (kc1, kc2), sum 1 per energy."""
+        u = capi.f64(upsf)
+        E = u.shape[1]
+        out = np.zeros((E, nk, nk))
+        capi.check(capi.lib().xts_psf_make_kernel(self._h, int(nk), C.byref(grid), float(kc1),
+                                                  float(kc2), capi.dptr(u), E, capi.dptr(out)))
+        return out
+
+    def fft_convolve_map_psf(self, image, kernel):
+        """FFT::convolve_map_psf (model_map/mm_fft/fft.convolve_map_psf.cpp:8-39):
+        image [C, H, W], kernel [C, KH, KW] -> [C, H + KH - 1, W + KW - 1]."""
+        im, k = capi.f64(image), capi.f64(kernel)
+        Cn, H, W = im.shape
+        _, KH, KW = k.shape
+        out = np.zeros((Cn, H + KH - 1, W + KW - 1))
+        capi.check(capi.lib().xts_fft_convolve_map_psf(self._h, capi.dptr(im), Cn, H, W,
+                                                       capi.dptr(k), KH, KW, capi.dptr(out)))
         return out
 
     def psf_boundary_radius(self, wcs: Wcs, src_sph):

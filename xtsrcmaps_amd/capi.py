@@ -66,6 +66,15 @@ class Footprint(C.Structure):
                 ("offset", C.c_int64)]
 
 
+class DiffuseGeom(C.Structure):
+    """xts_diffuse_geom: crop, resampled grid, kernel and transform sizes."""
+    _fields_ = [("crop_a1", C.c_int64), ("crop_a2", C.c_int64), ("crop_n1", C.c_int64),
+                ("crop_n2", C.c_int64), ("m1", C.c_int64), ("m2", C.c_int64),
+                ("step1", C.c_double), ("step2", C.c_double), ("grid", Wcs),
+                ("wrap", C.c_int64), ("nk", C.c_int64), ("kc1", C.c_double),
+                ("kc2", C.c_double), ("fft_p", C.c_int64), ("fft_q", C.c_int64)]
+
+
 class AeffGrid(C.Structure):
     _fields_ = [("params", _d), ("cosths", _d), ("logEs", _d), ("Mc", C.c_int64),
                 ("Me", C.c_int64), ("min_costheta", C.c_double)]
@@ -146,6 +155,14 @@ _SIGS = {
     "xts_read_irf_pars": [C.c_char_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                           _i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                           C.POINTER(C.c_float)],
+    "xts_diffuse_geometry": [C.POINTER(Wcs), C.POINTER(Wcs), C.c_double, C.c_double,
+                             C.POINTER(DiffuseGeom)],
+    "xts_diffuse_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), C.POINTER(Wcs), _vp, _i64, _vp,
+                                          _vp, _i64, C.c_double, C.c_double, _vp, C.c_int32],
+    "xts_diffuse_src_model_map_wcs": [_vp, C.POINTER(Wcs), C.POINTER(Wcs), _f, _i64, _d, _d,
+                                      _i64, C.c_double, C.c_double, _d],
+    "xts_psf_make_kernel": [_vp, _i64, C.POINTER(Wcs), C.c_double, C.c_double, _d, _i64, _d],
+    "xts_fft_convolve_map_psf": [_vp, _d, _i64, _i64, _i64, _d, _i64, _i64, _d],
     "xts_read_expcube": [C.c_char_p, C.c_char_p, C.POINTER(ExpcubeInfo), C.POINTER(C.c_float),
                          C.POINTER(C.c_float), C.POINTER(C.c_float)],
 }

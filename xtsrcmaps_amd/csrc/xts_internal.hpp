@@ -65,7 +65,7 @@ enum SlotId {
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
     kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotBilerp,
     kSlotMlpState, kSlotMlpPacked, kSlotSurrogate, kSlotSurList, kSlotSumFac,
-    kSlotSurrogate8, kSlotCount
+    kSlotSurrogate8, kSlotDiffSep, kSlotDiffSpec, kSlotDiffKspec, kSlotDiffConv, kSlotCount
 };
 
 }  // namespace xts
@@ -193,6 +193,9 @@ int model_map_dev_call(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sph, 
                        const double* expo, const double* upsf, const double* parint, int64_t E,
                        const double* rule, int64_t N, int32_t flags, void* out, int phase);
 int noalloc_scratch(int device, void** ptr, size_t* bytes);
+// Per-pixel solid angle [naxis1 * naxis2] of an image on the device, in the
+// geometry slot (xts_modelmap.hip).
+int image_solid_angle(xts_ctx* ctx, const xts_wcs* wcs, const double** omega);
 // Raise a kernel's dynamic LDS limit to at least `bytes` (once per kernel
 // and size: the attribute belongs to the function).
 int allow_lds(const void* fn, size_t bytes);

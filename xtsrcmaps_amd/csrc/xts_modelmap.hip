@@ -2458,6 +2458,14 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
     return XTS_OK;
 }
 
+int image_solid_angle(xts_ctx* ctx, const xts_wcs* wcs, const double** omega) {
+    XTS_TRY(check_wcs(wcs));
+    ImageGeom ig;
+    XTS_TRY(build_geometry(ctx, wcs, nullptr, 0, &ig));
+    *omega = ig.omega;
+    return XTS_OK;
+}
+
 // The PSF table in the cubature's precision, energies padded to Ep (a
 // multiple of 4); the same threads clear the nzero accumulators of the
 // model map.

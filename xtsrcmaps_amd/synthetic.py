@@ -234,3 +234,67 @@ def make_batch_inputs(cfg: BatchConfig) -> BatchInputs:
                               seed=cfg.seed, src_mode="uniform"))
     phys = Inputs(**{**base.__dict__, "src_sph": np.concatenate(srcs)})
     return BatchInputs(cfg, wcs, np.array(begin, np.int64), phys)
+
+
+# ------------------------------------------------------------- diffuse sources
+@dataclass
+class DiffuseConfig:
+    """A diffuse-source model map: an all-sky galactic CAR cube of the
+    galactic diffuse MapCube's shape (2880 x 1441 at 0.125 deg, 28 energy
+    planes) under a celestial CAR model map, its grid resampled to
+    target_pix_size."""
+    name: str
+    npix: int = 400
+    cdelt: float = 0.05
+    crval: tuple = (180.0, 0.0)
+    cube_shape: tuple = (2880, 1441)
+    cube_cdelt: float = 0.125
+    planes: int = 28
+    n_energy: int = 37
+    target: float = 0.05
+    seed: int = 7
+
+    @property
+    def values(self) -> int:
+        return self.n_energy * self.npix * self.npix
+
+
+DIFFUSE_CONFIGS = {
+    # the model map of the 2,000-source workload (400 x 400 at 0.05 deg,
+    # 37 energies) under the galactic diffuse cube
+    "diffuse": DiffuseConfig("diffuse"),
+    # the same on a 100 x 100 image at 0.2 deg over a 1 deg cube with 8
+    # planes (oracle-sized)
+    "diffuse_small": DiffuseConfig("diffuse_small", npix=100, cdelt=0.2, cube_shape=(360, 181),
+                                   cube_cdelt=1.0, planes=8, n_energy=6, target=0.5),
+}
+
+
+def diffuse_inputs(cfg: DiffuseConfig):
+    """(roi Wcs, cube Wcs, cube float32 [planes, N2, N1], upsf [401, E],
+    exposure [E]): a cube bright along the galactic plane with a power-law
+    spectrum and pixel-scale structure, a King-like PSF narrowing with
+    energy."""
+    rng = np.random.default_rng(cfg.seed)
+    n = cfg.npix
+    roi = Wcs.make((n, n), (n / 2 + 0.5, n / 2 + 0.5), cfg.crval, (-cfg.cdelt, cfg.cdelt), 0.0,
+                   "CAR")
+    N1, N2 = cfg.cube_shape
+    cube_wcs = Wcs.make((N1, N2), ((N1 + 1) / 2, (N2 + 1) / 2), (0.0, 0.0),
+                        (-cfg.cube_cdelt, cfg.cube_cdelt), 0.0, "CAR", galactic=True)
+    lon = cube_wcs.cdelt1 * (np.arange(1, N1 + 1) - cube_wcs.crpix1)
+    lat = cube_wcs.cdelt2 * (np.arange(1, N2 + 1) - cube_wcs.crpix2)
+    prof_b = np.exp(-np.abs(lat) / 3.0) + 0.2 * np.exp(-np.abs(lat) / 25.0)
+    prof_l = 1.0 + 0.5 * np.cos(np.radians(lon))
+    spec = (10.0 ** np.linspace(2, 6, cfg.planes) / 100.0) ** -2.7
+    tex = (1.0 + 0.1 * rng.standard_normal((N2, N1))).astype(np.float32)
+    cube = (spec[:, None, None] * (prof_b[:, None] * prof_l[None, :] * tex)[None]).astype(
+        np.float32)
+    sep = np.zeros(401)
+    sep[1:] = 1e-4 * np.exp(np.arange(400) * np.log(7e5) / 399.0)
+    energy = 10.0 ** np.linspace(2, 6, cfg.n_energy)
+    sigma = np.maximum(3.0 * (energy / 100.0) ** -0.8, 0.05)
+    g = 2.5
+    upsf = (1.0 + (sep[:, None] / sigma[None, :]) ** 2 / (2 * g)) ** -g
+    expo = 3e10 * (0.5 + 0.5 * np.log10(energy) / 6.0)
+    return roi, cube_wcs, cube, upsf, expo
