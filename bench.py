@@ -440,6 +440,129 @@ def cpu_baseline(cfg, inp, gpu_maps: dict, n_src: int):
     return base, accuracy
 
 
+DIFFUSE_METRIC = "diffuse srcmap pix·E values/s, 1 GPU"
+DIFFUSE_STAGES = ("diffuse_kernel", "diffuse_fft", "diffuse_cols", "diffuse_sample")
+
+
+def main_diffuse(args, cfg):
+    """The diffuse-source model map: crop and resample of the cube, the PSF
+    kernel's spectrum, the per-energy FFT convolution, sampling onto the
+    model map (xts_diffuse_src_model_map_wcs_dev), FP32 output."""
+    import torch
+    import xtsrcmaps_amd as xa
+    from xtsrcmaps_amd import synthetic
+    roi, cube_wcs, cube, upsf, expo = synthetic.diffuse_inputs(cfg)
+    E, H, W = cfg.n_energy, roi.naxis1, roi.naxis2
+    ctx = xa.Context(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx.set_stream(stream.cuda_stream)
+    # the step recomputes the model map's solid angles every call
+    ctx.set_tuning("geometry_cache", 0)
+    dev = torch.device("cuda:0")
+    t_cube, t_u, t_e = (torch.as_tensor(a, device=dev) for a in (cube, upsf, expo))
+    out = torch.empty((E, H, W), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def one(*_):
+        ctx.diffuse_src_model_map_wcs_dev(roi, cube_wcs, t_cube, t_u, t_e, cfg.target, out,
+                                          flags=xa.OUT_F32)
+    _log(f"bench: diffuse: warmup {args.warmup}, timing {args.steps} steps")
+    ms = timed(ctx, None, None, None, args.steps, args.warmup, 0, one)
+    ctx.enable_timing(True)
+    for name in DIFFUSE_STAGES:
+        ctx.kernel_times(name)
+    for _ in range(5):
+        one()
+    torch.cuda.synchronize()
+    stages = {name: ctx.kernel_times(name)[0] / 5 for name in DIFFUSE_STAGES}
+    ctx.enable_timing(False)
+    g = xa.diffuse_geometry(roi, cube_wcs, cfg.target)
+    values = E * H * W
+    _log(f"bench: diffuse: {ms:.3f} ms per map; stages {stages}")
+    # the dominant kernel: the convolution's column pass (forward transform,
+    # product with the kernel's spectrum, inverse); algorithmic bytes per
+    # launch: its row spectra read and written, the kernel's spectrum read
+    qh = g["fft_p"] // 2 + 1
+    col_bytes = E * qh * (2 * g["m2"] + g["fft_q"]) * 16
+    achieved = col_bytes / (stages["diffuse_cols"] / 1e3) / 1e9
+    result = {
+        "metric": DIFFUSE_METRIC,
+        "value": values / (ms / 1e3),
+        "unit": "values/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name}: the diffuse-source model map of a {H}x{W} CAR "
+                               f"{cfg.cdelt} deg celestial image at {E} energies over a "
+                               f"{cube_wcs.naxis1}x{cube_wcs.naxis2} galactic CAR cube "
+                               f"({cfg.cube_cdelt} deg, {cfg.planes} planes), resampled to "
+                               f"{cfg.target} deg: grid {g['m1']}x{g['m2']}, kernel "
+                               f"{g['nk']}x{g['nk']}, transforms {g['fft_p']}x{g['fft_q']}; "
+                               f"FP64 arithmetic, FP32 output",
+                   "per_step": "solid angles, crop and resample, PSF kernel and its spectrum, "
+                               "convolution, sampling, exposure"},
+        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fft_cols_kernel<1> (the convolution's column pass); algorithmic "
+                               "bytes per launch = E x (P/2+1) x (2 m2 + Q) x 16: the image's "
+                               "row spectra read and written, the kernel's spectrum read",
+                     "launch_ms": stages["diffuse_cols"], "bytes_per_launch": col_bytes},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"], result["accuracy"] = cpu_baseline_diffuse(cfg, out.cpu().numpy())
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline_diffuse(cfg, gpu_map, planes=None):
+    """The oracle's resample, kernel and sampling around scipy's FFT
+    convolution (the reference's FFTW algorithm: per-plane linear convolution
+    by FFT) on `planes` of the workload (default all), on the host cores;
+    those planes of the GPU map against it."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path[:0] = [here, os.path.join(here, "tests")]
+    from oracle import oracle as orc
+    from scipy.signal import fftconvolve
+    from xtsrcmaps_amd import synthetic
+    orc.lib()
+    roi, cube_wcs, cube, upsf, expo = synthetic.diffuse_inputs(cfg)
+    E = cfg.n_energy
+    planes = list(range(E)) if planes is None else [p for p in planes if p < E]
+
+    def ow(w):
+        return orc.make_wcs((w.naxis1, w.naxis2), (w.crpix1, w.crpix2), (w.crval1, w.crval2),
+                            (w.cdelt1, w.cdelt2), w.crota2, "CAR" if w.proj == 0 else "AIT")
+    t0 = time.perf_counter()
+    g = orc.diffuse_geometry(ow(roi), roi.is_galactic, ow(cube_wcs), cube_wcs.is_galactic,
+                             cfg.target)
+    img = orc.diffuse_resample(cube, ow(cube_wcs), g, E)
+    K = orc.psf_make_kernel(g.nk, g.grid, g.kc1, g.kc2, upsf[:, planes])
+    c = (g.nk - 1) // 2
+    conv = np.stack([fftconvolve(img[e], K[i], mode="full")[c:c + g.m1, c:c + g.m2]
+                     for i, e in enumerate(planes)])
+    om = orc.solid_angle(roi.naxis1, roi.naxis2, ow(roi))
+    ref = orc.diffuse_sample(conv, g, ow(roi), om, expo[planes])
+    dt = time.perf_counter() - t0
+    rel = np.abs(gpu_map[planes].astype(np.float64) - ref) / np.abs(ref)
+    model, phys = host_cpu()
+    n = len(planes) * roi.naxis1 * roi.naxis2
+    base = dict(value=n / dt, unit="values/s", cores=orc.num_threads(), kind="port",
+                cpu_model=model, physical_cores=phys,
+                sample=f"energy planes {planes} of {cfg.name} ({n:.3g} map values): the "
+                       f"oracle's crop and resample of all {E} planes, kernel, sampling (C, "
+                       f"OpenMP threads) around scipy FFT convolution (one thread), FP64, "
+                       f"{dt:.2f} s")
+    return base, {"max_rel_err_vs_fp64_port": float(rel.max()),
+                  "median_rel_err": float(np.median(rel)), "planes": planes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,6 +588,9 @@ def main():
     base = xa.LAYOUT_SEHW | xa.OUT_F32 | xa.COMPUTE_F32
     sur = not args.no_surrogate
     _log(f"bench: setup {args.config}")
+    if args.config in synthetic.DIFFUSE_CONFIGS:
+        main_diffuse(args, synthetic.DIFFUSE_CONFIGS[args.config])
+        return
     if args.config in synthetic.CONFIGS and synthetic.CONFIGS[args.config].extra.get("flux_decades"):
         main_summed(args, synthetic.CONFIGS[args.config])
         return

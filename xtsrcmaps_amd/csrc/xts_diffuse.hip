@@ -57,6 +57,7 @@ constexpr int kFftThreads = 256;
 struct FftPlan {
     int n = 0, nst = 0;
     int radix[16] = {};
+    const double2* tw = nullptr;  // device [n]: (cos, sin)(2 pi m / n)
 };
 
 // Transform length n as radix-4, -2, -3 and -5 passes; false if n has
@@ -140,67 +141,57 @@ template <> __device__ inline void dft<5>(double2* v, double sg) {
 }
 
 // One Stockham pass of radix R over nb transforms of length n laid out one
-// after the other in buf (ns: the product of the radices before this pass).
-// Butterfly j of a transform reads x[j + r n/R], twiddles by
-// exp(sg 2 pi i r k / (ns R)) with k = j mod ns, and writes its R outputs to
-// (j / ns) ns R + k + r ns.  Every thread holds its butterflies' inputs in
-// registers across the barrier, so one buffer serves as input and output.
+// after the other (ns: the product of the radices before this pass).
+// Butterfly j of a transform reads in[j + r n/R], twiddles by
+// exp(sg 2 pi i r k / (ns R)) with k = j mod ns (from the table tw of the
+// length: m = r k n / (ns R)), and writes its R outputs to out[(j / ns) ns R
+// + k + r ns].  Input and output are two LDS buffers, one barrier a pass.
 template <int R>
-__device__ void fft_pass(double2* buf, int n, int nb, int ns, double sg) {
-    constexpr int kMaxB = (kFftMax / R + kFftThreads - 1) / kFftThreads;
+__device__ void fft_pass(const double2* __restrict__ in, double2* __restrict__ out, int n,
+                         int nb, int ns, double sg, const double2* __restrict__ tw) {
     const int T = n / R;
     const int total = T * nb;
-    double2 v[kMaxB][R];
+    const int tws = n / (ns * R);
+#pragma unroll 2
+    for (int q = threadIdx.x; q < total; q += kFftThreads) {
+        const int t = q / T, j = q - t * T;
+        const double2* x = in + t * n + j;
+        double2 v[R];
 #pragma unroll
-    for (int b = 0; b < kMaxB; ++b) {
-        const int q = threadIdx.x + b * kFftThreads;
-        if (q < total) {
-            const int t = q / T, j = q - t * T;
-            const double2* x = buf + (size_t)t * n;
+        for (int r = 0; r < R; ++r) v[r] = x[r * T];
+        const int k = j % ns;
+        if (ns > 1) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = x[j + r * T];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < kMaxB; ++b) {
-        const int q = threadIdx.x + b * kFftThreads;
-        if (q < total) {
-            const int t = q / T, j = q - t * T;
-            const int k = j % ns;
-            if (ns > 1) {
-                double s, c;
-                sincospi(sg * 2.0 * k / (double)(ns * R), &s, &c);
-                const double2 w1 = make_double2(c, s);
-                double2 w = w1;
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    v[b][r] = cmul(v[b][r], w);
-                    w = cmul(w, w1);
-                }
+            for (int r = 1; r < R; ++r) {
+                const double2 w = tw[r * k * tws];
+                v[r] = cmul(v[r], make_double2(w.x, sg * w.y));
             }
-            dft<R>(v[b], sg);
-            double2* x = buf + (size_t)t * n;
-            const int base = (j / ns) * ns * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) x[base + r * ns] = v[b][r];
         }
+        dft<R>(v, sg);
+        double2* y = out + t * n + (j / ns) * ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) y[r * ns] = v[r];
     }
     __syncthreads();
 }
 
-// nb transforms of length p.n in buf, natural order in and out; sg -1
-// forward, +1 inverse (unscaled).
-__device__ void fft_run(double2* buf, const FftPlan& p, int nb, double sg) {
+// nb transforms of length p.n in a, natural order in and out (b: scratch of
+// the same size); sg -1 forward, +1 inverse (unscaled).  Returns the buffer
+// that holds the result (a or b).
+__device__ double2* fft_run(double2* a, double2* b, const FftPlan& p, int nb, double sg) {
     int ns = 1;
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.radix[s];
-        if (R == 4) fft_pass<4>(buf, p.n, nb, ns, sg);
-        else if (R == 2) fft_pass<2>(buf, p.n, nb, ns, sg);
-        else if (R == 3) fft_pass<3>(buf, p.n, nb, ns, sg);
-        else fft_pass<5>(buf, p.n, nb, ns, sg);
+        if (R == 4) fft_pass<4>(a, b, p.n, nb, ns, sg, p.tw);
+        else if (R == 2) fft_pass<2>(a, b, p.n, nb, ns, sg, p.tw);
+        else if (R == 3) fft_pass<3>(a, b, p.n, nb, ns, sg, p.tw);
+        else fft_pass<5>(a, b, p.n, nb, ns, sg, p.tw);
         ns *= R;
+        double2* t = a;
+        a = b;
+        b = t;
     }
+    return a;
 }
 
 // Deterministic block sum (fixed reduction order).
@@ -214,6 +205,15 @@ __device__ double block_sum(double v, double* red) {
     for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
     __syncthreads();
     return s;
+}
+
+// Twiddle table of a transform length n: tw[m] = (cos, sin)(2 pi m / n).
+__global__ void __launch_bounds__(256) twiddle_kernel(double2* __restrict__ tw, int n) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+        double s, c;
+        sincospi(2.0 * m / (double)n, &s, &c);
+        tw[m] = make_double2(c, s);
+    }
 }
 
 // Table node i (the upper one) and blend weight t of a separation s in
@@ -403,7 +403,7 @@ static int make_plan(const xts_wcs* roi_x, const xts_wcs* cube_x, double target,
 }
 
 // ------------------------------------------------------------------ kernels
-// Table node and weight of each kernel pixel: kernel pixel (h, w) is the
+// Table node and weight of each kernel pixel, [w][h]: kernel pixel (h, w) is the
 // grid's pixel (kc1 + h - c, kc2 + w - c), its separation taken from the
 // grid direction at (kc1, kc2) (psf/psf.make_kernel.cpp:21-37).
 __global__ void __launch_bounds__(256)
@@ -412,8 +412,9 @@ psf_sep_kernel(Wcs g, double kc1, double kc2, int nk, int c, int* __restrict__ n
     const int n = nk * nk;
     double d0[3];
     wcs_pix2dir(g, kc1, kc2, d0);
+    // stored by kernel column: k = w nk + h (the row pass runs along h)
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int h = k / nk, w = k - h * nk;
+        const int w = k / nk, h = k - w * nk;
         double d[3];
         int i = 0;
         double t = 0.0;
@@ -432,9 +433,9 @@ struct RowArgs {
     FftPlan q;
     int nrows, ncols;      // nonzero rows and columns of the input
     int nb;                // row pairs per block
-    int qh, qs;            // half-spectrum length, spectrum row stride
-    int rstride;           // spectrum rows per energy (>= nrows)
-    double2* spec;         // [E][rstride][qs]
+    int qh;                // half-spectrum length
+    int rstride;           // spectrum rows per column (>= nrows)
+    double2* spec;         // [E][qh][rstride]
     // cube source: trilinear resample of the crop (math/trilerp.hpp:9-56)
     const float* cube;
     long long cn1, cn2;    // cube axis lengths (axis 1 fastest in memory)
@@ -487,7 +488,9 @@ __device__ inline double cube_value(const RowArgs& a, int e, int i, int j) {
 template <int SRC>
 __device__ inline double row_value(const RowArgs& a, int e, int r, int col) {
     if (r >= a.nrows || col >= a.ncols) return 0.0;
-    if (SRC == kSrcCube) return cube_value(a, e, r, col);
+    // the cube's lines run along axis 1 (the cube's contiguous axis): line r
+    // is axis-2 index r
+    if (SRC == kSrcCube) return cube_value(a, e, col, r);
     if (SRC == kSrcPsf) {
         const size_t k = (size_t)r * a.ncols + col;
         const int i = a.node[k];
@@ -500,13 +503,16 @@ __device__ inline double row_value(const RowArgs& a, int e, int r, int col) {
 
 // Rows 2p and 2p + 1 as the real and imaginary parts of one transform; the
 // two half spectra X = (Z_k + conj Z_-k) / 2, Y = (Z_k - conj Z_-k) / 2i
-// (k = 0 .. q/2) to spec.  blockIdx.y: energy.
+// (k = 0 .. q/2) to spec, stored by column: spec[e][k][r], so that the
+// column pass reads whole columns and this pass writes runs of 2 nb rows.
+// blockIdx.y: energy.  LDS: two buffers of nb q.n values.
 template <int SRC>
 __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
-    extern __shared__ double2 buf[];
+    extern __shared__ double2 lds[];
     __shared__ double red[kFftThreads / 64];
     const int e = blockIdx.y, n = a.q.n;
     const int pair0 = blockIdx.x * a.nb;
+    double2* buf = lds;
     for (int idx = threadIdx.x; idx < a.nb * n; idx += kFftThreads) {
         const int t = idx / n, col = idx - t * n;
         const int r0 = 2 * (pair0 + t);
@@ -530,14 +536,19 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
             }
         }
     }
-    fft_run(buf, a.q, a.nb, -1.0);
-    for (int idx = threadIdx.x; idx < a.nb * a.qh; idx += kFftThreads) {
-        const int t = idx / a.qh, k = idx - t * a.qh;
-        const int r0 = 2 * (pair0 + t);
+    buf = fft_run(lds, lds + a.nb * n, a.q, a.nb, -1.0);
+    // column k of the spectrum: rows 2 pair0 .. 2 pair0 + 2 nb - 1, X and Y
+    // of each transform next to each other
+    const int run = 2 * a.nb;
+    double2* col0 = a.spec + (size_t)e * a.qh * a.rstride + 2 * pair0;
+    for (int idx = threadIdx.x; idx < run * a.qh; idx += kFftThreads) {
+        const int k = idx / run, s = idx - k * run;
+        const int t = s >> 1;
+        if (2 * pair0 + s >= a.nrows) continue;
         const double2 zk = buf[t * n + k], zm = buf[t * n + (n - k) % n];
-        double2* row = a.spec + ((size_t)e * a.rstride + r0) * a.qs + k;
-        if (r0 < a.nrows) *row = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-        if (r0 + 1 < a.nrows) row[a.qs] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
+        col0[(size_t)k * a.rstride + s] =
+            (s & 1) ? make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x))
+                    : make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
     }
 }
 
@@ -547,34 +558,37 @@ struct ColArgs {
     FftPlan p;
     int rows_in, rows_out;  // nonzero input rows, rows written back (from row c on)
     int c;                  // first output row of the circular convolution
-    int qh, qs, cb;         // half-spectrum columns, row stride, columns per block
-    int rstride;            // spectrum rows per energy
-    double2* spec;          // [E][rstride][qs]
+    int qh, cb;             // half-spectrum columns, columns per block
+    int rstride;            // spectrum rows per column
+    double2* spec;          // [E][qh][rstride]
     double2* kspec;         // [E][qh][P], the kernel's spectrum by column
     const double* rowsum;   // [E][rows_in]: the kernel's row sums (null: sum 1)
     double scale;           // 1 / (P Q)
 };
 
-// cb adjacent spectrum columns per block.  kColKernel: forward transforms,
-// scaled by 1 / (kernel sum P Q), stored by column in kspec.  kColConvolve:
-// forward, times the kernel's spectrum, inverse, rows c .. c + rows_out - 1
-// written back to rows 0 .. rows_out - 1 of the same columns.
+// cb spectrum columns per block (contiguous in spec).  kColKernel: forward
+// transforms, scaled by 1 / (kernel sum P Q), stored in kspec.
+// kColConvolve: forward, times the kernel's spectrum, inverse, rows c .. c +
+// rows_out - 1 written back to rows 0 .. rows_out - 1 of the same columns.
+// LDS: two buffers of cb p.n values.
 template <int MODE>
 __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
-    extern __shared__ double2 buf[];
+    extern __shared__ double2 lds[];
     __shared__ double red[kFftThreads / 64];
     const int e = blockIdx.y, n = a.p.n, cb = a.cb;
     const int k0 = blockIdx.x * cb;
-    double2* spec = a.spec + (size_t)e * a.rstride * a.qs;
+    const int ncol = min(cb, a.qh - k0);
+    double2* spec = a.spec + ((size_t)e * a.qh + k0) * a.rstride;
+    double2* b0 = lds;
+    double2* b1 = lds + cb * n;
     for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads) {
-        const int r = idx / cb, t = idx - r * cb;
-        const int k = k0 + t;
-        buf[t * n + r] = (r < a.rows_in && k < a.qh) ? spec[(size_t)r * a.qs + k]
-                                                     : make_double2(0.0, 0.0);
+        const int t = idx / n, r = idx - t * n;
+        b0[idx] = (t < ncol && r < a.rows_in) ? spec[(size_t)t * a.rstride + r]
+                                              : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_run(buf, a.p, cb, -1.0);
-    double2* ks = a.kspec + (size_t)e * a.qh * n;
+    double2* buf = fft_run(b0, b1, a.p, cb, -1.0);
+    double2* ks = a.kspec + ((size_t)e * a.qh + k0) * n;
     if (MODE == kColKernel) {
         double tot = 0.0;
         if (a.rowsum)
@@ -582,57 +596,55 @@ __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
                 tot += a.rowsum[(size_t)e * a.rows_in + h];
         tot = a.rowsum ? block_sum(tot, red) : 1.0;
         const double s = a.scale / tot;
-        for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads) {
-            const int t = idx / n, r = idx - t * n;
-            if (k0 + t < a.qh) {
-                const double2 v = buf[idx];
-                ks[(size_t)(k0 + t) * n + r] = make_double2(v.x * s, v.y * s);
-            }
+        for (int idx = threadIdx.x; idx < ncol * n; idx += kFftThreads) {
+            const double2 v = buf[idx];
+            ks[idx] = make_double2(v.x * s, v.y * s);
         }
         return;
     }
-    for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads) {
-        const int t = idx / n, r = idx - t * n;
-        if (k0 + t < a.qh) buf[idx] = cmul(buf[idx], ks[(size_t)(k0 + t) * n + r]);
-    }
+    for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads)
+        buf[idx] = idx < ncol * n ? cmul(buf[idx], ks[idx]) : make_double2(0.0, 0.0);
     __syncthreads();
-    fft_run(buf, a.p, cb, 1.0);
-    for (int idx = threadIdx.x; idx < cb * a.rows_out; idx += kFftThreads) {
-        const int r = idx / cb, t = idx - r * cb;
-        const int k = k0 + t;
-        if (k < a.qh) spec[(size_t)r * a.qs + k] = buf[t * n + r + a.c];
+    double2* other = buf == b0 ? b1 : b0;
+    buf = fft_run(buf, other, a.p, cb, 1.0);
+    for (int idx = threadIdx.x; idx < ncol * a.rows_out; idx += kFftThreads) {
+        const int t = idx / a.rows_out, r = idx - t * a.rows_out;
+        spec[(size_t)t * a.rstride + r] = buf[t * n + r + a.c];
     }
 }
 
 struct RowInvArgs {
     FftPlan q;
     int rows, cols_out, c;  // rows, columns written (from column c on)
-    int nb, qh, qs;
-    const double2* spec;    // [E][rstride][qs]
+    int nb, qh;
+    const double2* spec;    // [E][qh][rstride]
     int rstride;
     double* out;            // [E][rows][cols_out]
 };
 
 // Rows 2p, 2p + 1 from their half spectra: Z = X + iY over all k (X_-k =
 // conj X_k), inverse transform, real part row 2p, imaginary part 2p + 1.
+// LDS: two buffers of nb q.n values.
 __global__ void __launch_bounds__(kFftThreads) fft_rows_inv_kernel(RowInvArgs a) {
-    extern __shared__ double2 buf[];
+    extern __shared__ double2 lds[];
     const int e = blockIdx.y, n = a.q.n;
     const int pair0 = blockIdx.x * a.nb;
-    const double2* spec = a.spec + (size_t)e * a.rstride * a.qs;
-    for (int idx = threadIdx.x; idx < a.nb * n; idx += kFftThreads) {
-        const int t = idx / n, k = idx - t * n;
+    double2* buf = lds;
+    const double2* col0 = a.spec + (size_t)e * a.qh * a.rstride + 2 * pair0;
+    // Z_k of transform t (k < qh); reads run along t, two rows per thread
+    for (int idx = threadIdx.x; idx < a.nb * a.qh; idx += kFftThreads) {
+        const int k = idx / a.nb, t = idx - k * a.nb;
         const int r0 = 2 * (pair0 + t);
-        const bool lo = k < a.qh;
-        const int kk = lo ? k : n - k;
         double2 X = make_double2(0.0, 0.0), Y = make_double2(0.0, 0.0);
-        if (r0 < a.rows) X = spec[(size_t)r0 * a.qs + kk];
-        if (r0 + 1 < a.rows) Y = spec[(size_t)(r0 + 1) * a.qs + kk];
-        if (!lo) { X.y = -X.y; Y.y = -Y.y; }
-        buf[idx] = make_double2(X.x - Y.y, X.y + Y.x);
+        const double2* p = col0 + (size_t)k * a.rstride + 2 * t;
+        if (r0 < a.rows) X = p[0];
+        if (r0 + 1 < a.rows) Y = p[1];
+        buf[t * n + k] = make_double2(X.x - Y.y, X.y + Y.x);
+        // Z_{n-k} = conj X_k + i conj Y_k
+        if (k > 0 && n - k >= a.qh) buf[t * n + n - k] = make_double2(X.x + Y.y, Y.x - X.y);
     }
     __syncthreads();
-    fft_run(buf, a.q, a.nb, 1.0);
+    buf = fft_run(lds, lds + a.nb * n, a.q, a.nb, 1.0);
     for (int idx = threadIdx.x; idx < a.nb * a.cols_out; idx += kFftThreads) {
         const int t = idx / a.cols_out, j = idx - t * a.cols_out;
         const int r0 = 2 * (pair0 + t);
@@ -670,8 +682,9 @@ diffuse_sample_kernel(Wcs roi, Wcs up, int frame, double period, double vlo1, do
         const double w10 = fx * (1.0 - fy), w11 = fx * fy;
         const double om = omega[k];
         for (int e = 0; e < E; ++e) {
-            const double* c = conv + ((size_t)e * m1 + i0) * m2 + j0;
-            const double v = w00 * c[0] + w01 * c[1] + w10 * c[m2] + w11 * c[m2 + 1];
+            // conv [E][m2][m1]: lines of constant axis-2 index
+            const double* c = conv + ((size_t)e * m2 + j0) * m1 + i0;
+            const double v = w00 * c[0] + w01 * c[m1] + w10 * c[1] + w11 * c[m1 + 1];
             out[(size_t)e * n + k] = inside ? (OT)(v * om * expo[e]) : (OT)0;
         }
     }
@@ -702,10 +715,34 @@ psf_kernel_values_kernel(int nk, int E, const int* __restrict__ node,
     for (int k = threadIdx.x; k < n; k += blockDim.x) out[(size_t)e * n + k] *= inv;
 }
 
-// ------------------------------------------------------------------ host
-static int choose_nb(int n) { return std::max(1, std::min(8, kFftMax / n)); }
+// [E][w][h] -> [E][h][w]
+__global__ void __launch_bounds__(256)
+kernel_transpose_kernel(int nk, int E, const double* __restrict__ in, double* __restrict__ out) {
+    const size_t n = (size_t)nk * nk;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n * E;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = k / n, r = k - e * n;
+        const int w = (int)(r / nk), h = (int)(r - (size_t)w * nk);
+        out[e * n + (size_t)h * nk + w] = in[k];
+    }
+}
 
-static size_t lds_bytes(int nb, int n) { return (size_t)nb * n * sizeof(double2); }
+// ------------------------------------------------------------------ host
+// Transforms per block for length n: about 1024 values a block (two LDS
+// buffers of 16 KB), so that several blocks share a CU.
+static int choose_nb(int n) { return std::max(1, std::min(8, 1024 / n)); }
+static int spec_stride(int rows) { return (rows + 7) & ~7; }
+
+// The twiddle table of p into tw (p.n entries).
+static int make_twiddles(xts_ctx* ctx, FftPlan* p, double2* tw) {
+    twiddle_kernel<<<div_up(p->n, 256), 256, 0, ctx->stream>>>(tw, p->n);
+    XTS_HIP(hipGetLastError());
+    p->tw = tw;
+    return XTS_OK;
+}
+
+// two ping-pong buffers of nb n complex values
+static size_t lds_bytes(int nb, int n) { return 2 * (size_t)nb * n * sizeof(double2); }
 
 template <typename A>
 static int launch_lds(void (*kernel)(A), dim3 grid, size_t lds, hipStream_t st, A args) {
@@ -715,48 +752,64 @@ static int launch_lds(void (*kernel)(A), dim3 grid, size_t lds, hipStream_t st, 
     return XTS_OK;
 }
 
-// The kernel's spectrum, by column, into kspec; spec is scratch of at least
-// E x nk x qs complex values.
-static int psf_spectrum(xts_ctx* ctx, const DiffPlan& pl, const double* upsf, int E, int* node,
-                        double* wt, double* rowsum, double2* spec, double2* kspec) {
+// Row pass of E planes (rows ra.nrows) into spec, stored by column with
+// ra.rstride rows a column.
+static int rows_forward(hipStream_t st, RowArgs ra, const FftPlan& Qp, int E) {
+    const int Q = Qp.n;
+    ra.q = Qp;
+    ra.nb = choose_nb(Q);
+    ra.qh = Q / 2 + 1;
+    ra.E = E;
+    const dim3 g(div_up(div_up(ra.nrows, 2), ra.nb), E);
+    const size_t lds = lds_bytes(ra.nb, Q);
+    if (ra.node) return launch_lds(fft_rows_fwd_kernel<kSrcPsf>, g, lds, st, ra);
+    if (ra.arr) return launch_lds(fft_rows_fwd_kernel<kSrcArray>, g, lds, st, ra);
+    return launch_lds(fft_rows_fwd_kernel<kSrcCube>, g, lds, st, ra);
+}
+
+// Column pass computing the kernel's spectrum kspec [E][qh][P] from its row
+// spectra in spec (rows, stride rstride); rowsum: its row sums, or null.
+static int kernel_spectrum(hipStream_t st, const FftPlan& Pp, int qh, int E, int rows,
+                           int rstride, double2* spec, const double* rowsum, double2* kspec,
+                           double scale) {
+    ColArgs ca{};
+    ca.p = Pp;
+    ca.rows_in = rows;
+    ca.rows_out = 0;
+    ca.qh = qh;
+    ca.cb = choose_nb(Pp.n);
+    ca.rstride = rstride;
+    ca.spec = spec;
+    ca.kspec = kspec;
+    ca.rowsum = rowsum;
+    ca.scale = scale;
+    return launch_lds(fft_cols_kernel<kColKernel>, dim3(div_up(qh, ca.cb), E),
+                      lds_bytes(ca.cb, Pp.n), st, ca);
+}
+
+// The kernel's spectrum into kspec [E][P/2 + 1][Q], on stream st: lines
+// along axis 1 (kernel columns w, transforms of length P), then across them
+// (length Q); spec is scratch of at least E x (P/2 + 1) x spec_stride(nk)
+// values.
+static int psf_spectrum(hipStream_t st, const DiffPlan& pl, const double* upsf, int E,
+                        int* node, double* wt, double* rowsum, double2* spec, double2* kspec) {
     const xts_diffuse_geom& g = pl.g;
     const int nk = (int)g.nk, P = (int)g.fft_p, Q = (int)g.fft_q;
-    const int qh = Q / 2 + 1, qs = (qh + 7) & ~7;
-    psf_sep_kernel<<<std::min<unsigned>(div_up((int64_t)nk * nk, 256), 4096), 256, 0,
-                     ctx->stream>>>(pl.up, g.kc1, g.kc2, nk, pl.c, node, wt);
+    psf_sep_kernel<<<std::min<unsigned>(div_up((int64_t)nk * nk, 256), 4096), 256, 0, st>>>(
+        pl.up, g.kc1, g.kc2, nk, pl.c, node, wt);
     XTS_HIP(hipGetLastError());
     RowArgs ra{};
-    ra.q = pl.Q;
     ra.nrows = nk;
     ra.ncols = nk;
-    ra.nb = choose_nb(Q);
-    ra.qh = qh;
-    ra.qs = qs;
-    ra.rstride = nk;
+    ra.rstride = spec_stride(nk);
     ra.spec = spec;
-    ra.E = E;
     ra.node = node;
     ra.wt = wt;
     ra.upsf = upsf;
     ra.rowsum = rowsum;
-    const unsigned npair = div_up(nk, 2);
-    XTS_TRY(launch_lds(fft_rows_fwd_kernel<kSrcPsf>, dim3(div_up(npair, ra.nb), E),
-                       lds_bytes(ra.nb, Q), ctx->stream, ra));
-    ColArgs ca{};
-    ca.p = pl.P;
-    ca.rows_in = nk;
-    ca.rows_out = 0;
-    ca.qh = qh;
-    ca.qs = qs;
-    ca.cb = choose_nb(P);
-    ca.rstride = nk;
-    ca.spec = spec;
-    ca.kspec = kspec;
-    ca.rowsum = rowsum;
-    ca.scale = 1.0 / ((double)P * Q);
-    XTS_TRY(launch_lds(fft_cols_kernel<kColKernel>, dim3(div_up(qh, ca.cb), E),
-                       lds_bytes(ca.cb, P), ctx->stream, ca));
-    return XTS_OK;
+    XTS_TRY(rows_forward(st, ra, pl.P, E));
+    return kernel_spectrum(st, pl.Q, P / 2 + 1, E, nk, ra.rstride, spec, rowsum, kspec,
+                           1.0 / ((double)P * Q));
 }
 
 // Buffers of one diffuse map in the context's slots.
@@ -764,22 +817,23 @@ struct DiffBufs {
     int* node;
     double* wt;
     double* rowsum;
-    double2* spec;
-    double2* kspec;
+    double2* tw;          // twiddles of P, then of Q
+    double2* spec;        // row spectra (the kernel's, then the image's)
+    double2* kspec;       // the kernel's spectrum
     double* conv;
 };
 
 static int diffuse_bufs(xts_ctx* ctx, const DiffPlan& pl, int E, DiffBufs* b) {
     const xts_diffuse_geom& g = pl.g;
     const size_t nk = (size_t)g.nk, P = (size_t)g.fft_p, Q = (size_t)g.fft_q;
-    const size_t qh = Q / 2 + 1, qs = (qh + 7) & ~(size_t)7;
-    const size_t rows = std::max<size_t>(g.m1, nk);
+    const size_t qh = P / 2 + 1;
+    const size_t rows = (size_t)spec_stride((int)std::max<int64_t>(g.m2, g.nk));
     void *p0, *p1, *p2, *p3;
     const size_t sep_bytes = arena_bytes(nk * nk * 4) + arena_bytes(nk * nk * 8) +
-                             arena_bytes((size_t)E * nk * 8);
+                             arena_bytes((size_t)E * nk * 8) + arena_bytes((P + Q) * 16);
     XTS_TRY(slot_reserve(ctx, kSlotDiffSep, sep_bytes, &p0));
-    XTS_TRY(slot_reserve(ctx, kSlotDiffSpec, (size_t)E * rows * qs * 16, &p1));
-    XTS_TRY(slot_reserve(ctx, kSlotDiffKspec, (size_t)E * qh * P * 16, &p2));
+    XTS_TRY(slot_reserve(ctx, kSlotDiffSpec, (size_t)E * qh * rows * 16, &p1));
+    XTS_TRY(slot_reserve(ctx, kSlotDiffKspec, (size_t)E * qh * Q * 16, &p2));
     XTS_TRY(slot_reserve(ctx, kSlotDiffConv, (size_t)E * g.m1 * g.m2 * 8, &p3));
     char* c = (char*)p0;
     b->node = (int*)c;
@@ -787,6 +841,8 @@ static int diffuse_bufs(xts_ctx* ctx, const DiffPlan& pl, int E, DiffBufs* b) {
     b->wt = (double*)c;
     c += arena_bytes(nk * nk * 8);
     b->rowsum = (double*)c;
+    c += arena_bytes((size_t)E * nk * 8);
+    b->tw = (double2*)c;
     b->spec = (double2*)p1;
     b->kspec = (double2*)p2;
     b->conv = (double*)p3;
@@ -796,36 +852,31 @@ static int diffuse_bufs(xts_ctx* ctx, const DiffPlan& pl, int E, DiffBufs* b) {
 // Convolution of E planes (the cube's resample, or an array: ra.arr) with
 // the kernel whose spectrum is in kspec: rows c .. c + rows_out - 1 and
 // columns c .. c + cols_out - 1 of the linear convolution into out
-// [E][rows_out][cols_out].  spec holds E x max(rows, rows_out) x qs values.
+// [E][rows_out][cols_out].  spec holds E x qh x spec_stride(max(rows,
+// rows_out)) values.
 static int convolve(xts_ctx* ctx, const FftPlan& Pp, const FftPlan& Qp, RowArgs ra, int E,
                     int rows_out, int cols_out, int c, double2* spec, const double2* kspec,
                     double* out) {
     const int P = Pp.n, Q = Qp.n;
-    const int qh = Q / 2 + 1, qs = (qh + 7) & ~7;
-    const int rstride = std::max(ra.nrows, rows_out);
-    ra.q = Qp;
-    ra.nb = choose_nb(Q);
-    ra.qh = qh;
-    ra.qs = qs;
-    ra.rstride = rstride;
+    const int qh = Q / 2 + 1;
+    ra.rstride = spec_stride(std::max(ra.nrows, rows_out));
     ra.spec = spec;
-    ra.E = E;
-    const dim3 g1(div_up(div_up(ra.nrows, 2), ra.nb), E);
-    XTS_TRY(launch_lds(ra.arr ? fft_rows_fwd_kernel<kSrcArray> : fft_rows_fwd_kernel<kSrcCube>,
-                       g1, lds_bytes(ra.nb, Q), ctx->stream, ra));
+    XTS_TRY(rows_forward(ctx->stream, ra, Qp, E));
     ColArgs ca{};
     ca.p = Pp;
     ca.rows_in = ra.nrows;
     ca.rows_out = rows_out;
     ca.c = c;
     ca.qh = qh;
-    ca.qs = qs;
     ca.cb = choose_nb(P);
-    ca.rstride = rstride;
+    ca.rstride = ra.rstride;
     ca.spec = spec;
     ca.kspec = const_cast<double2*>(kspec);
-    XTS_TRY(launch_lds(fft_cols_kernel<kColConvolve>, dim3(div_up(qh, ca.cb), E),
-                       lds_bytes(ca.cb, P), ctx->stream, ca));
+    {
+        TimedRegion tr(ctx, "diffuse_cols");
+        XTS_TRY(launch_lds(fft_cols_kernel<kColConvolve>, dim3(div_up(qh, ca.cb), E),
+                           lds_bytes(ca.cb, P), ctx->stream, ca));
+    }
     RowInvArgs ri{};
     ri.q = Qp;
     ri.rows = rows_out;
@@ -833,13 +884,11 @@ static int convolve(xts_ctx* ctx, const FftPlan& Pp, const FftPlan& Qp, RowArgs 
     ri.c = c;
     ri.nb = choose_nb(Q);
     ri.qh = qh;
-    ri.qs = qs;
     ri.spec = spec;
-    ri.rstride = rstride;
+    ri.rstride = ra.rstride;
     ri.out = out;
-    XTS_TRY(launch_lds(fft_rows_inv_kernel, dim3(div_up(div_up(rows_out, 2), ri.nb), E),
-                       lds_bytes(ri.nb, Q), ctx->stream, ri));
-    return XTS_OK;
+    return launch_lds(fft_rows_inv_kernel, dim3(div_up(div_up(rows_out, 2), ri.nb), E),
+                      lds_bytes(ri.nb, Q), ctx->stream, ri);
 }
 
 }  // namespace xts
@@ -875,15 +924,19 @@ int xts_diffuse_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* roi, const xt
     XTS_TRY(image_solid_angle(ctx, roi, &omega));
     DiffBufs b;
     XTS_TRY(diffuse_bufs(ctx, pl, (int)E, &b));
+    XTS_TRY(make_twiddles(ctx, &pl.P, b.tw));
+    XTS_TRY(make_twiddles(ctx, &pl.Q, b.tw + pl.P.n));
     {
         TimedRegion tr(ctx, "diffuse_kernel");
-        XTS_TRY(psf_spectrum(ctx, pl, upsf, (int)E, b.node, b.wt, b.rowsum, b.spec, b.kspec));
+        XTS_TRY(psf_spectrum(ctx->stream, pl, upsf, (int)E, b.node, b.wt, b.rowsum, b.spec,
+                             b.kspec));
     }
     {
         TimedRegion tr(ctx, "diffuse_fft");
+        // lines of constant axis-2 index, along axis 1
         RowArgs ra{};
-        ra.nrows = (int)g.m1;
-        ra.ncols = (int)g.m2;
+        ra.nrows = (int)g.m2;
+        ra.ncols = (int)g.m1;
         ra.cube = cube;
         ra.cn1 = cube_wcs->naxis1;
         ra.cn2 = cube_wcs->naxis2;
@@ -897,7 +950,7 @@ int xts_diffuse_src_model_map_wcs_dev(xts_ctx* ctx, const xts_wcs* roi, const xt
         ra.s1 = (double)(g.crop_n1 - 1) / (double)(g.m1 - 1);
         ra.s2 = (double)(g.crop_n2 - 1) / (double)(g.m2 - 1);
         ra.se = E > 1 ? (double)(n_planes - 1) / (double)(E - 1) : 0.0;
-        XTS_TRY(convolve(ctx, pl.P, pl.Q, ra, (int)E, (int)g.m1, (int)g.m2, pl.c, b.spec,
+        XTS_TRY(convolve(ctx, pl.Q, pl.P, ra, (int)E, (int)g.m2, (int)g.m1, pl.c, b.spec,
                          b.kspec, b.conv));
     }
     {
@@ -947,16 +1000,19 @@ int xts_psf_make_kernel(xts_ctx* ctx, int64_t nk, const xts_wcs* grid, double kc
     XTS_CHECK(E >= 1 && E <= 4096, "bad energy count %lld", (long long)E);
     XTS_CHECK(grid->proj == XTS_PROJ_CAR || grid->proj == XTS_PROJ_AIT, "unknown projection");
     const size_t n = (size_t)nk * nk;
-    void *du, *dsep, *dout;
+    void *du, *dsep, *dout, *dtmp;
     XTS_TRY(to_device(ctx, kSlotIn0, upsf, (size_t)kND * E * 8, &du));
     XTS_TRY(slot_reserve(ctx, kSlotDiffSep, arena_bytes(n * 4) + arena_bytes(n * 8), &dsep));
     XTS_TRY(slot_reserve(ctx, kSlotOut0, (size_t)E * n * 8, &dout));
+    XTS_TRY(slot_reserve(ctx, kSlotOut1, (size_t)E * n * 8, &dtmp));
     int* node = (int*)dsep;
     double* wt = (double*)((char*)dsep + arena_bytes(n * 4));
     psf_sep_kernel<<<std::min<unsigned>(div_up((int64_t)n, 256), 4096), 256, 0, ctx->stream>>>(
         wcs_setup(*grid), kc1, kc2, (int)nk, (int)((nk - 1) / 2), node, wt);
     psf_kernel_values_kernel<<<(unsigned)E, 256, 0, ctx->stream>>>(
-        (int)nk, (int)E, node, wt, (const double*)du, (double*)dout);
+        (int)nk, (int)E, node, wt, (const double*)du, (double*)dtmp);
+    kernel_transpose_kernel<<<std::min<unsigned>(div_up((int64_t)E * n, 256), 8192), 256, 0,
+                              ctx->stream>>>((int)nk, (int)E, (const double*)dtmp, (double*)dout);
     XTS_HIP(hipGetLastError());
     XTS_HIP(hipMemcpyAsync(out, dout, (size_t)E * n * 8, hipMemcpyDeviceToHost, ctx->stream));
     XTS_HIP(hipStreamSynchronize(ctx->stream));
@@ -977,41 +1033,27 @@ int xts_fft_convolve_map_psf(xts_ctx* ctx, const double* image, int64_t C, int64
     FftPlan Pp, Qp;
     fft_plan(P, &Pp);
     fft_plan(Q, &Qp);
-    const int qh = Q / 2 + 1, qs = (qh + 7) & ~7;
-    const int64_t rows = std::max(OH, std::max(H, KH));
-    void *dimg, *dker, *dsp, *dks, *dout;
+    const int qh = Q / 2 + 1;
+    const int64_t rows = spec_stride((int)std::max(OH, std::max(H, KH)));
+    void *dimg, *dker, *dsp, *dks, *dout, *dtw;
+    XTS_TRY(slot_reserve(ctx, kSlotDiffSep, (size_t)(P + Q) * 16, &dtw));
+    XTS_TRY(make_twiddles(ctx, &Pp, (double2*)dtw));
+    XTS_TRY(make_twiddles(ctx, &Qp, (double2*)dtw + P));
     XTS_TRY(to_device(ctx, kSlotIn0, image, (size_t)C * H * W * 8, &dimg));
     XTS_TRY(to_device(ctx, kSlotIn1, kernel, (size_t)C * KH * KW * 8, &dker));
-    XTS_TRY(slot_reserve(ctx, kSlotDiffSpec, (size_t)C * rows * qs * 16, &dsp));
+    XTS_TRY(slot_reserve(ctx, kSlotDiffSpec, (size_t)C * qh * rows * 16, &dsp));
     XTS_TRY(slot_reserve(ctx, kSlotDiffKspec, (size_t)C * qh * P * 16, &dks));
     XTS_TRY(slot_reserve(ctx, kSlotOut0, (size_t)C * OH * OW * 8, &dout));
-    // the kernel's spectrum, scaled by 1 / (P Q) only
+    // the kernel's spectrum, divided by P Q only
     RowArgs ka{};
-    ka.q = Qp;
     ka.nrows = (int)KH;
     ka.ncols = (int)KW;
-    ka.nb = choose_nb(Q);
-    ka.qh = qh;
-    ka.qs = qs;
-    ka.rstride = (int)KH;
+    ka.rstride = spec_stride((int)KH);
     ka.spec = (double2*)dsp;
-    ka.E = (int)C;
     ka.arr = (const double*)dker;
-    XTS_TRY(launch_lds(fft_rows_fwd_kernel<kSrcArray>, dim3(div_up(div_up(KH, 2), ka.nb), C),
-                       lds_bytes(ka.nb, Q), ctx->stream, ka));
-    ColArgs kc{};
-    kc.p = Pp;
-    kc.rows_in = (int)KH;
-    kc.qh = qh;
-    kc.qs = qs;
-    kc.cb = choose_nb(P);
-    kc.rstride = (int)KH;
-    kc.spec = (double2*)dsp;
-    kc.kspec = (double2*)dks;
-    kc.rowsum = nullptr;
-    kc.scale = 1.0 / ((double)P * Q);
-    XTS_TRY(launch_lds(fft_cols_kernel<kColKernel>, dim3(div_up(qh, kc.cb), C), lds_bytes(kc.cb, P),
-                       ctx->stream, kc));
+    XTS_TRY(rows_forward(ctx->stream, ka, Qp, (int)C));
+    XTS_TRY(kernel_spectrum(ctx->stream, Pp, qh, (int)C, (int)KH, ka.rstride, (double2*)dsp,
+                            nullptr, (double2*)dks, 1.0 / ((double)P * Q)));
     RowArgs ra{};
     ra.nrows = (int)H;
     ra.ncols = (int)W;

@@ -211,18 +211,19 @@ hipEvent_t pooled_event(xts_ctx* ctx);
 struct TimedRegion {
     xts_ctx* ctx;
     const char* name;
+    hipStream_t st;
     hipEvent_t start = nullptr;
     bool open = false;
-    TimedRegion(xts_ctx* c, const char* n) : ctx(c), name(n) {
+    TimedRegion(xts_ctx* c, const char* n) : ctx(c), name(n), st(c->stream) {
         if (ctx->timing) {
             start = pooled_event(ctx);
-            if (start && hipEventRecord(start, ctx->stream) == hipSuccess) open = true;
+            if (start && hipEventRecord(start, st) == hipSuccess) open = true;
         }
     }
     void stop() {
         if (!open) return;
         hipEvent_t e = pooled_event(ctx);
-        if (e && hipEventRecord(e, ctx->stream) == hipSuccess)
+        if (e && hipEventRecord(e, st) == hipSuccess)
             ctx->timing_recs.push_back({name, start, e});
         open = false;
     }

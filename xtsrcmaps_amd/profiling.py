@@ -8,6 +8,7 @@ xtsrcmaps_amd.profiling <workload>``):
     stages --config c3 --precision 1 the PSF stages (0 FP64, 1 BF16x3 contractions)
     copy --mib 592 --calls 3         an FP32 copy kernel of known size: the
                                      calibration of FETCH_SIZE / WRITE_SIZE
+    diffuse --config diffuse --calls 5  the diffuse-source model map (FP32 out)
 
 Summaries of rocprofv3 CSV output (plain Python, no GPU):
 
@@ -71,6 +72,26 @@ def _stages(config: str, calls: int, precision: int) -> None:
         bench.psf_stages(ctx, inp, d, precision)
     torch.cuda.synchronize()
     print(f"stages: {calls} calls of {config} at precision {precision}", flush=True)
+
+
+def _diffuse(config: str, calls: int) -> None:
+    """The diffuse-source model map of a workload, `calls` times, FP32 output."""
+    import torch
+    import xtsrcmaps_amd as xa
+    from . import synthetic
+    cfg = synthetic.DIFFUSE_CONFIGS[config]
+    roi, cube_wcs, cube, upsf, expo = synthetic.diffuse_inputs(cfg)
+    ctx = xa.Context(0)
+    ctx.set_tuning("geometry_cache", 0)
+    dev = torch.device("cuda")
+    t_cube, t_u, t_e = (torch.as_tensor(a, device=dev) for a in (cube, upsf, expo))
+    out = torch.empty((cfg.n_energy, roi.naxis1, roi.naxis2), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(calls + 1):
+        ctx.diffuse_src_model_map_wcs_dev(roi, cube_wcs, t_cube, t_u, t_e, cfg.target, out,
+                                          flags=xa.OUT_F32)
+    ctx.synchronize()
+    print(f"diffuse: {calls + 1} calls of {config}", flush=True)
 
 
 def _copy(mib: int, calls: int) -> None:
@@ -242,6 +263,9 @@ def main(argv=None) -> None:
     p.add_argument("--config", default="c3")
     p.add_argument("--calls", type=int, default=5)
     p.add_argument("--precision", type=int, default=0)
+    p = sub.add_parser("diffuse")
+    p.add_argument("--config", default="diffuse")
+    p.add_argument("--calls", type=int, default=5)
     p = sub.add_parser("copy")
     p.add_argument("--mib", type=int, default=592)
     p.add_argument("--calls", type=int, default=3)
@@ -272,6 +296,8 @@ def main(argv=None) -> None:
         _stages(a.config, a.calls, a.precision)
     elif a.what == "copy":
         _copy(a.mib, a.calls)
+    elif a.what == "diffuse":
+        _diffuse(a.config, a.calls)
     elif a.what == "mfma":
         print(json.dumps(mfma_json(a.run_dir, a.kernel, a.cmd), indent=1))
     elif a.what == "stats":
