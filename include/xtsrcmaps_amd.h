@@ -422,7 +422,7 @@ int xts_sph2pix(xts_ctx* ctx, const xts_wcs* wcs, const double* sph, int64_t n,
  * source (psf/psf.make_kernel.cpp:8-90) through FFTs
  * (model_map/mm_fft/fft.convolve_map_psf.cpp:8-39), sampled bilinearly at
  * the model map's pixel centres and scaled by pixel solid angle and
- * exposure (diffuse_src_model_map_wcs.cxx:78-84). */
+ * exposure (diffuse_src_model_map_wcs.cxx:76, 88). */
 
 /* The crop, grid, kernel and transform sizes of a diffuse map. */
 typedef struct {
@@ -438,6 +438,17 @@ typedef struct {
                                    the model map's reference direction */
     int64_t fft_p, fft_q;       /* transform lengths along axis 1 and 2 */
 } xts_diffuse_geom;
+
+/* The MapCube the diffuse map reads (fits::read_allsky_cropped's input,
+ * fits/fits.read_allsky_cropped.cpp:13-41, 150-216): the primary image of
+ * a FITS file, BITPIX -32 or -64, NAXIS 2 or 3, CAR or AIT.  wcs gets
+ * readWcsParams's keywords (CROTA2 0 when absent; is_galactic when CTYPE1
+ * starts with GLON); planes NAXIS3 (1 for a 2-D image); data (optional)
+ * [planes][naxis2][naxis1] float; energies (optional) the first column of
+ * the ENERGIES table, *n_energies entries (0 without the table).  Call
+ * with data and energies NULL for the sizes. */
+int xts_read_mapcube(const char* path, xts_wcs* wcs, int64_t* planes, float* data,
+                     double* energies, int64_t* n_energies);
 
 /* Host only: the geometry xts_diffuse_src_model_map_wcs_dev uses.
  * margin_deg widens the crop on every side (0: the map's own box plus one

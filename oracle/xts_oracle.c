@@ -1018,15 +1018,15 @@ void or_point_src_model_map_wcs(long Nh, long Nw, const or_wcs* wc,
  * a function the reference declares but does not compile).  The steps it
  * names, with the choices where its sources are unfinished: the crop box
  * bounds the cube coordinates of the model map's whole border (the
- * reference's fits/fits.read_allsky_cropped.cpp:170-200 takes two corners),
+ * reference's fits/fits.read_allsky_cropped.cpp:163-192 takes two corners),
  * unwrapped across the seam of an all-sky CAR cube; the resampled grid is
  * trilerp's corner-aligned grid (math/trilerp.hpp:22-25); the kernel is
  * centred on the model map's reference direction and blends table nodes as
  * mm_cubature/convolve_pixel_psf_logsep.hpp:45-55,72-73 does (the
- * reference's psf/psf.make_kernel.cpp:38-40 swaps the two weights); the
+ * reference's psf/psf.make_kernel.cpp:50, 56 swaps the two weights); the
  * convolution is per energy plane; the convolved grid is sampled
  * bilinearly at the model map's pixel centres, then scaled by solid angle
- * and exposure (diffuse_src_model_map_wcs.cxx:78-84).
+ * and exposure (diffuse_src_model_map_wcs.cxx:76, 88).
  * ====================================================================== */
 
 /* J2000 galactic frame from the north galactic pole (192.85948, 27.12825)

@@ -1,7 +1,7 @@
 /*
  * xts_oracle.h -- FP64 CPU restatement of the xtsrcmaps point-source model-map
  * path: (a) exposure contraction, (b) mean-PSF build, (c) pixel cubature and
- * the model-map scaling passes.
+ * the model-map scaling passes, and the diffuse-source map.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
  * path is compared against.  Only tests/, __graft_entry__.smoke() and the
@@ -30,6 +30,15 @@
  * of a piecewise-linear integrand; or_pixel_logsep outside [1e-4, 70]
  * degrees against the edge segments' lines; or_psf_radius_integral's
  * exact-radius variant against FermiTools' ft_mm_upsf_radius_integ.bin.
+ *
+ * The diffuse-source map (tests/test_oracle_diffuse.py): or_cel2gal /
+ * or_gal2cel against the galactic centre's published J2000 position and the
+ * poles; or_diffuse_geometry by the crop holding every map pixel centre and
+ * the grid's pixel-to-cube map; or_diffuse_resample exact on trilinear
+ * cubes and equal across the longitude seam to a rolled cube;
+ * or_psf_make_kernel against a table linear in separation (closed form);
+ * or_convolve_same against scipy.signal.convolve2d; or_diffuse_sample
+ * exact on a linear grid.
  *
  * Parity unpinned: none.
  */

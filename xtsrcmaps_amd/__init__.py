@@ -56,6 +56,20 @@ def livetime_efficiency_factors(logEs, p0, p1) -> np.ndarray:
     return out
 
 
+def read_mapcube(path):
+    """The primary image of a MapCube FITS file (xts_read_mapcube): (wcs,
+    data float32 [planes, naxis2, naxis1], energies float64 or None)."""
+    w, planes, ne = Wcs(), C.c_int64(), C.c_int64()
+    p = str(path).encode()
+    capi.check(capi.lib().xts_read_mapcube(p, C.byref(w), C.byref(planes), None, None,
+                                           C.byref(ne)))
+    data = np.zeros((planes.value, w.naxis2, w.naxis1), np.float32)
+    en = np.zeros(max(ne.value, 1))
+    capi.check(capi.lib().xts_read_mapcube(p, C.byref(w), C.byref(planes), capi.fptr(data),
+                                           capi.dptr(en), C.byref(ne)))
+    return w, data, (en[:ne.value] if ne.value else None)
+
+
 def diffuse_geometry(roi: Wcs, cube_wcs: Wcs, target_pix_size: float,
                      margin_deg: float = 0.0) -> dict:
     """The crop, resampled grid, kernel and transform sizes of a diffuse map

@@ -155,6 +155,8 @@ _SIGS = {
     "xts_read_irf_pars": [C.c_char_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                           _i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                           C.POINTER(C.c_float)],
+    "xts_read_mapcube": [C.c_char_p, C.POINTER(Wcs), C.POINTER(C.c_int64), _f, _d,
+                         C.POINTER(C.c_int64)],
     "xts_diffuse_geometry": [C.POINTER(Wcs), C.POINTER(Wcs), C.c_double, C.c_double,
                              C.POINTER(DiffuseGeom)],
     "xts_diffuse_src_model_map_wcs_dev": [_vp, C.POINTER(Wcs), C.POINTER(Wcs), _vp, _i64, _vp,

@@ -12,7 +12,7 @@
 //   kernel    psf/psf.make_kernel.cpp:8-90
 //   convolve  model_map/mm_fft/fft.convolve_map_psf.cpp:8-39,
 //             math/convolve/fft/fft.fftw_convolve_CHW.cpp:7-90
-//   scale     model_map.diffuse_src_model_map_wcs.cxx:78-84 (solid angle,
+//   scale     model_map.diffuse_src_model_map_wcs.cxx:76, 88 (solid angle,
 //             exposure)
 // Where those sources are unfinished or inconsistent the steps here are:
 //   * the crop box bounds the cube coordinates of the whole border of the
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) twiddle_kernel(double2* __restrict__ tw, 
 
 // Table node i (the upper one) and blend weight t of a separation s in
 // degrees, value = (1 - t) lut[i - 1] + t lut[i]
-// (mm_cubature/convolve_pixel_psf_logsep.hpp:45-55, psf/psf.make_kernel.cpp:28-37).
+// (mm_cubature/convolve_pixel_psf_logsep.hpp:45-55, psf/psf.make_kernel.cpp:28-38).
 __host__ __device__ inline void sep_node(double s, int* i, double* t) {
     const double j = s < 1e-4 ? 1.0 : 1.0 + kRecipStep * (kLn1e4 + log(s));
     int k = 1 + (int)j;
@@ -383,7 +383,7 @@ static int make_plan(const xts_wcs* roi_x, const xts_wcs* cube_x, double target,
     pl->vlo2 = (0.5 - a2) / g.step2 + 1.0;
     pl->vhi2 = (N2 + 0.5 - a2) / g.step2 + 1.0;
     g.wrap = wrap ? 1 : 0;
-    // kernel: odd size min(m1, m2) (diffuse_src_model_map_wcs.cxx:55-63),
+    // kernel: odd size min(m1, m2) (diffuse_src_model_map_wcs.cxx:54-61),
     // centred at the model map's reference direction
     g.nk = std::min(g.m1, g.m2);
     if (g.nk % 2 == 0) g.nk -= 1;
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
     }
     __syncthreads();
     if (SRC == kSrcPsf) {
-        // row sums for the kernel's normalisation (psf.make_kernel.cpp:40-62)
+        // row sums for the kernel's normalisation (psf.make_kernel.cpp:52, 58, 63-72)
         for (int t = 0; t < a.nb; ++t) {
             double sx = 0.0, sy = 0.0;
             for (int col = threadIdx.x; col < n; col += kFftThreads) {
@@ -657,7 +657,7 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_inv_kernel(RowInvArgs a)
 
 // The convolved grid sampled bilinearly at each model-map pixel centre,
 // times the pixel's solid angle and the energy's exposure
-// (model_map.diffuse_src_model_map_wcs.cxx:78-84); zero where the pixel
+// (model_map.diffuse_src_model_map_wcs.cxx:76, 88); zero where the pixel
 // centre falls outside the cube.
 template <typename OT>
 __global__ void __launch_bounds__(256)
@@ -690,7 +690,7 @@ diffuse_sample_kernel(Wcs roi, Wcs up, int frame, double period, double vlo1, do
     }
 }
 
-// Normalised kernel values (psf/psf.make_kernel.cpp:40-88), one block per
+// Normalised kernel values (psf/psf.make_kernel.cpp:45-87), one block per
 // energy.
 __global__ void __launch_bounds__(256)
 psf_kernel_values_kernel(int nk, int E, const int* __restrict__ node,

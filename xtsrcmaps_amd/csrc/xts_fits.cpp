@@ -260,8 +260,17 @@ int64_t type_bytes(char t) {
     }
 }
 
-// The binary tables of a FITS file held in memory, by EXTNAME (upper case).
-int scan_tables(const std::vector<unsigned char>& buf, std::map<std::string, TableHdu>* out) {
+// The primary HDU: its keywords and data.
+struct PrimaryHdu {
+    std::map<std::string, std::string> keys;
+    const unsigned char* data = nullptr;
+    int64_t bytes = 0;
+};
+
+// The binary tables of a FITS file held in memory, by EXTNAME (upper case),
+// and (prim) its primary HDU.
+int scan_tables(const std::vector<unsigned char>& buf, std::map<std::string, TableHdu>* out,
+                PrimaryHdu* prim = nullptr) {
     size_t pos = 0;
     bool first = true;
     while (pos + kBlock <= buf.size()) {
@@ -317,6 +326,11 @@ int scan_tables(const std::vector<unsigned char>& buf, std::map<std::string, Tab
             __builtin_mul_overflow(n, std::max<int64_t>(gcount, 1), &n))
             return fail(XTS_ERR_INVALID, "FITS header: data size overflows");
         if ((uint64_t)n > buf.size() - pos) return fail(XTS_ERR_INVALID, "FITS data truncated");
+        if (prim && !keys.count("XTENSION")) {
+            prim->keys = keys;
+            prim->data = buf.data() + pos;
+            prim->bytes = n;
+        }
         if (keys.count("XTENSION") && upper(keys["XTENSION"]) == "BINTABLE") {
             TableHdu t;
             t.row_bytes = num("NAXIS1", 0);
@@ -467,6 +481,108 @@ extern "C" int xts_write_src_model(const char* path, const double* map, int64_t 
             return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
     }
     if (std::fflush(f.get()) != 0) return fail(XTS_ERR_INVALID, "xts_write_src_model: write failed");
+    return XTS_OK;
+}
+
+// fits::read_allsky_cropped's input (fits/fits.read_allsky_cropped.cpp:13-41,
+// 150-216): the primary image of a MapCube file, its WCS (readWcsParams's
+// keywords; CROTA2 0 when absent, the frame from CTYPE1) and the ENERGIES
+// table's first column.
+extern "C" int xts_read_mapcube(const char* path, xts_wcs* wcs, int64_t* planes, float* data,
+                                double* energies, int64_t* n_energies) {
+    if (!path || !wcs || !planes) return fail(XTS_ERR_INVALID, "xts_read_mapcube: null pointer");
+    std::vector<unsigned char> buf;
+    if (load_file(path, &buf) != XTS_OK) return XTS_ERR_INVALID;
+    std::map<std::string, TableHdu> tables;
+    PrimaryHdu prim;
+    if (scan_tables(buf, &tables, &prim) != XTS_OK) return XTS_ERR_INVALID;
+    auto& k = prim.keys;
+    auto inum = [&](const char* key, int64_t dflt) -> int64_t {
+        return k.count(key) ? std::strtoll(k[key].c_str(), nullptr, 10) : dflt;
+    };
+    bool bad = false;
+    auto real = [&](const char* key, bool required, double dflt) -> double {
+        if (!k.count(key)) {
+            if (required) bad = true;
+            return dflt;
+        }
+        const char* c = k[key].c_str();
+        char* e = nullptr;
+        const double v = std::strtod(c, &e);
+        if (e == c || *e != '\0' || !std::isfinite(v)) bad = true;
+        return v;
+    };
+    const int64_t naxis = inum("NAXIS", 0), bitpix = inum("BITPIX", 8);
+    const int64_t n1 = inum("NAXIS1", 0), n2 = inum("NAXIS2", 0);
+    const int64_t n3 = naxis >= 3 ? inum("NAXIS3", 1) : 1;
+    if ((naxis != 2 && naxis != 3) || n1 <= 0 || n2 <= 0 || n3 <= 0 || n1 > 65536 ||
+        n2 > 65536 || n3 > 65536)
+        return fail(XTS_ERR_INVALID, "'%s': primary image of NAXIS %lld (%lld x %lld x %lld)",
+                    path, (long long)naxis, (long long)n1, (long long)n2, (long long)n3);
+    if (bitpix != -32 && bitpix != -64)
+        return fail(XTS_ERR_INVALID, "'%s': BITPIX %lld (real images only)", path,
+                    (long long)bitpix);
+    const std::string ctype = upper(k.count("CTYPE1") ? k["CTYPE1"] : "");
+    const std::string proj = ctype.size() >= 8 ? ctype.substr(5, 3) : "";
+    if (proj != "CAR" && proj != "AIT")
+        return fail(XTS_ERR_INVALID, "'%s': CTYPE1 '%s' (CAR or AIT only)", path, ctype.c_str());
+    xts_wcs w{};
+    w.naxis1 = n1;
+    w.naxis2 = n2;
+    w.crpix1 = real("CRPIX1", true, 0.0);
+    w.crpix2 = real("CRPIX2", true, 0.0);
+    w.crval1 = real("CRVAL1", true, 0.0);
+    w.crval2 = real("CRVAL2", true, 0.0);
+    w.cdelt1 = real("CDELT1", true, 0.0);
+    w.cdelt2 = real("CDELT2", true, 0.0);
+    w.crota2 = real("CROTA2", false, 0.0);
+    w.proj = proj == "CAR" ? XTS_PROJ_CAR : XTS_PROJ_AIT;
+    w.is_galactic = ctype.compare(0, 4, "GLON") == 0 ? 1 : 0;
+    if (bad || w.cdelt1 == 0.0 || w.cdelt2 == 0.0)
+        return fail(XTS_ERR_INVALID, "'%s': missing or malformed CRPIX/CRVAL/CDELT", path);
+    *wcs = w;
+    *planes = n3;
+    if (data) {
+        const unsigned char* p = prim.data;
+        const size_t n = (size_t)n1 * n2 * n3;
+        for (size_t i = 0; i < n; ++i) {
+            if (bitpix == -32) {
+                const uint32_t u = be32(p + 4 * i);
+                std::memcpy(&data[i], &u, 4);
+            } else {
+                const uint64_t u = ((uint64_t)be32(p + 8 * i) << 32) | be32(p + 8 * i + 4);
+                double d;
+                std::memcpy(&d, &u, 8);
+                data[i] = (float)d;
+            }
+        }
+    }
+    if (n_energies || energies) {
+        auto it = tables.find("ENERGIES");
+        int64_t ne = 0;
+        if (it != tables.end() && !it->second.cols.empty()) {
+            const TableHdu& t = it->second;
+            const Column& c = t.cols[0];
+            if (c.type != 'E' && c.type != 'D')
+                return fail(XTS_ERR_INVALID, "'%s': ENERGIES column 1 is not real", path);
+            ne = t.nrows * c.repeat;
+            if (energies)
+                for (int64_t i = 0; i < ne; ++i) {
+                    const unsigned char* q = t.data + (i / c.repeat) * t.row_bytes + c.offset;
+                    const int64_t j = i % c.repeat;
+                    if (c.type == 'E') {
+                        const uint32_t u = be32(q + 4 * j);
+                        float f;
+                        std::memcpy(&f, &u, 4);
+                        energies[i] = f;
+                    } else {
+                        const uint64_t u = ((uint64_t)be32(q + 8 * j) << 32) | be32(q + 8 * j + 4);
+                        std::memcpy(&energies[i], &u, 8);
+                    }
+                }
+        }
+        if (n_energies) *n_energies = ne;
+    }
     return XTS_OK;
 }
 
