@@ -42,7 +42,8 @@
 //                        back in the same launch
 //   fft_rows_inv_kernel  two rows per inverse transform, real rows out
 //   diffuse_sample_kernel  bilinear sampling, solid angle, exposure
-// The transforms are mixed-radix (2, 3, 4, 5) Stockham passes in LDS, FP64.
+// The transforms are mixed-radix (2, 3, 4, 5) Stockham passes in LDS, in place,
+// FP64.
 #include "xts_internal.hpp"
 
 #include <algorithm>
@@ -141,57 +142,67 @@ template <> __device__ inline void dft<5>(double2* v, double sg) {
 }
 
 // One Stockham pass of radix R over nb transforms of length n laid out one
-// after the other (ns: the product of the radices before this pass).
-// Butterfly j of a transform reads in[j + r n/R], twiddles by
+// after the other in buf (ns: the product of the radices before this pass;
+// nb n <= NMAX).  Butterfly j of a transform reads x[j + r n/R], twiddles by
 // exp(sg 2 pi i r k / (ns R)) with k = j mod ns (from the table tw of the
-// length: m = r k n / (ns R)), and writes its R outputs to out[(j / ns) ns R
-// + k + r ns].  Input and output are two LDS buffers, one barrier a pass.
-template <int R>
-__device__ void fft_pass(const double2* __restrict__ in, double2* __restrict__ out, int n,
-                         int nb, int ns, double sg, const double2* __restrict__ tw) {
+// length: m = r k n / (ns R)), and writes its R outputs to (j / ns) ns R + k
+// + r ns.  Every thread holds its butterflies' inputs in registers across
+// the barrier, so one LDS buffer serves as input and output; NMAX bounds the
+// butterflies a thread holds.
+template <int R, int NMAX>
+__device__ void fft_pass(double2* buf, int n, int nb, int ns, double sg,
+                         const double2* __restrict__ tw) {
+    constexpr int kMaxB = (NMAX / R + kFftThreads - 1) / kFftThreads;
     const int T = n / R;
     const int total = T * nb;
     const int tws = n / (ns * R);
-#pragma unroll 2
-    for (int q = threadIdx.x; q < total; q += kFftThreads) {
-        const int t = q / T, j = q - t * T;
-        const double2* x = in + t * n + j;
-        double2 v[R];
+    double2 v[kMaxB][R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = x[r * T];
-        const int k = j % ns;
-        if (ns > 1) {
+    for (int b = 0; b < kMaxB; ++b) {
+        const int q = threadIdx.x + b * kFftThreads;
+        if (q < total) {
+            const int t = q / T, j = q - t * T;
+            const double2* x = buf + t * n + j;
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const double2 w = tw[r * k * tws];
-                v[r] = cmul(v[r], make_double2(w.x, sg * w.y));
-            }
+            for (int r = 0; r < R; ++r) v[b][r] = x[r * T];
         }
-        dft<R>(v, sg);
-        double2* y = out + t * n + (j / ns) * ns * R + k;
+    }
+    __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; ++r) y[r * ns] = v[r];
+    for (int b = 0; b < kMaxB; ++b) {
+        const int q = threadIdx.x + b * kFftThreads;
+        if (q < total) {
+            const int t = q / T, j = q - t * T;
+            const int k = j % ns;
+            if (ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const double2 w = tw[r * k * tws];
+                    v[b][r] = cmul(v[b][r], make_double2(w.x, sg * w.y));
+                }
+            }
+            dft<R>(v[b], sg);
+            double2* y = buf + t * n + (j / ns) * ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) y[r * ns] = v[b][r];
+        }
     }
     __syncthreads();
 }
 
-// nb transforms of length p.n in a, natural order in and out (b: scratch of
-// the same size); sg -1 forward, +1 inverse (unscaled).  Returns the buffer
-// that holds the result (a or b).
-__device__ double2* fft_run(double2* a, double2* b, const FftPlan& p, int nb, double sg) {
+// nb transforms of length p.n in buf, natural order in and out, in place;
+// sg -1 forward, +1 inverse (unscaled).
+template <int NMAX>
+__device__ void fft_run(double2* buf, const FftPlan& p, int nb, double sg) {
     int ns = 1;
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.radix[s];
-        if (R == 4) fft_pass<4>(a, b, p.n, nb, ns, sg, p.tw);
-        else if (R == 2) fft_pass<2>(a, b, p.n, nb, ns, sg, p.tw);
-        else if (R == 3) fft_pass<3>(a, b, p.n, nb, ns, sg, p.tw);
-        else fft_pass<5>(a, b, p.n, nb, ns, sg, p.tw);
+        if (R == 4) fft_pass<4, NMAX>(buf, p.n, nb, ns, sg, p.tw);
+        else if (R == 2) fft_pass<2, NMAX>(buf, p.n, nb, ns, sg, p.tw);
+        else if (R == 3) fft_pass<3, NMAX>(buf, p.n, nb, ns, sg, p.tw);
+        else fft_pass<5, NMAX>(buf, p.n, nb, ns, sg, p.tw);
         ns *= R;
-        double2* t = a;
-        a = b;
-        b = t;
     }
-    return a;
 }
 
 // Deterministic block sum (fixed reduction order).
@@ -505,8 +516,8 @@ __device__ inline double row_value(const RowArgs& a, int e, int r, int col) {
 // two half spectra X = (Z_k + conj Z_-k) / 2, Y = (Z_k - conj Z_-k) / 2i
 // (k = 0 .. q/2) to spec, stored by column: spec[e][k][r], so that the
 // column pass reads whole columns and this pass writes runs of 2 nb rows.
-// blockIdx.y: energy.  LDS: two buffers of nb q.n values.
-template <int SRC>
+// blockIdx.y: energy.  LDS: nb q.n values.
+template <int SRC, int NMAX>
 __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
     extern __shared__ double2 lds[];
     __shared__ double red[kFftThreads / 64];
@@ -536,7 +547,7 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
             }
         }
     }
-    buf = fft_run(lds, lds + a.nb * n, a.q, a.nb, -1.0);
+    fft_run<NMAX>(buf, a.q, a.nb, -1.0);
     // column k of the spectrum: rows 2 pair0 .. 2 pair0 + 2 nb - 1, X and Y
     // of each transform next to each other
     const int run = 2 * a.nb;
@@ -570,8 +581,8 @@ struct ColArgs {
 // transforms, scaled by 1 / (kernel sum P Q), stored in kspec.
 // kColConvolve: forward, times the kernel's spectrum, inverse, rows c .. c +
 // rows_out - 1 written back to rows 0 .. rows_out - 1 of the same columns.
-// LDS: two buffers of cb p.n values.
-template <int MODE>
+// LDS: cb p.n values.
+template <int MODE, int NMAX>
 __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
     extern __shared__ double2 lds[];
     __shared__ double red[kFftThreads / 64];
@@ -579,15 +590,14 @@ __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
     const int k0 = blockIdx.x * cb;
     const int ncol = min(cb, a.qh - k0);
     double2* spec = a.spec + ((size_t)e * a.qh + k0) * a.rstride;
-    double2* b0 = lds;
-    double2* b1 = lds + cb * n;
+    double2* buf = lds;
     for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads) {
         const int t = idx / n, r = idx - t * n;
-        b0[idx] = (t < ncol && r < a.rows_in) ? spec[(size_t)t * a.rstride + r]
-                                              : make_double2(0.0, 0.0);
+        buf[idx] = (t < ncol && r < a.rows_in) ? spec[(size_t)t * a.rstride + r]
+                                               : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    double2* buf = fft_run(b0, b1, a.p, cb, -1.0);
+    fft_run<NMAX>(buf, a.p, cb, -1.0);
     double2* ks = a.kspec + ((size_t)e * a.qh + k0) * n;
     if (MODE == kColKernel) {
         double tot = 0.0;
@@ -605,8 +615,7 @@ __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
     for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads)
         buf[idx] = idx < ncol * n ? cmul(buf[idx], ks[idx]) : make_double2(0.0, 0.0);
     __syncthreads();
-    double2* other = buf == b0 ? b1 : b0;
-    buf = fft_run(buf, other, a.p, cb, 1.0);
+    fft_run<NMAX>(buf, a.p, cb, 1.0);
     for (int idx = threadIdx.x; idx < ncol * a.rows_out; idx += kFftThreads) {
         const int t = idx / a.rows_out, r = idx - t * a.rows_out;
         spec[(size_t)t * a.rstride + r] = buf[t * n + r + a.c];
@@ -624,7 +633,8 @@ struct RowInvArgs {
 
 // Rows 2p, 2p + 1 from their half spectra: Z = X + iY over all k (X_-k =
 // conj X_k), inverse transform, real part row 2p, imaginary part 2p + 1.
-// LDS: two buffers of nb q.n values.
+// LDS: nb q.n values.
+template <int NMAX>
 __global__ void __launch_bounds__(kFftThreads) fft_rows_inv_kernel(RowInvArgs a) {
     extern __shared__ double2 lds[];
     const int e = blockIdx.y, n = a.q.n;
@@ -644,7 +654,7 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_inv_kernel(RowInvArgs a)
         if (k > 0 && n - k >= a.qh) buf[t * n + n - k] = make_double2(X.x + Y.y, Y.x - X.y);
     }
     __syncthreads();
-    buf = fft_run(lds, lds + a.nb * n, a.q, a.nb, 1.0);
+    fft_run<NMAX>(buf, a.q, a.nb, 1.0);
     for (int idx = threadIdx.x; idx < a.nb * a.cols_out; idx += kFftThreads) {
         const int t = idx / a.cols_out, j = idx - t * a.cols_out;
         const int r0 = 2 * (pair0 + t);
@@ -728,9 +738,32 @@ kernel_transpose_kernel(int nk, int E, const double* __restrict__ in, double* __
 }
 
 // ------------------------------------------------------------------ host
-// Transforms per block for length n: about 1024 values a block (two LDS
-// buffers of 16 KB), so that several blocks share a CU.
-static int choose_nb(int n) { return std::max(1, std::min(8, 1024 / n)); }
+// Transforms per block for length n: at most 1536 values a block (LDS
+// 24 KB), so that several blocks share a CU; NMAX: the kernel variant whose
+// register arrays hold a block of nb n values.
+static int choose_nb(int n) { return std::max(1, std::min(8, 1536 / n)); }
+static int pick_nmax(int elems) {
+    return elems <= 1024 ? 1024 : (elems <= 1536 ? 1536 : (elems <= 2048 ? 2048 : 4096));
+}
+template <typename F>
+static F by_nmax(int elems, F f1024, F f1536, F f2048, F f4096) {
+    const int m = pick_nmax(elems);
+    return m == 1024 ? f1024 : (m == 1536 ? f1536 : (m == 2048 ? f2048 : f4096));
+}
+template <int SRC>
+static void (*rows_fwd_fn(int elems))(RowArgs) {
+    return by_nmax(elems, fft_rows_fwd_kernel<SRC, 1024>, fft_rows_fwd_kernel<SRC, 1536>,
+                   fft_rows_fwd_kernel<SRC, 2048>, fft_rows_fwd_kernel<SRC, 4096>);
+}
+template <int MODE>
+static void (*cols_fn(int elems))(ColArgs) {
+    return by_nmax(elems, fft_cols_kernel<MODE, 1024>, fft_cols_kernel<MODE, 1536>,
+                   fft_cols_kernel<MODE, 2048>, fft_cols_kernel<MODE, 4096>);
+}
+static void (*rows_inv_fn(int elems))(RowInvArgs) {
+    return by_nmax(elems, fft_rows_inv_kernel<1024>, fft_rows_inv_kernel<1536>,
+                   fft_rows_inv_kernel<2048>, fft_rows_inv_kernel<4096>);
+}
 static int spec_stride(int rows) { return (rows + 7) & ~7; }
 
 // The twiddle table of p into tw (p.n entries).
@@ -741,8 +774,7 @@ static int make_twiddles(xts_ctx* ctx, FftPlan* p, double2* tw) {
     return XTS_OK;
 }
 
-// two ping-pong buffers of nb n complex values
-static size_t lds_bytes(int nb, int n) { return 2 * (size_t)nb * n * sizeof(double2); }
+static size_t lds_bytes(int nb, int n) { return (size_t)nb * n * sizeof(double2); }
 
 template <typename A>
 static int launch_lds(void (*kernel)(A), dim3 grid, size_t lds, hipStream_t st, A args) {
@@ -762,9 +794,10 @@ static int rows_forward(hipStream_t st, RowArgs ra, const FftPlan& Qp, int E) {
     ra.E = E;
     const dim3 g(div_up(div_up(ra.nrows, 2), ra.nb), E);
     const size_t lds = lds_bytes(ra.nb, Q);
-    if (ra.node) return launch_lds(fft_rows_fwd_kernel<kSrcPsf>, g, lds, st, ra);
-    if (ra.arr) return launch_lds(fft_rows_fwd_kernel<kSrcArray>, g, lds, st, ra);
-    return launch_lds(fft_rows_fwd_kernel<kSrcCube>, g, lds, st, ra);
+    const int el = ra.nb * Q;
+    if (ra.node) return launch_lds(rows_fwd_fn<kSrcPsf>(el), g, lds, st, ra);
+    if (ra.arr) return launch_lds(rows_fwd_fn<kSrcArray>(el), g, lds, st, ra);
+    return launch_lds(rows_fwd_fn<kSrcCube>(el), g, lds, st, ra);
 }
 
 // Column pass computing the kernel's spectrum kspec [E][qh][P] from its row
@@ -783,7 +816,7 @@ static int kernel_spectrum(hipStream_t st, const FftPlan& Pp, int qh, int E, int
     ca.kspec = kspec;
     ca.rowsum = rowsum;
     ca.scale = scale;
-    return launch_lds(fft_cols_kernel<kColKernel>, dim3(div_up(qh, ca.cb), E),
+    return launch_lds(cols_fn<kColKernel>(ca.cb * Pp.n), dim3(div_up(qh, ca.cb), E),
                       lds_bytes(ca.cb, Pp.n), st, ca);
 }
 
@@ -874,7 +907,7 @@ static int convolve(xts_ctx* ctx, const FftPlan& Pp, const FftPlan& Qp, RowArgs 
     ca.kspec = const_cast<double2*>(kspec);
     {
         TimedRegion tr(ctx, "diffuse_cols");
-        XTS_TRY(launch_lds(fft_cols_kernel<kColConvolve>, dim3(div_up(qh, ca.cb), E),
+        XTS_TRY(launch_lds(cols_fn<kColConvolve>(ca.cb * P), dim3(div_up(qh, ca.cb), E),
                            lds_bytes(ca.cb, P), ctx->stream, ca));
     }
     RowInvArgs ri{};
@@ -887,7 +920,7 @@ static int convolve(xts_ctx* ctx, const FftPlan& Pp, const FftPlan& Qp, RowArgs 
     ri.spec = spec;
     ri.rstride = ra.rstride;
     ri.out = out;
-    return launch_lds(fft_rows_inv_kernel, dim3(div_up(div_up(rows_out, 2), ri.nb), E),
+    return launch_lds(rows_inv_fn(ri.nb * Q), dim3(div_up(div_up(rows_out, 2), ri.nb), E),
                       lds_bytes(ri.nb, Q), ctx->stream, ri);
 }
 
