@@ -191,3 +191,25 @@ def test_gpu_diffuse_benchmark_shape(ctx, orc):
     r = np.abs(got[planes] - ref) / np.abs(ref)
     _log(f"  planes {planes}: max rel {r.max():.2e}")
     assert np.isfinite(got).all() and (got > 0).all() and r.max() < 1e-9
+
+
+def test_gpu_diffuse_map_from_mapcube_file(ctx, tmp_path):
+    """A MapCube FITS file (primary image, ENERGIES table) through
+    xts_read_mapcube into the diffuse map: bit for bit the map of the same
+    cube handed over as an array."""
+    import xtsrcmaps_amd as xa
+    from test_fits_mapcube import _mapcube
+    _log("test_gpu_diffuse_map_from_mapcube_file")
+    rng = np.random.default_rng(8)
+    data = rng.uniform(0.5, 1.5, (3, 181, 360)).astype(np.float32)
+    f = tmp_path / "cube.fits"
+    _mapcube(f, data, energies=[1e2, 1e3, 1e4])
+    w, cube, energies = xa.read_mapcube(f)
+    assert w.is_galactic == 1 and energies is not None and np.array_equal(cube, data)
+    roi = xa.Wcs.make((40, 36), (20.5, 18.5), (193.98, -5.82), (-0.25, 0.25), 0.0, "CAR")
+    E = 4
+    upsf = (1.0 + (_sep()[:, None] / np.linspace(0.5, 2.0, E)[None, :]) ** 2 / 5.0) ** -2.5
+    expo = np.full(E, 1e10)
+    a = ctx.diffuse_src_model_map_wcs(roi, w, cube, upsf, expo, 0.5)
+    b = ctx.diffuse_src_model_map_wcs(roi, w, data, upsf, expo, 0.5)
+    assert np.array_equal(a, b) and np.isfinite(a).all() and (a > 0).all()
