@@ -153,3 +153,60 @@ def test_cpp_pipeline_from_fits_inputs(orc, tmp_path):
     ref = orc.point_src_model_map_wcs(48, 48, oracle_wcs(orc, wcs), src, expo, upsf, parint,
                                       DEG7)
     assert max_rel(got["map"], ref) < 1e-5
+
+
+DIFFUSE_DRIVER = os.path.join(ROOT, "tests", "cpp", "diffuse_driver")
+
+
+def _read_arrays(path):
+    """int64 rank, int64 extents, float64 data, repeated."""
+    buf = open(path, "rb").read()
+    out, pos = [], 0
+    while pos < len(buf):
+        rank = int(np.frombuffer(buf, np.int64, 1, pos)[0])
+        shape = tuple(int(x) for x in np.frombuffer(buf, np.int64, rank, pos + 8))
+        pos += 8 * (1 + rank)
+        n = int(np.prod(shape))
+        out.append(np.frombuffer(buf, np.float64, n, pos).reshape(shape))
+        pos += 8 * n
+    return out
+
+
+def test_diffuse_driver_built_and_usage():
+    if not os.path.exists(DIFFUSE_DRIVER):
+        pytest.fail(f"{DIFFUSE_DRIVER} missing: run python -m xtsrcmaps_amd.build")
+    r = subprocess.run([DIFFUSE_DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_diffuse_steps(tmp_path):
+    """Fermi::ModelMap::diffuse_src_model_map_wcs, Psf::make_kernel and
+    FFT::convolve_map_psf through the C++ layer (tests/cpp/diffuse_driver)
+    equal the C ABI's results on the same inputs; the convolution equals
+    scipy's within 1e-12 of its largest value."""
+    import xtsrcmaps_amd as xa
+    from scipy.signal import fftconvolve
+    from test_fits_mapcube import _mapcube
+    rng = np.random.default_rng(9)
+    data = rng.uniform(0.5, 1.5, (3, 181, 360)).astype(np.float32)
+    f = tmp_path / "cube.fits"
+    _mapcube(f, data, energies=[1e2, 1e3, 1e4])
+    print("test_cpp_diffuse_steps: running the C++ driver", flush=True)
+    r = subprocess.run([DIFFUSE_DRIVER, str(f), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    upsf, expo, cmap, kernel, img, ker, conv = _read_arrays(tmp_path / "out.bin")
+    roi = xa.Wcs.make((40, 36), (20.5, 18.5), (193.98, -5.82), (-0.25, 0.25), 0.0, "CAR")
+    w, cube, _ = xa.read_mapcube(f)
+    with xa.Context(0) as ctx:
+        ref = ctx.diffuse_src_model_map_wcs(roi, w, cube, upsf[0], expo[0], 0.5)
+        kref = ctx.psf_make_kernel(15, roi, roi.crpix1, roi.crpix2, upsf[0])
+        cref = ctx.fft_convolve_map_psf(img, ker)
+    assert cmap.shape == (1, 40, 36, 4)
+    np.testing.assert_array_equal(cmap[0], ref.transpose(1, 2, 0))
+    np.testing.assert_array_equal(kernel, kref)
+    np.testing.assert_array_equal(conv, cref)
+    for c in range(2):
+        s = fftconvolve(img[c], ker[c], mode="full")
+        assert np.abs(conv[c] - s).max() < 1e-12 * np.abs(s).max()

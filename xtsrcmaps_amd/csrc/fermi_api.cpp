@@ -470,6 +470,20 @@ auto compute_psf_data(Obs::XtObs const& obs, Irf::XtIrf const& irf, Exposure::Xt
     return {upsf, par};
 }
 
+auto make_kernel(size_t const Nk, SkyGeom<double> const& skygeom,
+                 Tensor<double, 3> const& uPsf) -> Tensor<double, 3> {
+    require(Nk % 2 == 1 && uPsf.extent(0) >= 1 && uPsf.extent(1) == sep_arr_len,
+            "make_kernel: Nk must be odd and uPsf [S, 401, E]");
+    const size_t E = uPsf.extent(2);
+    const xts_wcs w = skygeom.c_wcs();
+    Tensor<double, 3> out(E, Nk, Nk);
+    // uPsf[0, :, :] is the first sep_arr_len x E block
+    check(xts_psf_make_kernel(xts_context(), (int64_t)Nk, &w, w.crpix1, w.crpix2, uPsf.data(),
+                              (int64_t)E, out.data()),
+          "make_kernel");
+    return out;
+}
+
 }  // namespace Psf
 
 // ------------------------------------------------------------------ cubature
@@ -599,7 +613,57 @@ auto map_integral(Tensor<double, 4> const& model_map, Tensor<double, 2> const& s
     return out;
 }
 
+auto diffuse_src_model_map_wcs(size_t const Nh, size_t const Nw, SkyGeom<double> const& skygeom,
+                               Tensor<double, 2> const& src_sph,
+                               std::vector<std::string> const& src_names,
+                               Tensor<double, 2> const& exposures, Tensor<double, 3> const& uPsf,
+                               Tensor<double, 3> const& partial_integrals,
+                               double const target_pix_size) -> Tensor<double, 4> {
+    require(!src_names.empty() && exposures.extent(0) >= 1 && uPsf.extent(0) >= 1 &&
+                uPsf.extent(1) == Psf::sep_arr_len && uPsf.extent(2) == exposures.extent(1),
+            "diffuse_src_model_map_wcs: a MapCube file name, exposures [S, E], uPsf [S, 401, E]");
+    (void)src_sph;            // the kernel is centred on skygeom's reference direction
+    (void)partial_integrals;  // no point-source correction factor for a diffuse source
+    const size_t E = exposures.extent(1);
+    xts_wcs cube_wcs{};
+    int64_t planes = 0;
+    check(xts_read_mapcube(src_names[0].c_str(), &cube_wcs, &planes, nullptr, nullptr, nullptr),
+          "read_mapcube");
+    std::vector<float> cube((size_t)planes * cube_wcs.naxis1 * cube_wcs.naxis2);
+    check(xts_read_mapcube(src_names[0].c_str(), &cube_wcs, &planes, cube.data(), nullptr,
+                           nullptr),
+          "read_mapcube");
+    const xts_wcs w = image_wcs(skygeom, Nh, Nw);
+    std::vector<double> ehw(E * Nh * Nw);
+    // row 0 of exposures and uPsf: the diffuse source's
+    check(xts_diffuse_src_model_map_wcs(xts_context(), &w, &cube_wcs, cube.data(), planes,
+                                        uPsf.data(), exposures.data(), (int64_t)E,
+                                        target_pix_size, 0.0, ehw.data()),
+          "diffuse_src_model_map_wcs");
+    Tensor<double, 4> out(1, Nh, Nw, E);
+    for (size_t e = 0; e < E; ++e)
+        for (size_t h = 0; h < Nh; ++h)
+            for (size_t x = 0; x < Nw; ++x) out[0, h, x, e] = ehw[(e * Nh + h) * Nw + x];
+    return out;
+}
+
 }  // namespace ModelMap
+
+namespace FFT {
+auto convolve_map_psf(Tensor<double, 3> const& image, Tensor<double, 3> const& kernel)
+    -> Tensor<double, 3> {
+    const size_t C = image.extent(0), H = image.extent(1), W = image.extent(2);
+    const size_t KH = kernel.extent(1), KW = kernel.extent(2);
+    require(kernel.extent(0) == C, "convolve_map_psf: channel dimensions of image and kernel "
+                                   "must match");
+    Tensor<double, 3> out(C, H + KH - 1, W + KW - 1);
+    check(xts_fft_convolve_map_psf(xts_context(), image.data(), (int64_t)C, (int64_t)H,
+                                   (int64_t)W, kernel.data(), (int64_t)KH, (int64_t)KW,
+                                   out.data()),
+          "convolve_map_psf");
+    return out;
+}
+}  // namespace FFT
 
 // ---------------------------------------------------------------- fits
 namespace fits {

@@ -36,6 +36,20 @@ auto map_integral(Tensor<double, 4> const& model_map, Tensor<double, 2> const& s
                   SkyGeom<double> const& skygeom, Tensor<double, 1> const& psf_radius,
                   std::vector<bool> const& is_in_fov) -> Tensor<double, 2>;
 
+// ModelMap::diffuse_src_model_map_wcs (model_map.diffuse_src_model_map_wcs.cxx:8-97,
+// declared but not compiled by the reference): src_names[0] is the MapCube
+// FITS file (the galactic diffuse cube), exposures [S, E] and uPsf [S, 401,
+// E] the diffuse source's (row 0: at the image's reference direction); the
+// cube is cropped, resampled to target_pix_size, convolved per energy with
+// the PSF kernel by FFTs, sampled at the image's pixel centres and scaled by
+// solid angle and exposure.  Returns [1, Nh, Nw, E].
+auto diffuse_src_model_map_wcs(size_t const Nh, size_t const Nw, SkyGeom<double> const& skygeom,
+                               Tensor<double, 2> const& src_sph,
+                               std::vector<std::string> const& src_names,
+                               Tensor<double, 2> const& exposures, Tensor<double, 3> const& uPsf,
+                               Tensor<double, 3> const& partial_integrals,
+                               double const target_pix_size) -> Tensor<double, 4>;
+
 // model_map.compute_src_maps.cxx:6-22
 template <Source::SourceConcept T>
 auto compute_srcmaps(Obs::XtObs const& obs, Source::SourceData<T> src,
@@ -45,3 +59,11 @@ auto compute_srcmaps(Obs::XtObs const& obs, Source::SourceData<T> src,
 }
 
 }  // namespace Fermi::ModelMap
+
+namespace Fermi::FFT {
+// FFT::convolve_map_psf (model_map/mm_fft/fft.convolve_map_psf.cpp:8-39): the
+// full linear convolution of each plane of image [C, H, W] with the same
+// plane of kernel [C, KH, KW]: [C, H + KH - 1, W + KW - 1].
+auto convolve_map_psf(Tensor<double, 3> const& image, Tensor<double, 3> const& kernel)
+    -> Tensor<double, 3>;
+}  // namespace Fermi::FFT

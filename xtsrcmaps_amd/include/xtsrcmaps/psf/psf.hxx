@@ -9,6 +9,7 @@
 #include "xtsrcmaps/exposure/exposure.hxx"
 #include "xtsrcmaps/irf/irf_types.hxx"
 #include "xtsrcmaps/observation/obs_types.hxx"
+#include "xtsrcmaps/sky_geom/sky_geom.hxx"
 #include "xtsrcmaps/tensor/tensor.hpp"
 
 namespace Fermi::Psf {
@@ -64,5 +65,15 @@ auto psf_lookup_table_and_partial_integrals(
 
 auto compute_psf_data(Obs::XtObs const& obs, Irf::XtIrf const& irf,
                       Exposure::XtExp const& exp) -> XtPsf;
+
+// Psf::make_kernel (psf/psf.make_kernel.cpp:8-90): the PSF of uPsf's first
+// source [S, 401, E] on an Nk x Nk block (Nk odd) of skygeom's pixel grid
+// centred at its reference pixel (CRPIX), valued at each pixel's separation
+// from the reference direction and normalised to sum 1 per energy: [E, Nk,
+// Nk].  The reference's block starts at pixel (1, 1), which is the same
+// block when CRPIX is ((Nk + 1) / 2, (Nk + 1) / 2); its two table nodes are
+// blended as the point-source pixel integral blends them.
+auto make_kernel(size_t const Nk, SkyGeom<double> const& skygeom,
+                 Tensor<double, 3> const& uPsf) -> Tensor<double, 3>;
 
 }  // namespace Fermi::Psf
