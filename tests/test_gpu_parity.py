@@ -422,6 +422,7 @@ sys.exit(0 if ok and rel.max() < 1e-5 else 1)
 
 _SPLIT_CHILD = r"""
 import sys
+import numpy as np
 import torch
 sys.path.insert(0, sys.argv[1])
 import bench
@@ -475,9 +476,26 @@ try:
                                        d["out"], inp.rule, flags=flags)
 except Exception:
     refused += 1
+# so does a call that rebuilds the geometry slots (the solid angle of another
+# image, the diffuse map)
+for other in ("solid_angle", "diffuse"):
+    ctx.point_src_model_map_rows_dev(inp.wcs, d["src"], E, inp.rule, flags=flags)
+    if other == "solid_angle":
+        ctx.solid_angle(xa.Wcs.make((16, 12), (8.5, 6.5), (10.0, 20.0), (-0.1, 0.1)))
+    else:
+        cube = xa.Wcs.make((36, 19), (18.5, 10.0), (0.0, 0.0), (-10.0, 10.0), galactic=True)
+        ctx.diffuse_src_model_map_wcs(xa.Wcs.make((16, 12), (8.5, 6.5), (10.0, 20.0),
+                                                  (-0.5, 0.5), galactic=True),
+                                      cube, np.ones((2, 19, 36), np.float32),
+                                      np.ones((401, 2)), np.ones(2), 5.0)
+    try:
+        ctx.point_src_model_map_finish_dev(inp.wcs, d["src"], d["expo"], d["upsf"], d["parint"],
+                                           d["out"], inp.rule, flags=flags)
+    except Exception:
+        refused += 1
 torch.cuda.synchronize()
 print(f"refused {refused}", flush=True)
-ok = rel(a, b) < 2e-7 and rel(c, b) < 2e-7 and refused == 4
+ok = rel(a, b) < 2e-7 and rel(c, b) < 2e-7 and refused == 6
 sys.exit(0 if ok else 1)
 """
 
@@ -487,8 +505,10 @@ def test_gpu_model_map_split_halves():
     equals the one-call entry point, also with the PSF stages computed on a
     second context and stream between the halves, as the bench runs them;
     atomically accumulated map integrals may differ in the last bit, hence
-    2e-7.  A second half without a matching first half is refused.  (Child
-    process: torch before the library.)"""
+    2e-7.  A second half without a matching first half is refused, and so is
+    one after another model map, a change of stream, or a call that rebuilds
+    the geometry (solid angle, diffuse map).  (Child process: torch before
+    the library.)"""
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     _log("test_gpu_model_map_split_halves: c1, two streams (child process)")

@@ -2460,6 +2460,8 @@ static int build_geometry(xts_ctx* ctx, const xts_wcs* wcs, const double* src_sp
 
 int image_solid_angle(xts_ctx* ctx, const xts_wcs* wcs, const double** omega) {
     XTS_TRY(check_wcs(wcs));
+    // the geometry slots are the pending first half's of a split model map
+    ctx->pending_rows.valid = false;
     ImageGeom ig;
     XTS_TRY(build_geometry(ctx, wcs, nullptr, 0, &ig));
     *omega = ig.omega;
@@ -3353,6 +3355,7 @@ int xts_cubature_convolve_map_psf(xts_ctx* ctx, const xts_wcs* wcs, const double
 int xts_solid_angle(xts_ctx* ctx, const xts_wcs* wcs, double* out) {
     XTS_TRY(activate(ctx));
     XTS_TRY(check_wcs(wcs));
+    ctx->pending_rows.valid = false;   // rebuilds the geometry slots
     XTS_CHECK(out != nullptr, "null out");
     ImageGeom ig;
     XTS_TRY(build_geometry(ctx, wcs, nullptr, 0, &ig));
@@ -3367,6 +3370,7 @@ int xts_psf_boundary_radius(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     XTS_TRY(activate(ctx));
     XTS_TRY(check_wcs(wcs));
     XTS_CHECK(src_sph && radius && in_fov && S > 0, "bad arguments");
+    ctx->pending_rows.valid = false;   // rebuilds the geometry slots
     void* dsrc;
     XTS_TRY(to_device(ctx, kSlotIn0, src_sph, S * 2 * 8, &dsrc));
     ImageGeom ig;
@@ -3387,6 +3391,7 @@ int xts_map_integral(xts_ctx* ctx, const xts_wcs* wcs, const double* map, const 
     XTS_TRY(activate(ctx));
     XTS_TRY(check_wcs(wcs));
     XTS_CHECK(map && src_sph && radius && in_fov && out && S > 0 && E > 0, "bad arguments");
+    ctx->pending_rows.valid = false;   // rebuilds the geometry slots
     const int H = (int)wcs->naxis1, W = (int)wcs->naxis2;
     void *dsrc, *dmap, *dg;
     XTS_TRY(to_device(ctx, kSlotIn0, src_sph, S * 2 * 8, &dsrc));
