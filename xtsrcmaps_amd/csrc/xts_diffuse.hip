@@ -58,8 +58,14 @@ constexpr int kFftThreads = 256;
 struct FftPlan {
     int n = 0, nst = 0;
     int radix[16] = {};
-    const double2* tw = nullptr;  // device [n]: (cos, sin)(2 pi m / n)
+    int ns[16] = {};                       // product of the radices before pass s
+    int tws[16] = {};                      // n / (ns R): twiddle table stride
+    unsigned long long inv_t[16] = {};     // ceil(2^32 / (n / R)) of pass s
+    unsigned long long inv_ns[16] = {};    // ceil(2^32 / ns) of pass s
+    const double2* tw = nullptr;           // device [n]: (cos, sin)(2 pi m / n)
 };
+
+static unsigned long long inv32(unsigned d) { return ((1ULL << 32) + d - 1) / d; }
 
 // Transform length n as radix-4, -2, -3 and -5 passes; false if n has
 // another prime factor.
@@ -71,6 +77,15 @@ static bool fft_plan(int n, FftPlan* p) {
     while (m % 2 == 0) { p->radix[p->nst++] = 2; m /= 2; }
     while (m % 3 == 0) { p->radix[p->nst++] = 3; m /= 3; }
     while (m % 5 == 0) { p->radix[p->nst++] = 5; m /= 5; }
+    int ns = 1;
+    for (int k = 0; k < p->nst; ++k) {
+        const int R = p->radix[k];
+        p->ns[k] = ns;
+        p->tws[k] = n / (ns * R);
+        p->inv_t[k] = inv32((unsigned)(n / R));
+        p->inv_ns[k] = inv32((unsigned)ns);
+        ns *= R;
+    }
     return m == 1 && n >= 1;
 }
 
@@ -141,6 +156,13 @@ template <> __device__ inline void dft<5>(double2* v, double sg) {
     v[3] = csub(t2, u2);
 }
 
+// floor(x / d) from inv = ceil(2^32 / d): exact for x d < 2^32 (the error
+// x (inv d - 2^32) / (d 2^32) stays below 1 / d), which holds for the
+// butterfly and position indices here (x < 2^13, d < 2^13).
+__device__ inline int fdiv(int x, unsigned long long inv) {
+    return (int)(((unsigned long long)(unsigned)x * inv) >> 32);
+}
+
 // One Stockham pass of radix R over nb transforms of length n laid out one
 // after the other in buf (ns: the product of the radices before this pass;
 // nb n <= NMAX).  Butterfly j of a transform reads x[j + r n/R], twiddles by
@@ -150,18 +172,18 @@ template <> __device__ inline void dft<5>(double2* v, double sg) {
 // the barrier, so one LDS buffer serves as input and output; NMAX bounds the
 // butterflies a thread holds.
 template <int R, int NMAX>
-__device__ void fft_pass(double2* buf, int n, int nb, int ns, double sg,
+__device__ void fft_pass(double2* buf, int n, int nb, int ns, int tws,
+                         unsigned long long inv_t, unsigned long long inv_ns, double sg,
                          const double2* __restrict__ tw) {
     constexpr int kMaxB = (NMAX / R + kFftThreads - 1) / kFftThreads;
     const int T = n / R;
     const int total = T * nb;
-    const int tws = n / (ns * R);
     double2 v[kMaxB][R];
 #pragma unroll
     for (int b = 0; b < kMaxB; ++b) {
         const int q = threadIdx.x + b * kFftThreads;
         if (q < total) {
-            const int t = q / T, j = q - t * T;
+            const int t = fdiv(q, inv_t), j = q - t * T;
             const double2* x = buf + t * n + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = x[r * T];
@@ -172,8 +194,9 @@ __device__ void fft_pass(double2* buf, int n, int nb, int ns, double sg,
     for (int b = 0; b < kMaxB; ++b) {
         const int q = threadIdx.x + b * kFftThreads;
         if (q < total) {
-            const int t = q / T, j = q - t * T;
-            const int k = j % ns;
+            const int t = fdiv(q, inv_t), j = q - t * T;
+            const int quo = fdiv(j, inv_ns);
+            const int k = j - quo * ns;
             if (ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -182,7 +205,7 @@ __device__ void fft_pass(double2* buf, int n, int nb, int ns, double sg,
                 }
             }
             dft<R>(v[b], sg);
-            double2* y = buf + t * n + (j / ns) * ns * R + k;
+            double2* y = buf + t * n + quo * ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) y[r * ns] = v[b][r];
         }
@@ -194,14 +217,13 @@ __device__ void fft_pass(double2* buf, int n, int nb, int ns, double sg,
 // sg -1 forward, +1 inverse (unscaled).
 template <int NMAX>
 __device__ void fft_run(double2* buf, const FftPlan& p, int nb, double sg) {
-    int ns = 1;
     for (int s = 0; s < p.nst; ++s) {
-        const int R = p.radix[s];
-        if (R == 4) fft_pass<4, NMAX>(buf, p.n, nb, ns, sg, p.tw);
-        else if (R == 2) fft_pass<2, NMAX>(buf, p.n, nb, ns, sg, p.tw);
-        else if (R == 3) fft_pass<3, NMAX>(buf, p.n, nb, ns, sg, p.tw);
-        else fft_pass<5, NMAX>(buf, p.n, nb, ns, sg, p.tw);
-        ns *= R;
+        const int R = p.radix[s], ns = p.ns[s], tws = p.tws[s];
+        const unsigned long long it = p.inv_t[s], in = p.inv_ns[s];
+        if (R == 4) fft_pass<4, NMAX>(buf, p.n, nb, ns, tws, it, in, sg, p.tw);
+        else if (R == 2) fft_pass<2, NMAX>(buf, p.n, nb, ns, tws, it, in, sg, p.tw);
+        else if (R == 3) fft_pass<3, NMAX>(buf, p.n, nb, ns, tws, it, in, sg, p.tw);
+        else fft_pass<5, NMAX>(buf, p.n, nb, ns, tws, it, in, sg, p.tw);
     }
 }
 
@@ -524,10 +546,11 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
     const int e = blockIdx.y, n = a.q.n;
     const int pair0 = blockIdx.x * a.nb;
     double2* buf = lds;
-    for (int idx = threadIdx.x; idx < a.nb * n; idx += kFftThreads) {
-        const int t = idx / n, col = idx - t * n;
+    for (int t = 0; t < a.nb; ++t) {
         const int r0 = 2 * (pair0 + t);
-        buf[idx] = make_double2(row_value<SRC>(a, e, r0, col), row_value<SRC>(a, e, r0 + 1, col));
+        for (int col = threadIdx.x; col < n; col += kFftThreads)
+            buf[t * n + col] =
+                make_double2(row_value<SRC>(a, e, r0, col), row_value<SRC>(a, e, r0 + 1, col));
     }
     __syncthreads();
     if (SRC == kSrcPsf) {
@@ -550,13 +573,14 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_fwd_kernel(RowArgs a) {
     fft_run<NMAX>(buf, a.q, a.nb, -1.0);
     // column k of the spectrum: rows 2 pair0 .. 2 pair0 + 2 nb - 1, X and Y
     // of each transform next to each other
+    // run = 2 nb rows, a power of two dividing the block: each thread keeps
+    // its row s and steps through the columns
     const int run = 2 * a.nb;
     double2* col0 = a.spec + (size_t)e * a.qh * a.rstride + 2 * pair0;
-    for (int idx = threadIdx.x; idx < run * a.qh; idx += kFftThreads) {
-        const int k = idx / run, s = idx - k * run;
-        const int t = s >> 1;
-        if (2 * pair0 + s >= a.nrows) continue;
-        const double2 zk = buf[t * n + k], zm = buf[t * n + (n - k) % n];
+    const int s = threadIdx.x & (run - 1), t = s >> 1;
+    if (2 * pair0 + s < a.nrows)
+        for (int k = threadIdx.x / run; k < a.qh; k += kFftThreads / run) {
+        const double2 zk = buf[t * n + k], zm = buf[t * n + (k == 0 ? 0 : n - k)];
         col0[(size_t)k * a.rstride + s] =
             (s & 1) ? make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x))
                     : make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
@@ -591,11 +615,10 @@ __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
     const int ncol = min(cb, a.qh - k0);
     double2* spec = a.spec + ((size_t)e * a.qh + k0) * a.rstride;
     double2* buf = lds;
-    for (int idx = threadIdx.x; idx < cb * n; idx += kFftThreads) {
-        const int t = idx / n, r = idx - t * n;
-        buf[idx] = (t < ncol && r < a.rows_in) ? spec[(size_t)t * a.rstride + r]
-                                               : make_double2(0.0, 0.0);
-    }
+    for (int t = 0; t < cb; ++t)
+        for (int r = threadIdx.x; r < n; r += kFftThreads)
+            buf[t * n + r] = (t < ncol && r < a.rows_in) ? spec[(size_t)t * a.rstride + r]
+                                                         : make_double2(0.0, 0.0);
     __syncthreads();
     fft_run<NMAX>(buf, a.p, cb, -1.0);
     double2* ks = a.kspec + ((size_t)e * a.qh + k0) * n;
@@ -616,10 +639,9 @@ __global__ void __launch_bounds__(kFftThreads) fft_cols_kernel(ColArgs a) {
         buf[idx] = idx < ncol * n ? cmul(buf[idx], ks[idx]) : make_double2(0.0, 0.0);
     __syncthreads();
     fft_run<NMAX>(buf, a.p, cb, 1.0);
-    for (int idx = threadIdx.x; idx < ncol * a.rows_out; idx += kFftThreads) {
-        const int t = idx / a.rows_out, r = idx - t * a.rows_out;
-        spec[(size_t)t * a.rstride + r] = buf[t * n + r + a.c];
-    }
+    for (int t = 0; t < ncol; ++t)
+        for (int r = threadIdx.x; r < a.rows_out; r += kFftThreads)
+            spec[(size_t)t * a.rstride + r] = buf[t * n + r + a.c];
 }
 
 struct RowInvArgs {
@@ -642,8 +664,10 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_inv_kernel(RowInvArgs a)
     double2* buf = lds;
     const double2* col0 = a.spec + (size_t)e * a.qh * a.rstride + 2 * pair0;
     // Z_k of transform t (k < qh); reads run along t, two rows per thread
-    for (int idx = threadIdx.x; idx < a.nb * a.qh; idx += kFftThreads) {
-        const int k = idx / a.nb, t = idx - k * a.nb;
+    // nb, a power of two dividing the block: each thread keeps its transform
+    // t and steps through the columns
+    const int t = threadIdx.x & (a.nb - 1);
+    for (int k = threadIdx.x / a.nb; k < a.qh; k += kFftThreads / a.nb) {
         const int r0 = 2 * (pair0 + t);
         double2 X = make_double2(0.0, 0.0), Y = make_double2(0.0, 0.0);
         const double2* p = col0 + (size_t)k * a.rstride + 2 * t;
@@ -655,13 +679,14 @@ __global__ void __launch_bounds__(kFftThreads) fft_rows_inv_kernel(RowInvArgs a)
     }
     __syncthreads();
     fft_run<NMAX>(buf, a.q, a.nb, 1.0);
-    for (int idx = threadIdx.x; idx < a.nb * a.cols_out; idx += kFftThreads) {
-        const int t = idx / a.cols_out, j = idx - t * a.cols_out;
-        const int r0 = 2 * (pair0 + t);
-        const double2 z = buf[t * n + j + a.c];
-        double* o = a.out + ((size_t)e * a.rows + r0) * a.cols_out + j;
-        if (r0 < a.rows) *o = z.x;
-        if (r0 + 1 < a.rows) o[a.cols_out] = z.y;
+    for (int tt = 0; tt < a.nb; ++tt) {
+        const int r0 = 2 * (pair0 + tt);
+        double* o = a.out + ((size_t)e * a.rows + r0) * a.cols_out;
+        for (int j = threadIdx.x; j < a.cols_out; j += kFftThreads) {
+            const double2 z = buf[tt * n + j + a.c];
+            if (r0 < a.rows) o[j] = z.x;
+            if (r0 + 1 < a.rows) o[a.cols_out + j] = z.y;
+        }
     }
 }
 
@@ -741,7 +766,11 @@ kernel_transpose_kernel(int nk, int E, const double* __restrict__ in, double* __
 // Transforms per block for length n: at most 1536 values a block (LDS
 // 24 KB), so that several blocks share a CU; NMAX: the kernel variant whose
 // register arrays hold a block of nb n values.
-static int choose_nb(int n) { return std::max(1, std::min(8, 1536 / n)); }
+static int choose_nb(int n) {
+    int nb = 1;
+    while (nb < 8 && 2 * nb * n <= 1536) nb *= 2;
+    return nb;
+}
 static int pick_nmax(int elems) {
     return elems <= 1024 ? 1024 : (elems <= 1536 ? 1536 : (elems <= 2048 ? 2048 : 4096));
 }
