@@ -9,7 +9,8 @@ reference-shaped steps, through the C ABI:
 * the whole map against the FP64 oracle (direct convolution) on small
   cubes: same frame, a celestial map over a galactic cube, a map across the
   longitude seam, a rotated AIT map, a map holding the galactic pole, a map
-  partly outside a cube that does not wrap; within 1e-10 of each value
+  partly outside a cube that does not wrap, an all-sky AIT cube, a cube of
+  one plane at one energy; within 1e-10 of each value
   and 1e-11 of the largest (the model map around the pole, where a CAR
   cube's longitude stretches without bound, takes FP64 rounding of the
   pixel positions to 1e-12), zeros where the oracle has zeros, bitwise
@@ -108,6 +109,17 @@ def _case(xa, name):
         roi = xa.Wcs.make((40, 40), (20.5, 20.5), (0.0, 88.0), (-0.25, 0.25), 0.0, "AIT",
                           galactic=True)
         return roi, cube, 1.0, 0.0
+    if name == "ait_cube":
+        # an all-sky AIT cube (no longitude wrap in its pixels) under a
+        # celestial map
+        ait = xa.Wcs.make((360, 180), (180.5, 90.5), (0.0, 0.0), (-1.0, 1.0), 0.0, "AIT",
+                          galactic=True)
+        roi = xa.Wcs.make((44, 40), (22.5, 20.5), (266.4, -28.9), (-0.25, 0.25), 0.0, "CAR")
+        return roi, ait, 0.5, 0.0
+    if name == "one_plane":
+        roi = xa.Wcs.make((30, 26), (15.5, 13.5), (100.0, -40.0), (-0.5, 0.5), 0.0, "CAR",
+                          galactic=True)
+        return roi, cube, 1.0, 0.0
     # a cube covering 100 x 60 deg that does not wrap, the map across its edge
     part = xa.Wcs.make((100, 60), (50.5, 30.5), (20.0, 0.0), (-1.0, 1.0), 0.0, "CAR",
                        galactic=True)
@@ -117,7 +129,7 @@ def _case(xa, name):
 
 
 @pytest.mark.parametrize("name", ["same", "cel_over_gal", "seam", "rotated_ait", "pole",
-                                  "partial"])
+                                  "partial", "ait_cube", "one_plane"])
 def test_gpu_diffuse_map_vs_oracle(ctx, orc, name):
     import xtsrcmaps_amd as xa
     roi, cube, target, margin = _case(xa, name)
@@ -125,13 +137,18 @@ def test_gpu_diffuse_map_vs_oracle(ctx, orc, name):
     _log(f"test_gpu_diffuse_map_vs_oracle[{name}]: grid {g['m1']}x{g['m2']}, kernel {g['nk']}, "
          f"transforms {g['fft_p']}x{g['fft_q']}")
     rng = np.random.default_rng(11)
-    planes, E = 5, 7
+    planes, E = (1, 1) if name == "one_plane" else (5, 7)
     lat = cube.cdelt2 * (np.arange(1, cube.naxis2 + 1) - cube.crpix2)
     data = (np.exp(-np.abs(lat) / 10.0)[None, :, None] *
             rng.uniform(0.5, 1.5, (planes, cube.naxis2, cube.naxis1))).astype(np.float32)
     sep = _sep()
     upsf = (1.0 + (sep[:, None] / np.linspace(0.3, 3.0, E)[None, :]) ** 2 / 5.0) ** -2.5
     expo = np.linspace(1e10, 3e10, E)
+    if name == "ait_cube":
+        # AIT pixels outside the ellipse hold no sky
+        y = np.arange(1, cube.naxis2 + 1)[:, None] - cube.crpix2
+        x = np.arange(1, cube.naxis1 + 1)[None, :] - cube.crpix1
+        data *= ((x / 180.0) ** 2 + (y / 90.0) ** 2 <= 1.0)[None].astype(np.float32)
     got = ctx.diffuse_src_model_map_wcs(roi, cube, data, upsf, expo, target, margin)
     ref = orc.diffuse_src_model_map_wcs(_ow(orc, roi), roi.is_galactic, _ow(orc, cube),
                                         cube.is_galactic, data, upsf, expo, target, margin)
