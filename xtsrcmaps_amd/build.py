@@ -25,6 +25,10 @@ CXX = os.environ.get("XTS_CXX", "/opt/rocm/llvm/bin/clang++")
 # Packed FP32 ops are disabled for every device pass: the hardware forms of
 # v_pk_{add,mul,fma}_f32 are kept out of generated code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Per-file flags.  xts_modelmap.hip: the surrogate network's matrix-core
+# results stay in VGPRs (the AGPR form added a v_accvgpr_read per
+# accumulator value: 48 per 16 pixels).
+FILE_FLAGS = {"xts_modelmap.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 COMMON = ["-O3", "-fPIC", "-std=c++20", f"--offload-arch={ARCH}",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC,
           "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
@@ -37,7 +41,8 @@ def sources() -> list[str]:
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON, *DEVICE_FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *DEVICE_FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src,
+           "-o", obj]
     if src.endswith(".cpp"):
         # host-only C++23 (the Tensor API uses multi-index operator[])
         cmd = [CXX, "-x", "c++", "-O3", "-fPIC", "-std=c++23", "-I", os.path.join(ROOT, "include"),

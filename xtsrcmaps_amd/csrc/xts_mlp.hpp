@@ -58,10 +58,22 @@ __host__ __device__ inline float mlp_from_bf16(uint16_t h) {
     return f;
 }
 
-// x / (1 + e^-x) from the hardware base-2 exponential and reciprocal
-// (about 1 ulp each; the hidden activations are rounded to BF16 anyway)
-__device__ inline float silu(float x) {
-    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+// The inference records hold the network in base-2 units: the first
+// layer's weights and every layer's biases times -log2(e), the output
+// weights times -ln(2).  A layer's accumulator then holds z = -log2(e) x for
+// the trained pre-activation x, and mlp_act(z) = z / (1 + 2^z) is
+// -log2(e) silu(x), because silu(x) = x / (1 + e^-x) and e^-x = 2^z.  The
+// next layer's products with the unchanged hidden weights, plus its scaled
+// bias, are again that layer's z (the factors -ln(2) and -log2(e) cancel),
+// and the output's weights take the last -ln(2).  This saves the multiply
+// by -log2(e) per activation.  The hardware base-2 exponential and the
+// reciprocal are about 1 ulp each; hidden activations are rounded to BF16
+// (or FP8) anyway.  For large |z|, 2^z goes to 0 (act = z) or to +inf
+// (act = 0), as silu does.
+constexpr float kMlpNegLog2e = -1.4426950408889634f;
+constexpr float kMlpNegLn2 = -0.6931471805599453f;
+__device__ inline float mlp_act(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z));
 }
 
 // Inference record in the matrix cores' operand order.  A 16x16 weight tile
@@ -92,14 +104,15 @@ struct MlpPacked {
 };
 static_assert(sizeof(MlpPacked) % 16 == 0, "MlpPacked size");
 
-// Pack FP32 parameters into the inference record (one block of 256 threads).
+// Pack FP32 parameters into the inference record (one block of 256 threads),
+// in the base-2 units of mlp_act.
 __device__ inline void mlp_pack(const float* __restrict__ p, const float* __restrict__ norm,
                                 MlpPacked* __restrict__ out, int tid, int nthr) {
     for (int k = tid; k < 4 * 64; k += nthr) {
         const int To = k / 64, l = k % 64, g = l / 16, o = 16 * To + l % 16;
         mlp_bf16x4 v;
         for (int j = 0; j < 4; ++j) {
-            const float w = p[kOffW1 + o * kMlpIn + j];
+            const float w = kMlpNegLog2e * p[kOffW1 + o * kMlpIn + j];
             const uint16_t hi = mlp_bf16(w), lo = mlp_bf16(w - mlp_from_bf16(hi));
             v[j] = (short)(g <= 1 ? hi : (g == 2 ? lo : 0));
         }
@@ -121,10 +134,10 @@ __device__ inline void mlp_pack(const float* __restrict__ p, const float* __rest
         dst[1][t][l] = lo8;
     }
     for (int k = tid; k < kMlpH; k += nthr) {
-        out->b1[k] = p[kOffB1 + k];
-        out->b2[k] = p[kOffB2 + k];
-        out->b3[k] = p[kOffB3 + k];
-        out->w4[k] = p[kOffW4 + k];
+        out->b1[k] = kMlpNegLog2e * p[kOffB1 + k];
+        out->b2[k] = kMlpNegLog2e * p[kOffB2 + k];
+        out->b3[k] = kMlpNegLog2e * p[kOffB3 + k];
+        out->w4[k] = kMlpNegLn2 * p[kOffW4 + k];
     }
     if (tid == 0) {
         out->b4 = p[kOffB4];
@@ -146,8 +159,8 @@ __device__ inline mlp_bf16x4 mlp_to_bf16x4(float a, float b, float c, float d) {
     return __builtin_bit_cast(mlp_bf16x4, u);
 }
 
-__device__ inline mlp_bf16x4 mlp_silu_bf16(mlp_f32x4& acc) {
-    return mlp_to_bf16x4(silu(acc[0]), silu(acc[1]), silu(acc[2]), silu(acc[3]));
+__device__ inline mlp_bf16x4 mlp_act_bf16(mlp_f32x4& acc) {
+    return mlp_to_bf16x4(mlp_act(acc[0]), mlp_act(acc[1]), mlp_act(acc[2]), mlp_act(acc[3]));
 }
 
 // Two of the previous layer's 16-output tiles as one 32-wide input chunk.
@@ -190,7 +203,7 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = m->b1[16 * To + 4 * g + i];
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w1[To][lane], in, acc, 0, 0, 0);
-        H[To] = mlp_silu_bf16(acc);
+        H[To] = mlp_act_bf16(acc);
     }
     // hidden layer 2
     mlp_bf16x8 X[2] = {mlp_chunk(H[0], H[1]), mlp_chunk(H[2], H[3])};
@@ -206,7 +219,7 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
         }
     }
 #pragma unroll
-    for (int To = 0; To < 4; ++To) H[To] = mlp_silu_bf16(acc2[To]);
+    for (int To = 0; To < 4; ++To) H[To] = mlp_act_bf16(acc2[To]);
     X[0] = mlp_chunk(H[0], H[1]);
     X[1] = mlp_chunk(H[2], H[3]);
     // hidden layer 3, its activations kept in FP32 for the output dot product
@@ -222,7 +235,7 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
             acc = mlp_mfma32(m->w3[0][To * 2 + Ti][lane], X[Ti], acc);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], silu(acc[i]), y);
+        for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], mlp_act(acc[i]), y);
     }
     // the four lane groups hold disjoint outputs of the same sample
     y += __shfl_xor(y, 16);
@@ -265,9 +278,10 @@ __device__ inline mlp_fp8x8 mlp_fp8_pack8(const float (&v)[8]) {
     return (mlp_fp8x8)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
-// Pack FP32 parameters into the FP8 record (one block); sw2, sw3: the
-// hidden layers' weight scales (largest |w| / kFp8Max, computed by the
-// caller over each layer).
+// Pack FP32 parameters into the FP8 record (one block), in the base-2 units
+// of mlp_act (the hidden weights are unchanged); sw2, sw3: the hidden
+// layers' weight scales (largest |w| / kFp8Max, computed by the caller over
+// each layer).
 __device__ inline void mlp_pack8(const float* __restrict__ p, const float* __restrict__ norm,
                                  float sw2, float sw3, MlpPacked8* __restrict__ out, int tid,
                                  int nthr) {
@@ -275,7 +289,7 @@ __device__ inline void mlp_pack8(const float* __restrict__ p, const float* __res
         const int To = k / 64, l = k % 64, g = l / 16, o = 16 * To + l % 16;
         mlp_bf16x4 v;
         for (int j = 0; j < 4; ++j) {
-            const float w = p[kOffW1 + o * kMlpIn + j];
+            const float w = kMlpNegLog2e * p[kOffW1 + o * kMlpIn + j];
             const uint16_t hi = mlp_bf16(w), lo = mlp_bf16(w - mlp_from_bf16(hi));
             v[j] = (short)(g <= 1 ? hi : (g == 2 ? lo : 0));
         }
@@ -292,10 +306,10 @@ __device__ inline void mlp_pack8(const float* __restrict__ p, const float* __res
         (layer == 0 ? out->w2 : out->w3)[t][l] = mlp_fp8_pack8(v);
     }
     for (int k = tid; k < kMlpH; k += nthr) {
-        out->b1[k] = p[kOffB1 + k];
-        out->b2[k] = p[kOffB2 + k];
-        out->b3[k] = p[kOffB3 + k];
-        out->w4[k] = p[kOffW4 + k];
+        out->b1[k] = kMlpNegLog2e * p[kOffB1 + k];
+        out->b2[k] = kMlpNegLog2e * p[kOffB2 + k];
+        out->b3[k] = kMlpNegLog2e * p[kOffB3 + k];
+        out->w4[k] = kMlpNegLn2 * p[kOffW4 + k];
     }
     if (tid == 0) {
         out->b4 = p[kOffB4];
@@ -365,19 +379,19 @@ __device__ inline float mlp_infer16_fp8(const MlpPacked8* __restrict__ m, const 
         for (int i = 0; i < 4; ++i) acc[i] = m->b1[16 * To + 4 * g + i];
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(m->w1[To][lane], in, acc, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h[To][i] = silu(acc[i]);
+        for (int i = 0; i < 4; ++i) h[To][i] = mlp_act(acc[i]);
     }
     mlp_fp8_layer(m->w2, m->sw2, m->b2, h, z);
 #pragma unroll
     for (int To = 0; To < 4; ++To)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h[To][i] = silu(z[To][i]);
+        for (int i = 0; i < 4; ++i) h[To][i] = mlp_act(z[To][i]);
     mlp_fp8_layer(m->w3, m->sw3, m->b3, h, z);
     float y = 0.0f;
 #pragma unroll
     for (int To = 0; To < 4; ++To)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], silu(z[To][i]), y);
+        for (int i = 0; i < 4; ++i) y = fmaf(m->w4[16 * To + 4 * g + i], mlp_act(z[To][i]), y);
     y += __shfl_xor(y, 16);
     y += __shfl_xor(y, 32);
     return fmaf(y + m->b4, m->norm[8], m->norm[9]);
