@@ -6,6 +6,8 @@
 //   fits_check expcube <file> <table> xts_read_expcube (keywords, then arrays)
 //   fits_check write <file>           xts_write_src_model of a small map,
 //                                     then its HDU read back as a table
+//   fits_check mapcube <file>         xts_read_mapcube (sizes, then the
+//                                     planes and the energies)
 // Prints "rc <code>" per call; exit status 0 unless the sanitizers abort.
 #include <cstdarg>
 #include <cstdint>
@@ -54,6 +56,24 @@ int main(int argc, char** argv) {
             std::vector<float> cb(npix * (size_t)info.nbrbins), ra(npix), dec(npix);
             rc = xts_read_expcube(argv[2], argv[3], &info, cb.data(), ra.data(), dec.data());
             std::printf("rc %d nside %lld\n", rc, (long long)info.nside);
+        }
+        return 0;
+    }
+    if (mode == "mapcube") {
+        xts_wcs w{};
+        int64_t planes = 0, ne = 0;
+        int rc = xts_read_mapcube(argv[2], &w, &planes, nullptr, nullptr, &ne);
+        std::printf("rc %d\n", rc);
+        if (rc == 0) {
+            std::vector<float> data((size_t)(planes * w.naxis1 * w.naxis2));
+            std::vector<double> en((size_t)ne);
+            rc = xts_read_mapcube(argv[2], &w, &planes, data.data(), en.data(), &ne);
+            double sum = 0, esum = 0;
+            for (float v : data) sum += v;
+            for (double v : en) esum += v;
+            std::printf("rc %d planes %lld size %lld x %lld energies %lld sum %g esum %g\n", rc,
+                        (long long)planes, (long long)w.naxis1, (long long)w.naxis2,
+                        (long long)ne, sum, esum);
         }
         return 0;
     }

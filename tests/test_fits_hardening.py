@@ -1,8 +1,9 @@
 """The hand-written FITS parser against malformed headers, built with
 AddressSanitizer and UndefinedBehaviorSanitizer (host code only: the FITS
 entry points and a small driver, tests/cpp_asan/fits_check.cpp).  Every
-crafted file must be refused with an error code; the reference's IRF files
-must still read; no sanitizer may report."""
+crafted file (binary tables, livetime cubes, MapCube images) must be refused
+with an error code; the reference's IRF files must still read; no sanitizer
+may report."""
 import os
 import shutil
 import struct
@@ -137,3 +138,59 @@ def test_reference_irf_files_and_writer_under_sanitizers(driver, tmp_path):
             assert "rc 0 rows" in out, out
     out = _run(driver, "write", str(tmp_path / "maps.fits"))
     assert out.splitlines()[0] == "rc 0", out
+
+
+def _mapcube(n1=6, n2=4, n3=2, bitpix=-32, naxis=3, extra=(), data=None, etform=None, ne=3):
+    cards = [("SIMPLE", "T"), ("BITPIX", bitpix), ("NAXIS", naxis), ("NAXIS1", n1),
+             ("NAXIS2", n2)] + [(f"NAXIS{k}", n3 if k == 3 else 1) for k in range(3, naxis + 1)]
+    cards += [("CTYPE1", "GLON-CAR"), ("CTYPE2", "GLAT-CAR"), ("CRPIX1", 3.5), ("CRPIX2", 2.5),
+              ("CRVAL1", 0.0), ("CRVAL2", 0.0), ("CDELT1", -1.0), ("CDELT2", 1.0)]
+    cards += list(extra)
+    if data is None:
+        dt = {-32: ">f4", -64: ">f8", 16: ">i2"}[bitpix]
+        data = np.arange(n1 * n2 * n3).astype(dt).tobytes()
+    out = _hdu(cards, data)
+    if etform is not None:
+        rep = int(etform[:-1])
+        w = rep * {"D": 8, "E": 4, "J": 4}[etform[-1]]
+        t = [("XTENSION", "BINTABLE"), ("BITPIX", 8), ("NAXIS", 2), ("NAXIS1", w),
+             ("NAXIS2", ne), ("PCOUNT", 0), ("GCOUNT", 1), ("TFIELDS", 1), ("TFORM1", etform),
+             ("EXTNAME", "ENERGIES")]
+        out += _hdu(t, b"\x40" * (w * ne))
+    return out
+
+
+MAPCUBES = {
+    # 12,800 bytes of planes, the file cut after the first data block
+    "truncated_data": _mapcube(n1=40, n2=40, n3=2)[:2 * 2880],
+    "naxis_4": _mapcube(naxis=4),
+    "huge_planes": _mapcube(n3=2 ** 40, data=b""),
+    "bitpix_16": _mapcube(bitpix=16),
+    "zero_cdelt": _mapcube(extra=[("CDELT1", 0.0)]),
+    "malformed_crpix": _mapcube(extra=[("CRPIX1", "abc")]),
+    "tan_projection": _mapcube(extra=[("CTYPE1", "RA---TAN")]),
+    "integer_energies": _mapcube(etform="1J"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MAPCUBES))
+def test_crafted_mapcube_is_refused(driver, tmp_path, name):
+    p = tmp_path / f"{name}.fits"
+    p.write_bytes(MAPCUBES[name])
+    out = _run(driver, "mapcube", str(p))
+    assert out.splitlines()[-1].startswith("rc 1"), out
+
+
+def test_well_formed_mapcubes_read(driver, tmp_path):
+    # 2 planes of 6 x 4 (sum of 0..47 = 1128); energies as 'D' and as a
+    # repeat-2 'E' column (bytes 0x40...: 3.0 as a double, 3.0039 as a float)
+    for name, blob, want in (
+            ("d", _mapcube(etform="1D"), "rc 0 planes 2 size 6 x 4 energies 3 sum 1128"),
+            ("e2", _mapcube(etform="2E", ne=2), "rc 0 planes 2 size 6 x 4 energies 4 sum 1128"),
+            ("none", _mapcube(bitpix=-64), "rc 0 planes 2 size 6 x 4 energies 0 sum 1128"),
+            ("empty_column", _mapcube(etform="0D"), "energies 0 sum 1128"),
+            ("image_2d", _mapcube(naxis=2, n3=1), "rc 0 planes 1 size 6 x 4 energies 0 sum 276")):
+        p = tmp_path / f"{name}.fits"
+        p.write_bytes(blob)
+        out = _run(driver, "mapcube", str(p))
+        assert want in out, (name, out)
