@@ -626,6 +626,7 @@ def main():
                  "batched cubature": (step_batched, base)}
         if sur:
             paths["surrogate"] = (step, base | xa.SURROGATE)
+            paths["surrogate, one pass"] = (step, base | xa.SURROGATE, {"pipeline_batches": 1})
             paths["fp8 surrogate"] = (step, base | xa.SURROGATE | xa.SURROGATE_FP8)
             paths["batched surrogate"] = (step_batched, base | xa.SURROGATE)
         evals = cfg.evals

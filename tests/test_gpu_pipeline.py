@@ -166,18 +166,22 @@ def test_gpu_pipeline_split_halves(ctx, orc, c1):
 
 def test_gpu_pipeline_surrogate(ctx, orc, c1):
     """The surrogate map (BF16 and FP8 networks) in 4 batches, the network
-    pass of each batch on the context stream: equal to the one-pass surrogate
-    map with the same network (a short fit on the 200x200 image) within 2e-6,
-    and finite and positive."""
+    pass of each batch on the context stream, or (tuning key
+    "surrogate_beside") on the second stream as 2 resident blocks per CU:
+    equal to the one-pass surrogate map with the same network (a short fit on
+    the 200x200 image) within 2e-6, and finite and positive."""
     import xtsrcmaps_amd as xa
     from pipelines import oracle_stages
     from xtsrcmaps_amd import surrogate as sg
     _log("test_gpu_pipeline_surrogate: 100 x 37 x 200 x 200, 4 batches")
     o = oracle_stages(orc, c1)
     sg.fit(ctx, c1.wcs, c1.rule, n_samples=1 << 18, steps=300, seed=3)
-    for extra in (xa.SURROGATE, xa.SURROGATE | xa.SURROGATE_FP8):
-        a, b = _maps(ctx, 4, c1.wcs, o["src"], o["expo"], o["upsf"], o["parint"], c1.rule,
-                     extra=extra)
-        r = np.max(_rel(b, a.astype(np.float64)))
-        _log(f"  flags {extra:#x}: batched vs one pass max rel {r:.3e}")
-        assert np.isfinite(b).all() and (b > 0).all() and r < 2e-6
+    for beside in (0, 2):
+        ctx.set_tuning("surrogate_beside", beside)
+        for extra in (xa.SURROGATE, xa.SURROGATE | xa.SURROGATE_FP8):
+            a, b = _maps(ctx, 4, c1.wcs, o["src"], o["expo"], o["upsf"], o["parint"], c1.rule,
+                         extra=extra)
+            r = np.max(_rel(b, a.astype(np.float64)))
+            _log(f"  beside {beside}, flags {extra:#x}: batched vs one pass max rel {r:.3e}")
+            assert np.isfinite(b).all() and (b > 0).all() and r < 2e-6
+    ctx.set_tuning("surrogate_beside", 0)

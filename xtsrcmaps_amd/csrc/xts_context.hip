@@ -283,6 +283,11 @@ int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
         ctx->tune_rw_frames_cu = value;
         return XTS_OK;
     }
+    if (strcmp(key, "surrogate_beside") == 0) {
+        XTS_CHECK(value >= 0 && value <= 8, "surrogate_beside is 0 .. 8");
+        ctx->tune_sur_beside = value;
+        return XTS_OK;
+    }
     if (strcmp(key, "pipeline_batches") == 0) {
         ctx->tune_pipe_batches = value;
         return XTS_OK;

@@ -147,6 +147,8 @@ struct xts_ctx {
     int64_t tune_rw_legacy = 0;            // 1: FP32 row weights by per-source tile blocks
     int64_t tune_emit_wide_apart = 0;      // 1: the pipeline's wide pixels in a launch of their own
     int64_t tune_near_emit_gx = 0;         // >0: the pipeline's near-pixel blocks per source
+    int64_t tune_sur_beside = 4;           // the pipeline's network pass: 0 on the context stream,
+                                           // n on the second stream, n resident blocks per CU
     // the pipelined map's second stream (row weights of the next batch) and
     // its events: [0, B) row weights done, [B, 2B) batch emitted, 2B fork
     hipStream_t pipe_stream = nullptr;

@@ -1338,18 +1338,68 @@ rowweight_frames_kernel(CubArgs a, int nsplit) {
 // wave evaluates the network for 16 pixels at a time (xts_mlp.hpp), forms
 // B1 = |A| + max(y, 0) (y = 0 without the network), the three row weights
 // (poly_rows), stores them x solid angle and adds the in-radius ones to the
-// block's LDS histogram, added to the source's once.  Grid: (blocks
-// striding over the list, S).  Rec: MlpPacked (BF16) or MlpPacked8 (FP8).
+// block's LDS histogram, added to the source's histogram.  Grid: (blocks
+// striding over the tiles, S), the histogram added once per block; or
+// (resident != 0) a 1-D grid of resident blocks striding over (source, run
+// of four tiles) items, the histogram added after each item (the pipelined
+// map's second stream, beside the emit).  Each group's list entries are
+// loaded while the previous group's network runs.  Rec: MlpPacked (BF16) or
+// MlpPacked8 (FP8).
 constexpr int kSurThreads = 256;
 template <typename Rec>
+__device__ inline void sur_tile(const CubArgs& a, const Rec* net, double* hblk, int s, int t,
+                                int ntiles, bool use_net, int lane) {
+    constexpr bool F8 = sizeof(Rec) == sizeof(MlpPacked8);
+    const int count = a.sur_count[(size_t)s * ntiles + t];
+    if (count <= 0) return;
+    const float4* list = a.sur_list + 2 * ((size_t)s * ntiles + t) * kThreads;
+    float4* rw = reinterpret_cast<float4*>(a.rw) + (size_t)s * a.H * a.W;
+    const float wsum = (float)a.wsum;
+    // the tile's entries in groups of 16 (the last group's spare lanes
+    // evaluate the tile's last entry and store nothing)
+    int ic = min(lane & 15, count - 1);
+    float4 f = list[2 * ic], e = list[2 * ic + 1];
+    for (int i0 = 0; i0 < count; i0 += 16) {
+        const int i = i0 + (lane & 15);
+        float4 fn = f, en = e;
+        if (i0 + 16 < count) {
+            ic = min(i + 16, count - 1);
+            fn = list[2 * ic];
+            en = list[2 * ic + 1];
+        }
+        float y = 0.0f;
+        if (use_net) {
+            const float x[4] = {f.x, f.y, f.z, f.w};
+            if constexpr (F8) y = mlp_infer16_fp8(net, x);
+            else y = mlp_infer16(net, x);
+        }
+        if (lane < 16 && i < count) {
+            const float A = e.x, om = e.z;
+            const int pix = __float_as_int(e.y), rf = __float_as_int(e.w);
+            float W[4];
+            poly_rows<1>(wsum, A, fabsf(A) + fmaxf(y, 0.0f), 0.0f, W);
+            const float4 wv = float4{om * W[0], om * W[1], om * W[2], om * W[3]};
+            rw[pix] = wv;
+            if (rf & 1) {
+                const int r0 = rf >> 1;
+                atomicAdd(&hblk[r0], (double)wv.x);
+                atomicAdd(&hblk[r0 + 1], (double)wv.y);
+                atomicAdd(&hblk[r0 + 2], (double)wv.z);
+            }
+        }
+        f = fn;
+        e = en;
+    }
+}
+
+template <typename Rec>
 __global__ void __launch_bounds__(kSurThreads)
-sur_rows_kernel(CubArgs a) {
+sur_rows_kernel(CubArgs a, int resident) {
     __shared__ Rec net;
     __shared__ double hblk[kNDPad];
-    const int s = blockIdx.y;
     const int ntiles = a.tiles_w * a.tiles_h;
     const int waves = kSurThreads / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-    if ((int)blockIdx.x * waves >= ntiles) return;
+    if (!resident && (int)blockIdx.x * waves >= ntiles) return;
     constexpr bool F8 = sizeof(Rec) == sizeof(MlpPacked8);
     const bool use_net = a.sur == (F8 ? 3 : 1);
     if (use_net) {
@@ -1360,43 +1410,30 @@ sur_rows_kernel(CubArgs a) {
     }
     for (int k = threadIdx.x; k < kND; k += kSurThreads) hblk[k] = 0.0;
     __syncthreads();
-    const SrcGeom g = a.geo[s];
-    const bool do_hist = a.hist && g.in_fov;
-    const size_t plane = (size_t)a.H * a.W;
-    float4* rw = reinterpret_cast<float4*>(a.rw) + (size_t)s * plane;
-    const float wsum = (float)a.wsum;
-    // each wave takes whole tiles: their entries in groups of 16 (the last
-    // group's spare lanes evaluate the tile's last entry and store nothing)
-    for (int t = blockIdx.x * waves + wave; t < ntiles; t += gridDim.x * waves) {
-        const int count = a.sur_count[(size_t)s * ntiles + t];
-        const float4* list = a.sur_list + 2 * ((size_t)s * ntiles + t) * kThreads;
-        for (int i0 = 0; i0 < count; i0 += 16) {
-            const int i = i0 + (lane & 15);
-            const int ic = min(i, count - 1);
-            const float4 f = list[2 * ic], e = list[2 * ic + 1];
-            float y = 0.0f;
-            if (use_net) {
-                const float x[4] = {f.x, f.y, f.z, f.w};
-                if constexpr (F8) y = mlp_infer16_fp8(&net, x);
-                else y = mlp_infer16(&net, x);
-            }
-            if (lane < 16 && i < count) {
-                const float A = e.x, om = e.z;
-                const int pix = __float_as_int(e.y), rf = __float_as_int(e.w);
-                float W[4];
-                poly_rows<1>(wsum, A, fabsf(A) + fmaxf(y, 0.0f), 0.0f, W);
-                const float4 wv = float4{om * W[0], om * W[1], om * W[2], om * W[3]};
-                rw[pix] = wv;
-                if (rf & 1) {
-                    const int r0 = rf >> 1;
-                    atomicAdd(&hblk[r0], (double)wv.x);
-                    atomicAdd(&hblk[r0 + 1], (double)wv.y);
-                    atomicAdd(&hblk[r0 + 2], (double)wv.z);
+    if (resident) {
+        const int64_t per_src = (ntiles + waves - 1) / waves;
+        const int64_t nitem = (int64_t)a.S * per_src;
+        for (int64_t it = blockIdx.x; it < nitem; it += gridDim.x) {
+            const int s = (int)(it / per_src);
+            const int t = (int)(it % per_src) * waves + wave;
+            if (t < ntiles) sur_tile<Rec>(a, &net, hblk, s, t, ntiles, use_net, lane);
+            __syncthreads();
+            const bool do_hist = a.hist && a.geo[s].in_fov;
+            for (int k = threadIdx.x; k < kND; k += kSurThreads) {
+                const double v = hblk[k];
+                if (v != 0.0) {
+                    if (do_hist) atomicAdd(&a.hist[(size_t)s * kND + k], v);
+                    hblk[k] = 0.0;
                 }
             }
+            __syncthreads();
         }
+        return;
     }
-    if (do_hist) {
+    const int s = blockIdx.y;
+    for (int t = blockIdx.x * waves + wave; t < ntiles; t += gridDim.x * waves)
+        sur_tile<Rec>(a, &net, hblk, s, t, ntiles, use_net, lane);
+    if (a.hist && a.geo[s].in_fov) {
         __syncthreads();
         for (int k = threadIdx.x; k < kND; k += kSurThreads)
             if (hblk[k] != 0.0) atomicAdd(&a.hist[(size_t)s * kND + k], hblk[k]);
@@ -2601,19 +2638,17 @@ static size_t model_map_counter_doubles(int64_t S) { return (size_t)S; }
 // Source batches of the pipelined map for this call, or 1 (the passes over
 // all sources at once): the FP32 cubature into an FP32 [S, E, H, W] map,
 // full model map, not from a caller's arena; tuning key
-// "pipeline_batches" (0: the default, 1: off, n: n batches).  The surrogate
-// map runs in batches only when asked (its network pass on M between the
-// emits leaves the pipeline no faster: 21.4 against 20.9 ms).  By default a
+// "pipeline_batches" (0: the default, 1: off, n: n batches).  By default a
 // map of at least kPipeMinValues values runs in kPipeDefaultBatches
 // batches (2,000 sources x 37 x 400 x 400: 15.1 ms against 16.4 for the
-// same kernels one after the other, on one MI355X).
+// same kernels one after the other, on one MI355X; with the BF16 surrogate
+// 19.0 against 20.1).
 constexpr int64_t kPipeDefaultBatches = 8;
 constexpr int64_t kPipeMinSources = 16;   // sources per batch at least
 constexpr double kPipeMinValues = 4e9;
 static int64_t pipeline_batches(const xts_ctx* ctx, int64_t S, int64_t plane, int64_t E,
-                                int32_t flags, bool full, int phase, int sur) {
+                                int32_t flags, bool full, int phase) {
     const bool ok = full && !(flags & XTS_COMPUTE_F64) && (flags & XTS_OUT_F32) &&
-                    (!sur || ctx->tune_pipe_batches > 1) &&
                     (flags & XTS_LAYOUT_SEHW) && !(phase & kPhaseSum) &&
                     ctx->arena.base == nullptr && phase != kPhaseEmit;
     if (!ok) return 1;
@@ -2634,6 +2669,24 @@ static int64_t rw_frames_per_block(const xts_ctx* ctx, int64_t S, int64_t ntiles
 // Launches the tile-frame row-weight pass over the sources of c on stream
 // st: items of one tile and a run of about `per_block` sources, one block
 // per item, or per_cu resident blocks per CU striding over the items.
+// Launches the surrogate's network pass over the sources of c on stream st:
+// blocks striding over each source's tiles (about 4 x want_blocks in all),
+// or per_cu resident blocks per CU striding over (source, run of four tiles)
+// items.
+static int launch_sur(xts_ctx* ctx, const CubArgs& c, int64_t want_blocks, int64_t per_cu,
+                      hipStream_t st) {
+    const int64_t tiles = (int64_t)c.tiles_w * c.tiles_h;
+    const int64_t runs = div_up(tiles, kSurThreads / 64);
+    dim3 grid((unsigned)std::min<int64_t>(std::max<int64_t>(1, 4 * want_blocks / c.S), runs),
+              (unsigned)c.S);
+    if (per_cu > 0) grid = dim3((unsigned)std::min<int64_t>(c.S * runs, per_cu * ctx->cu_count));
+    const int resident = per_cu > 0 ? 1 : 0;
+    if (c.sur == 3) sur_rows_kernel<MlpPacked8><<<grid, kSurThreads, 0, st>>>(c, resident);
+    else sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, st>>>(c, resident);
+    XTS_HIP(hipGetLastError());
+    return XTS_OK;
+}
+
 static int launch_rw_frames(xts_ctx* ctx, const CubArgs& c, int64_t per_block, int64_t per_cu,
                             hipStream_t st) {
     const int64_t ntiles = (int64_t)c.tiles_w * c.tiles_h;
@@ -2647,9 +2700,10 @@ static int launch_rw_frames(xts_ctx* ctx, const CubArgs& c, int64_t per_block, i
 }
 
 // The pipelined map (FP32 [S, E, H, W], B source batches of Sb): a second
-// stream computes the row weights and near rows of batch b + 1 while the
+// stream computes the row weights and near rows of batch b + 1 (with the
+// surrogate, and its network pass: tuning surrogate_beside) while the
 // context stream takes batch b's correction factors, map values and near
-// pixels (with the surrogate, first batch b's network pass).  The row-weight pass (rowweight_frames_kernel) reads no global
+// pixels.  The row-weight pass (rowweight_frames_kernel) reads no global
 // memory per source, so it keeps its pace beside the store-bound emit.  Row
 // weights and lists of two batches alternate between two buffers (parity
 // b & 1): batch b + 1's row weights wait for batch b - 1's map.  kPhaseRows
@@ -2734,6 +2788,13 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         const CubArgs c = batch(b);
         // the first batch's row weights run alone: one block per item
         XTS_TRY(launch_rw_frames(ctx, c, per_block, b == 0 ? 0 : per_cu, R));
+        // the network pass on R as surrogate_beside resident blocks per CU
+        // (4 by default: with 3 to 6 the map takes 19.0-19.1 ms at 2,000
+        // sources x 37 x 400 x 400, with 2 20.9, with 1 30.9; on M, 20.0;
+        // the network needs the occupancy, so the gain is the surrogate row
+        // weights beside the emit)
+        if (c.sur && ctx->tune_sur_beside > 0)
+            XTS_TRY(launch_sur(ctx, c, want_blocks, b == 0 ? 0 : ctx->tune_sur_beside, R));
         near_rows_kernel<float><<<dim3((unsigned)near_gx(c), (unsigned)c.S), kNearThreads,
                                   sm_near, R>>>(c);
         XTS_HIP(hipGetLastError());
@@ -2745,18 +2806,10 @@ static int run_pipeline(xts_ctx* ctx, const CubArgs& ca, int64_t B, int64_t Sb, 
         const CubArgs c = batch(b);
         const size_t b0 = (size_t)(b * Sb);
         XTS_HIP(hipStreamWaitEvent(M, evR[b], 0));
-        if (c.sur) {
-            // the network pass at full occupancy on M (beside the emit its
-            // matrix and VALU work would run at a fraction of the CUs)
-            const int64_t tiles = (int64_t)c.tiles_w * c.tiles_h;
-            const dim3 grid(
-                (unsigned)std::min<int64_t>(std::max<int64_t>(1, 4 * want_blocks / c.S),
-                                            div_up(tiles, kSurThreads / 64)),
-                (unsigned)c.S);
+        if (c.sur && ctx->tune_sur_beside == 0) {
+            // the network pass at full occupancy on M
             TimedRegion tr(ctx, "surrogate");
-            if (c.sur == 3) sur_rows_kernel<MlpPacked8><<<grid, kSurThreads, 0, M>>>(c);
-            else sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, M>>>(c);
-            XTS_HIP(hipGetLastError());
+            XTS_TRY(launch_sur(ctx, c, want_blocks, 0, M));
         }
         {
             TimedRegion tr(ctx, "factor");
@@ -2856,7 +2909,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     // the pipelined map (run_pipeline): the FP32 [S, E, H, W] model map in
     // pipe_b source batches of pipe_sb, row weights of two batches resident
     const int64_t pipe_b = pipe ? pipeline_batches(ctx, S, (int64_t)wcs->naxis1 * wcs->naxis2, E,
-                                                   flags, full, phase, sur)
+                                                   flags, full, phase)
                                 : 1;
     const int64_t pipe_sb = pipe_b > 1 ? (S + pipe_b - 1) / pipe_b : S;
     // the radii are launched with the table conversion below; a pipelined
@@ -3015,14 +3068,8 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     }
     // pass 1b: the surrogate's pixels, blocks stride over each source's list
     if (sur && (phase & kPhaseRows)) {
-        const int64_t tiles = (int64_t)ca.tiles_w * ca.tiles_h;
-        const dim3 grid((unsigned)std::min<int64_t>(std::max<int64_t>(1, 4 * want_blocks / S),
-                                                   div_up(tiles, kSurThreads / 64)),
-                        (unsigned)S);
         TimedRegion tr(ctx, "surrogate");
-        if (sur == 3) sur_rows_kernel<MlpPacked8><<<grid, kSurThreads, 0, ctx->stream>>>(ca);
-        else sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, ctx->stream>>>(ca);
-        XTS_HIP(hipGetLastError());
+        XTS_TRY(launch_sur(ctx, ca, want_blocks, 0, ctx->stream));
     }
     // pass 2a: blocks stride over each source's list
     if (phase & kPhaseRows) {
