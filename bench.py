@@ -717,7 +717,7 @@ def main():
             "target_speedup": 5.0,
             "training_s": n.seconds, "training_samples": n.n_samples,
             "training_steps": n.steps, "final_loss": n.final_loss,
-            "network": "4-64-64-64-1 SiLU, BF16 inference on MFMA (high and low weight parts)"}
+            "network": "4-64-64-64-1 SiLU, BF16 inference on MFMA (layer 1 with the inputs' and weights' high and low BF16 parts, hidden layers plain BF16)"}
     if not args.no_cpu_baseline and cfg is not None:
         ncpu = min(args.cpu_sources, S)
         _log(f"bench: cpu baseline (oracle, {ncpu} sources)")

@@ -112,7 +112,8 @@ int xts_abi_version(void);
  * of the named model-map stages ("cubature": the row-weight pass, "near":
  * the near-pixel rows, "factor", "emit", "wide" and "near_emit": the map
  * values of the far, wide and near pixels; the row weights of the pipelined
- * map run on its second stream and are not timed) are bracketed by events on the
+ * map and the surrogate's network pass there run on its second stream and
+ * are not timed) are bracketed by events on the
  * context stream; xts_ctx_kernel_times synchronises and returns (and
  * clears) the summed milliseconds and the launch count of one stage. */
 int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
@@ -123,9 +124,14 @@ int xts_ctx_enable_timing(xts_ctx* ctx, int enable);
  * maps of that image; 0: recomputed by every call); "pipeline_batches" (the
  * FP32 [S, E, H, W] model map in this many source batches, each batch's row
  * weights computed by the same launch as the previous batch's map values;
- * 1: the passes over all sources at once; 0: the default).  Results do not
- * depend on any of them.  Setting a key cancels a pending first half of a
- * split model map. */
+ * 1: the passes over all sources at once; 0: the default); "surrogate_beside"
+ * (the pipelined map's network pass on the second stream as this many
+ * resident blocks per CU, default 4; 0: on the context stream).  Results do
+ * not depend on any of them.  "surrogate_bf16_lo" (0, the default: the BF16
+ * network's hidden weights as plain BF16; 1: as BF16 high and low parts,
+ * FP32-level weights) changes the surrogate's values by rounding (about
+ * 3e-6 relative on the 2,000-source 400x400 map).  Setting a key cancels a
+ * pending first half of a split model map. */
 int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value);
 int xts_ctx_kernel_times(xts_ctx* ctx, const char* name, double* total_ms,
                          int64_t* count);

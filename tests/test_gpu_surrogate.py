@@ -3,14 +3,16 @@
   FP64 backpropagation of tests/mlp_ref.py on a fixed minibatch (relative
   to each parameter block's largest entry, 1e-4: FP32 sums of 200 terms);
 - one Adam step against the update rule (Kingma & Ba 2015);
-- BF16 matrix-core inference against the FP32 forward pass;
+- BF16 matrix-core inference (high and low weight parts) against the FP32
+  forward pass;
 - the network trained on the device from the device's FP64 cubature of the
   2,000-source 400x400 0.05-degree image, then the FP32 model map with the
   surrogate against the FP64 oracle's full cubature for 16 sources at full
   size and 37 energies: at most 1e-4 relative per map value (the metric's
-  bound), and the same pipeline without the network (the first moment's
-  absolute value, i.e. the value at the points' mean separation) at least
-  ten times worse.
+  bound), with plain BF16 hidden weights (the default), with their high and
+  low parts (within 2e-5 of each other) and with FP8, and the same pipeline
+  without the network (the first moment's absolute value, i.e. the value at
+  the points' mean separation) at least ten times worse.
 Child process: torch (device buffers) before the library."""
 import os
 import subprocess
@@ -128,14 +130,14 @@ o = oracle_stages(orc, sub)
 ref = orc.point_src_model_map_wcs(cfg.npix, cfg.npix, oracle_wcs(orc, inp.wcs), o["src"],
                                   o["expo"], o["upsf"], o["parint"], oracle_rule(inp))
 base = xa.LAYOUT_SHWE | xa.OUT_F64 | xa.COMPUTE_F32
-res = {}
+res, maps = {}, {}
 for name, fl in (("cubature", base), ("surrogate", base | xa.SURROGATE),
                  ("fp8 surrogate", base | xa.SURROGATE | xa.SURROGATE_FP8),
                  ("no network", base | xa.SURROGATE | xa.SURROGATE_NO_NET)):
     got = ctx.point_src_model_map_wcs(inp.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
                                       inp.rule, flags=fl)
     rel = np.abs(got - ref) / np.abs(ref)
-    res[name] = rel
+    res[name], maps[name] = rel, got
     print(f"{name}: max rel {rel.max():.3e} median {np.median(rel):.3e} "
           f"p99.99 {np.quantile(rel, 0.9999):.3e}", flush=True)
 check("cubature", res["cubature"].max() < 1e-5, "plain FP32 cubature within 1e-5")
@@ -144,6 +146,18 @@ check("ablation", res["no network"].max() > 10 * res["surrogate"].max(),
       "without the network at least ten times worse")
 check("fp8 map", res["fp8 surrogate"].max() <= 1e-4,
       "FP8 surrogate within 1e-4")
+# the BF16 record with the hidden weights' high and low parts (the default
+# takes their high parts alone)
+ctx.set_tuning("surrogate_bf16_lo", 1)
+hl = ctx.point_src_model_map_wcs(inp.wcs, o["src"], o["expo"], o["upsf"], o["parint"],
+                                 inp.rule, flags=base | xa.SURROGATE)
+ctx.set_tuning("surrogate_bf16_lo", 0)
+rel = np.abs(hl - ref) / np.abs(ref)
+dhl = np.max(np.abs(maps["surrogate"] - hl) / np.abs(hl))
+print(f"surrogate, high and low weight parts: max rel {rel.max():.3e}; plain BF16 weights "
+      f"differ from it by at most {dhl:.2e}", flush=True)
+check("hi/lo map", rel.max() <= 1e-4 and dhl < 2e-5,
+      "surrogate with high and low weight parts within 1e-4, and within 2e-5 of the default")
 sys.exit(0 if ok else 1)
 """
 

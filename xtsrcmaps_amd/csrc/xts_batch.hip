@@ -188,6 +188,8 @@ static int ensure_aux(xts_ctx* ctx) {
     a->tune_tiles_per_block = ctx->tune_tiles_per_block;
     a->surrogate = ctx->surrogate;
     a->surrogate8 = ctx->surrogate8;
+    a->surrogate_h = ctx->surrogate_h;
+    a->tune_sur_lo = ctx->tune_sur_lo;
     a->surrogate_ready = ctx->surrogate_ready;
     return XTS_OK;
 }

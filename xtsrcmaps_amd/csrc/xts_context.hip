@@ -283,6 +283,11 @@ int xts_ctx_set_tuning(xts_ctx* ctx, const char* key, int64_t value) {
         ctx->tune_rw_frames_cu = value;
         return XTS_OK;
     }
+    if (strcmp(key, "surrogate_bf16_lo") == 0) {
+        XTS_CHECK(value == 0 || value == 1, "surrogate_bf16_lo is 0 or 1");
+        ctx->tune_sur_lo = value;
+        return XTS_OK;
+    }
     if (strcmp(key, "surrogate_beside") == 0) {
         XTS_CHECK(value >= 0 && value <= 8, "surrogate_beside is 0 .. 8");
         ctx->tune_sur_beside = value;

@@ -65,7 +65,7 @@ enum SlotId {
     kSlotGeom, kSlotLut, kSlotAcc, kSlotWork0, kSlotWork1, kSlotWork2,
     kSlotRows, kSlotConst, kSlotRule, kSlotSrcGeo, kSlotNearPool, kSlotBilerp,
     kSlotMlpState, kSlotMlpPacked, kSlotSurrogate, kSlotSurList, kSlotSumFac,
-    kSlotSurrogate8, kSlotDiffSep, kSlotDiffSpec, kSlotDiffKspec, kSlotDiffConv, kSlotCount
+    kSlotSurrogate8, kSlotSurrogateH, kSlotDiffSep, kSlotDiffSpec, kSlotDiffKspec, kSlotDiffConv, kSlotCount
 };
 
 }  // namespace xts
@@ -120,6 +120,7 @@ struct xts_ctx {
     // the surrogate network in its matrix-core record (xts_ctx_set_surrogate)
     void* surrogate = nullptr;
     void* surrogate8 = nullptr;   // the same network's FP8 record
+    void* surrogate_h = nullptr;  // its BF16 record without the weights' low parts
     bool surrogate_ready = false;
     // batched engine: the second context its odd batches run on (own slots,
     // own stream), fork/join events, the tuned launch shapes per workload
@@ -147,6 +148,8 @@ struct xts_ctx {
     int64_t tune_rw_legacy = 0;            // 1: FP32 row weights by per-source tile blocks
     int64_t tune_emit_wide_apart = 0;      // 1: the pipeline's wide pixels in a launch of their own
     int64_t tune_near_emit_gx = 0;         // >0: the pipeline's near-pixel blocks per source
+    int64_t tune_sur_lo = 0;               // 0: BF16 network with plain BF16 hidden weights,
+                                           // 1: with their high and low parts
     int64_t tune_sur_beside = 4;           // the pipeline's network pass: 0 on the context stream,
                                            // n on the second stream, n resident blocks per CU
     // the pipelined map's second stream (row weights of the next batch) and

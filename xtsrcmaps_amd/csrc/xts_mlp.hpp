@@ -29,6 +29,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 namespace xts {
 
@@ -94,20 +95,29 @@ __device__ inline float mlp_act(float z) {
 __host__ __device__ inline int mlp_kslot(int Ti, int g, int j) {
     return 32 * Ti + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
 }
-struct MlpPacked {
+// P = 2: the hidden layers' weights as BF16 high and low parts (MlpPacked,
+// FP32-level weights, 35.9 KB); P = 1: the high parts alone (MlpPackedH,
+// plain BF16 weights, 19.5 KB: half the hidden layers' matrix instructions
+// and LDS reads).
+template <int P>
+struct MlpPackedT {
+    static constexpr int kParts = P;
     mlp_bf16x4 w1[4][64];          // [To][lane]
-    mlp_s16x8 w2[2][8][64];        // [hi/lo][To * 2 + Ti][lane]
-    mlp_s16x8 w3[2][8][64];
+    mlp_s16x8 w2[P][8][64];        // [hi/lo][To * 2 + Ti][lane]
+    mlp_s16x8 w3[P][8][64];
     float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
     float b4, norm[kMlpNorm];
     float pad[5];
 };
-static_assert(sizeof(MlpPacked) % 16 == 0, "MlpPacked size");
+using MlpPacked = MlpPackedT<2>;
+using MlpPackedH = MlpPackedT<1>;
+static_assert(sizeof(MlpPacked) % 16 == 0 && sizeof(MlpPackedH) % 16 == 0, "MlpPacked size");
 
 // Pack FP32 parameters into the inference record (one block of 256 threads),
 // in the base-2 units of mlp_act.
+template <int P>
 __device__ inline void mlp_pack(const float* __restrict__ p, const float* __restrict__ norm,
-                                MlpPacked* __restrict__ out, int tid, int nthr) {
+                                MlpPackedT<P>* __restrict__ out, int tid, int nthr) {
     for (int k = tid; k < 4 * 64; k += nthr) {
         const int To = k / 64, l = k % 64, g = l / 16, o = 16 * To + l % 16;
         mlp_bf16x4 v;
@@ -131,7 +141,7 @@ __device__ inline void mlp_pack(const float* __restrict__ p, const float* __rest
         }
         mlp_s16x8(*dst)[8][64] = layer == 0 ? out->w2 : out->w3;
         dst[0][t][l] = hi8;
-        dst[1][t][l] = lo8;
+        if constexpr (P == 2) dst[1][t][l] = lo8;
     }
     for (int k = tid; k < kMlpH; k += nthr) {
         out->b1[k] = kMlpNegLog2e * p[kOffB1 + k];
@@ -192,7 +202,8 @@ __device__ inline mlp_bf16x4 mlp_input_parts(const float* norm, const float (&x)
 // One wave evaluates 16 samples: lane l carries sample l % 16's raw
 // features x[4] (every lane of the sample the same values); the result, for
 // sample l % 16, is returned in every lane.  m: the record (LDS or global).
-__device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float (&x)[4]) {
+template <int P>
+__device__ inline float mlp_infer16(const MlpPackedT<P>* __restrict__ m, const float (&x)[4]) {
     const int lane = threadIdx.x & 63, g = lane >> 4;
     // normalised inputs, split into BF16 high and low parts by lane group
     mlp_bf16x4 in = mlp_input_parts(m->norm, x, g);
@@ -214,7 +225,8 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
         for (int i = 0; i < 4; ++i) acc2[To][i] = m->b2[16 * To + 4 * g + i];
 #pragma unroll
         for (int Ti = 0; Ti < 2; ++Ti) {
-            acc2[To] = mlp_mfma32(m->w2[1][To * 2 + Ti][lane], X[Ti], acc2[To]);
+            if constexpr (P == 2)
+                acc2[To] = mlp_mfma32(m->w2[1][To * 2 + Ti][lane], X[Ti], acc2[To]);
             acc2[To] = mlp_mfma32(m->w2[0][To * 2 + Ti][lane], X[Ti], acc2[To]);
         }
     }
@@ -231,7 +243,7 @@ __device__ inline float mlp_infer16(const MlpPacked* __restrict__ m, const float
         for (int i = 0; i < 4; ++i) acc[i] = m->b3[16 * To + 4 * g + i];
 #pragma unroll
         for (int Ti = 0; Ti < 2; ++Ti) {
-            acc = mlp_mfma32(m->w3[1][To * 2 + Ti][lane], X[Ti], acc);
+            if constexpr (P == 2) acc = mlp_mfma32(m->w3[1][To * 2 + Ti][lane], X[Ti], acc);
             acc = mlp_mfma32(m->w3[0][To * 2 + Ti][lane], X[Ti], acc);
         }
 #pragma unroll

@@ -212,6 +212,7 @@ struct CubArgs {
     int* sur_count;            // [S][tiles] entries of each tile's 256 list slots
     const MlpPacked* sur_net;
     const MlpPacked8* sur_net8;
+    const MlpPackedH* sur_neth;
     // summed model (xts_point_src_summed_model_dev): per-source flux and the
     // FP32 [E][H W] sum the batch's maps x flux are added to; else null
     const double* flux;
@@ -1349,7 +1350,7 @@ constexpr int kSurThreads = 256;
 template <typename Rec>
 __device__ inline void sur_tile(const CubArgs& a, const Rec* net, double* hblk, int s, int t,
                                 int ntiles, bool use_net, int lane) {
-    constexpr bool F8 = sizeof(Rec) == sizeof(MlpPacked8);
+    constexpr bool F8 = std::is_same_v<Rec, MlpPacked8>;
     const int count = a.sur_count[(size_t)s * ntiles + t];
     if (count <= 0) return;
     const float4* list = a.sur_list + 2 * ((size_t)s * ntiles + t) * kThreads;
@@ -1400,10 +1401,13 @@ sur_rows_kernel(CubArgs a, int resident) {
     const int ntiles = a.tiles_w * a.tiles_h;
     const int waves = kSurThreads / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
     if (!resident && (int)blockIdx.x * waves >= ntiles) return;
-    constexpr bool F8 = sizeof(Rec) == sizeof(MlpPacked8);
+    constexpr bool F8 = std::is_same_v<Rec, MlpPacked8>;
     const bool use_net = a.sur == (F8 ? 3 : 1);
     if (use_net) {
-        const float4* src = reinterpret_cast<const float4*>(F8 ? a.sur_net8 : (const void*)a.sur_net);
+        const void* rec = F8 ? (const void*)a.sur_net8
+                             : std::is_same_v<Rec, MlpPackedH> ? (const void*)a.sur_neth
+                                                                : (const void*)a.sur_net;
+        const float4* src = reinterpret_cast<const float4*>(rec);
         float4* dst = reinterpret_cast<float4*>(&net);
         for (int k = threadIdx.x; k < (int)(sizeof(Rec) / 16); k += kSurThreads)
             dst[k] = src[k];
@@ -2682,7 +2686,8 @@ static int launch_sur(xts_ctx* ctx, const CubArgs& c, int64_t want_blocks, int64
     if (per_cu > 0) grid = dim3((unsigned)std::min<int64_t>(c.S * runs, per_cu * ctx->cu_count));
     const int resident = per_cu > 0 ? 1 : 0;
     if (c.sur == 3) sur_rows_kernel<MlpPacked8><<<grid, kSurThreads, 0, st>>>(c, resident);
-    else sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, st>>>(c, resident);
+    else if (ctx->tune_sur_lo) sur_rows_kernel<MlpPacked><<<grid, kSurThreads, 0, st>>>(c, resident);
+    else sur_rows_kernel<MlpPackedH><<<grid, kSurThreads, 0, st>>>(c, resident);
     XTS_HIP(hipGetLastError());
     return XTS_OK;
 }
@@ -2990,6 +2995,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     ca.sur = sur;
     ca.sur_net = sur == 1 ? (const MlpPacked*)ctx->surrogate : nullptr;
     ca.sur_net8 = sur == 3 ? (const MlpPacked8*)ctx->surrogate8 : nullptr;
+    ca.sur_neth = sur == 1 ? (const MlpPackedH*)ctx->surrogate_h : nullptr;
     ca.sur_list = nullptr;
     ca.sur_count = nullptr;
     if (sur) {

@@ -390,9 +390,10 @@ __global__ void fill_kernel(float* __restrict__ x, long long n, float val) {
 }
 
 // ------------------------------------------------------------ BF16 inference
+template <int NP>
 __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ P,
                                                    const float* __restrict__ norm,
-                                                   MlpPacked* __restrict__ out) {
+                                                   MlpPackedT<NP>* __restrict__ out) {
     mlp_pack(P, norm, out, threadIdx.x, blockDim.x);
 }
 
@@ -447,7 +448,7 @@ infer_kernel(const Rec* __restrict__ mp, const float* __restrict__ X, int ldx,
 #pragma unroll
         for (int i = 0; i < 4; ++i) x[i] = X[rr * ldx + i];
         float out;
-        if constexpr (sizeof(Rec) == sizeof(MlpPacked8)) out = mlp_infer16_fp8(&m, x);
+        if constexpr (std::is_same_v<Rec, MlpPacked8>) out = mlp_infer16_fp8(&m, x);
         else out = mlp_infer16(&m, x);
         if (lane < 16 && r < B) y[r] = out;
     }
@@ -581,7 +582,7 @@ int xts_mlp_infer_bf16_dev(xts_ctx* ctx, const float* params, const float* norm,
     XTS_CHECK(y && ldx >= kMlpIn, "null output or row stride below 4");
     void* mp;
     XTS_TRY(slot_reserve(ctx, kSlotMlpPacked, sizeof(MlpPacked), &mp));
-    pack_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked*)mp);
+    pack_kernel<2><<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked*)mp);
     const unsigned grid = std::min<unsigned>(div_up(B, 64), 4096);
     infer_kernel<MlpPacked><<<grid, 256, 0, ctx->stream>>>((const MlpPacked*)mp, x, (int)ldx, B,
                                                            y);
@@ -611,15 +612,18 @@ int xts_ctx_set_surrogate(xts_ctx* ctx, const float* params, const float* norm) 
         return XTS_OK;
     }
     XTS_CHECK(norm, "null normalisation record");
-    void *mp, *mp8;
+    void *mp, *mph, *mp8;
     XTS_TRY(slot_reserve(ctx, kSlotSurrogate, sizeof(MlpPacked), &mp));
+    XTS_TRY(slot_reserve(ctx, kSlotSurrogateH, sizeof(MlpPackedH), &mph));
     XTS_TRY(slot_reserve(ctx, kSlotSurrogate8, sizeof(MlpPacked8), &mp8));
-    pack_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked*)mp);
+    pack_kernel<2><<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked*)mp);
+    pack_kernel<1><<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPackedH*)mph);
     pack8_kernel<<<1, 256, 0, ctx->stream>>>(params, norm, (MlpPacked8*)mp8);
     XTS_HIP(hipGetLastError());
     // packed before any later model map reads it, whatever stream that uses
     XTS_HIP(hipStreamSynchronize(ctx->stream));
     ctx->surrogate = mp;
+    ctx->surrogate_h = mph;
     ctx->surrogate8 = mp8;
     ctx->surrogate_ready = true;
     ++ctx->generation;
