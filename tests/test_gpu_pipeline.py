@@ -167,7 +167,8 @@ def test_gpu_pipeline_split_halves(ctx, orc, c1):
 def test_gpu_pipeline_surrogate(ctx, orc, c1):
     """The surrogate map (BF16 and FP8 networks) in 4 batches, the network
     pass of each batch on the context stream, or (tuning key
-    "surrogate_beside") on the second stream as 2 resident blocks per CU:
+    "surrogate_beside", default 4) on the second stream as 2 resident blocks
+    per CU (out-of-range values of the surrogate's tuning keys refused):
     equal to the one-pass surrogate map with the same network (a short fit on
     the 200x200 image) within 2e-6, and finite and positive."""
     import xtsrcmaps_amd as xa
@@ -184,4 +185,7 @@ def test_gpu_pipeline_surrogate(ctx, orc, c1):
             r = np.max(_rel(b, a.astype(np.float64)))
             _log(f"  beside {beside}, flags {extra:#x}: batched vs one pass max rel {r:.3e}")
             assert np.isfinite(b).all() and (b > 0).all() and r < 2e-6
-    ctx.set_tuning("surrogate_beside", 0)
+    ctx.set_tuning("surrogate_beside", 4)
+    for key, bad in (("surrogate_beside", 9), ("surrogate_bf16_lo", 2)):
+        with pytest.raises(xa.XtsError):
+            ctx.set_tuning(key, bad)
