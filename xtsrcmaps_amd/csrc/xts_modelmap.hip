@@ -2645,19 +2645,24 @@ static size_t model_map_counter_doubles(int64_t S) { return (size_t)S; }
 // "pipeline_batches" (0: the default, 1: off, n: n batches).  By default a
 // map of at least kPipeMinValues values runs in kPipeDefaultBatches
 // batches (2,000 sources x 37 x 400 x 400: 15.1 ms against 16.4 for the
-// same kernels one after the other, on one MI355X; with the BF16 surrogate
-// 19.0 against 20.1).
+// same kernels one after the other, on one MI355X), a surrogate map in
+// kPipeSurBatches (its first batch's row weights and network, and its last
+// emit, run with nothing beside them: 12 batches took 17.9-18.75 ms against
+// 18.1-18.97 for 8 in four same-box comparisons; 19.1-20.0 as one pass).
 constexpr int64_t kPipeDefaultBatches = 8;
+constexpr int64_t kPipeSurBatches = 12;
 constexpr int64_t kPipeMinSources = 16;   // sources per batch at least
 constexpr double kPipeMinValues = 4e9;
 static int64_t pipeline_batches(const xts_ctx* ctx, int64_t S, int64_t plane, int64_t E,
-                                int32_t flags, bool full, int phase) {
+                                int32_t flags, bool full, int phase, int sur) {
     const bool ok = full && !(flags & XTS_COMPUTE_F64) && (flags & XTS_OUT_F32) &&
                     (flags & XTS_LAYOUT_SEHW) && !(phase & kPhaseSum) &&
                     ctx->arena.base == nullptr && phase != kPhaseEmit;
     if (!ok) return 1;
     int64_t B = ctx->tune_pipe_batches;
-    if (B == 0) B = (double)S * plane * E >= kPipeMinValues ? kPipeDefaultBatches : 1;
+    if (B == 0)
+        B = (double)S * plane * E >= kPipeMinValues ? (sur ? kPipeSurBatches : kPipeDefaultBatches)
+                                                    : 1;
     B = std::min<int64_t>(B, S / kPipeMinSources);
     return B > 1 ? B : 1;
 }
@@ -2914,7 +2919,7 @@ static int model_map_device(xts_ctx* ctx, const xts_wcs* wcs, const double* src_
     // the pipelined map (run_pipeline): the FP32 [S, E, H, W] model map in
     // pipe_b source batches of pipe_sb, row weights of two batches resident
     const int64_t pipe_b = pipe ? pipeline_batches(ctx, S, (int64_t)wcs->naxis1 * wcs->naxis2, E,
-                                                   flags, full, phase)
+                                                   flags, full, phase, sur)
                                 : 1;
     const int64_t pipe_sb = pipe_b > 1 ? (S + pipe_b - 1) / pipe_b : S;
     // the radii are launched with the table conversion below; a pipelined
