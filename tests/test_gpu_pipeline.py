@@ -165,7 +165,8 @@ def test_gpu_pipeline_split_halves(ctx, orc, c1):
 
 
 def test_gpu_pipeline_surrogate(ctx, orc, c1):
-    """The surrogate map (BF16 and FP8 networks) in 4 batches, the network
+    """The surrogate map (BF16 and FP8 networks, and the same pixels without
+    the network) in 4 batches, the network
     pass of each batch on the context stream, or (tuning key
     "surrogate_beside", default 4) on the second stream as 2 resident blocks
     per CU (out-of-range values of the surrogate's tuning keys refused):
@@ -179,7 +180,8 @@ def test_gpu_pipeline_surrogate(ctx, orc, c1):
     sg.fit(ctx, c1.wcs, c1.rule, n_samples=1 << 18, steps=300, seed=3)
     for beside in (0, 2):
         ctx.set_tuning("surrogate_beside", beside)
-        for extra in (xa.SURROGATE, xa.SURROGATE | xa.SURROGATE_FP8):
+        for extra in (xa.SURROGATE, xa.SURROGATE | xa.SURROGATE_FP8,
+                      xa.SURROGATE | xa.SURROGATE_NO_NET):
             a, b = _maps(ctx, 4, c1.wcs, o["src"], o["expo"], o["upsf"], o["parint"], c1.rule,
                          extra=extra)
             r = np.max(_rel(b, a.astype(np.float64)))
